@@ -1,0 +1,227 @@
+"""Benchmark of the PM force step on one MI355X.
+
+A step is one PM force evaluation over the whole particle set: cell sort, CIC scatter, r2c FFT,
+fused mask x Green x gradient, the three c2r FFTs and the CIC gather (``pm_gravity``), with the
+particles already resident in HBM.  Default workload: 512^3 particles on a 1024^3 float32 mesh
+in a 1000 Mpc/h box (LCDM Omega_m=0.3, Omega_b=0.05, h=0.7, n_s=0.96, sigma_8=0.8), positions
+= Lagrangian grid + U(-1/2,1/2) l_p jitter (seed 1) + 2LPT displacement at a=1 (white-noise
+seed 0).  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5|c2] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CONFIGS = {
+    # name: (particles per side, mesh ratio, box Mpc/h)
+    "c5": (512, 2, 1000.0),
+    "c2": (256, 2, 256.0),
+    "small": (64, 2, 128.0),
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name, L, n_p):
+    """Algorithmic (compulsory) HBM bytes of one launch of each kernel of the step."""
+    R = 4 * L["nreal"]
+    Ch = 8 * L["nx"] * L["ny"] * (L["nz"] // 2 + 1)
+    table = {
+        "fft_z_r2c": R + Ch,
+        "fft_cols_fwd": 2 * Ch,
+        "fft_cols_green": 3 * Ch,
+        "fft_cols_inv2": 3 * Ch,
+        "fft_cols_inv": 2 * Ch,
+        "fft_z_c2r": Ch + R,
+        "k_fill": R,
+        "k_scatter_sorted": 19 * n_p + R,
+        "k_gather3_sorted": (19 + 4 + 12) * n_p + 3 * R,
+        "k_payload": (4 + 19 + 19) * n_p,
+    }
+    return table.get(name)
+
+
+def official_step_bytes(n_p, L):
+    """Step bytes under the accounting of the reference analysis: each 3-D FFT as 3 axis passes."""
+    R = 4 * L["nreal"]
+    C = 8 * L["nx"] * L["ny"] * (L["nz"] // 2 + 1)
+    return 66 * n_p + (18 * n_p + R) + (R + 5 * C) + 3 * (R + 5 * C) + (34 * n_p + 3 * R)
+
+
+def cpu_baseline(dev, omega_m):
+    """Float32 NumPy/SciPy port of the force step, timed on host cores on a 128^3/256^3 sample."""
+    import numpy as np
+
+    from oracle.cpu_baseline import force_step
+    from pmpp_amd import Configuration
+    from pmpp_amd.initial_conditions import LCDM, particles_2lpt
+
+    ng, ratio, box = 128, 2, 250.0  # same construction and density as the 512^3/1000 Mpc/h workload
+    conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio)
+    ptcl = particles_2lpt(conf, LCDM(), seed=0, jitter_seed=1, device=dev)
+    pm = ptcl.pmid.cpu().numpy()
+    dp = ptcl.disp.cpu().numpy()
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    args = (pm, dp, omega_m, conf.mesh_shape, conf.cell_size, conf.ptcl_grid_shape, conf.ptcl_spacing, cores)
+    force_step(*args)  # warm-up
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        force_step(*args)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": conf.ptcl_num / med,
+        "unit": "particles/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"median of 5 float32 NumPy/SciPy force steps at {ng}^3 particles on a {ng * ratio}^3 mesh "
+                  f"({box:g} Mpc/h, same construction and particle density as the GPU workload; "
+                  f"{med:.2f} s/step, {1.0 / med:.3f} steps/s); CPU: {model or platform.processor()}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU x-slab path is a "
+              "design note, not built in this tree", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from pmpp_amd import Configuration, _lib
+    from pmpp_amd._torch import ensure_init, ptr, stream_handle, workspace
+    from pmpp_amd.initial_conditions import LCDM, particles_2lpt
+    from pmpp_amd.nbody.gravity import gravity_desc
+
+    dev = torch.device("cuda", 0)
+    ng, ratio, box = CONFIGS[args.config]
+    cosmo = LCDM()
+    conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio)
+    log(f"[bench] building 2LPT input: {ng}^3 particles, {ng * ratio}^3 mesh, {box:g} Mpc/h")
+    ptcl = particles_2lpt(conf, cosmo, seed=0, jitter_seed=1, device=dev)
+    torch.cuda.synchronize()
+    ensure_init(dev)
+    d = gravity_desc(conf, cosmo.Omega_m)
+    n = len(ptcl)
+    pm = ptcl.pmid.contiguous()
+    dp = ptcl.disp.contiguous()
+    nbytes = _lib.size_query("pm_gravity_workspace", d, n)
+    ws = workspace(nbytes, dev)
+    acc = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    stream = stream_handle(dev)
+    lib = _lib.load_library()
+    log(f"[bench] workspace {nbytes / 2**30:.2f} GiB; warmup {args.warmup}")
+
+    def step():
+        _lib.check(lib.pm_gravity(ptr(pm), ptr(dp), None, n, d, ptr(acc), ptr(ws), nbytes, stream), "pm_gravity")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    max_events = args.steps * 48 + 16
+    _lib.call("pm_timing_enable", max_events)
+    _lib.call("pm_timing_mark", b"begin", stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (ctypes.c_float * max_events)()
+    names = (ctypes.c_char_p * max_events)()
+    cnt = ctypes.c_int32(0)
+    _lib.call("pm_timing_read", ms, names, max_events, ctypes.byref(cnt))
+    _lib.call("pm_timing_enable", 0)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    log(f"[bench] {ms_per_step:.3f} ms/step over {args.steps} steps")
+
+    L = {"nx": conf.mesh_shape[0], "ny": conf.mesh_shape[1], "nz": conf.mesh_shape[2],
+         "nreal": conf.mesh_size}
+    per = {}
+    for i in range(cnt.value):
+        k = names[i].decode()
+        t, c = per.get(k, (0.0, 0))
+        per[k] = (t + ms[i], c + 1)
+    kernels = {}
+    for k, (t, c) in sorted(per.items(), key=lambda kv: -kv[1][0]):
+        b = kernel_bytes(k, L, n)
+        avg = t / c
+        kernels[k] = {"ms_per_step": t / args.steps, "launches_per_step": c / args.steps, "avg_ms": avg,
+                      "GBps": (b / (avg * 1e-3) / 1e9) if b else None}
+        log(f"[bench]   {k:22s} {t / args.steps:8.3f} ms/step  x{c / args.steps:.0f}  avg {avg:.3f} ms"
+            + (f"  {b / (avg * 1e-3) / 1e9:8.1f} GB/s" if b else ""))
+    dominant = next(k for k in kernels if kernel_bytes(k, L, n))
+    dk = kernels[dominant]
+    achieved = dk["GBps"]
+    off_bytes = official_step_bytes(n, L)
+    roofline = {
+        "bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "algorithmic_bytes_per_launch": kernel_bytes(dominant, L, n), "avg_launch_ms": round(dk["avg_ms"], 4),
+        "step_official_bytes": off_bytes,
+        "step_frac": round(off_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+    }
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (float32 NumPy/SciPy port)")
+        cpu = cpu_baseline(dev, cosmo.Omega_m)
+    value = n / (ms_per_step * 1e-3)
+    out = {
+        "metric": "PM force steps/sec & particles/s (512^3 on 1024^3 mesh, fp32)",
+        "value": value,
+        "unit": "particles/s",
+        "steps_per_s": 1e3 / ms_per_step,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (2LPT LCDM field from seeded white noise + lattice jitter)",
+        "config": {"workload": f"PM force step: {ng}^3 particles, {ng * ratio}^3 mesh, {box:g} Mpc/h",
+                   "particles": n, "mesh": list(conf.mesh_shape), "box": box, "brick": list(conf.brick)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kernels.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+/*
+ * pmpp_amd -- C ABI of the MI355X (gfx950) particle-mesh force-step library.
+ *
+ * Every entry point is extern "C", takes caller-owned device pointers plus plain
+ * sizes, enqueues work on the caller's HIP stream (passed as void*, a hipStream_t),
+ * never synchronises, never allocates device memory and returns a status code; the
+ * text of the last error on the calling host thread is available from pm_last_error().
+ * Temporaries come only from a caller-provided workspace whose size is reported by the
+ * matching *_workspace() query.  This is the ownership/threading contract of the
+ * reference's native FFI handlers (cuda/route_kernels.cu:1-7, 738-800; XLA owns every
+ * buffer and the handler only enqueues on the supplied stream).
+ *
+ * Layouts (all row-major, C order):
+ *   pmid      (n,3) int16   particle mesh anchors      (nbody/particles.py:33-127)
+ *   disp      (n,3) float32 physical displacements     (nbody/particles.py:33-127)
+ *   valid     (n)   uint8   1 = active slot, 0 = padding; NULL = all active
+ *   mesh      (X,Y,Z[,C]) float32
+ *   spectrum  (X,Y,Z/2+1) complex64 interleaved (re,im), the layout of rfftn
+ */
+#ifndef PMPP_AMD_H
+#define PMPP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM_ABI_VERSION 1
+
+enum pm_status {
+  PM_SUCCESS = 0,
+  PM_ERROR_INVALID_ARGUMENT = 1, /* ffi::Error::InvalidArgument analogue */
+  PM_ERROR_UNSUPPORTED = 2,      /* shape outside what the kernels implement */
+  PM_ERROR_WORKSPACE = 3,        /* workspace missing or too small */
+  PM_ERROR_HIP = 4               /* a HIP runtime call or launch failed (ffi::Error::Internal) */
+};
+
+/* CIC geometry shared by the scatter/gather kernels.
+ * Mirrors the keyword arguments of the reference kernel seam
+ * pallas_scatter / pallas_gather (src/pmpp/cic/pallas.py:412-478):
+ *   local_shape  = mesh.shape[:3]          (corners outside it are dropped, pallas.py:261-265)
+ *   global_shape = global_shape            (periodic wrap of the particle mesh)
+ *   offset       = offset (physical units, per axis; pallas.py:107-110)
+ *   particle_cell= particle_cell_size      (conf.cell_size, a1)
+ *   mesh_cell    = cell_size               (explicit a2, <= 0 means None; pallas.py:192-223) */
+typedef struct pm_cic_geom {
+  int32_t local_shape[3];
+  int32_t global_shape[3];
+  float offset[3];
+  float particle_cell;
+  float mesh_cell;
+} pm_cic_geom;
+
+/* Description of one PM force evaluation (src/pmpp/nbody/gravity.py:786-814).
+ *   mesh_shape      conf.mesh_shape
+ *   cell_size       conf.cell_size                       (core/configuration.py:463-466)
+ *   ptcl_spacing    conf.ptcl_spacing (particle-Nyquist mask, core/utils.py:15-37)
+ *   mask_active     1 when conf.mesh_shape != conf.ptcl_grid_shape (utils.py:31-32)
+ *   omega_m         cosmo.Omega_m (cast to float32 as at gravity.py:630)
+ *   particle_value  conf.mesh_size / conf.ptcl_num       (cic/scatter.py:258-265)
+ *   brick           cell-sort granularity (cells per brick along each axis) */
+typedef struct pm_gravity_desc {
+  int32_t mesh_shape[3];
+  float cell_size;
+  float ptcl_spacing;
+  int32_t mask_active;
+  float omega_m;
+  float particle_value;
+  int32_t brick[3];
+} pm_gravity_desc;
+
+const char* pm_last_error(void);
+int pm_abi_version(void);
+/* Prepares per-device constant tables on the current device; called implicitly by
+ * every entry point, exposed so a caller can do it before HIP graph capture. */
+int pm_init(void);
+
+/* Per-kernel timing for measurement harnesses (no counterpart in the reference, whose
+ * only timer is timeit around a jitted call, core/utils.py:347-374).  pm_timing_enable(n)
+ * creates n HIP events for the calling host thread (0 disables); afterwards every library
+ * kernel launch on that thread records the next event on its stream, and pm_timing_mark
+ * records one explicitly.  pm_timing_read waits for the last event and returns, for each
+ * recorded event after the first, its name and the milliseconds since the previous one. */
+int pm_timing_enable(int32_t max_events);
+int pm_timing_mark(const char* name, void* stream);
+int pm_timing_read(float* ms, const char** names, int32_t capacity, int32_t* count);
+
+/* ---------------------------------------------------------------- CIC ----
+ * pm_cic_scatter   replaces pallas_scatter       (src/pmpp/cic/pallas.py:444-478)
+ *                  and the JAX fallback _scatter  (src/pmpp/cic/scatter.py:278-338).
+ *   mesh += sum_p val_p W_p ; val is (n,C) when non-NULL, else the scalar val_scalar.
+ * pm_cic_gather    replaces pallas_gather        (pallas.py:412-441, gather.py:414-472).
+ *   out (n,C) = sum_corners W mesh ; invalid particles read 0.
+ * pm_cic_gather_bwd replaces pallas_gather_bwd   (pallas.py:481-519, gather.py:475-549).
+ *   disp_cot (n,3) is overwritten; mesh_cot is accumulated into (caller zeroes it).
+ * pm_cic_scatter_bwd replaces pallas_scatter_bwd (pallas.py:522-571, scatter.py:341-437).
+ *   disp_cot (n,3) overwritten; val_cot (n,C) overwritten when val != NULL, otherwise
+ *   val_cot[0] is accumulated into (the scalar-value cotangent; caller zeroes it). */
+int pm_cic_scatter(const int16_t* pmid, const float* disp, const uint8_t* valid, const float* val,
+                   float val_scalar, int64_t n, int32_t channels, const pm_cic_geom* geom, float* mesh,
+                   void* stream);
+int pm_cic_gather(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                  int32_t channels, const pm_cic_geom* geom, const float* mesh, float* out,
+                  void* stream);
+int pm_cic_gather_bwd(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                      int32_t channels, const pm_cic_geom* geom, const float* mesh,
+                      const float* val_cot, float* disp_cot, float* mesh_cot, void* stream);
+int pm_cic_scatter_bwd(const int16_t* pmid, const float* disp, const uint8_t* valid,
+                       const float* val, float val_scalar, int64_t n, int32_t channels,
+                       const pm_cic_geom* geom, const float* mesh_cot, float* disp_cot,
+                       float* val_cot, void* stream);
+
+/* ---------------------------------------------------------- cell sort ----
+ * Stable counting sort of particles by the raveled brick key of their containing
+ * cell pmid + floor(disp/cell) (wrapped); invalid particles take the sentinel key
+ * num_bricks and land last.  Replaces the single-device remnant of the canonical
+ * key/order contract: pmid_to_idx (src/pmpp/core/utils.py:292-326), the sentinel
+ * (src/pmpp/distributed/routing.py:112-116) and the stable block-scan compaction
+ * (cuda/route_kernels.cu:194-225).  perm[j] = input index of the j-th sorted particle;
+ * bin_offsets (optional, num_bricks+2 int32) delimits each brick. */
+int pm_sort_cells_workspace(int64_t n, const int32_t mesh_shape[3], const int32_t brick[3],
+                            size_t* bytes);
+int pm_sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                  const int32_t mesh_shape[3], float cell_size, const int32_t brick[3],
+                  int32_t* perm, int32_t* bin_offsets, void* workspace, size_t workspace_bytes,
+                  void* stream);
+
+/* --------------------------------------------------------------- FFT -----
+ * pm_rfft3d  = jnp.fft.rfftn  (unnormalised forward)     used at nbody/gravity.py:436
+ * pm_irfft3d = jnp.fft.irfftn (1/N normalised inverse)   used at nbody/gravity.py:528
+ * Power-of-two axes from 2 to 2048, last axis even. In-place and out-of-place both work;
+ * pm_irfft3d overwrites its input spectrum. */
+int pm_fft_workspace(const int32_t shape[3], size_t* bytes);
+int pm_rfft3d(const float* in, float* out_complex, const int32_t shape[3], void* workspace,
+              size_t workspace_bytes, void* stream);
+int pm_irfft3d(float* in_complex, float* out, const int32_t shape[3], void* workspace,
+               size_t workspace_bytes, void* stream);
+
+/* Fused spectral force multiplier on a standard-layout spectrum:
+ *   out = D_axis(k) * mask(k) * (-1/k^2) * in,  D_axis = -i k_axis (Nyquist derivative 0)
+ * i.e. apply_particle_nyquist_filter + laplace_transposed + neg_ik[axis]
+ * (nbody/gravity.py:174-196, 263-321; core/configuration.py:365-376).  adjoint != 0
+ * multiplies by the conjugate symbol (the spectral adjoint).  axis = -1 gives the
+ * potential (no derivative).  Elementwise, so in-place is allowed; the workspace holds the
+ * float32 wavenumber/mask tables (pm_spectral_workspace). */
+int pm_spectral_workspace(const pm_gravity_desc* desc, size_t* bytes);
+int pm_spectral_force_multiply(const float* in_complex, float* out_complex,
+                               const pm_gravity_desc* desc, int32_t axis, int32_t adjoint,
+                               void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------ gravity ----
+ * The whole single-device force step (nbody/gravity.py:786-814 on the streamed branch
+ * :597-606, 506-543): cell sort, CIC scatter of particle_value, S = 1.5*Omega_m*(rho-1),
+ * r2c FFT, particle-Nyquist mask x Poisson x (-i k_i), three c2r FFTs and the CIC gather.
+ * acc (n,3) float32 in caller order; invalid particles get 0.
+ * pm_gravity_vjp pulls acc_cot back to disp_cot (n,3, overwritten) and accumulates
+ * d/dOmega_m into omega_cot[0] (device float, caller zeroes) -- the force VJP walked by
+ * force_adj (nbody/integrator.py:620-686). */
+int pm_gravity_workspace(const pm_gravity_desc* desc, int64_t n, size_t* bytes);
+int pm_gravity(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+               const pm_gravity_desc* desc, float* acc, void* workspace, size_t workspace_bytes,
+               void* stream);
+int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                   const pm_gravity_desc* desc, const float* acc_cot, float* disp_cot,
+                   float* omega_cot, void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMPP_AMD_H */

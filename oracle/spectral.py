@@ -1,0 +1,153 @@
+"""Float64 NumPy restatement of PM++'s single-device spectral force (gravity).
+
+Follows (read as text, never imported):
+
+* ``src/pmpp/numerics/fft.py:4-44``           -- ``fftfreq``: angular wavenumbers, rfft last axis.
+* ``src/pmpp/core/utils.py:15-37``            -- ``build_particle_nyquist_filter``.
+* ``src/pmpp/core/configuration.py:365-376``  -- cached ``neg_ik`` with the Nyquist derivative zeroed.
+* ``src/pmpp/nbody/gravity.py:74-119, 263-321`` -- ``k^2`` and ``laplace_transposed`` (-S/k^2, k=0 -> 0).
+* ``src/pmpp/nbody/gravity.py:174-196``       -- ``apply_particle_nyquist_filter``.
+* ``src/pmpp/nbody/gravity.py:626-633, 597-606, 506-543, 786-814`` -- the single-device
+  streamed force: scatter, delta*1.5*Omega_m, rfftn, mask, Poisson, -ik_i, irfftn, gather.
+* ``src/pmpp/cic/scatter.py:258-265``        -- default particle value N_m/N_p on valid particles.
+
+The FFTs are ``numpy.fft.rfftn``/``irfftn`` (pocketfft, a library routine), pinned in
+``tests/test_oracle_spectral.py`` against a brute-force O(N^2) DFT on small meshes.  The
+force is pinned by the closed form of a single plane wave (a = -A k0 sin(k0.x)/k0^2 of the
+source A cos(k0.x)), by zero force on a uniform lattice, by momentum conservation, and by the
+linearity in Omega_m (gravity.py:630).  ``gravity_vjp`` is pinned by central finite differences.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .cic import scatter, gather, scatter_adj, gather_adj
+
+F32_EPS = float(np.finfo(np.float32).eps)
+
+
+def fftfreq_angular(n, spacing, last=False):
+    """Angular wavenumbers of one axis (numerics/fft.py:30-41), float64."""
+    period = 2 * np.pi / spacing
+    f = np.fft.rfftfreq(n) if last else np.fft.fftfreq(n)
+    return f * period
+
+
+def kvec(mesh_shape, cell_size):
+    """Per-axis angular wavenumbers (kx, ky, kz_half) in float64."""
+    nx, ny, nz = mesh_shape
+    return (fftfreq_angular(nx, cell_size), fftfreq_angular(ny, cell_size),
+            fftfreq_angular(nz, cell_size, last=True))
+
+
+def particle_nyquist_masks(mesh_shape, ptcl_grid_shape, ptcl_spacing, cell_size):
+    """Per-axis 0/1 masks keeping |k| <= pi/l_p (1 + 8 eps_f32) (core/utils.py:15-37).
+
+    The support is decided in float32 exactly as the reference computes it, so the
+    set of kept modes is identical; an empty tuple means "no filter" (mesh == particle grid).
+    """
+    if tuple(mesh_shape) == tuple(ptcl_grid_shape):
+        return ()
+    k_nyq = np.float32(np.pi / ptcl_spacing)
+    limit = k_nyq + k_nyq * np.float32(8 * F32_EPS)
+    out = []
+    for k in kvec(mesh_shape, cell_size):
+        k32 = k.astype(np.float32)
+        out.append((np.abs(k32) <= limit).astype(np.float64))
+    return tuple(out)
+
+
+def neg_k(mesh_shape, cell_size):
+    """-k_i with the Nyquist derivative zeroed (configuration.py:369-376), float64.
+
+    The multiplier of the spectral gradient is i * neg_k; the zero test is done in
+    float32 as in the reference: | |k| - pi/cell | <= (pi/cell) eps_f32.
+    """
+    nyq = np.float32(np.pi / cell_size)
+    eps = nyq * np.float32(F32_EPS)
+    out = []
+    for k in kvec(mesh_shape, cell_size):
+        k32 = k.astype(np.float32)
+        zero = np.abs(np.abs(k32) - nyq) <= eps
+        out.append(np.where(zero, 0.0, -k))
+    return tuple(out)
+
+
+def green_multiplier(mesh_shape, cell_size, masks=()):
+    """mask(k) * (-1/k^2) with the k=0 mode set to 0 (gravity.py:263-287 + :174-196)."""
+    kx, ky, kz = kvec(mesh_shape, cell_size)
+    k2 = kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2
+    g = np.where(k2 != 0, -1.0 / np.where(k2 == 0, 1.0, k2), 0.0)
+    if masks:
+        g = g * masks[0][:, None, None] * masks[1][None, :, None] * masks[2][None, None, :]
+    return g
+
+
+def force_meshes(density, omega_m, mesh_shape, cell_size, masks=()):
+    """Real-space force components (F_x, F_y, F_z) from a density mesh (gravity.py:626-633, 506-543)."""
+    src = (np.asarray(density, dtype=np.float64) - 1.0) * (1.5 * float(omega_m))
+    src_hat = np.fft.rfftn(src)
+    pot = src_hat * green_multiplier(mesh_shape, cell_size, masks)
+    nk = neg_k(mesh_shape, cell_size)
+    shapes = [(-1, 1, 1), (1, -1, 1), (1, 1, -1)]
+    return tuple(np.fft.irfftn(1j * nk[i].reshape(shapes[i]) * pot, s=tuple(mesh_shape), axes=(0, 1, 2))
+                 for i in range(3))
+
+
+def default_value(n, mesh_shape, ptcl_num, valid=None):
+    """Per-particle CIC value valid * N_m / N_p (scatter.py:258-265)."""
+    v = np.full((n,), float(np.prod(mesh_shape)) / float(ptcl_num))
+    if valid is not None:
+        v = np.where(np.asarray(valid, dtype=bool), v, 0.0)
+    return v
+
+
+def gravity(pmid, disp, omega_m, mesh_shape, cell_size, ptcl_num, masks=(), valid=None):
+    """Accelerations (N,3) of the single-device PM force (gravity.py:786-814), float64."""
+    n = np.asarray(pmid).shape[0]
+    val = default_value(n, mesh_shape, ptcl_num, valid)
+    dens = scatter(pmid, disp, val, np.zeros(tuple(mesh_shape)), cell_size, mesh_shape, valid=valid)
+    comps = force_meshes(dens, omega_m, mesh_shape, cell_size, masks)
+    acc = np.stack([gather(pmid, disp, c, cell_size, mesh_shape, valid=valid) for c in comps], axis=-1)
+    return acc
+
+
+def gravity_vjp(pmid, disp, omega_m, acc_cot, mesh_shape, cell_size, ptcl_num, masks=(), valid=None):
+    """Reverse of ``gravity`` w.r.t. ``disp`` and ``omega_m`` (call stack (2) of the force VJP).
+
+    Order of the pullbacks: gather^T per component (disp cotangent + mesh cotangent), the
+    irfftn transpose, the -ik_i multiply transpose, the self-adjoint Poisson symbol
+    (gravity.py:307-318), the mask, the rfftn transpose, the 1.5 Omega_m scale, and the
+    scatter^T.  Implemented through the exact linear adjoints of numpy's real FFTs.
+    Returns ``(disp_cot (N,3), omega_m_cot)``.
+    """
+    mesh_shape = tuple(mesh_shape)
+    n = np.asarray(pmid).shape[0]
+    val = default_value(n, mesh_shape, ptcl_num, valid)
+    dens = scatter(pmid, disp, val, np.zeros(mesh_shape), cell_size, mesh_shape, valid=valid)
+    comps = force_meshes(dens, omega_m, mesh_shape, cell_size, masks)
+    acc_cot = np.asarray(acc_cot, dtype=np.float64)
+    disp_cot = np.zeros((n, 3))
+    mesh_cots = []
+    for i in range(3):
+        dc, mc = gather_adj(pmid, disp, comps[i], acc_cot[:, i], cell_size, mesh_shape, valid=valid)
+        disp_cot += dc
+        mesh_cots.append(mc)
+    # F_i = irfftn(D_i * G * rfftn(S)), D_i = i neg_k_i.  For a real linear map
+    # F = A S its transpose A^T is obtained from the adjoint-of-irfftn / rfftn pair:
+    # A^T g = irfftn(conj(D_i) G rfftn(g)) exactly, because the composite is a real
+    # convolution whose kernel transposes to its mirror image (conj of a Hermitian symbol).
+    g = green_multiplier(mesh_shape, cell_size, masks)
+    nk = neg_k(mesh_shape, cell_size)
+    shapes = [(-1, 1, 1), (1, -1, 1), (1, 1, -1)]
+    src_cot = np.zeros(mesh_shape)
+    for i in range(3):
+        d = 1j * nk[i].reshape(shapes[i])
+        src_cot += np.fft.irfftn(np.conj(d) * g * np.fft.rfftn(mesh_cots[i]), s=mesh_shape, axes=(0, 1, 2))
+    src = dens - 1.0
+    omega_cot = 1.5 * float(np.sum(src_cot * src))
+    dens_cot = src_cot * (1.5 * float(omega_m))
+    dc, _ = scatter_adj(pmid, disp, val, dens_cot, cell_size, mesh_shape, valid=valid)
+    disp_cot += dc
+    return disp_cot, omega_cot

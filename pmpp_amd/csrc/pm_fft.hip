@@ -1,0 +1,501 @@
+// FFT passes (real<->half-spectrum along z, complex along y/x) and the fused spectral
+// force passes of the PM step.  See pm_fft.h for the decomposition.
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+#include "pm_fft.h"
+#include "pm_fft_launch.h"
+
+namespace pm {
+namespace fft {
+
+// ------------------------------------------------------------ twiddle table ----
+static __device__ float2 g_twiddle[kTwN];
+
+namespace {
+std::mutex g_tw_mutex;
+const float2* g_tw_ptr[64] = {nullptr};
+}  // namespace
+
+const float2* twiddle_table() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  return g_tw_ptr[dev];
+}
+
+}  // namespace fft
+
+int ensure_device_tables() {
+  int dev = 0;
+  PM_HIP_TRY(hipGetDevice(&dev));
+  PM_RETURN_IF(dev < 0 || dev >= 64, PM_ERROR_UNSUPPORTED, "device ordinal >= 64");
+  std::lock_guard<std::mutex> lock(fft::g_tw_mutex);
+  if (fft::g_tw_ptr[dev]) return PM_SUCCESS;
+  std::vector<float2> host(fft::kTwN);
+  for (int i = 0; i < fft::kTwN; ++i) {
+    // exp(-2 pi i j / 4096) in double, rounded once to float
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)i / fft::kTwN;
+    host[i] = make_float2((float)cosl(a), (float)(-sinl(a)));
+  }
+  host[0] = make_float2(1.f, 0.f);
+  host[fft::kTwN / 4] = make_float2(0.f, -1.f);
+  host[fft::kTwN / 2] = make_float2(-1.f, 0.f);
+  host[3 * fft::kTwN / 4] = make_float2(0.f, 1.f);
+  PM_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(fft::g_twiddle), host.data(), sizeof(float2) * fft::kTwN));
+  void* ptr = nullptr;
+  PM_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(fft::g_twiddle)));
+  PM_HIP_TRY(hipDeviceSynchronize());
+  fft::g_tw_ptr[dev] = static_cast<const float2*>(ptr);
+  return PM_SUCCESS;
+}
+
+namespace fft {
+namespace {
+
+// ----------------------------------------------------------- column passes ----
+template <int N>
+struct ColCfg {
+  static constexpr int T = (N >= 2048) ? 8 : 16;  // adjacent columns per workgroup
+  static constexpr int P = Plan<N>::P;
+  static constexpr int THREADS = T * P;
+  static constexpr int PADK = T;  // per-k1-block LDS pad: spreads the strided rows over banks
+  static constexpr int LDS_FLOATS = N * T + Plan<N>::N1 * PADK;
+};
+
+template <int N, int MODE>
+__global__ void __launch_bounds__(ColCfg<N>::THREADS)
+    k_cols(ColPass io, SpecTables tb, int dax, const float2* __restrict__ tw) {
+  using PL = Plan<N>;
+  using CF = ColCfg<N>;
+  constexpr int T = CF::T, E = PL::E, N2 = PL::N2;
+  __shared__ float lds[CF::LDS_FLOATS];
+  const int col = threadIdx.x % T;
+  const int p = threadIdx.x / T;
+  const int cg = blockIdx.x * T + col;
+  const int64_t b = blockIdx.y;
+  const bool cok = cg < io.ncols;
+  const int64_t base = b * io.s_batch + (int64_t)cg * io.s_col;
+  auto idx = [&](int row) { return row * T + col + (row / N2) * CF::PADK; };
+
+  float2 v[E];
+  if constexpr (MODE == kColFwd || MODE == kColGreen) {
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+      v[s] = cok ? io.in[base + (int64_t)nrow<N>(p, s) * io.s_row] : make_float2(0.f, 0.f);
+    forward_core<N>(v, lds, p, idx, tw);
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+      v[s] = cok ? io.in[base + (int64_t)xrow<N>(p, s) * io.s_row] : make_float2(0.f, 0.f);
+  }
+
+  if constexpr (MODE == kColFwd) {
+    if (cok) {
+#pragma unroll
+      for (int s = 0; s < E; ++s) io.out0[base + (int64_t)xrow<N>(p, s) * io.s_row] = v[s];
+    }
+  } else if constexpr (MODE == kColInv) {
+    inverse_core<N>(v, lds, p, idx, tw);
+    if (cok) {
+#pragma unroll
+      for (int s = 0; s < E; ++s) io.out0[base + (int64_t)nrow<N>(p, s) * io.s_row] = v[s];
+    }
+  } else {
+    float2 w[E];
+    if constexpr (MODE == kColGreen) {
+      // mask x Poisson symbol (gravity.py:174-196, 263-287), then D_x = i*negk_x
+      int ky = (int)(b * io.ky_b) + cg * io.ky_c;
+      int kz = (int)(b * io.kz_b) + cg * io.kz_c + io.kz0;
+      float kyv = cok ? tb.k[1][ky] : 0.f, kzv = cok ? tb.k[2][kz] : 0.f;
+      bool keep_yz = cok && tb.mask[1][ky] != 0.f && tb.mask[2][kz] != 0.f;
+#pragma unroll
+      for (int s = 0; s < E; ++s) {
+        int kx = xrow<N>(p, s);
+        float kxv = tb.k[0][kx];
+        float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+        bool keep = keep_yz && tb.mask[0][kx] != 0.f && k2 != 0.f;
+        float2 phi = keep ? make_float2(-v[s].x / k2, -v[s].y / k2) : make_float2(0.f, 0.f);
+        v[s] = phi;
+        w[s] = c_imul(tb.negk[0][kx], phi);
+      }
+    } else {  // kColInv2: second output multiplied by i*negk_dax along this axis
+#pragma unroll
+      for (int s = 0; s < E; ++s) w[s] = c_imul(tb.negk[dax][xrow<N>(p, s)], v[s]);
+    }
+    inverse_core<N>(v, lds, p, idx, tw);
+    if (cok) {
+#pragma unroll
+      for (int s = 0; s < E; ++s) io.out0[base + (int64_t)nrow<N>(p, s) * io.s_row] = v[s];
+    }
+    inverse_core<N>(w, lds, p, idx, tw);
+    if (cok) {
+#pragma unroll
+      for (int s = 0; s < E; ++s) io.out1[base + (int64_t)nrow<N>(p, s) * io.s_row] = w[s];
+    }
+  }
+}
+
+// --------------------------------------------------------------- z passes ----
+template <int M>
+struct ZCfg {
+  static constexpr int P = Plan<M>::P;
+  static constexpr int LINES = 256 / P;
+  static constexpr int THREADS = 256;
+  static constexpr int LSTR = 2 * (M + 2);  // floats per line slot (holds M+1 float2)
+  static constexpr int LDS_FLOATS = LINES * LSTR;
+};
+
+template <int M, bool C2R>
+__global__ void __launch_bounds__(256) k_zpass(ZPass zp, SpecTables tb, const float2* __restrict__ tw) {
+  using PL = Plan<M>;
+  using ZC = ZCfg<M>;
+  constexpr int P = PL::P, E = PL::E, N2 = PL::N2;
+  __shared__ float lds[ZC::LDS_FLOATS];
+  const int p = threadIdx.x % P;
+  const int line = threadIdx.x / P;
+  const int64_t lg = (int64_t)blockIdx.x * ZC::LINES + line;
+  const bool ok = lg < zp.nlines;
+  float* slot = lds + line * ZC::LSTR;
+  float2* nat = reinterpret_cast<float2*>(slot);
+  auto idx = [&](int row) { return line * ZC::LSTR + row + row / N2; };
+  float2 v[E];
+
+  if constexpr (!C2R) {
+    const float* src = zp.rin + lg * zp.pitch_r;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      int m = nrow<M>(p, s);
+      float2 r = ok ? *reinterpret_cast<const float2*>(src + 2 * m) : make_float2(0.f, 0.f);
+      if (ok) r = make_float2((r.x + zp.src_shift) * zp.src_scale, (r.y + zp.src_shift) * zp.src_scale);
+      v[s] = r;
+    }
+    forward_core<M>(v, lds, p, idx, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) nat[xrow<M>(p, s)] = v[s];
+    __syncthreads();
+    float2* dst = zp.cmain + lg * zp.pitch_c;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      int k = p + P * j;
+      float2 a = nat[k];
+      float2 bc = c_conj(nat[(M - k) & (M - 1)]);
+      float2 d = c_sub(a, bc);
+      float2 fe = c_scale(c_add(a, bc), 0.5f);
+      float2 fo = make_float2(0.5f * d.y, -0.5f * d.x);  // -i (a - bc) / 2
+      float2 x = c_add(fe, c_mul(table_tw<2 * M, false>(tw, k), fo));
+      if (ok) dst[k] = x;
+    }
+    if (ok && p == 0) {
+      float2 z0 = nat[0];
+      zp.cnyq[lg * zp.nyq_stride] = make_float2(z0.x - z0.y, 0.f);
+    }
+  } else {
+    const float2* srcm = zp.cmain + lg * zp.pitch_c;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      int k = p + P * j;
+      float2 x = ok ? srcm[k] : make_float2(0.f, 0.f);
+      if (zp.deriv) x = c_imul(tb.negk[2][k], x);
+      nat[k] = x;
+    }
+    if (p == 0) {
+      float2 x = ok ? zp.cnyq[lg * zp.nyq_stride] : make_float2(0.f, 0.f);
+      if (zp.deriv) x = c_imul(tb.negk[2][M], x);
+      nat[M] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      int k = xrow<M>(p, s);
+      float2 xk = nat[k];
+      float2 xm = nat[M - k];
+      if (k == 0) {  // irfft uses only the real parts of the DC and Nyquist bins
+        xk.y = 0.f;
+        xm.y = 0.f;
+      }
+      float2 e = c_scale(c_add(xk, c_conj(xm)), 0.5f);
+      float2 o = c_scale(c_mulconj(c_sub(xk, c_conj(xm)), table_tw<2 * M, false>(tw, k)), 0.5f);
+      v[s] = make_float2(e.x - o.y, e.y + o.x);  // e + i o
+    }
+    __syncthreads();
+    inverse_core<M>(v, lds, p, idx, tw);
+    if (ok) {
+      float* dst = zp.rout + lg * zp.pitch_r;
+#pragma unroll
+      for (int s = 0; s < E; ++s) {
+        int m = nrow<M>(p, s);
+        *reinterpret_cast<float2*>(dst + 2 * m) = make_float2(v[s].x * zp.out_scale, v[s].y * zp.out_scale);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ spectral tables ----
+// Angular wavenumbers k = m * (1/n) * (2 pi / cell) evaluated in double and rounded to
+// float (numerics/fft.py:30-41), the particle-Nyquist masks (core/utils.py:33-37) and
+// negk with the derivative zeroed at the mesh Nyquist (configuration.py:369-376), all
+// decided in float32 like the reference.
+__global__ void k_spec_tables(float* out, int n0, int n1, int n2, double period, float k_nyq_ptcl,
+                              int mask_active, float nyq_mesh) {
+  int n[3] = {n0, n1, n2};
+  int len[3] = {n0, n1, n2 / 2 + 1};
+  int off = 0;
+  const float eps32 = 1.1920928955078125e-07f;
+  for (int a = 0; a < 3; ++a) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len[a]; i += gridDim.x * blockDim.x) {
+      int m = (a == 2) ? i : (i < (n[a] + 1) / 2 ? i : i - n[a]);
+      double k64 = (double)m * (1.0 / (double)n[a]) * period;
+      float k = (float)k64;
+      float* kk = out + off;
+      float* mk = out + off + len[a];
+      float* nk = out + off + 2 * len[a];
+      kk[i] = k;
+      float limit = k_nyq_ptcl + k_nyq_ptcl * (8.0f * eps32);
+      mk[i] = (!mask_active || fabsf(k) <= limit) ? 1.f : 0.f;
+      float eps = nyq_mesh * eps32;
+      nk[i] = (fabsf(fabsf(k) - nyq_mesh) <= eps) ? 0.f : -k;
+    }
+    off += 3 * len[a];
+  }
+}
+
+// out = (adjoint ? conj(D) : D) * G * in on a standard-layout spectrum; axis < 0: D = 1.
+__global__ void k_spectral_multiply(const float2* __restrict__ in, float2* __restrict__ out, int nx,
+                                    int ny, int nzh, SpecTables tb, int axis, int adjoint) {
+  int64_t total = (int64_t)nx * ny * nzh;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int kz = (int)(i % nzh);
+    int ky = (int)((i / nzh) % ny);
+    int kx = (int)(i / ((int64_t)nzh * ny));
+    float kxv = tb.k[0][kx], kyv = tb.k[1][ky], kzv = tb.k[2][kz];
+    float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+    bool keep = tb.mask[0][kx] != 0.f && tb.mask[1][ky] != 0.f && tb.mask[2][kz] != 0.f && k2 != 0.f;
+    float2 v = in[i];
+    float2 phi = keep ? make_float2(-v.x / k2, -v.y / k2) : make_float2(0.f, 0.f);
+    if (axis >= 0) {
+      int ki = axis == 0 ? kx : (axis == 1 ? ky : kz);
+      float nk = tb.negk[axis][ki];
+      phi = c_imul(adjoint ? -nk : nk, phi);
+    }
+    out[i] = phi;
+  }
+}
+
+template <int N>
+int launch_cols_n(int mode, const ColPass& cp, int batches, const SpecTables& tb, int dax,
+                  hipStream_t s) {
+  using CF = ColCfg<N>;
+  const float2* tw = twiddle_table();
+  PM_RETURN_IF(tw == nullptr, PM_ERROR_HIP, "twiddle table not initialised");
+  dim3 grid((unsigned)ceil_div(cp.ncols, CF::T), (unsigned)batches);
+  dim3 block(CF::THREADS);
+  switch (mode) {
+    case kColFwd:
+      hipLaunchKernelGGL((k_cols<N, kColFwd>), grid, block, 0, s, cp, tb, dax, tw);
+      PM_LAUNCHED("fft_cols_fwd", s);
+      break;
+    case kColInv:
+      hipLaunchKernelGGL((k_cols<N, kColInv>), grid, block, 0, s, cp, tb, dax, tw);
+      PM_LAUNCHED("fft_cols_inv", s);
+      break;
+    case kColInv2:
+      hipLaunchKernelGGL((k_cols<N, kColInv2>), grid, block, 0, s, cp, tb, dax, tw);
+      PM_LAUNCHED("fft_cols_inv2", s);
+      break;
+    case kColGreen:
+      hipLaunchKernelGGL((k_cols<N, kColGreen>), grid, block, 0, s, cp, tb, dax, tw);
+      PM_LAUNCHED("fft_cols_green", s);
+      break;
+    default: return fail(PM_ERROR_INVALID_ARGUMENT, "bad column mode");
+  }
+  return PM_SUCCESS;
+}
+
+template <int M>
+int launch_z_n(bool c2r, const ZPass& zp, const SpecTables& tb, hipStream_t s) {
+  using ZC = ZCfg<M>;
+  const float2* tw = twiddle_table();
+  PM_RETURN_IF(tw == nullptr, PM_ERROR_HIP, "twiddle table not initialised");
+  if (zp.nlines == 0) return PM_SUCCESS;
+  dim3 grid((unsigned)ceil_div(zp.nlines, ZC::LINES));
+  if (c2r) {
+    hipLaunchKernelGGL((k_zpass<M, true>), grid, dim3(ZC::THREADS), 0, s, zp, tb, tw);
+    PM_LAUNCHED("fft_z_c2r", s);
+  } else {
+    hipLaunchKernelGGL((k_zpass<M, false>), grid, dim3(ZC::THREADS), 0, s, zp, tb, tw);
+    PM_LAUNCHED("fft_z_r2c", s);
+  }
+  return PM_SUCCESS;
+}
+
+}  // namespace
+
+#define PM_POW2_CASES(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
+
+int launch_cols(int n, int mode, const ColPass& cp, int batches, const SpecTables& tb, int dax,
+                hipStream_t s) {
+  if (batches == 0 || cp.ncols == 0) return PM_SUCCESS;
+  switch (n) {
+#define PM_CASE(V) \
+  case V: return launch_cols_n<V>(mode, cp, batches, tb, dax, s);
+    PM_POW2_CASES(PM_CASE)
+#undef PM_CASE
+    default: return fail(PM_ERROR_UNSUPPORTED, "FFT axis length must be a power of two in [2, 2048]");
+  }
+}
+
+int launch_z(int nz, bool c2r, const ZPass& zp, const SpecTables& tb, hipStream_t s) {
+  if (nz < 2 || nz > 2048 || !is_pow2(nz))
+    return fail(PM_ERROR_UNSUPPORTED, "last FFT axis must be a power of two in [2, 2048]");
+  switch (nz / 2) {
+#define PM_CASE(V) \
+  case V: return launch_z_n<V>(c2r, zp, tb, s);
+    PM_CASE(1) PM_CASE(2) PM_CASE(4) PM_CASE(8) PM_CASE(16) PM_CASE(32) PM_CASE(64) PM_CASE(128)
+    PM_CASE(256) PM_CASE(512) PM_CASE(1024)
+#undef PM_CASE
+    default: return fail(PM_ERROR_UNSUPPORTED, "last FFT axis must be a power of two in [2, 2048]");
+  }
+}
+
+bool supported_shape(const int32_t* shape) {
+  for (int a = 0; a < 3; ++a)
+    if (shape[a] < 2 || shape[a] > 2048 || !is_pow2(shape[a])) return false;
+  return true;
+}
+
+size_t spec_table_floats(const int32_t* shape) {
+  return 3 * ((size_t)shape[0] + shape[1] + (shape[2] / 2 + 1));
+}
+
+SpecTables spec_tables_view(const float* base, const int32_t* shape) {
+  SpecTables tb;
+  int len[3] = {shape[0], shape[1], shape[2] / 2 + 1};
+  size_t off = 0;
+  for (int a = 0; a < 3; ++a) {
+    tb.k[a] = base + off;
+    tb.mask[a] = base + off + len[a];
+    tb.negk[a] = base + off + 2 * len[a];
+    off += 3 * (size_t)len[a];
+  }
+  return tb;
+}
+
+int launch_spec_tables(float* out, const int32_t* shape, double cell_size, double ptcl_spacing,
+                       int mask_active, hipStream_t s) {
+  double period = 2.0 * 3.14159265358979323846 / cell_size;
+  float k_nyq_ptcl = (float)(3.14159265358979323846 / ptcl_spacing);
+  float nyq_mesh = (float)(3.14159265358979323846 / cell_size);
+  hipLaunchKernelGGL(k_spec_tables, dim3(8), dim3(256), 0, s, out, shape[0], shape[1], shape[2], period,
+                     k_nyq_ptcl, mask_active, nyq_mesh);
+  PM_LAUNCHED("k_spec_tables", s);
+  return PM_SUCCESS;
+}
+
+int launch_spectral_multiply(const float2* in, float2* out, const int32_t* shape, const SpecTables& tb,
+                             int axis, int adjoint, hipStream_t s) {
+  int64_t total = (int64_t)shape[0] * shape[1] * (shape[2] / 2 + 1);
+  unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(total, 256), 65536);
+  hipLaunchKernelGGL(k_spectral_multiply, dim3(blocks), dim3(256), 0, s, in, out, shape[0], shape[1],
+                     shape[2] / 2 + 1, tb, axis, adjoint);
+  PM_LAUNCHED("k_spectral_multiply", s);
+  return PM_SUCCESS;
+}
+
+// Forward 3D r2c on a half spectrum with row pitch `pitch` (>= nz/2+1), in place over
+// `spec` (z pass writes it, y and x passes update it).
+int rfft3d_pitched(const float* in, float2* spec, const int32_t* shape, int64_t pitch, float shift,
+                   float scale, const SpecTables& tb, hipStream_t s) {
+  const int nx = shape[0], ny = shape[1], nz = shape[2], m = nz / 2;
+  ZPass zp{};
+  zp.rin = in;
+  zp.cmain = spec;
+  zp.cnyq = spec + m;
+  zp.pitch_r = nz;
+  zp.pitch_c = pitch;
+  zp.nyq_stride = pitch;
+  zp.nlines = (int64_t)nx * ny;
+  zp.src_shift = shift;
+  zp.src_scale = scale;
+  int st = launch_z(nz, false, zp, tb, s);
+  if (st) return st;
+  ColPass cy{spec, spec, nullptr, pitch, 1, (int64_t)ny * pitch, m + 1, 0, 0, 0, 0, 0};
+  st = launch_cols(ny, kColFwd, cy, nx, tb, 0, s);
+  if (st) return st;
+  ColPass cx{spec, spec, nullptr, (int64_t)ny * pitch, 1, pitch, m + 1, 0, 0, 0, 0, 0};
+  return launch_cols(nx, kColFwd, cx, ny, tb, 0, s);
+}
+
+// Inverse 3D c2r (1/N normalised); the spectrum is used as scratch.
+int irfft3d_pitched(float2* spec, float* out, const int32_t* shape, int64_t pitch, const SpecTables& tb,
+                    hipStream_t s) {
+  const int nx = shape[0], ny = shape[1], nz = shape[2], m = nz / 2;
+  ColPass cx{spec, spec, nullptr, (int64_t)ny * pitch, 1, pitch, m + 1, 0, 0, 0, 0, 0};
+  int st = launch_cols(nx, kColInv, cx, ny, tb, 0, s);
+  if (st) return st;
+  ColPass cy{spec, spec, nullptr, pitch, 1, (int64_t)ny * pitch, m + 1, 0, 0, 0, 0, 0};
+  st = launch_cols(ny, kColInv, cy, nx, tb, 0, s);
+  if (st) return st;
+  ZPass zp{};
+  zp.rout = out;
+  zp.cmain = spec;
+  zp.cnyq = spec + m;
+  zp.pitch_r = nz;
+  zp.pitch_c = pitch;
+  zp.nyq_stride = pitch;
+  zp.nlines = (int64_t)nx * ny;
+  zp.out_scale = (float)(2.0 / ((double)nx * ny * nz));
+  return launch_z(nz, true, zp, tb, s);
+}
+
+}  // namespace fft
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+int pm_fft_workspace(const int32_t shape[3], size_t* bytes) {
+  PM_RETURN_IF(shape == nullptr || bytes == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF(!fft::supported_shape(shape), PM_ERROR_UNSUPPORTED,
+               "FFT axes must be powers of two in [2, 2048]");
+  *bytes = 0;
+  return PM_SUCCESS;
+}
+
+int pm_rfft3d(const float* in, float* out_complex, const int32_t shape[3], void* workspace,
+              size_t workspace_bytes, void* stream) {
+  (void)workspace;
+  (void)workspace_bytes;
+  PM_RETURN_IF(shape == nullptr || in == nullptr || out_complex == nullptr, PM_ERROR_INVALID_ARGUMENT,
+               "NULL argument");
+  PM_RETURN_IF(!fft::supported_shape(shape), PM_ERROR_UNSUPPORTED,
+               "FFT axes must be powers of two in [2, 2048]");
+  int st = ensure_device_tables();
+  if (st) return st;
+  PM_RETURN_IF((const void*)in == (const void*)out_complex, PM_ERROR_INVALID_ARGUMENT,
+               "pm_rfft3d needs distinct input and output buffers");
+  fft::SpecTables tb{};
+  return fft::rfft3d_pitched(in, reinterpret_cast<float2*>(out_complex), shape, shape[2] / 2 + 1, 0.f,
+                             1.f, tb, as_stream(stream));
+}
+
+int pm_irfft3d(float* in_complex, float* out, const int32_t shape[3], void* workspace,
+               size_t workspace_bytes, void* stream) {
+  (void)workspace;
+  (void)workspace_bytes;
+  PM_RETURN_IF(shape == nullptr || in_complex == nullptr || out == nullptr, PM_ERROR_INVALID_ARGUMENT,
+               "NULL argument");
+  PM_RETURN_IF(!fft::supported_shape(shape), PM_ERROR_UNSUPPORTED,
+               "FFT axes must be powers of two in [2, 2048]");
+  int st = ensure_device_tables();
+  if (st) return st;
+  PM_RETURN_IF((const void*)in_complex == (const void*)out, PM_ERROR_INVALID_ARGUMENT,
+               "pm_irfft3d needs distinct input and output buffers");
+  fft::SpecTables tb{};
+  return fft::irfft3d_pitched(reinterpret_cast<float2*>(in_complex), out, shape, shape[2] / 2 + 1, tb,
+                              as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,534 @@
+// The single-device PM force step and its reverse (force VJP).
+//
+// Forward (nbody/gravity.py:786-814 on the streamed single-device branch :597-606, 506-543):
+//   1. cell sort (pm_sort.hip) and a gather of the particle payload into key order;
+//   2. CIC scatter of v_p = valid * N_m/N_p from sorted particles (spatially coherent atomics);
+//   3. z pass r2c with S = (rho - 1) * (1.5 Omega_m) fused into the load;
+//   4. y pass forward;
+//   5. x pass forward, mask x (-1/k^2), then two inverse x transforms in registers:
+//        A <- IFFT_x(phi),  B <- IFFT_x(-i k_x phi)
+//   6. y pass inverse on A with two outputs: A <- IFFT_y(A), C <- IFFT_y(-i k_y A);
+//      y pass inverse on B;
+//   7. z pass c2r of B -> F_x, C -> F_y, (-i k_z) A -> F_z  (1/N normalisation fused);
+//   8. CIC gather of (F_x, F_y, F_z) from sorted particles, written back through the
+//      permutation in caller order.
+// The derivative operators commute with the transforms along the other axes, so the three
+// inverse transforms of the reference (one per component, :535-543) share their x and y
+// passes: 4R + 14C bytes of mesh traffic instead of 4R + 20C.
+#include <algorithm>
+
+#include "pm_common.h"
+#include "pm_fft_launch.h"
+#include "pm_sort.h"
+
+namespace pm {
+namespace grav {
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void k_fill(float* __restrict__ p, int64_t n, float v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_payload(const int32_t* __restrict__ perm,
+                                                    const int16_t* __restrict__ pmid,
+                                                    const float* __restrict__ disp,
+                                                    const uint8_t* __restrict__ valid, int64_t n,
+                                                    int16_t* __restrict__ pmid_s, float* __restrict__ disp_s,
+                                                    uint8_t* __restrict__ valid_s) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int64_t i = perm[j];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    pmid_s[j * 3 + a] = pmid[i * 3 + a];
+    disp_s[j * 3 + a] = disp[i * 3 + a];
+  }
+  valid_s[j] = valid ? valid[i] : (uint8_t)1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter_sorted(const int16_t* __restrict__ pmid,
+                                                           const float* __restrict__ disp,
+                                                           const uint8_t* __restrict__ valid, int64_t n,
+                                                           float val, Geom g, float* __restrict__ mesh) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || !valid[j]) return;
+  AxisCic ax[3];
+  particle_cic(g, pmid, disp, j, ax);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    int64_t flat;
+    corner_index(g, ax, c, flat);
+    atomic_add_f32(mesh + flat, val * corner_weight(ax, c));
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather3_sorted(const int16_t* __restrict__ pmid,
+                                                           const float* __restrict__ disp,
+                                                           const uint8_t* __restrict__ valid,
+                                                           const int32_t* __restrict__ perm, int64_t n,
+                                                           Geom g, const float* __restrict__ fx,
+                                                           const float* __restrict__ fy,
+                                                           const float* __restrict__ fz,
+                                                           float* __restrict__ acc) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float r[3] = {0.f, 0.f, 0.f};
+  if (valid[j]) {
+    AxisCic ax[3];
+    particle_cic(g, pmid, disp, j, ax);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      int64_t flat;
+      corner_index(g, ax, c, flat);
+      float w = corner_weight(ax, c);
+      r[0] = r[0] + fx[flat] * w;
+      r[1] = r[1] + fy[flat] * w;
+      r[2] = r[2] + fz[flat] * w;
+    }
+  }
+  int64_t i = perm[j];
+  acc[i * 3 + 0] = r[0];
+  acc[i * 3 + 1] = r[1];
+  acc[i * 3 + 2] = r[2];
+}
+
+// Reverse of the three gathers (gather^T = scatter of acc_cot, plus the displacement
+// cotangent from the mesh values), pallas.py:321-355 per component.
+__global__ void __launch_bounds__(kBlock) k_gather3_bwd_sorted(
+    const int16_t* __restrict__ pmid, const float* __restrict__ disp, const uint8_t* __restrict__ valid,
+    const int32_t* __restrict__ perm, int64_t n, Geom g, const float* __restrict__ fx,
+    const float* __restrict__ fy, const float* __restrict__ fz, const float* __restrict__ acc_cot,
+    float* __restrict__ gx, float* __restrict__ gy, float* __restrict__ gz, float* __restrict__ disp_cot_s) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float dc[3] = {0.f, 0.f, 0.f};
+  if (valid[j]) {
+    int64_t i = perm[j];
+    float ac[3] = {acc_cot[i * 3 + 0], acc_cot[i * 3 + 1], acc_cot[i * 3 + 2]};
+    AxisCic ax[3];
+    particle_cic(g, pmid, disp, j, ax);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      int64_t flat;
+      corner_index(g, ax, c, flat);
+      float w = corner_weight(ax, c);
+      float gr[3];
+      corner_grad(ax, c, gr);
+      atomic_add_f32(gx + flat, ac[0] * w);
+      atomic_add_f32(gy + flat, ac[1] * w);
+      atomic_add_f32(gz + flat, ac[2] * w);
+      float inner = ac[0] * fx[flat] + ac[1] * fy[flat] + ac[2] * fz[flat];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) dc[a] = dc[a] + inner * gr[a];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) disp_cot_s[j * 3 + a] = dc[a] / g.a1;
+}
+
+// Scatter adjoint for the density: disp_cot += v * sum_c dens_cot[c] grad_c / cell, then the
+// total is written back in caller order (scatter.py:341-375, pallas.py:358-409).
+__global__ void __launch_bounds__(kBlock) k_scatter_bwd_sorted(
+    const int16_t* __restrict__ pmid, const float* __restrict__ disp, const uint8_t* __restrict__ valid,
+    const int32_t* __restrict__ perm, int64_t n, Geom g, float val, const float* __restrict__ dens_cot,
+    const float* __restrict__ disp_cot_s, float* __restrict__ disp_cot) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float dc[3] = {0.f, 0.f, 0.f};
+  if (valid[j] && val != 0.f) {
+    AxisCic ax[3];
+    particle_cic(g, pmid, disp, j, ax);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      int64_t flat;
+      corner_index(g, ax, c, flat);
+      float gr[3];
+      corner_grad(ax, c, gr);
+      float mv = dens_cot[flat] * val;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) dc[a] = dc[a] + mv * gr[a];
+    }
+  }
+  int64_t i = perm[j];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) disp_cot[i * 3 + a] = disp_cot_s[j * 3 + a] + dc[a] / g.a1;
+}
+
+// src_hat <- G(k) * sum_i conj(D_i(k)) * hat_i on a pitched half spectrum.
+__global__ void k_adjoint_combine(float2* __restrict__ a, const float2* __restrict__ b,
+                                  const float2* __restrict__ c, int nx, int ny, int nzh, int64_t pitch,
+                                  fft::SpecTables tb) {
+  int64_t total = (int64_t)nx * ny * nzh;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int kz = (int)(t % nzh);
+    int ky = (int)((t / nzh) % ny);
+    int kx = (int)(t / ((int64_t)nzh * ny));
+    int64_t i = ((int64_t)kx * ny + ky) * pitch + kz;
+    float kxv = tb.k[0][kx], kyv = tb.k[1][ky], kzv = tb.k[2][kz];
+    float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+    bool keep = tb.mask[0][kx] != 0.f && tb.mask[1][ky] != 0.f && tb.mask[2][kz] != 0.f && k2 != 0.f;
+    // conj(i*negk) = -i*negk
+    float2 s = fft::c_imul(-tb.negk[0][kx], a[i]);
+    s = fft::c_add(s, fft::c_imul(-tb.negk[1][ky], b[i]));
+    s = fft::c_add(s, fft::c_imul(-tb.negk[2][kz], c[i]));
+    a[i] = keep ? make_float2(-s.x / k2, -s.y / k2) : make_float2(0.f, 0.f);
+  }
+}
+
+// omega_cot += 1.5 * sum_x src_cot(x) * (rho(x) - 1); dens_cot = src_cot * scale, in place
+__global__ void __launch_bounds__(kBlock) k_source_bwd(float* __restrict__ src_cot,
+                                                       const float* __restrict__ rho, int64_t n,
+                                                       float scale, float* __restrict__ omega_cot) {
+  float acc = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float sc = src_cot[i];
+    acc += sc * (rho[i] - 1.f);
+    src_cot[i] = sc * scale;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  __shared__ float part[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int w = 0; w < kBlock / 64; ++w) t += part[w];
+    atomic_add_f32(omega_cot, 1.5f * t);
+  }
+}
+
+struct Layout {
+  int nx, ny, nz, m;
+  int64_t pitch;  // complex row pitch (float2), >= m+1, multiple of 16
+  int64_t nreal;  // nx*ny*nz
+  int64_t ncplx;  // nx*ny*pitch
+};
+
+Layout make_layout(const int32_t* s) {
+  Layout L;
+  L.nx = s[0];
+  L.ny = s[1];
+  L.nz = s[2];
+  L.m = s[2] / 2;
+  L.pitch = (int64_t)align_up((size_t)L.m + 1, 16);
+  L.nreal = (int64_t)L.nx * L.ny * L.nz;
+  L.ncplx = (int64_t)L.nx * L.ny * L.pitch;
+  return L;
+}
+
+int check_desc(const pm_gravity_desc* d, int64_t n) {
+  PM_RETURN_IF(d == nullptr, PM_ERROR_INVALID_ARGUMENT, "desc is NULL");
+  PM_RETURN_IF(n < 0 || n > INT32_MAX, PM_ERROR_UNSUPPORTED, "particle count must be in [0, 2^31)");
+  PM_RETURN_IF(!fft::supported_shape(d->mesh_shape), PM_ERROR_UNSUPPORTED,
+               "mesh axes must be powers of two in [2, 2048]");
+  PM_RETURN_IF(!(d->cell_size > 0.f), PM_ERROR_INVALID_ARGUMENT, "cell_size must be positive");
+  PM_RETURN_IF(d->mask_active && !(d->ptcl_spacing > 0.f), PM_ERROR_INVALID_ARGUMENT,
+               "ptcl_spacing must be positive when the particle-Nyquist mask is active");
+  return sort::check_sort_args(n, d->mesh_shape, d->brick);
+}
+
+struct Buffers {
+  int32_t* perm;
+  int16_t* pmid_s;
+  float* disp_s;
+  uint8_t* valid_s;
+  float* tables;
+  float* r0;  // density, then F_x
+  float* r1;  // F_y
+  float* r2;  // F_z
+  float2* ca;
+  float2* cb;
+  float2* cc;
+  // reverse only: F_x (the forward keeps rho in r0), three mesh cotangents
+  float* g0;
+  float* g1;
+  float* g2;
+  float* g3;
+  float* disp_cot_s;
+};
+
+bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, bool reverse, Buffers& b) {
+  b.perm = ws.take<int32_t>(n);
+  b.pmid_s = ws.take<int16_t>(3 * n);
+  b.disp_s = ws.take<float>(3 * n);
+  b.valid_s = ws.take<uint8_t>(n);
+  b.tables = ws.take<float>(fft::spec_table_floats(shape));
+  b.r0 = ws.take<float>(L.nreal);
+  b.r1 = ws.take<float>(L.nreal);
+  b.r2 = ws.take<float>(L.nreal);
+  b.ca = ws.take<float2>(L.ncplx);
+  b.cb = ws.take<float2>(L.ncplx);
+  b.cc = ws.take<float2>(L.ncplx);
+  if (reverse) {
+    b.g0 = ws.take<float>(L.nreal);
+    b.g1 = ws.take<float>(L.nreal);
+    b.g2 = ws.take<float>(L.nreal);
+    b.g3 = ws.take<float>(L.nreal);
+    b.disp_cot_s = ws.take<float>(3 * n);
+  }
+  return ws.ok();
+}
+
+size_t workspace_bytes(const pm_gravity_desc* d, int64_t n, bool reverse) {
+  Layout L = make_layout(d->mesh_shape);
+  Arena probe(reinterpret_cast<void*>(uintptr_t(256)), SIZE_MAX / 2);
+  Buffers b;
+  carve(probe, L, n, d->mesh_shape, reverse, b);
+  // the sort's temporaries are carved after the step buffers
+  return probe.used + 256 + sort::workspace_bytes(n, d->mesh_shape, d->brick) + 256;
+}
+
+Geom step_geom(const pm_gravity_desc* d) {
+  Geom g;
+  for (int a = 0; a < 3; ++a) {
+    g.ls[a] = d->mesh_shape[a];
+    g.gs[a] = d->mesh_shape[a];
+    g.off[a] = 0.f;
+  }
+  g.a1 = d->cell_size;
+  g.a2 = 0.f;
+  return g;
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, ceil_div(n, kBlock)); }
+
+int fill(float* p, int64_t n, hipStream_t s) {
+  if (n == 0) return PM_SUCCESS;
+  unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, kBlock * 4), 1 << 16);
+  hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(kBlock), 0, s, p, n, 0.f);
+  PM_LAUNCHED("k_fill", s);
+  return PM_SUCCESS;
+}
+
+#define PM_TRY(expr)        \
+  do {                      \
+    int _st = (expr);       \
+    if (_st) return _st;    \
+  } while (0)
+
+// Sort, payload, density.  Leaves rho in b.r0.
+int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                  const pm_gravity_desc* d, const Layout& L, Buffers& b, Arena& ws, hipStream_t s) {
+  const Geom g = step_geom(d);
+  PM_TRY(sort::sort_cells(pmid, disp, valid, n, d->mesh_shape, d->cell_size, d->brick, b.perm, nullptr, ws, s));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_payload, dim3(grid_for(n)), dim3(kBlock), 0, s, b.perm, pmid, disp, valid, n, b.pmid_s,
+                       b.disp_s, b.valid_s);
+    PM_LAUNCHED("k_payload", s);
+  }
+  PM_TRY(fill(b.r0, L.nreal, s));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter_sorted, dim3(grid_for(n)), dim3(kBlock), 0, s, b.pmid_s, b.disp_s, b.valid_s,
+                       n, d->particle_value, g, b.r0);
+    PM_LAUNCHED("k_scatter_sorted", s);
+  }
+  return PM_SUCCESS;
+}
+
+// Density (b.r0) -> force meshes F_x (b.r0), F_y (b.r1), F_z (b.r2).
+int force_mesh_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b, const fft::SpecTables& tb,
+                     hipStream_t s, bool keep_density) {
+  using namespace fft;
+  const float coef = 1.5f * d->omega_m;  // 1.5 * Omega_m in float32 (gravity.py:630)
+  ZPass z1{};
+  z1.rin = b.r0;
+  z1.cmain = b.ca;
+  z1.cnyq = b.ca + L.m;
+  z1.pitch_r = L.nz;
+  z1.pitch_c = L.pitch;
+  z1.nyq_stride = L.pitch;
+  z1.nlines = (int64_t)L.nx * L.ny;
+  z1.src_shift = -1.f;
+  z1.src_scale = coef;
+  PM_TRY(launch_z(L.nz, false, z1, tb, s));
+  const int ncols = L.m + 1;
+  ColPass yf{b.ca, b.ca, nullptr, L.pitch, 1, (int64_t)L.ny * L.pitch, ncols, 0, 0, 0, 0, 0};
+  PM_TRY(launch_cols(L.ny, kColFwd, yf, L.nx, tb, 1, s));
+  // x: forward, Green, two inverses; columns are kz, batches are ky
+  ColPass xg{b.ca, b.ca, b.cb, (int64_t)L.ny * L.pitch, 1, L.pitch, ncols, 1, 0, 0, 1, 0};
+  PM_TRY(launch_cols(L.nx, kColGreen, xg, L.ny, tb, 0, s));
+  ColPass yi2{b.ca, b.ca, b.cc, L.pitch, 1, (int64_t)L.ny * L.pitch, ncols, 0, 0, 0, 0, 0};
+  PM_TRY(launch_cols(L.ny, kColInv2, yi2, L.nx, tb, 1, s));
+  ColPass yi{b.cb, b.cb, nullptr, L.pitch, 1, (int64_t)L.ny * L.pitch, ncols, 0, 0, 0, 0, 0};
+  PM_TRY(launch_cols(L.ny, kColInv, yi, L.nx, tb, 1, s));
+  const float norm = (float)(2.0 / ((double)L.nx * L.ny * L.nz));
+  ZPass zc{};
+  zc.pitch_r = L.nz;
+  zc.pitch_c = L.pitch;
+  zc.nyq_stride = L.pitch;
+  zc.nlines = (int64_t)L.nx * L.ny;
+  zc.out_scale = norm;
+  float* fx = keep_density ? b.g0 : b.r0;  // reverse keeps rho for the Omega_m cotangent
+  zc.cmain = b.cb;
+  zc.cnyq = b.cb + L.m;
+  zc.rout = fx;
+  zc.deriv = 0;
+  PM_TRY(launch_z(L.nz, true, zc, tb, s));
+  zc.cmain = b.cc;
+  zc.cnyq = b.cc + L.m;
+  zc.rout = b.r1;
+  PM_TRY(launch_z(L.nz, true, zc, tb, s));
+  zc.cmain = b.ca;
+  zc.cnyq = b.ca + L.m;
+  zc.rout = b.r2;
+  zc.deriv = 1;
+  PM_TRY(launch_z(L.nz, true, zc, tb, s));
+  return PM_SUCCESS;
+}
+
+int prepare(const pm_gravity_desc* d, int64_t n, void* workspace, size_t workspace_bytes_, bool reverse,
+            Layout& L, Buffers& b, Arena& ws, fft::SpecTables& tb, hipStream_t s) {
+  PM_TRY(check_desc(d, n));
+  PM_TRY(ensure_device_tables());
+  size_t need = workspace_bytes(d, n, reverse);
+  PM_RETURN_IF(workspace == nullptr || workspace_bytes_ < need, PM_ERROR_WORKSPACE,
+               "gravity workspace too small: need " + std::to_string(need) + " bytes");
+  L = make_layout(d->mesh_shape);
+  PM_RETURN_IF(!carve(ws, L, n, d->mesh_shape, reverse, b), PM_ERROR_WORKSPACE, "workspace carve failed");
+  PM_TRY(fft::launch_spec_tables(b.tables, d->mesh_shape, d->cell_size, d->ptcl_spacing, d->mask_active, s));
+  tb = fft::spec_tables_view(b.tables, d->mesh_shape);
+  return PM_SUCCESS;
+}
+
+}  // namespace
+}  // namespace grav
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+int pm_gravity_workspace(const pm_gravity_desc* desc, int64_t n, size_t* bytes) {
+  int st = grav::check_desc(desc, n);
+  if (st) return st;
+  PM_RETURN_IF(bytes == nullptr, PM_ERROR_INVALID_ARGUMENT, "bytes is NULL");
+  size_t f = grav::workspace_bytes(desc, n, false);
+  size_t r = grav::workspace_bytes(desc, n, true);
+  *bytes = std::max(f, r);
+  return PM_SUCCESS;
+}
+
+int pm_gravity(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+               const pm_gravity_desc* desc, float* acc, void* workspace, size_t workspace_bytes,
+               void* stream) {
+  hipStream_t s = as_stream(stream);
+  grav::Layout L;
+  grav::Buffers b{};
+  Arena ws(workspace, workspace_bytes);
+  fft::SpecTables tb;
+  int st = grav::prepare(desc, n, workspace, workspace_bytes, false, L, b, ws, tb, s);
+  if (st) return st;
+  PM_RETURN_IF(n > 0 && (pmid == nullptr || disp == nullptr || acc == nullptr), PM_ERROR_INVALID_ARGUMENT,
+               "pmid/disp/acc NULL");
+  st = grav::density_stage(pmid, disp, valid, n, desc, L, b, ws, s);
+  if (st) return st;
+  st = grav::force_mesh_stage(desc, L, b, tb, s, false);
+  if (st) return st;
+  if (n > 0) {
+    hipLaunchKernelGGL(grav::k_gather3_sorted, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s, b.pmid_s,
+                       b.disp_s, b.valid_s, b.perm, n, grav::step_geom(desc), b.r0, b.r1, b.r2, acc);
+    PM_LAUNCHED("k_gather3_sorted", s);
+  }
+  return PM_SUCCESS;
+}
+
+int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                   const pm_gravity_desc* desc, const float* acc_cot, float* disp_cot,
+                   float* omega_cot, void* workspace, size_t workspace_bytes, void* stream) {
+  hipStream_t s = as_stream(stream);
+  grav::Layout L;
+  grav::Buffers b{};
+  Arena ws(workspace, workspace_bytes);
+  fft::SpecTables tb;
+  int st = grav::prepare(desc, n, workspace, workspace_bytes, true, L, b, ws, tb, s);
+  if (st) return st;
+  PM_RETURN_IF(n > 0 && (pmid == nullptr || disp == nullptr || acc_cot == nullptr || disp_cot == nullptr),
+               PM_ERROR_INVALID_ARGUMENT, "pmid/disp/acc_cot/disp_cot NULL");
+  PM_RETURN_IF(omega_cot == nullptr, PM_ERROR_INVALID_ARGUMENT, "omega_cot NULL");
+  const Geom g = grav::step_geom(desc);
+  // recompute the forward meshes: rho stays in r0, F_x goes to g0, F_y r1, F_z r2
+  st = grav::density_stage(pmid, disp, valid, n, desc, L, b, ws, s);
+  if (st) return st;
+  st = grav::force_mesh_stage(desc, L, b, tb, s, true);
+  if (st) return st;
+  // gather^T: acc_cot scattered into three cotangent meshes, displacement cotangent from F
+  st = grav::fill(b.g1, L.nreal, s);
+  if (st) return st;
+  st = grav::fill(b.g2, L.nreal, s);
+  if (st) return st;
+  st = grav::fill(b.g3, L.nreal, s);
+  if (st) return st;
+  if (n > 0) {
+    hipLaunchKernelGGL(grav::k_gather3_bwd_sorted, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s, b.pmid_s,
+                       b.disp_s, b.valid_s, b.perm, n, g, b.g0, b.r1, b.r2, acc_cot, b.g1, b.g2, b.g3,
+                       b.disp_cot_s);
+    PM_LAUNCHED("k_gather3_bwd_sorted", s);
+  }
+  // spectral adjoint: src_cot = irfftn(G * sum_i conj(D_i) rfftn(g_i))  (the symbol is
+  // Hermitian, so the transpose of irfftn(H rfftn(.)) is irfftn(conj(H) rfftn(.)))
+  fft::SpecTables none{};
+  st = fft::rfft3d_pitched(b.g1, b.ca, desc->mesh_shape, L.pitch, 0.f, 1.f, none, s);
+  if (st) return st;
+  st = fft::rfft3d_pitched(b.g2, b.cb, desc->mesh_shape, L.pitch, 0.f, 1.f, none, s);
+  if (st) return st;
+  st = fft::rfft3d_pitched(b.g3, b.cc, desc->mesh_shape, L.pitch, 0.f, 1.f, none, s);
+  if (st) return st;
+  {
+    int64_t total = (int64_t)L.nx * L.ny * (L.m + 1);
+    unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(total, 256), 65536);
+    hipLaunchKernelGGL(grav::k_adjoint_combine, dim3(blocks), dim3(256), 0, s, b.ca, b.cb, b.cc, L.nx, L.ny,
+                       L.m + 1, L.pitch, tb);
+    PM_LAUNCHED("k_adjoint_combine", s);
+  }
+  st = fft::irfft3d_pitched(b.ca, b.g1, desc->mesh_shape, L.pitch, none, s);
+  if (st) return st;
+  {
+    const float coef = 1.5f * desc->omega_m;
+    unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(L.nreal, 256 * 4), 4096);
+    hipLaunchKernelGGL(grav::k_source_bwd, dim3(blocks), dim3(256), 0, s, b.g1, b.r0, L.nreal, coef, omega_cot);
+    PM_LAUNCHED("k_source_bwd", s);
+  }
+  if (n > 0) {
+    hipLaunchKernelGGL(grav::k_scatter_bwd_sorted, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s, b.pmid_s,
+                       b.disp_s, b.valid_s, b.perm, n, g, desc->particle_value, b.g1, b.disp_cot_s, disp_cot);
+    PM_LAUNCHED("k_scatter_bwd_sorted", s);
+  }
+  return PM_SUCCESS;
+}
+
+int pm_spectral_workspace(const pm_gravity_desc* desc, size_t* bytes) {
+  PM_RETURN_IF(desc == nullptr || bytes == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF(!fft::supported_shape(desc->mesh_shape), PM_ERROR_UNSUPPORTED,
+               "mesh axes must be powers of two in [2, 2048]");
+  *bytes = align_up(sizeof(float) * fft::spec_table_floats(desc->mesh_shape), 256);
+  return PM_SUCCESS;
+}
+
+int pm_spectral_force_multiply(const float* in_complex, float* out_complex, const pm_gravity_desc* desc,
+                               int32_t axis, int32_t adjoint, void* workspace, size_t workspace_bytes,
+                               void* stream) {
+  PM_RETURN_IF(desc == nullptr || in_complex == nullptr || out_complex == nullptr,
+               PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF(axis < -1 || axis > 2, PM_ERROR_INVALID_ARGUMENT, "axis must be -1, 0, 1 or 2");
+  size_t need = 0;
+  int st = pm_spectral_workspace(desc, &need);
+  if (st) return st;
+  PM_RETURN_IF(desc->mask_active && !(desc->ptcl_spacing > 0.f), PM_ERROR_INVALID_ARGUMENT,
+               "ptcl_spacing must be positive when the particle-Nyquist mask is active");
+  PM_RETURN_IF(!(desc->cell_size > 0.f), PM_ERROR_INVALID_ARGUMENT, "cell_size must be positive");
+  PM_RETURN_IF(workspace == nullptr || workspace_bytes < need, PM_ERROR_WORKSPACE,
+               "spectral workspace too small: need " + std::to_string(need) + " bytes");
+  hipStream_t s = as_stream(stream);
+  float* tables = static_cast<float*>(workspace);
+  st = fft::launch_spec_tables(tables, desc->mesh_shape, desc->cell_size, desc->ptcl_spacing, desc->mask_active, s);
+  if (st) return st;
+  fft::SpecTables tb = fft::spec_tables_view(tables, desc->mesh_shape);
+  return fft::launch_spectral_multiply(reinterpret_cast<const float2*>(in_complex),
+                                       reinterpret_cast<float2*>(out_complex), desc->mesh_shape, tb, axis,
+                                       adjoint, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,353 @@
+// Stable, deterministic cell sort (least-significant-digit radix, 8-bit digits).
+//
+// Each pass is a stable counting sort of one digit:
+//   1. k_digit_hist   one wave per contiguous segment of kSeg items counts its digits;
+//   2. exclusive scan of the digit-major (digit, segment) count matrix;
+//   3. k_digit_scatter the same waves walk their segment in order, rank each item among
+//      equal digits of its 64-item group with ballot masks, and place it at
+//      offset[digit][segment] + running count -- input order is preserved inside every
+//      key, so the result equals numpy.argsort(keys, kind="stable") bit for bit.
+// The key of the first pass is computed on the fly from (pmid, disp): the raveled brick
+// index of the containing cell, with invalid particles mapped to the sentinel.
+#include "pm_common.h"
+#include "pm_sort.h"
+
+namespace pm {
+namespace sort {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kIters = 128;               // 64-item groups per segment
+constexpr int kSeg = kWave * kIters;      // items per wave segment
+constexpr int kRadix = 256;
+
+struct KeyGeom {
+  int n[3];
+  int b[3];
+  int nb[3];
+  float cell;
+  int sentinel;
+};
+
+__device__ __forceinline__ int cell_key(const KeyGeom& g, const int16_t* __restrict__ pmid,
+                                        const float* __restrict__ disp,
+                                        const uint8_t* __restrict__ valid, int64_t i) {
+  if (valid && !valid[i]) return g.sentinel;
+  int c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    // containing cell: the CIC base index with offset 0 (cic/pallas.py:227-236)
+    float t = disp[i * 3 + a] / g.cell;
+    int base = (int)pmid[i * 3 + a] + (int)floorf(t);
+    c[a] = pos_mod(base, g.n[a]) / g.b[a];
+  }
+  return (c[0] * g.nb[1] + c[1]) * g.nb[2] + c[2];
+}
+
+template <bool FIRST>
+__global__ void __launch_bounds__(kWave * kWavesPerBlock)
+    k_digit_hist(const int32_t* __restrict__ keys, const int16_t* __restrict__ pmid,
+                 const float* __restrict__ disp, const uint8_t* __restrict__ valid, KeyGeom g,
+                 int64_t n, int shift, int64_t nseg, int32_t* __restrict__ hist) {
+  __shared__ int32_t h[kWavesPerBlock][kRadix];
+  const int lane = threadIdx.x % kWave;
+  const int w = threadIdx.x / kWave;
+  for (int d = lane; d < kRadix; d += kWave) h[w][d] = 0;
+  __syncthreads();
+  const int64_t seg = (int64_t)blockIdx.x * kWavesPerBlock + w;
+  if (seg < nseg) {
+    const int64_t start = seg * kSeg;
+    for (int it = 0; it < kIters; ++it) {
+      int64_t i = start + it * kWave + lane;
+      if (i < n) {
+        int key = FIRST ? cell_key(g, pmid, disp, valid, i) : keys[i];
+        atomicAdd(&h[w][(key >> shift) & (kRadix - 1)], 1);
+      }
+    }
+  }
+  __syncthreads();
+  if (seg < nseg)
+    for (int d = lane; d < kRadix; d += kWave) hist[(int64_t)d * nseg + seg] = h[w][d];
+}
+
+template <bool FIRST>
+__global__ void __launch_bounds__(kWave * kWavesPerBlock)
+    k_digit_scatter(const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in,
+                    const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+                    const uint8_t* __restrict__ valid, KeyGeom g, int64_t n, int shift, int64_t nseg,
+                    const int32_t* __restrict__ offsets, int32_t* __restrict__ keys_out,
+                    int32_t* __restrict__ vals_out) {
+  __shared__ int32_t cnt[kWavesPerBlock][kRadix];
+  const int lane = threadIdx.x % kWave;
+  const int w = threadIdx.x / kWave;
+  const int64_t seg = (int64_t)blockIdx.x * kWavesPerBlock + w;
+  if (seg >= nseg) return;  // whole wave exits together; no block barriers below
+  for (int d = lane; d < kRadix; d += kWave) cnt[w][d] = offsets[(int64_t)d * nseg + seg];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t start = seg * kSeg;
+  for (int it = 0; it < kIters; ++it) {
+    int64_t i = start + it * kWave + lane;
+    bool active = i < n;
+    int key = 0, val = 0;
+    if (active) {
+      if (FIRST) {
+        key = cell_key(g, pmid, disp, valid, i);
+        val = (int)i;
+      } else {
+        key = keys_in[i];
+        val = vals_in[i];
+      }
+    }
+    int d = (key >> shift) & (kRadix - 1);
+    uint64_t peers = __ballot(active);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      bool set = (d >> bit) & 1;
+      uint64_t m = __ballot(set);
+      peers &= set ? m : ~m;
+    }
+    int rank = __popcll(peers & lt_mask);
+    int base = 0;
+    if (active) base = cnt[w][d];
+    __builtin_amdgcn_wave_barrier();
+    if (active) {
+      int pos = base + rank;
+      if (keys_out) keys_out[pos] = key;
+      vals_out[pos] = val;
+      if (rank == 0) cnt[w][d] = base + __popcll(peers);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  }
+}
+
+// ---- exclusive scan of int32 counts (3 phases) ----
+constexpr int kScanBlock = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanChunk = kScanBlock * kScanPer;
+
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* sh, int32_t& total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    int32_t a = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  total = sh[kScanBlock - 1];
+  int32_t ex = sh[t] - v;
+  __syncthreads();
+  return ex;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const int32_t* __restrict__ in, int64_t len,
+                                                            int32_t* __restrict__ partial) {
+  __shared__ int32_t sh[kScanBlock];
+  int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+  int32_t s = 0;
+  for (int j = 0; j < kScanPer; ++j)
+    if (base + j < len) s += in[base + j];
+  int32_t total;
+  block_exclusive_scan(s, sh, total);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_partials(int32_t* __restrict__ partial, int64_t len) {
+  __shared__ int32_t sh[kScanBlock];
+  int32_t carry = 0;
+  for (int64_t base = 0; base < len; base += kScanBlock) {
+    int64_t i = base + threadIdx.x;
+    int32_t v = i < len ? partial[i] : 0;
+    int32_t total;
+    int32_t ex = block_exclusive_scan(v, sh, total);
+    if (i < len) partial[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_apply(const int32_t* __restrict__ in, int64_t len,
+                                                           const int32_t* __restrict__ partial,
+                                                           int32_t* __restrict__ out) {
+  __shared__ int32_t sh[kScanBlock];
+  int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+  int32_t vals[kScanPer];
+  int32_t s = 0;
+  for (int j = 0; j < kScanPer; ++j) {
+    vals[j] = (base + j < len) ? in[base + j] : 0;
+    s += vals[j];
+  }
+  int32_t total;
+  int32_t ex = block_exclusive_scan(s, sh, total) + partial[blockIdx.x];
+  for (int j = 0; j < kScanPer; ++j) {
+    if (base + j < len) out[base + j] = ex;
+    ex += vals[j];
+  }
+}
+
+// bin_offsets[b] = first sorted position with key >= b, for b in [0, num_keys]
+__global__ void k_bin_bounds(const int32_t* __restrict__ keys, int64_t n, int num_keys,
+                             int32_t* __restrict__ bounds) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n) return;
+  int lo = (j == 0) ? 0 : keys[j - 1] + 1;
+  int hi = (j == n) ? num_keys : keys[j];
+  for (int b = lo; b <= hi; ++b) bounds[b] = (int32_t)j;
+}
+
+int scan_exclusive(const int32_t* in, int32_t* out, int64_t len, int32_t* partial, hipStream_t s) {
+  if (len == 0) return PM_SUCCESS;
+  int64_t blocks = ceil_div(len, kScanChunk);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)blocks), dim3(kScanBlock), 0, s, in, len, partial);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlock), 0, s, partial, blocks);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)blocks), dim3(kScanBlock), 0, s, in, len, partial, out);
+  PM_LAUNCHED("scan", s);
+  return PM_SUCCESS;
+}
+
+KeyGeom make_key_geom(const int32_t* mesh_shape, float cell, const int32_t* brick) {
+  KeyGeom g;
+  int64_t nbins = 1;
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = mesh_shape[a];
+    g.b[a] = brick[a];
+    g.nb[a] = (int)ceil_div(mesh_shape[a], brick[a]);
+    nbins *= g.nb[a];
+  }
+  g.cell = cell;
+  g.sentinel = (int)nbins;
+  return g;
+}
+
+int key_bits(int64_t max_key) {
+  int b = 1;
+  while ((int64_t(1) << b) <= max_key) ++b;
+  return b;
+}
+
+}  // namespace
+
+int check_sort_args(int64_t n, const int32_t* mesh_shape, const int32_t* brick) {
+  PM_RETURN_IF(n < 0 || n > INT32_MAX, PM_ERROR_UNSUPPORTED, "particle count must be in [0, 2^31)");
+  PM_RETURN_IF(mesh_shape == nullptr || brick == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL shape");
+  int64_t nbins = 1;
+  for (int a = 0; a < 3; ++a) {
+    PM_RETURN_IF(mesh_shape[a] < 1 || brick[a] < 1, PM_ERROR_INVALID_ARGUMENT,
+                 "mesh_shape and brick must be positive");
+    nbins *= ceil_div(mesh_shape[a], brick[a]);
+  }
+  PM_RETURN_IF(nbins >= INT32_MAX, PM_ERROR_UNSUPPORTED,
+               "more than 2^31-1 bricks: use a coarser brick for this mesh");
+  return PM_SUCCESS;
+}
+
+size_t workspace_bytes(int64_t n, const int32_t* mesh_shape, const int32_t* brick) {
+  int64_t nseg = ceil_div(n, kSeg);
+  int64_t hist = (int64_t)kRadix * nseg;
+  size_t b = 0;
+  b += align_up(sizeof(int32_t) * n, 256) * 3;  // keys a/b, vals
+  b += align_up(sizeof(int32_t) * hist, 256) * 2;
+  b += align_up(sizeof(int32_t) * (ceil_div(hist, kScanChunk) + 1), 256);
+  (void)mesh_shape;
+  (void)brick;
+  return b + 256;
+}
+
+int sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+               const int32_t* mesh_shape, float cell_size, const int32_t* brick, int32_t* perm,
+               int32_t* bin_offsets, Arena& ws, hipStream_t s) {
+  KeyGeom g = make_key_geom(mesh_shape, cell_size, brick);
+  if (n == 0) {
+    if (bin_offsets) {
+      hipLaunchKernelGGL(k_bin_bounds, dim3(1), dim3(1), 0, s, (const int32_t*)nullptr, (int64_t)0,
+                         g.sentinel + 1, bin_offsets);
+      PM_LAUNCHED("k_bin_bounds", s);
+    }
+    return PM_SUCCESS;
+  }
+  const int64_t nseg = ceil_div(n, kSeg);
+  const int64_t hist_len = (int64_t)kRadix * nseg;
+  int32_t* keys_a = ws.take<int32_t>(n);
+  int32_t* keys_b = ws.take<int32_t>(n);
+  int32_t* vals_t = ws.take<int32_t>(n);
+  int32_t* hist = ws.take<int32_t>(hist_len);
+  int32_t* offs = ws.take<int32_t>(hist_len);
+  int32_t* partial = ws.take<int32_t>(ceil_div(hist_len, kScanChunk) + 1);
+  PM_RETURN_IF(!ws.ok(), PM_ERROR_WORKSPACE, "sort workspace too small");
+
+  const int bits = key_bits(g.sentinel);
+  const int passes = (bits + 7) / 8;
+  const dim3 grid((unsigned)ceil_div(nseg, kWavesPerBlock)), block(kWave * kWavesPerBlock);
+  // value ping-pong arranged so the last pass writes `perm`
+  int32_t* vbuf[2] = {(passes % 2 == 1) ? perm : vals_t, (passes % 2 == 1) ? vals_t : perm};
+  int32_t* kbuf[2] = {keys_a, keys_b};
+  for (int pass = 0; pass < passes; ++pass) {
+    const int shift = 8 * pass;
+    const bool last = pass == passes - 1;
+    int32_t* kin = kbuf[(pass + 1) % 2];
+    int32_t* kout = (last && !bin_offsets) ? nullptr : kbuf[pass % 2];
+    int32_t* vin = vbuf[(pass + 1) % 2];
+    int32_t* vout = vbuf[pass % 2];
+    if (pass == 0)
+      hipLaunchKernelGGL(k_digit_hist<true>, grid, block, 0, s, nullptr, pmid, disp, valid, g, n, shift,
+                         nseg, hist);
+    else
+      hipLaunchKernelGGL(k_digit_hist<false>, grid, block, 0, s, kin, pmid, disp, valid, g, n, shift,
+                         nseg, hist);
+    PM_LAUNCHED("k_digit_hist", s);
+    int st = scan_exclusive(hist, offs, hist_len, partial, s);
+    if (st) return st;
+    if (pass == 0)
+      hipLaunchKernelGGL(k_digit_scatter<true>, grid, block, 0, s, nullptr, nullptr, pmid, disp, valid, g,
+                         n, shift, nseg, offs, kout, vout);
+    else
+      hipLaunchKernelGGL(k_digit_scatter<false>, grid, block, 0, s, kin, vin, pmid, disp, valid, g, n,
+                         shift, nseg, offs, kout, vout);
+    PM_LAUNCHED("k_digit_scatter", s);
+    if (last && bin_offsets) {
+      hipLaunchKernelGGL(k_bin_bounds, dim3((unsigned)ceil_div(n + 1, 256)), dim3(256), 0, s, kout, n,
+                         g.sentinel + 1, bin_offsets);
+      PM_LAUNCHED("k_bin_bounds", s);
+    }
+  }
+  return PM_SUCCESS;
+}
+
+}  // namespace sort
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+int pm_sort_cells_workspace(int64_t n, const int32_t mesh_shape[3], const int32_t brick[3],
+                            size_t* bytes) {
+  int st = sort::check_sort_args(n, mesh_shape, brick);
+  if (st) return st;
+  PM_RETURN_IF(bytes == nullptr, PM_ERROR_INVALID_ARGUMENT, "bytes is NULL");
+  *bytes = sort::workspace_bytes(n, mesh_shape, brick);
+  return PM_SUCCESS;
+}
+
+int pm_sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                  const int32_t mesh_shape[3], float cell_size, const int32_t brick[3],
+                  int32_t* perm, int32_t* bin_offsets, void* workspace, size_t workspace_bytes,
+                  void* stream) {
+  int st = sort::check_sort_args(n, mesh_shape, brick);
+  if (st) return st;
+  PM_RETURN_IF(!(cell_size > 0.f), PM_ERROR_INVALID_ARGUMENT, "cell_size must be positive");
+  PM_RETURN_IF(n > 0 && (pmid == nullptr || disp == nullptr || perm == nullptr),
+               PM_ERROR_INVALID_ARGUMENT, "pmid/disp/perm NULL");
+  size_t need = sort::workspace_bytes(n, mesh_shape, brick);
+  PM_RETURN_IF(workspace == nullptr || workspace_bytes < need, PM_ERROR_WORKSPACE,
+               "sort workspace too small: need " + std::to_string(need) + " bytes");
+  Arena ws(workspace, workspace_bytes);
+  return sort::sort_cells(pmid, disp, valid, n, mesh_shape, cell_size, brick, perm, bin_offsets, ws,
+                          as_stream(stream));
+}
+
+}  // extern "C"

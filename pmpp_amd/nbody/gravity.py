@@ -1,0 +1,85 @@
+"""PM gravity on the HIP library (drop-in for ``pmpp.nbody.gravity``).
+
+``gravity(a, ptcl, cosmo, conf, correction=None)`` follows ``src/pmpp/nbody/gravity.py:786-814``
+on the single-device streamed branch (:597-606, :506-543): CIC density of ``N_m/N_p`` per valid
+particle, source 1.5 Omega_m (rho - 1), rfftn, particle-Nyquist mask, -1/k^2, -i k_i, irfftn and
+CIC gather per component.  The whole step is one call into ``pm_gravity``; its reverse is
+``pm_gravity_vjp`` (the VJP walked by ``force_adj``, ``nbody/integrator.py:620-686``), exposed
+through ``torch.autograd`` with cotangents for ``disp`` and ``Omega_m``.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from .. import _lib
+from .._torch import as_contig, ensure_init, ptr, stream_handle, workspace
+
+
+def gravity_desc(conf, omega_m) -> _lib.GravityDesc:
+    d = _lib.GravityDesc()
+    for a in range(3):
+        d.mesh_shape[a] = int(conf.mesh_shape[a])
+        d.brick[a] = int(conf.brick[a])
+    d.cell_size = float(conf.cell_size)
+    d.ptcl_spacing = float(conf.ptcl_spacing)
+    d.mask_active = int(conf.mask_active)
+    d.omega_m = float(torch.as_tensor(omega_m, dtype=torch.float32))  # cast as gravity.py:630
+    d.particle_value = float(conf.mesh_size / conf.ptcl_num)
+    return d
+
+
+def _valid_u8(ptcl):
+    if ptcl.unused_index is None:
+        return None
+    return (~ptcl.unused_index.detach()).to(torch.uint8).contiguous()
+
+
+class _GravityFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, disp, omega_m, pmid, valid, conf):
+        dev = disp.device
+        ensure_init(dev)
+        d = gravity_desc(conf, omega_m.detach())
+        pm = as_contig(pmid, torch.int16, "pmid")
+        dp = as_contig(disp.detach(), torch.float32, "disp")
+        n = pm.shape[0]
+        nbytes = _lib.size_query("pm_gravity_workspace", d, n)
+        ws = workspace(nbytes, dev)
+        acc = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        _lib.call("pm_gravity", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(acc), ptr(ws), nbytes, stream_handle(dev))
+        ctx.save_for_backward(dp, omega_m, pm, valid)
+        ctx.conf = conf
+        return acc
+
+    @staticmethod
+    def backward(ctx, acc_cot):
+        dp, omega_m, pm, valid = ctx.saved_tensors
+        conf = ctx.conf
+        dev = dp.device
+        d = gravity_desc(conf, omega_m.detach())
+        n = pm.shape[0]
+        nbytes = _lib.size_query("pm_gravity_workspace", d, n)
+        ws = workspace(nbytes, dev)
+        ac = as_contig(acc_cot, torch.float32, "acc_cot")
+        disp_cot = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        omega_cot = torch.zeros((), dtype=torch.float32, device=dev)
+        _lib.call("pm_gravity_vjp", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(ac), ptr(disp_cot), ptr(omega_cot),
+                  ptr(ws), nbytes, stream_handle(dev))
+        return disp_cot, omega_cot.to(omega_m.dtype).reshape(omega_m.shape), None, None, None
+
+
+def gravity(a, ptcl, cosmo, conf, correction=None):
+    """Gravitational accelerations of particles in [H_0^2] (gravity.py:786-814).
+
+    ``cosmo`` needs an ``Omega_m`` (float or tensor; a tensor with ``requires_grad`` gets a
+    cotangent).  ``a`` is unused by the uncorrected force, as in the reference.
+    """
+    del a
+    if correction is not None:
+        from ..corrections import apply_correction_gravity
+        return apply_correction_gravity(ptcl, cosmo, conf, correction)
+    omega = cosmo.Omega_m if hasattr(cosmo, "Omega_m") else cosmo
+    if not isinstance(omega, torch.Tensor):
+        omega = torch.tensor(float(omega), dtype=torch.float32, device=ptcl.disp.device)
+    return _GravityFn.apply(ptcl.disp, omega, ptcl.pmid.detach(), _valid_u8(ptcl), conf)

@@ -1,0 +1,58 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol, rejects bad input."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from pmpp_amd import _lib
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "pmpp_amd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(pm_\w+)\s*\(", text, flags=re.M)
+
+
+def test_header_declares_what_the_binding_types():
+    names = declared_functions()
+    assert len(names) == len(set(names))
+    assert set(names) == set(_lib.EXPORTED_SYMBOLS)
+
+
+def test_library_loads_and_exports_every_symbol():
+    lib = _lib.load_library()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    assert lib.pm_abi_version() == _lib.ABI_VERSION
+
+
+def test_queries_validate_without_a_gpu():
+    lib = _lib.load_library()
+    out = ctypes.c_size_t(0)
+    assert lib.pm_fft_workspace(_lib.int3((8, 8, 8)), ctypes.byref(out)) == _lib.PM_SUCCESS
+    assert lib.pm_fft_workspace(_lib.int3((6, 8, 8)), ctypes.byref(out)) == _lib.PM_ERROR_UNSUPPORTED
+    assert b"powers of two" in lib.pm_last_error()
+    assert lib.pm_sort_cells_workspace(1000, _lib.int3((8, 8, 8)), _lib.int3((2, 2, 2)),
+                                       ctypes.byref(out)) == _lib.PM_SUCCESS
+    assert out.value > 4 * 1000
+    assert lib.pm_sort_cells_workspace(10, _lib.int3((8, 0, 8)), _lib.int3((2, 2, 2)),
+                                       ctypes.byref(out)) == _lib.PM_ERROR_INVALID_ARGUMENT
+    from pmpp_amd import Configuration
+    from pmpp_amd.nbody.gravity import gravity_desc
+
+    d = gravity_desc(Configuration(1.0, (16, 16, 16), mesh_shape=2), 0.3)
+    assert lib.pm_gravity_workspace(d, 4096, ctypes.byref(out)) == _lib.PM_SUCCESS
+    assert out.value > 3 * 4 * 32 ** 3
+    with pytest.raises(NotImplementedError, match="powers of two"):
+        _lib.call("pm_gravity_workspace", gravity_desc(Configuration(1.0, (12, 12, 12), mesh_shape=1), 0.3),
+                  10, ctypes.byref(out))
+
+
+def test_null_arguments_are_rejected_before_any_launch():
+    lib = _lib.load_library()
+    assert lib.pm_gravity(None, None, None, 10, None, None, None, 0, None) == _lib.PM_ERROR_INVALID_ARGUMENT
+    assert lib.pm_cic_gather(None, None, None, 5, 1, None, None, None, None) == _lib.PM_ERROR_INVALID_ARGUMENT

@@ -1,0 +1,107 @@
+"""HIP PM force step vs the float64 oracle, closed forms and the reference's own gates."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import spectral
+from pmpp_amd import Configuration, Particles, gravity
+
+pytestmark = pytest.mark.gpu
+
+
+def _ptcl(conf, disp_scale, seed, device, valid_frac=None):
+    base = Particles.gen_grid(conf, device=device)
+    rng = np.random.default_rng(seed)
+    d = base.disp.cpu().numpy() + (rng.normal(size=base.disp.shape) * disp_scale * conf.ptcl_spacing)
+    unused = None
+    if valid_frac is not None:
+        unused = torch.as_tensor(rng.random(len(base)) > valid_frac, device=device)
+    return base.replace(disp=torch.as_tensor(d.astype(np.float32), device=device),
+                        unused_index=unused if unused is not None else base.unused_index)
+
+
+def _oracle_acc(conf, ptcl, omega):
+    pm = ptcl.pmid.cpu().numpy().astype(np.int64)
+    dp = ptcl.disp.cpu().numpy().astype(np.float64)
+    valid = None if ptcl.unused_index is None else ~ptcl.unused_index.cpu().numpy()
+    masks = spectral.particle_nyquist_masks(conf.mesh_shape, conf.ptcl_grid_shape, conf.ptcl_spacing,
+                                            conf.cell_size)
+    return spectral.gravity(pm, dp, omega, conf.mesh_shape, conf.cell_size, conf.ptcl_num, masks, valid), masks
+
+
+@pytest.mark.parametrize("ng,ratio,box,scale,valid_frac", [
+    (8, 1, 8.0, 0.3, None),
+    (8, 2, 16.0, 0.3, None),
+    (16, 2, 64.0, 1.5, None),
+    (8, 4, 8.0, 0.2, None),
+    (16, 2, 32.0, 0.5, 0.8),
+    (32, 2, 128.0, 2.0, None),
+])
+def test_force_matches_float64_oracle(device, ng, ratio, box, scale, valid_frac):
+    conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio)
+    ptcl = _ptcl(conf, scale, ng * ratio, device, valid_frac)
+    acc = gravity(1.0, ptcl, 0.3, conf).cpu().numpy()
+    ref, _ = _oracle_acc(conf, ptcl, 0.3)
+    # float32 path vs float64: relative error of the field, stated tolerance 1e-5
+    err = np.linalg.norm(acc - ref) / np.linalg.norm(ref)
+    assert err < 1e-5, err
+    np.testing.assert_allclose(acc, ref, atol=3e-5 * np.abs(ref).max())
+    if valid_frac is not None:
+        assert np.all(acc[ptcl.unused_index.cpu().numpy()] == 0)
+
+
+def test_uniform_lattice_is_force_free(device):
+    # tests/test_deep_gravity_spectral.py:253-264
+    conf = Configuration(1.25, (4, 4, 4), mesh_shape=1)
+    grid = Particles.gen_grid(conf, device=device)
+    acc = gravity(0.8, grid, 0.3, conf)
+    assert float(acc.abs().max()) <= 3e-7
+    disp = grid.disp.clone()
+    disp[0, 0] = 1e-3
+    pert = gravity(0.8, grid.replace(disp=disp), 0.3, conf).cpu().numpy()
+    assert np.linalg.norm(pert) > 1e-7
+    assert np.linalg.norm(pert.sum(axis=0)) < 2e-5
+
+
+def test_refined_force_mesh_does_not_alias_particle_lattice(device):
+    # tests/test_gravity_particle_nyquist_filter.py:11-40 restated
+    res, box, amp = 16, 1000.0, 0.5
+
+    def modes(mesh_shape):
+        conf = Configuration(box / res, (res,) * 3, mesh_shape=mesh_shape)
+        axis = np.arange(res, dtype=np.float32) * np.float32(box / res)
+        x, y, z = np.meshgrid(axis, axis, axis, indexing="ij")
+        pos = np.stack([x, y, z], -1).reshape(-1, 3)
+        pos[:, 0] = (pos[:, 0] + np.float32(amp) * np.sin(np.float32(2 * np.pi / box) * pos[:, 0])) % np.float32(box)
+        ptcl = Particles.from_pos(conf, torch.as_tensor(pos, device=device))
+        acc = gravity(1.0, ptcl, 0.3, conf).cpu().numpy()
+        mean_x = acc[:, 0].reshape(res, res, res).mean(axis=(1, 2))
+        return np.abs(np.fft.rfft(mean_x)) / (res / 2)
+
+    base = modes(1)
+    for ms in (2, 4):
+        m = modes(ms)
+        assert np.isclose(m[1], base[1], rtol=0.03)
+        assert m[2] < 0.10 * base[1]
+
+
+@pytest.mark.parametrize("ng,ratio", [(8, 2), (16, 1)])
+def test_force_vjp_matches_oracle(device, ng, ratio):
+    conf = Configuration(2.0, (ng,) * 3, mesh_shape=ratio)
+    ptcl = _ptcl(conf, 0.4, 7, device)
+    disp = ptcl.disp.clone().requires_grad_(True)
+    omega = torch.tensor(0.3, device=device, requires_grad=True)
+    acc = gravity(1.0, ptcl.replace(disp=disp), type("C", (), {"Omega_m": omega})(), conf)
+    rng = np.random.default_rng(11)
+    ac = rng.normal(size=acc.shape).astype(np.float32)
+    (acc * torch.as_tensor(ac, device=device)).sum().backward()
+    masks = spectral.particle_nyquist_masks(conf.mesh_shape, conf.ptcl_grid_shape, conf.ptcl_spacing,
+                                            conf.cell_size)
+    dref, oref = spectral.gravity_vjp(ptcl.pmid.cpu().numpy().astype(np.int64),
+                                      ptcl.disp.cpu().numpy().astype(np.float64), 0.3, ac, conf.mesh_shape,
+                                      conf.cell_size, conf.ptcl_num, masks)
+    got = disp.grad.cpu().numpy()
+    # gradients within 1e-3 (relative, field norm)
+    assert np.linalg.norm(got - dref) / np.linalg.norm(dref) < 1e-3
+    assert float(omega.grad) == pytest.approx(oref, rel=1e-3)

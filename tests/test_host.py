@@ -1,0 +1,41 @@
+"""Host-side logic: configuration geometry and spectral caches vs the oracle."""
+
+import numpy as np
+import pytest
+
+from oracle import spectral
+from pmpp_amd import Configuration
+
+
+def test_configuration_geometry_and_validation():
+    conf = Configuration(1000 / 512, (512,) * 3, mesh_shape=2)
+    assert conf.mesh_shape == (1024,) * 3
+    assert conf.cell_size == pytest.approx(1000 / 1024)
+    assert conf.mesh_size == 1024 ** 3 and conf.ptcl_num == 512 ** 3
+    assert conf.brick == (32, 32, 32)
+    with pytest.raises(ValueError, match="cannot be smaller"):
+        Configuration(1.0, (8, 8, 8), mesh_shape=(4, 8, 8))
+    with pytest.raises(ValueError, match="aspect ratios"):
+        Configuration(1.0, (8, 8, 8), mesh_shape=(16, 8, 8))
+    with pytest.raises(ValueError, match="symplectic"):
+        Configuration(1.0, (8, 8, 8), symp_splits=((0, 0.5), (1, 0.6)))
+
+
+@pytest.mark.parametrize("ratio", [1, 2, 4])
+def test_spectral_caches_match_oracle(ratio):
+    conf = Configuration(2.0, (8, 8, 8), mesh_shape=ratio)
+    for a, k in enumerate(conf.kvec):
+        np.testing.assert_allclose(k, spectral.kvec(conf.mesh_shape, conf.cell_size)[a].astype(np.float32))
+    masks = spectral.particle_nyquist_masks(conf.mesh_shape, conf.ptcl_grid_shape, conf.ptcl_spacing,
+                                            conf.cell_size)
+    assert len(conf.particle_nyquist_masks) == len(masks)
+    for m, mo in zip(conf.particle_nyquist_masks, masks):
+        np.testing.assert_array_equal(m, mo)
+    for a, nik in enumerate(conf.neg_ik):
+        np.testing.assert_allclose(nik.imag, spectral.neg_k(conf.mesh_shape, conf.cell_size)[a], rtol=1e-6)
+
+
+def test_nbody_schedule():
+    conf = Configuration(2.0, (8, 8, 8), a_start=1 / 64, a_stop=1.0, a_nbody_maxstep=(1 - 1 / 64) / 10)
+    a = conf.a_nbody
+    assert len(a) == 11 and a[0] == pytest.approx(1 / 64) and a[-1] == pytest.approx(1.0)
