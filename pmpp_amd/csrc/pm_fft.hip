@@ -54,28 +54,42 @@ namespace fft {
 namespace {
 
 // ----------------------------------------------------------- column passes ----
-template <int N>
+// T adjacent columns per workgroup half; two-output passes use two halves, each of which
+// loads the same input tile (the second read is served by L2), runs the forward transform
+// and inverts one of the two outputs, so no thread keeps both spectra live.
+template <int N, int T, int HALVES>
 struct ColCfg {
-  static constexpr int T = (N >= 2048) ? 8 : 16;  // adjacent columns per workgroup
   static constexpr int P = Plan<N>::P;
-  static constexpr int THREADS = T * P;
+  static constexpr int HALF_THREADS = T * P;
+  static constexpr int THREADS = HALF_THREADS * HALVES;
   static constexpr int PADK = T;  // per-k1-block LDS pad: spreads the strided rows over banks
   static constexpr int LDS_FLOATS = N * T + Plan<N>::N1 * PADK;
 };
 
-template <int N, int MODE>
-__global__ void __launch_bounds__(ColCfg<N>::THREADS)
-    k_cols(ColPass io, SpecTables tb, int dax, const float2* __restrict__ tw) {
+// Column tiles are visited in an XCD-aware order: workgroups are dispatched round-robin
+// over the 8 XCDs, so logical tile q is given to workgroup (q % g8) * 8 + q / g8 ... i.e.
+// consecutive logical tiles (which share cache lines when T*8 < 128 bytes) run on the same
+// XCD at nearly the same time.
+template <int N, int T, int MODE, int HALVES>
+__global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
+    k_cols(ColPass io, SpecTables tb, int dax, const float2* __restrict__ tw, int gx, int gy, int g8) {
   using PL = Plan<N>;
-  using CF = ColCfg<N>;
-  constexpr int T = CF::T, E = PL::E, N2 = PL::N2;
-  __shared__ float lds[CF::LDS_FLOATS];
-  const int col = threadIdx.x % T;
-  const int p = threadIdx.x / T;
-  const int cg = blockIdx.x * T + col;
-  const int64_t b = blockIdx.y;
+  using CF = ColCfg<N, T, HALVES>;
+  constexpr int E = PL::E, N2 = PL::N2;
+  __shared__ float lds[HALVES * CF::LDS_FLOATS];
+  const int L = blockIdx.x;
+  const int q = (L % 8) * g8 + L / 8;
+  const int tilex = q % gx;
+  const int64_t b = q / gx;
+  if (b >= gy) return;  // padding workgroup of the XCD remap: uniform exit before any barrier
+  const int half = HALVES > 1 ? (int)(threadIdx.x / CF::HALF_THREADS) : 0;
+  const int lt = threadIdx.x % CF::HALF_THREADS;
+  const int col = lt % T;
+  const int p = lt / T;
+  const int cg = tilex * T + col;
   const bool cok = cg < io.ncols;
   const int64_t base = b * io.s_batch + (int64_t)cg * io.s_col;
+  float* my = lds + half * CF::LDS_FLOATS;
   auto idx = [&](int row) { return row * T + col + (row / N2) * CF::PADK; };
 
   float2 v[E];
@@ -83,7 +97,7 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
 #pragma unroll
     for (int s = 0; s < E; ++s)
       v[s] = cok ? io.in[base + (int64_t)nrow<N>(p, s) * io.s_row] : make_float2(0.f, 0.f);
-    forward_core<N>(v, lds, p, idx, tw);
+    forward_core<N>(v, my, p, idx, tw);
   } else {
 #pragma unroll
     for (int s = 0; s < E; ++s)
@@ -95,16 +109,11 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
 #pragma unroll
       for (int s = 0; s < E; ++s) io.out0[base + (int64_t)xrow<N>(p, s) * io.s_row] = v[s];
     }
-  } else if constexpr (MODE == kColInv) {
-    inverse_core<N>(v, lds, p, idx, tw);
-    if (cok) {
-#pragma unroll
-      for (int s = 0; s < E; ++s) io.out0[base + (int64_t)nrow<N>(p, s) * io.s_row] = v[s];
-    }
+    return;
   } else {
-    float2 w[E];
     if constexpr (MODE == kColGreen) {
-      // mask x Poisson symbol (gravity.py:174-196, 263-287), then D_x = i*negk_x
+      // mask x Poisson symbol (gravity.py:174-196, 263-287); the second half also applies
+      // D_x = i*negk_x (configuration.py:365-376)
       int ky = (int)(b * io.ky_b) + cg * io.ky_c;
       int kz = (int)(b * io.kz_b) + cg * io.kz_c + io.kz0;
       float kyv = cok ? tb.k[1][ky] : 0.f, kzv = cok ? tb.k[2][kz] : 0.f;
@@ -116,22 +125,19 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
         float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
         bool keep = keep_yz && tb.mask[0][kx] != 0.f && k2 != 0.f;
         float2 phi = keep ? make_float2(-v[s].x / k2, -v[s].y / k2) : make_float2(0.f, 0.f);
-        v[s] = phi;
-        w[s] = c_imul(tb.negk[0][kx], phi);
+        v[s] = (half == 1) ? c_imul(tb.negk[0][kx], phi) : phi;
       }
-    } else {  // kColInv2: second output multiplied by i*negk_dax along this axis
+    } else if constexpr (MODE == kColInv2) {
+      if (half == 1) {
 #pragma unroll
-      for (int s = 0; s < E; ++s) w[s] = c_imul(tb.negk[dax][xrow<N>(p, s)], v[s]);
+        for (int s = 0; s < E; ++s) v[s] = c_imul(tb.negk[dax][xrow<N>(p, s)], v[s]);
+      }
     }
-    inverse_core<N>(v, lds, p, idx, tw);
+    inverse_core<N>(v, my, p, idx, tw);
+    float2* out = (half == 1) ? io.out1 : io.out0;
     if (cok) {
 #pragma unroll
-      for (int s = 0; s < E; ++s) io.out0[base + (int64_t)nrow<N>(p, s) * io.s_row] = v[s];
-    }
-    inverse_core<N>(w, lds, p, idx, tw);
-    if (cok) {
-#pragma unroll
-      for (int s = 0; s < E; ++s) io.out1[base + (int64_t)nrow<N>(p, s) * io.s_row] = w[s];
+      for (int s = 0; s < E; ++s) out[base + (int64_t)nrow<N>(p, s) * io.s_row] = v[s];
     }
   }
 }
@@ -283,34 +289,35 @@ __global__ void k_spectral_multiply(const float2* __restrict__ in, float2* __res
   }
 }
 
+template <int N, int T, int MODE, int HALVES>
+int launch_cols_cfg(const ColPass& cp, int batches, const SpecTables& tb, int dax, hipStream_t s,
+                    const char* name) {
+  using CF = ColCfg<N, T, HALVES>;
+  const float2* tw = twiddle_table();
+  PM_RETURN_IF(tw == nullptr, PM_ERROR_HIP, "twiddle table not initialised");
+  const int gx = (int)ceil_div(cp.ncols, T);
+  const int64_t tiles = (int64_t)gx * batches;
+  const int64_t g = ceil_div(tiles, 8) * 8;
+  PM_RETURN_IF(g > INT32_MAX, PM_ERROR_UNSUPPORTED, "too many FFT column tiles");
+  hipLaunchKernelGGL((k_cols<N, T, MODE, HALVES>), dim3((unsigned)g), dim3(CF::THREADS), 0, s, cp, tb, dax, tw,
+                     gx, batches, (int)(g / 8));
+  PM_LAUNCH_CHECK(name);
+  PM_MARK(name, s);
+  return PM_SUCCESS;
+}
+
 template <int N>
 int launch_cols_n(int mode, const ColPass& cp, int batches, const SpecTables& tb, int dax,
                   hipStream_t s) {
-  using CF = ColCfg<N>;
-  const float2* tw = twiddle_table();
-  PM_RETURN_IF(tw == nullptr, PM_ERROR_HIP, "twiddle table not initialised");
-  dim3 grid((unsigned)ceil_div(cp.ncols, CF::T), (unsigned)batches);
-  dim3 block(CF::THREADS);
+  constexpr int T1 = (N >= 2048) ? 8 : 16;  // one-output passes: 128-byte row segments
+  constexpr int T2 = (N >= 2048) ? 4 : 8;   // two-output passes: two halves of T2 columns
   switch (mode) {
-    case kColFwd:
-      hipLaunchKernelGGL((k_cols<N, kColFwd>), grid, block, 0, s, cp, tb, dax, tw);
-      PM_LAUNCHED("fft_cols_fwd", s);
-      break;
-    case kColInv:
-      hipLaunchKernelGGL((k_cols<N, kColInv>), grid, block, 0, s, cp, tb, dax, tw);
-      PM_LAUNCHED("fft_cols_inv", s);
-      break;
-    case kColInv2:
-      hipLaunchKernelGGL((k_cols<N, kColInv2>), grid, block, 0, s, cp, tb, dax, tw);
-      PM_LAUNCHED("fft_cols_inv2", s);
-      break;
-    case kColGreen:
-      hipLaunchKernelGGL((k_cols<N, kColGreen>), grid, block, 0, s, cp, tb, dax, tw);
-      PM_LAUNCHED("fft_cols_green", s);
-      break;
+    case kColFwd: return launch_cols_cfg<N, T1, kColFwd, 1>(cp, batches, tb, dax, s, "fft_cols_fwd");
+    case kColInv: return launch_cols_cfg<N, T1, kColInv, 1>(cp, batches, tb, dax, s, "fft_cols_inv");
+    case kColInv2: return launch_cols_cfg<N, T2, kColInv2, 2>(cp, batches, tb, dax, s, "fft_cols_inv2");
+    case kColGreen: return launch_cols_cfg<N, T2, kColGreen, 2>(cp, batches, tb, dax, s, "fft_cols_green");
     default: return fail(PM_ERROR_INVALID_ARGUMENT, "bad column mode");
   }
-  return PM_SUCCESS;
 }
 
 template <int M>

@@ -95,6 +95,428 @@ __global__ void __launch_bounds__(kBlock) k_gather3_sorted(const int16_t* __rest
   acc[i * 3 + 2] = r[2];
 }
 
+// ----------------------------------------------------------------------------------------
+// Brick-tiled CIC on sorted particles.  The mesh is tiled by bricks of (BX,BY,BZ) cells; the
+// cell sort groups particles by the brick of their base (corner-0) cell, so every corner a
+// particle touches lies in its brick's tile extended by one cell on the upper side.
+//
+// Scatter (owner computes): one workgroup per brick accumulates its particles into an LDS
+// tile of (BX+1)(BY+1)(BZ+1) floats with LDS atomics, then stores the BX*BY*BZ owned cells
+// with plain stores (every mesh cell is written exactly once; no zero fill, no global
+// atomics) and the upper halo shell into a per-brick shell buffer.  A second pass folds the
+// shells of the 7 lower neighbours into each brick's lower faces (pull, deterministic).
+// Gather: one workgroup per brick stages each force component's extended tile in LDS and
+// interpolates its particles from there.
+constexpr int kTileMaxFloats = 19100;  // (32+1)*(16+1)*(32+2) = 19074 fits (76.3 KB: 2 workgroups/CU)
+
+struct Bricks {
+  int n[3];   // mesh shape
+  int b[3];   // brick shape
+  int nb[3];  // bricks per axis
+  int t[3];   // tile shape b+1
+  int fx, fy, fz, sh;  // shell face sizes and total
+};
+
+inline Bricks make_bricks(const int32_t* mesh, const int32_t* brick) {
+  Bricks k;
+  for (int a = 0; a < 3; ++a) {
+    k.n[a] = mesh[a];
+    k.b[a] = brick[a];
+    k.nb[a] = mesh[a] / brick[a];
+    k.t[a] = brick[a] + 1;
+  }
+  k.fx = (k.b[1] + 1) * (k.b[2] + 1);
+  k.fy = k.b[0] * (k.b[2] + 1);
+  k.fz = k.b[0] * k.b[1];
+  k.sh = k.fx + k.fy + k.fz;
+  return k;
+}
+
+inline bool tiled_ok(const int32_t* mesh, const int32_t* brick) {
+  for (int a = 0; a < 3; ++a)
+    if (brick[a] < 1 || mesh[a] % brick[a] != 0) return false;
+  // LDS tile rows have a z pitch of b2 + 2 floats
+  return (int64_t)(brick[0] + 1) * (brick[1] + 1) * (brick[2] + 2) <= kTileMaxFloats;
+}
+
+inline bool is_brick(const Bricks& k, int x, int y, int z) { return k.b[0] == x && k.b[1] == y && k.b[2] == z; }
+
+// index of an upper-shell cell (x == bx or y == by or z == bz) in a brick's shell block
+__device__ __forceinline__ int shell_index(const Bricks& k, int x, int y, int z) {
+  if (x == k.b[0]) return y * (k.b[2] + 1) + z;
+  if (y == k.b[1]) return k.fx + x * (k.b[2] + 1) + z;
+  return k.fx + k.fy + x * k.b[1] + y;
+}
+
+__device__ __forceinline__ void brick_coords(const Bricks& k, int64_t bid, int c[3]) {
+  c[2] = (int)(bid % k.nb[2]);
+  c[1] = (int)((bid / k.nb[2]) % k.nb[1]);
+  c[0] = (int)(bid / ((int64_t)k.nb[2] * k.nb[1]));
+}
+
+constexpr int kTileThreads = 512;
+constexpr int kBatch = 8;  // independent loads kept in flight per thread in the runtime-shape tile loops
+
+// Compile-time brick shape when BX > 0 (integer divisions by the tile extents become
+// shifts/multiplies); BX = 0 reads the shape from the runtime descriptor.
+//
+// LDS tile layout: tile[(x * t1 + y) * pitch + z], z in [0, b2], pitch = b2 + 2, so every
+// corner of a particle is its corner-0 address plus a compile-time offset, and rows are
+// 8-byte aligned for float2 LDS stores when b2 is even.
+template <int BX, int BY, int BZ>
+struct BrickShape {
+  int b0, b1, b2;
+  static constexpr bool kVec = BX > 0 && BZ % 4 == 0;
+  __device__ __forceinline__ explicit BrickShape(const Bricks& k)
+      : b0(BX ? BX : k.b[0]), b1(BY ? BY : k.b[1]), b2(BZ ? BZ : k.b[2]) {}
+  __device__ __forceinline__ int t0() const { return b0 + 1; }
+  __device__ __forceinline__ int t1() const { return b1 + 1; }
+  __device__ __forceinline__ int t2() const { return b2 + 1; }
+  __device__ __forceinline__ int pitch() const { return b2 + 2; }
+  __device__ __forceinline__ int rows() const { return (b0 + 1) * (b1 + 1); }
+  __device__ __forceinline__ int tfloats() const { return (b0 + 1) * (b1 + 1) * (b2 + 2); }
+  __device__ __forceinline__ int cell(int x, int y, int z) const { return (x * (b1 + 1) + y) * (b2 + 2) + z; }
+  // offset of corner c relative to corner 0
+  __device__ __forceinline__ int corner_off(int c) const {
+    return ((c & 1) * (b1 + 1) + ((c >> 1) & 1)) * (b2 + 2) + ((c >> 2) & 1);
+  }
+};
+
+// Base cell and fractional offset of a particle for the offset-0, same-cell-size CIC of the
+// force step: bit-identical to axis_cic_same_cell with offset 0 (rem = 0 there).
+__device__ __forceinline__ void base_frac(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+                                          int64_t j, float a1, const int n[3], int base[3], float f[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float t = disp[j * 3 + a] / a1;
+    float lf = floorf(t);
+    base[a] = pos_mod((int)pmid[j * 3 + a] + (int)lf, n[a]);
+    f[a] = t - lf;
+  }
+}
+
+// CIC weight of corner c from the fractional offsets, in the reference's product order
+__device__ __forceinline__ float frac_weight(const float f[3], int c) {
+  float wx = (c & 1) ? 1.f - fabsf(f[0] - 1.f) : 1.f - fabsf(f[0]);
+  float wy = ((c >> 1) & 1) ? 1.f - fabsf(f[1] - 1.f) : 1.f - fabsf(f[1]);
+  float wz = ((c >> 2) & 1) ? 1.f - fabsf(f[2] - 1.f) : 1.f - fabsf(f[2]);
+  return wx * wy * wz;
+}
+
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kTileThreads) k_scatter_tiles(
+    const int16_t* __restrict__ pmid, const float* __restrict__ disp, const int32_t* __restrict__ bin_off,
+    float val, Geom g, Bricks k, float* __restrict__ mesh, float* __restrict__ shell) {
+  __shared__ __align__(16) float tile[kTileMaxFloats];
+  const BrickShape<BX, BY, BZ> B(k);
+  const int64_t bid = blockIdx.x;
+  int bc[3];
+  brick_coords(k, bid, bc);
+  const int o[3] = {bc[0] * B.b0, bc[1] * B.b1, bc[2] * B.b2};
+  const int tf = B.tfloats();
+  for (int i = threadIdx.x; i < tf; i += kTileThreads) tile[i] = 0.f;
+  __syncthreads();
+  const int64_t j0 = bin_off[bid], j1 = bin_off[bid + 1];
+  constexpr int PPT = 4;
+  for (int64_t c0 = j0; c0 < j1; c0 += (int64_t)kTileThreads * PPT) {
+    int bs[PPT][3];
+    float fr[PPT][3];
+    bool act[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {  // all loads of the chunk first
+      int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
+      act[q] = j < j1;
+      if (act[q]) base_frac(pmid, disp, j, g.a1, k.n, bs[q], fr[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      if (!act[q]) continue;
+      int l0 = bs[q][0] - o[0], l1 = bs[q][1] - o[1], l2 = bs[q][2] - o[2];
+      bool inside = l0 >= 0 && l0 < B.b0 && l1 >= 0 && l1 < B.b1 && l2 >= 0 && l2 < B.b2;
+      if (inside) {
+        const int c0i = B.cell(l0, l1, l2);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) atomicAdd(&tile[c0i + B.corner_off(c)], val * frac_weight(fr[q], c));
+      } else {  // cannot happen when the key and the CIC agree; stay correct regardless
+        int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
+        AxisCic ax[3];
+        particle_cic(g, pmid, disp, j, ax);
+        for (int c = 0; c < 8; ++c) {
+          int64_t flat;
+          corner_index(g, ax, c, flat);
+          atomic_add_f32(mesh + flat, val * corner_weight(ax, c));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // owned cells x < b0, y < b1, z < b2: main rows, moved as float4 when aligned
+  if constexpr (BrickShape<BX, BY, BZ>::kVec) {
+    constexpr int C4 = BZ / 4;
+    constexpr int ITEMS = BX * BY * C4;
+    for (int it = threadIdx.x; it < ITEMS; it += kTileThreads) {
+      int r = it / C4, c4 = it - r * C4;
+      int x = r / BY, y = r - x * BY;
+      const float* src = &tile[B.cell(x, y, 4 * c4)];
+      float2 lo = *reinterpret_cast<const float2*>(src), hi = *reinterpret_cast<const float2*>(src + 2);
+      *reinterpret_cast<float4*>(&mesh[((int64_t)(o[0] + x) * k.n[1] + (o[1] + y)) * k.n[2] + o[2] + 4 * c4]) =
+          make_float4(lo.x, lo.y, hi.x, hi.y);
+    }
+  } else {
+    const int rows = B.b0 * B.b1;
+    for (int i = threadIdx.x; i < rows * B.b2; i += kTileThreads) {
+      int r = i / B.b2, z = i - r * B.b2;
+      int x = r / B.b1, y = r - x * B.b1;
+      mesh[((int64_t)(o[0] + x) * k.n[1] + (o[1] + y)) * k.n[2] + (o[2] + z)] = tile[B.cell(x, y, z)];
+    }
+  }
+  // upper shell: face x = b0 (t1*t2), face y = b1 (b0*t2), face z = b2 (b0*b1)
+  float* sh = shell + bid * k.sh;
+  const int t2 = B.t2();
+  const int fx = B.t1() * t2, fy = B.b0 * t2, fz = B.b0 * B.b1;
+  for (int i = threadIdx.x; i < fx + fy + fz; i += kTileThreads) {
+    int x, y, z;
+    if (i < fx) {
+      x = B.b0;
+      y = i / t2;
+      z = i - y * t2;
+    } else if (i < fx + fy) {
+      int r = i - fx;
+      y = B.b1;
+      x = r / t2;
+      z = r - x * t2;
+    } else {
+      int r = i - fx - fy;
+      z = B.b2;
+      x = r / B.b1;
+      y = r - x * B.b1;
+    }
+    sh[i] = tile[B.cell(x, y, z)];
+  }
+}
+
+// Fold the upper shells of the 7 lower neighbours into this brick's lower faces.
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kTileThreads) k_scatter_halo(Bricks k, const float* __restrict__ shell,
+                                                                float* __restrict__ mesh) {
+  const BrickShape<BX, BY, BZ> B(k);
+  const int64_t bid = blockIdx.x;
+  int bc[3];
+  brick_coords(k, bid, bc);
+  const int o[3] = {bc[0] * B.b0, bc[1] * B.b1, bc[2] * B.b2};
+  int64_t nbr[8];  // shell block of the lower neighbour bid - (dx,dy,dz)
+#pragma unroll
+  for (int d = 1; d < 8; ++d) {
+    int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
+    int sb0 = bc[0] - dx, sb1 = bc[1] - dy, sb2 = bc[2] - dz;
+    sb0 += sb0 < 0 ? k.nb[0] : 0;
+    sb1 += sb1 < 0 ? k.nb[1] : 0;
+    sb2 += sb2 < 0 ? k.nb[2] : 0;
+    nbr[d] = (((int64_t)sb0 * k.nb[1] + sb1) * k.nb[2] + sb2) * k.sh;
+  }
+  const int t2 = B.t2();
+  const int fxs = B.t1() * t2, fys = B.b0 * t2;
+  // lower-face cells: x==0 (b1*b2), then y==0 & x>0 ((b0-1)*b2), then z==0 & x>0 & y>0
+  const int f0 = B.b1 * B.b2;
+  const int f1 = (B.b0 - 1) * B.b2;
+  const int f2 = (B.b0 - 1) * (B.b1 - 1);
+  for (int i = threadIdx.x; i < f0 + f1 + f2; i += kTileThreads) {
+    int x, y, z;
+    if (i < f0) {
+      x = 0;
+      y = i / B.b2;
+      z = i - y * B.b2;
+    } else if (i < f0 + f1) {
+      int r = i - f0;
+      y = 0;
+      x = 1 + r / B.b2;
+      z = r - (x - 1) * B.b2;
+    } else {
+      int r = i - f0 - f1;
+      z = 0;
+      x = 1 + r / (B.b1 - 1);
+      y = 1 + r - (x - 1) * (B.b1 - 1);
+    }
+    float acc = 0.f;
+#pragma unroll
+    for (int d = 1; d < 8; ++d) {
+      int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
+      if ((dx && x != 0) || (dy && y != 0) || (dz && z != 0)) continue;
+      int sx = x + dx * B.b0, sy = y + dy * B.b1, sz = z + dz * B.b2;
+      int si = (sx == B.b0) ? sy * t2 + sz : (sy == B.b1) ? fxs + sx * t2 + sz : fxs + fys + sx * B.b1 + sy;
+      acc += shell[nbr[d] + si];
+    }
+    int64_t flat = ((int64_t)(o[0] + x) * k.n[1] + (o[1] + y)) * k.n[2] + (o[2] + z);
+    mesh[flat] += acc;
+  }
+}
+
+// Stage one force component's extended tile (b+1)^3 in LDS.
+template <int BX, int BY, int BZ>
+__device__ __forceinline__ void load_tile(float* tile, const float* __restrict__ F, const BrickShape<BX, BY, BZ>& B,
+                                          const Bricks& k, const int o[3]) {
+  if constexpr (BrickShape<BX, BY, BZ>::kVec) {
+    constexpr int C4 = BZ / 4;
+    constexpr int ROWS = (BX + 1) * (BY + 1);
+    constexpr int ITEMS = ROWS * C4;
+    constexpr int ROUND = 4;  // float4 loads in flight per thread per round (64 B/lane)
+#pragma unroll 1
+    for (int base = 0; base < ITEMS; base += ROUND * kTileThreads) {
+      float4 v[ROUND];
+#pragma unroll
+      for (int u = 0; u < ROUND; ++u) {  // all row loads of the round before any LDS store
+        int it = base + threadIdx.x + u * kTileThreads;
+        if (it < ITEMS) {
+          int r = it / C4, c4 = it - r * C4;
+          int x = r / (BY + 1), y = r - x * (BY + 1);
+          int gx = o[0] + x, gy = o[1] + y;
+          gx -= gx >= k.n[0] ? k.n[0] : 0;
+          gy -= gy >= k.n[1] ? k.n[1] : 0;
+          v[u] = *reinterpret_cast<const float4*>(&F[((int64_t)gx * k.n[1] + gy) * k.n[2] + o[2] + 4 * c4]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < ROUND; ++u) {
+        int it = base + threadIdx.x + u * kTileThreads;
+        if (it < ITEMS) {
+          int r = it / C4, c4 = it - r * C4;
+          float* dst = &tile[r * (BZ + 2) + 4 * c4];
+          *reinterpret_cast<float2*>(dst) = make_float2(v[u].x, v[u].y);
+          *reinterpret_cast<float2*>(dst + 2) = make_float2(v[u].z, v[u].w);
+        }
+      }
+    }
+    // z == b2 column (first cell of the next brick along z, wrapped)
+    int gz = o[2] + BZ;
+    gz -= gz >= k.n[2] ? k.n[2] : 0;
+#pragma unroll 1
+    for (int r = threadIdx.x; r < ROWS; r += kTileThreads) {
+      int x = r / (BY + 1), y = r - x * (BY + 1);
+      int gx = o[0] + x, gy = o[1] + y;
+      gx -= gx >= k.n[0] ? k.n[0] : 0;
+      gy -= gy >= k.n[1] ? k.n[1] : 0;
+      tile[r * (BZ + 2) + BZ] = F[((int64_t)gx * k.n[1] + gy) * k.n[2] + gz];
+    }
+  } else {
+    const int t1 = B.t1(), t2 = B.t2();
+    const int tyz = t1 * t2, tvol = B.t0() * tyz;
+    for (int i0 = 0; i0 < tvol; i0 += kTileThreads * kBatch) {
+      float v[kBatch];
+      int dst[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {  // issue kBatch independent loads, then store
+        int i = i0 + u * kTileThreads + threadIdx.x;
+        v[u] = 0.f;
+        dst[u] = -1;
+        if (i < tvol) {
+          int x = i / tyz, rem = i - x * tyz;
+          int y = rem / t2, z = rem - y * t2;
+          int gx = o[0] + x, gy = o[1] + y, gz = o[2] + z;
+          gx -= gx >= k.n[0] ? k.n[0] : 0;
+          gy -= gy >= k.n[1] ? k.n[1] : 0;
+          gz -= gz >= k.n[2] ? k.n[2] : 0;
+          v[u] = F[((int64_t)gx * k.n[1] + gy) * k.n[2] + gz];
+          dst[u] = B.cell(x, y, z);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u)
+        if (dst[u] >= 0) tile[dst[u]] = v[u];
+    }
+  }
+}
+
+constexpr int kGatherPPT = 4;  // particles per thread per chunk
+
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kTileThreads, 4) k_gather3_tiles(
+    const int16_t* __restrict__ pmid, const float* __restrict__ disp, const int32_t* __restrict__ bin_off,
+    const int32_t* __restrict__ perm, Geom g, Bricks k, const float* __restrict__ f0,
+    const float* __restrict__ f1, const float* __restrict__ f2, float* __restrict__ acc) {
+  __shared__ __align__(16) float tile[kTileMaxFloats];
+  const BrickShape<BX, BY, BZ> B(k);
+  const int64_t bid = blockIdx.x;
+  int bc[3];
+  brick_coords(k, bid, bc);
+  const int o[3] = {bc[0] * B.b0, bc[1] * B.b1, bc[2] * B.b2};
+  const int64_t j0 = bin_off[bid], j1 = bin_off[bid + 1];
+  for (int64_t c0 = j0; c0 < j1; c0 += (int64_t)kTileThreads * kGatherPPT) {
+    int cell0[kGatherPPT];  // LDS cell of corner 0; -1 outside the tile, -2 inactive
+    float w[kGatherPPT][8];                 // corner weights, computed once for the 3 components
+    float r[kGatherPPT][3];
+#pragma unroll
+    for (int q = 0; q < kGatherPPT; ++q) {
+      int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
+      cell0[q] = -2;
+      r[q][0] = r[q][1] = r[q][2] = 0.f;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) w[q][c] = 0.f;
+      if (j < j1) {
+        int bs[3];
+        float fr[3];
+        base_frac(pmid, disp, j, g.a1, k.n, bs, fr);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) w[q][c] = frac_weight(fr, c);
+        int l0 = bs[0] - o[0], l1 = bs[1] - o[1], l2 = bs[2] - o[2];
+        bool in = l0 >= 0 && l0 < B.b0 && l1 >= 0 && l1 < B.b1 && l2 >= 0 && l2 < B.b2;
+        cell0[q] = in ? B.cell(l0, l1, l2) : -1;
+      }
+    }
+    // one force component: stage its tile, interpolate this thread's particles
+    auto component = [&](const float* __restrict__ F, int comp) {
+      __syncthreads();
+      load_tile<BX, BY, BZ>(tile, F, B, k, o);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kGatherPPT; ++q) {
+        if (cell0[q] == -2) continue;
+        float s = 0.f;
+        if (cell0[q] >= 0) {
+#pragma unroll
+          for (int c = 0; c < 8; ++c) s = s + tile[cell0[q] + B.corner_off(c)] * w[q][c];
+        } else {
+          int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
+          AxisCic ax[3];
+          particle_cic(g, pmid, disp, j, ax);
+          for (int c = 0; c < 8; ++c) {
+            int64_t flat;
+            corner_index(g, ax, c, flat);
+            s = s + F[flat] * corner_weight(ax, c);
+          }
+        }
+        if (comp == 0) r[q][0] = s;
+        if (comp == 1) r[q][1] = s;
+        if (comp == 2) r[q][2] = s;
+      }
+    };
+    component(f0, 0);
+    component(f1, 1);
+    component(f2, 2);
+#pragma unroll
+    for (int q = 0; q < kGatherPPT; ++q) {
+      int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
+      if (j < j1) {
+        int64_t i = perm[j];
+        acc[i * 3 + 0] = r[q][0];
+        acc[i * 3 + 1] = r[q][1];
+        acc[i * 3 + 2] = r[q][2];
+      }
+    }
+  }
+}
+
+// invalid particles (sorted last, past the last brick) get a zero acceleration
+__global__ void k_zero_invalid(const int32_t* __restrict__ perm, const int32_t* __restrict__ bin_off,
+                               int64_t nbins, int64_t n, float* __restrict__ acc) {
+  for (int64_t j = bin_off[nbins] + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = perm[j];
+    acc[i * 3 + 0] = 0.f;
+    acc[i * 3 + 1] = 0.f;
+    acc[i * 3 + 2] = 0.f;
+  }
+}
+
 // Reverse of the three gathers (gather^T = scatter of acc_cot, plus the displacement
 // cotangent from the mesh values), pallas.py:321-355 per component.
 __global__ void __launch_bounds__(kBlock) k_gather3_bwd_sorted(
@@ -236,6 +658,8 @@ struct Buffers {
   float* disp_s;
   uint8_t* valid_s;
   float* tables;
+  int32_t* bins;   // brick offsets of the sorted particles (tiled path)
+  float* shell;    // per-brick upper halo shells (tiled scatter)
   float* r0;  // density, then F_x
   float* r1;  // F_y
   float* r2;  // F_z
@@ -250,7 +674,16 @@ struct Buffers {
   float* disp_cot_s;
 };
 
-bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, bool reverse, Buffers& b) {
+bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const int32_t* brick, bool reverse,
+           Buffers& b) {
+  b.bins = nullptr;
+  b.shell = nullptr;
+  if (tiled_ok(shape, brick)) {
+    Bricks k = make_bricks(shape, brick);
+    int64_t nbins = (int64_t)k.nb[0] * k.nb[1] * k.nb[2];
+    b.bins = ws.take<int32_t>(nbins + 2);
+    b.shell = ws.take<float>(nbins * k.sh);
+  }
   b.perm = ws.take<int32_t>(n);
   b.pmid_s = ws.take<int16_t>(3 * n);
   b.disp_s = ws.take<float>(3 * n);
@@ -276,7 +709,7 @@ size_t workspace_bytes(const pm_gravity_desc* d, int64_t n, bool reverse) {
   Layout L = make_layout(d->mesh_shape);
   Arena probe(reinterpret_cast<void*>(uintptr_t(256)), SIZE_MAX / 2);
   Buffers b;
-  carve(probe, L, n, d->mesh_shape, reverse, b);
+  carve(probe, L, n, d->mesh_shape, d->brick, reverse, b);
   // the sort's temporaries are carved after the step buffers
   return probe.used + 256 + sort::workspace_bytes(n, d->mesh_shape, d->brick) + 256;
 }
@@ -313,11 +746,29 @@ int fill(float* p, int64_t n, hipStream_t s) {
 int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                   const pm_gravity_desc* d, const Layout& L, Buffers& b, Arena& ws, hipStream_t s) {
   const Geom g = step_geom(d);
-  PM_TRY(sort::sort_cells(pmid, disp, valid, n, d->mesh_shape, d->cell_size, d->brick, b.perm, nullptr, ws, s));
+  PM_TRY(sort::sort_cells(pmid, disp, valid, n, d->mesh_shape, d->cell_size, d->brick, b.perm, b.bins, ws, s));
   if (n > 0) {
     hipLaunchKernelGGL(k_payload, dim3(grid_for(n)), dim3(kBlock), 0, s, b.perm, pmid, disp, valid, n, b.pmid_s,
                        b.disp_s, b.valid_s);
     PM_LAUNCHED("k_payload", s);
+  }
+  if (b.bins) {
+    Bricks k = make_bricks(d->mesh_shape, d->brick);
+    const unsigned nbins = (unsigned)((int64_t)k.nb[0] * k.nb[1] * k.nb[2]);
+    if (is_brick(k, 32, 16, 32)) {
+      hipLaunchKernelGGL((k_scatter_tiles<32, 16, 32>), dim3(nbins), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
+                         b.bins, d->particle_value, g, k, b.r0, b.shell);
+      PM_LAUNCHED("k_scatter_tiles", s);
+      hipLaunchKernelGGL((k_scatter_halo<32, 16, 32>), dim3(nbins), dim3(kTileThreads), 0, s, k, b.shell, b.r0);
+      PM_LAUNCHED("k_scatter_halo", s);
+    } else {
+      hipLaunchKernelGGL((k_scatter_tiles<0, 0, 0>), dim3(nbins), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
+                         b.bins, d->particle_value, g, k, b.r0, b.shell);
+      PM_LAUNCHED("k_scatter_tiles", s);
+      hipLaunchKernelGGL((k_scatter_halo<0, 0, 0>), dim3(nbins), dim3(kTileThreads), 0, s, k, b.shell, b.r0);
+      PM_LAUNCHED("k_scatter_halo", s);
+    }
+    return PM_SUCCESS;
   }
   PM_TRY(fill(b.r0, L.nreal, s));
   if (n > 0) {
@@ -325,6 +776,34 @@ int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, 
                        n, d->particle_value, g, b.r0);
     PM_LAUNCHED("k_scatter_sorted", s);
   }
+  return PM_SUCCESS;
+}
+
+int gather_stage(const int16_t* pmid_s, const float* disp_s, const uint8_t* valid_s, const uint8_t* valid,
+                 int64_t n, const pm_gravity_desc* d, const Buffers& b, const float* fx, const float* fy,
+                 const float* fz, float* acc, hipStream_t s) {
+  if (n == 0) return PM_SUCCESS;
+  const Geom g = step_geom(d);
+  if (b.bins) {
+    Bricks k = make_bricks(d->mesh_shape, d->brick);
+    const int64_t nbins = (int64_t)k.nb[0] * k.nb[1] * k.nb[2];
+    if (is_brick(k, 32, 16, 32))
+      hipLaunchKernelGGL((k_gather3_tiles<32, 16, 32>), dim3((unsigned)nbins), dim3(kTileThreads), 0, s, pmid_s,
+                         disp_s, b.bins, b.perm, g, k, fx, fy, fz, acc);
+    else
+      hipLaunchKernelGGL((k_gather3_tiles<0, 0, 0>), dim3((unsigned)nbins), dim3(kTileThreads), 0, s, pmid_s,
+                         disp_s, b.bins, b.perm, g, k, fx, fy, fz, acc);
+    PM_LAUNCHED("k_gather3_tiles", s);
+    if (valid) {
+      unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), 4096);
+      hipLaunchKernelGGL(k_zero_invalid, dim3(blocks), dim3(kBlock), 0, s, b.perm, b.bins, nbins, n, acc);
+      PM_LAUNCHED("k_zero_invalid", s);
+    }
+    return PM_SUCCESS;
+  }
+  hipLaunchKernelGGL(k_gather3_sorted, dim3(grid_for(n)), dim3(kBlock), 0, s, pmid_s, disp_s, valid_s, b.perm, n, g,
+                     fx, fy, fz, acc);
+  PM_LAUNCHED("k_gather3_sorted", s);
   return PM_SUCCESS;
 }
 
@@ -387,7 +866,8 @@ int prepare(const pm_gravity_desc* d, int64_t n, void* workspace, size_t workspa
   PM_RETURN_IF(workspace == nullptr || workspace_bytes_ < need, PM_ERROR_WORKSPACE,
                "gravity workspace too small: need " + std::to_string(need) + " bytes");
   L = make_layout(d->mesh_shape);
-  PM_RETURN_IF(!carve(ws, L, n, d->mesh_shape, reverse, b), PM_ERROR_WORKSPACE, "workspace carve failed");
+  PM_RETURN_IF(!carve(ws, L, n, d->mesh_shape, d->brick, reverse, b), PM_ERROR_WORKSPACE,
+               "workspace carve failed");
   PM_TRY(fft::launch_spec_tables(b.tables, d->mesh_shape, d->cell_size, d->ptcl_spacing, d->mask_active, s));
   tb = fft::spec_tables_view(b.tables, d->mesh_shape);
   return PM_SUCCESS;
@@ -427,12 +907,7 @@ int pm_gravity(const int16_t* pmid, const float* disp, const uint8_t* valid, int
   if (st) return st;
   st = grav::force_mesh_stage(desc, L, b, tb, s, false);
   if (st) return st;
-  if (n > 0) {
-    hipLaunchKernelGGL(grav::k_gather3_sorted, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s, b.pmid_s,
-                       b.disp_s, b.valid_s, b.perm, n, grav::step_geom(desc), b.r0, b.r1, b.r2, acc);
-    PM_LAUNCHED("k_gather3_sorted", s);
-  }
-  return PM_SUCCESS;
+  return grav::gather_stage(b.pmid_s, b.disp_s, b.valid_s, valid, n, desc, b, b.r0, b.r1, b.r2, acc, s);
 }
 
 int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,

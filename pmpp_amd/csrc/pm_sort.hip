@@ -1,4 +1,4 @@
-// Stable, deterministic cell sort (least-significant-digit radix, 8-bit digits).
+// Stable, deterministic cell sort (least-significant-digit radix, digits of up to 10 bits).
 //
 // Each pass is a stable counting sort of one digit:
 //   1. k_digit_hist   one wave per contiguous segment of kSeg items counts its digits;
@@ -9,6 +9,8 @@
 //      key, so the result equals numpy.argsort(keys, kind="stable") bit for bit.
 // The key of the first pass is computed on the fly from (pmid, disp): the raveled brick
 // index of the containing cell, with invalid particles mapped to the sentinel.
+#include <algorithm>
+
 #include "pm_common.h"
 #include "pm_sort.h"
 
@@ -20,7 +22,8 @@ constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
 constexpr int kIters = 128;               // 64-item groups per segment
 constexpr int kSeg = kWave * kIters;      // items per wave segment
-constexpr int kRadix = 256;
+constexpr int kMaxDigitBits = 10;
+constexpr int kMaxRadix = 1 << kMaxDigitBits;
 
 struct KeyGeom {
   int n[3];
@@ -49,11 +52,12 @@ template <bool FIRST>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     k_digit_hist(const int32_t* __restrict__ keys, const int16_t* __restrict__ pmid,
                  const float* __restrict__ disp, const uint8_t* __restrict__ valid, KeyGeom g,
-                 int64_t n, int shift, int64_t nseg, int32_t* __restrict__ hist) {
-  __shared__ int32_t h[kWavesPerBlock][kRadix];
+                 int64_t n, int shift, int dbits, int64_t nseg, int32_t* __restrict__ hist) {
+  __shared__ int32_t h[kWavesPerBlock][kMaxRadix];
   const int lane = threadIdx.x % kWave;
   const int w = threadIdx.x / kWave;
-  for (int d = lane; d < kRadix; d += kWave) h[w][d] = 0;
+  const int radix = 1 << dbits;
+  for (int d = lane; d < radix; d += kWave) h[w][d] = 0;
   __syncthreads();
   const int64_t seg = (int64_t)blockIdx.x * kWavesPerBlock + w;
   if (seg < nseg) {
@@ -62,28 +66,29 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
       int64_t i = start + it * kWave + lane;
       if (i < n) {
         int key = FIRST ? cell_key(g, pmid, disp, valid, i) : keys[i];
-        atomicAdd(&h[w][(key >> shift) & (kRadix - 1)], 1);
+        atomicAdd(&h[w][(key >> shift) & (radix - 1)], 1);
       }
     }
   }
   __syncthreads();
   if (seg < nseg)
-    for (int d = lane; d < kRadix; d += kWave) hist[(int64_t)d * nseg + seg] = h[w][d];
+    for (int d = lane; d < radix; d += kWave) hist[(int64_t)d * nseg + seg] = h[w][d];
 }
 
 template <bool FIRST>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     k_digit_scatter(const int32_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in,
                     const int16_t* __restrict__ pmid, const float* __restrict__ disp,
-                    const uint8_t* __restrict__ valid, KeyGeom g, int64_t n, int shift, int64_t nseg,
-                    const int32_t* __restrict__ offsets, int32_t* __restrict__ keys_out,
+                    const uint8_t* __restrict__ valid, KeyGeom g, int64_t n, int shift, int dbits,
+                    int64_t nseg, const int32_t* __restrict__ offsets, int32_t* __restrict__ keys_out,
                     int32_t* __restrict__ vals_out) {
-  __shared__ int32_t cnt[kWavesPerBlock][kRadix];
+  __shared__ int32_t cnt[kWavesPerBlock][kMaxRadix];
   const int lane = threadIdx.x % kWave;
   const int w = threadIdx.x / kWave;
+  const int radix = 1 << dbits;
   const int64_t seg = (int64_t)blockIdx.x * kWavesPerBlock + w;
   if (seg >= nseg) return;  // whole wave exits together; no block barriers below
-  for (int d = lane; d < kRadix; d += kWave) cnt[w][d] = offsets[(int64_t)d * nseg + seg];
+  for (int d = lane; d < radix; d += kWave) cnt[w][d] = offsets[(int64_t)d * nseg + seg];
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -101,10 +106,9 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
         val = vals_in[i];
       }
     }
-    int d = (key >> shift) & (kRadix - 1);
+    int d = (key >> shift) & (radix - 1);
     uint64_t peers = __ballot(active);
-#pragma unroll
-    for (int bit = 0; bit < 8; ++bit) {
+    for (int bit = 0; bit < dbits; ++bit) {
       bool set = (d >> bit) & 1;
       uint64_t m = __ballot(set);
       peers &= set ? m : ~m;
@@ -247,7 +251,7 @@ int check_sort_args(int64_t n, const int32_t* mesh_shape, const int32_t* brick) 
 
 size_t workspace_bytes(int64_t n, const int32_t* mesh_shape, const int32_t* brick) {
   int64_t nseg = ceil_div(n, kSeg);
-  int64_t hist = (int64_t)kRadix * nseg;
+  int64_t hist = (int64_t)kMaxRadix * nseg;
   size_t b = 0;
   b += align_up(sizeof(int32_t) * n, 256) * 3;  // keys a/b, vals
   b += align_up(sizeof(int32_t) * hist, 256) * 2;
@@ -255,6 +259,11 @@ size_t workspace_bytes(int64_t n, const int32_t* mesh_shape, const int32_t* bric
   (void)mesh_shape;
   (void)brick;
   return b + 256;
+}
+
+int digit_bits_for(int bits, int& passes) {
+  passes = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  return (bits + passes - 1) / passes;
 }
 
 int sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
@@ -270,7 +279,11 @@ int sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int
     return PM_SUCCESS;
   }
   const int64_t nseg = ceil_div(n, kSeg);
-  const int64_t hist_len = (int64_t)kRadix * nseg;
+  // without a validity mask no particle takes the sentinel, so the key range ends at nbins-1
+  const int bits = key_bits(valid ? g.sentinel : std::max(g.sentinel - 1, 1));
+  int passes = 1;
+  const int dbits = digit_bits_for(bits, passes);
+  const int64_t hist_len = ((int64_t)1 << dbits) * nseg;
   int32_t* keys_a = ws.take<int32_t>(n);
   int32_t* keys_b = ws.take<int32_t>(n);
   int32_t* vals_t = ws.take<int32_t>(n);
@@ -279,14 +292,12 @@ int sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int
   int32_t* partial = ws.take<int32_t>(ceil_div(hist_len, kScanChunk) + 1);
   PM_RETURN_IF(!ws.ok(), PM_ERROR_WORKSPACE, "sort workspace too small");
 
-  const int bits = key_bits(g.sentinel);
-  const int passes = (bits + 7) / 8;
   const dim3 grid((unsigned)ceil_div(nseg, kWavesPerBlock)), block(kWave * kWavesPerBlock);
   // value ping-pong arranged so the last pass writes `perm`
   int32_t* vbuf[2] = {(passes % 2 == 1) ? perm : vals_t, (passes % 2 == 1) ? vals_t : perm};
   int32_t* kbuf[2] = {keys_a, keys_b};
   for (int pass = 0; pass < passes; ++pass) {
-    const int shift = 8 * pass;
+    const int shift = dbits * pass;
     const bool last = pass == passes - 1;
     int32_t* kin = kbuf[(pass + 1) % 2];
     int32_t* kout = (last && !bin_offsets) ? nullptr : kbuf[pass % 2];
@@ -294,19 +305,19 @@ int sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int
     int32_t* vout = vbuf[pass % 2];
     if (pass == 0)
       hipLaunchKernelGGL(k_digit_hist<true>, grid, block, 0, s, nullptr, pmid, disp, valid, g, n, shift,
-                         nseg, hist);
+                         dbits, nseg, hist);
     else
       hipLaunchKernelGGL(k_digit_hist<false>, grid, block, 0, s, kin, pmid, disp, valid, g, n, shift,
-                         nseg, hist);
+                         dbits, nseg, hist);
     PM_LAUNCHED("k_digit_hist", s);
     int st = scan_exclusive(hist, offs, hist_len, partial, s);
     if (st) return st;
     if (pass == 0)
       hipLaunchKernelGGL(k_digit_scatter<true>, grid, block, 0, s, nullptr, nullptr, pmid, disp, valid, g,
-                         n, shift, nseg, offs, kout, vout);
+                         n, shift, dbits, nseg, offs, kout, vout);
     else
       hipLaunchKernelGGL(k_digit_scatter<false>, grid, block, 0, s, kin, vin, pmid, disp, valid, g, n,
-                         shift, nseg, offs, kout, vout);
+                         shift, dbits, nseg, offs, kout, vout);
     PM_LAUNCHED("k_digit_scatter", s);
     if (last && bin_offsets) {
       hipLaunchKernelGGL(k_bin_bounds, dim3((unsigned)ceil_div(n + 1, 256)), dim3(256), 0, s, kout, n,

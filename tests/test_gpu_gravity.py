@@ -51,6 +51,31 @@ def test_force_matches_float64_oracle(device, ng, ratio, box, scale, valid_frac)
         assert np.all(acc[ptcl.unused_index.cpu().numpy()] == 0)
 
 
+@pytest.mark.parametrize("brick", [(4, 2, 8), (1, 1, 1), (16, 16, 16), (32, 1, 2), (3, 3, 3), (32, 32, 32)])
+def test_force_is_independent_of_the_brick_tiling(device, brick):
+    # (3,3,3) does not divide the mesh and (32,32,32) overflows the LDS tile: both take the
+    # untiled sorted path; the others exercise periodic brick neighbours including nb = 1
+    conf = Configuration(2.0, (16, 16, 16), mesh_shape=2, brick=brick)
+    ptcl = _ptcl(conf, 1.0, 3, device, valid_frac=0.9)
+    acc = gravity(1.0, ptcl, 0.3, conf).cpu().numpy()
+    ref, _ = _oracle_acc(conf, ptcl, 0.3)
+    assert np.linalg.norm(acc - ref) / np.linalg.norm(ref) < 1e-5
+    assert np.all(acc[ptcl.unused_index.cpu().numpy()] == 0)
+
+
+def test_clustered_particles_exceed_one_gather_chunk(device):
+    # every particle in one brick: several gather chunks and heavy LDS-atomic contention
+    conf = Configuration(1.0, (32, 32, 32), mesh_shape=1, brick=(8, 8, 8))
+    rng = np.random.default_rng(9)
+    n = 32 ** 3
+    pmid = torch.as_tensor(rng.integers(0, 8, (n, 3)).astype(np.int16), device=device)
+    disp = torch.as_tensor(rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32), device=device)
+    ptcl = Particles(conf, pmid, disp)
+    acc = gravity(1.0, ptcl, 0.3, conf).cpu().numpy()
+    ref, _ = _oracle_acc(conf, ptcl, 0.3)
+    assert np.linalg.norm(acc - ref) / np.linalg.norm(ref) < 1e-5
+
+
 def test_uniform_lattice_is_force_free(device):
     # tests/test_deep_gravity_spectral.py:253-264
     conf = Configuration(1.25, (4, 4, 4), mesh_shape=1)
