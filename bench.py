@@ -15,6 +15,7 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
+import math
 import os
 import platform
 import sys
@@ -36,21 +37,48 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, L, n_p):
-    """Algorithmic (compulsory) HBM bytes of one launch of each kernel of the step."""
+def sort_passes(mesh, brick):
+    """Radix passes of the brick-key sort (digits of at most 10 bits; no validity mask)."""
+    nb = 1
+    for m, b in zip(mesh, brick):
+        nb *= m // b
+    bits = max(1, (nb - 1).bit_length())
+    return -(-bits // 10), nb
+
+
+def kernel_step_bytes(name, L, n_p, brick):
+    """Algorithmic (compulsory) HBM bytes per step of every launch of one kernel name.
+
+    R = real float32 mesh; S1 / S2 = the complex64 spectrum band the force passes keep
+    (kz < nkz; all y rows / the packed |ky| band), pitch excluded; n_p particles of 18 B
+    (int16 pmid + float32 disp).  Brick shells are the upper faces of every brick that
+    the tiled scatter hands to the halo fold; tile reads of the gather count the mesh once.
+    """
     R = 4 * L["nreal"]
     Ch = 8 * L["nx"] * L["ny"] * (L["nz"] // 2 + 1)
+    S1 = 8 * L["nx"] * L["ny"] * L["nkz"]  # banded spectrum, full y rows
+    S2 = 8 * L["nx"] * L["nky"] * L["nkz"]  # banded spectrum, packed y band
+    passes, nb = sort_passes((L["nx"], L["ny"], L["nz"]), brick)
+    b0, b1, b2 = brick
+    shell = 4 * nb * ((b1 + 1) * (b2 + 1) + b0 * (b2 + 1) + b0 * b1)
+    faces = 8 * nb * (b1 * b2 + (b0 - 1) * b2 + (b0 - 1) * (b1 - 1))  # lower faces read + written
     table = {
-        "fft_z_r2c": R + Ch,
-        "fft_cols_fwd": 2 * Ch,
-        "fft_cols_green": 3 * Ch,
-        "fft_cols_inv2": 3 * Ch,
-        "fft_cols_inv": 2 * Ch,
-        "fft_z_c2r": Ch + R,
+        "fft_z_r2c": R + S1,
+        "fft_cols_fwd": S1 + S2,
+        "fft_cols_green": 3 * S2,
+        "fft_cols_inv2": S2 + 2 * S1,
+        "fft_cols_inv": S2 + S1,
+        "fft_z_c2r": 3 * (S1 + R),
+        "k_digit_hist": (18 + 4 * (passes - 1)) * n_p,
+        "k_digit_scatter": (26 + 16 * (passes - 1)) * n_p,
+        "k_bin_bounds": 4 * n_p + 4 * (nb + 2),
+        "k_payload": (4 + 18 + 18 + 1) * n_p,
+        "k_scatter_tiles": 18 * n_p + R + shell,
+        "k_scatter_halo": shell + faces,
+        "k_gather3_tiles": (18 + 4 + 12) * n_p + 3 * R,
         "k_fill": R,
         "k_scatter_sorted": 19 * n_p + R,
         "k_gather3_sorted": (19 + 4 + 12) * n_p + 3 * R,
-        "k_payload": (4 + 19 + 19) * n_p,
     }
     return table.get(name)
 
@@ -60,6 +88,49 @@ def official_step_bytes(n_p, L):
     R = 4 * L["nreal"]
     C = 8 * L["nx"] * L["ny"] * (L["nz"] // 2 + 1)
     return 66 * n_p + (18 * n_p + R) + (R + 5 * C) + 3 * (R + 5 * C) + (34 * n_p + 3 * R)
+
+
+def spectral_band(conf):
+    """Stored kz columns and y rows of the force passes: with the particle-Nyquist filter only
+    |k_i| <= pi/l_p (1 + 8 eps_f32) survive the Green multiply (same rule as the library)."""
+    nz, ny = conf.mesh_shape[2], conf.mesh_shape[1]
+    nkz, nky = nz // 2 + 1, ny
+    if conf.mask_active:
+        def half_width(n):
+            kmax = math.pi / conf.ptcl_spacing * (1 + 8 * 1.1920928955078125e-07) * (1 + 1e-4)
+            return math.floor(kmax / (2 * math.pi / (n * conf.cell_size)))
+        kz, ky = half_width(nz), half_width(ny)
+        if kz < nz // 2:
+            nkz = kz + 1
+        if 2 * ky + 1 < ny:
+            nky = 2 * ky + 1
+    return {"nkz": nkz, "nky": nky}
+
+
+def strict_step_bytes(n_p, L):
+    """Lower bound with every 3-D FFT as a single read + write of its input and output."""
+    R = 4 * L["nreal"]
+    C = 8 * L["nx"] * L["ny"] * (L["nz"] // 2 + 1)
+    return 66 * n_p + (18 * n_p + R) + (R + C) + 3 * (C + R) + (34 * n_p + 3 * R)
+
+
+def measured_traffic(kernel, config):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary, if any.
+
+    ``profiles/traffic_<config>.txt`` holds lines ``<kernel> <FETCH KiB> <WRITE KiB> <factor>``
+    (per dispatch, averaged; factor = copy-kernel calibration of the counters).
+    """
+    path = os.path.join(ROOT, "profiles", f"traffic_{config}.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == kernel:
+                    fetch, write, factor = (float(v) for v in parts[1:])
+                    return round((fetch + write) * 1024 * factor)
+    except (OSError, ValueError):
+        return None
+    return None
 
 
 def cpu_baseline(dev, omega_m):
@@ -112,6 +183,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy 3 times (the known-size kernel that "
+                         "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU x-slab path is a "
@@ -120,6 +194,15 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copied 3 x 1 GiB (read 1 GiB + write 1 GiB per copy)")
+        return
 
     from pmpp_amd import Configuration, _lib
     from pmpp_amd._torch import ensure_init, ptr, stream_handle, workspace
@@ -170,29 +253,35 @@ def main():
 
     L = {"nx": conf.mesh_shape[0], "ny": conf.mesh_shape[1], "nz": conf.mesh_shape[2],
          "nreal": conf.mesh_size}
+    L.update(spectral_band(conf))
     per = {}
     for i in range(cnt.value):
         k = names[i].decode()
         t, c = per.get(k, (0.0, 0))
         per[k] = (t + ms[i], c + 1)
     kernels = {}
+    brick = tuple(conf.brick)
     for k, (t, c) in sorted(per.items(), key=lambda kv: -kv[1][0]):
-        b = kernel_bytes(k, L, n)
-        avg = t / c
-        kernels[k] = {"ms_per_step": t / args.steps, "launches_per_step": c / args.steps, "avg_ms": avg,
-                      "GBps": (b / (avg * 1e-3) / 1e9) if b else None}
-        log(f"[bench]   {k:22s} {t / args.steps:8.3f} ms/step  x{c / args.steps:.0f}  avg {avg:.3f} ms"
-            + (f"  {b / (avg * 1e-3) / 1e9:8.1f} GB/s" if b else ""))
-    dominant = next(k for k in kernels if kernel_bytes(k, L, n))
+        sb = kernel_step_bytes(k, L, n, brick)
+        step_ms = t / args.steps
+        kernels[k] = {"ms_per_step": step_ms, "launches_per_step": c / args.steps, "avg_ms": t / c,
+                      "bytes_per_step": sb, "GBps": (sb / (step_ms * 1e-3) / 1e9) if sb else None}
+        log(f"[bench]   {k:22s} {step_ms:8.3f} ms/step  x{c / args.steps:.0f}  avg {t / c:.3f} ms"
+            + (f"  {sb / (step_ms * 1e-3) / 1e9:8.1f} GB/s" if sb else ""))
+    dominant = next(iter(kernels))
     dk = kernels[dominant]
-    achieved = dk["GBps"]
+    per_launch = dk["bytes_per_step"] / dk["launches_per_step"]
+    achieved = per_launch / (dk["avg_ms"] * 1e-3) / 1e9
     off_bytes = official_step_bytes(n, L)
+    strict_bytes = strict_step_bytes(n, L)
     roofline = {
         "bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "algorithmic_bytes_per_launch": kernel_bytes(dominant, L, n), "avg_launch_ms": round(dk["avg_ms"], 4),
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dominant, args.config),
+        "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": round(dk["avg_ms"], 4),
         "step_official_bytes": off_bytes,
         "step_frac": round(off_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+        "strict": {"step_bytes": strict_bytes,
+                   "frac": round(strict_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)},
     }
     cpu = None
     if not args.no_cpu_baseline:

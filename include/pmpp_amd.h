@@ -157,8 +157,11 @@ int pm_spectral_force_multiply(const float* in_complex, float* out_complex,
  * acc (n,3) float32 in caller order; invalid particles get 0.
  * pm_gravity_vjp pulls acc_cot back to disp_cot (n,3, overwritten) and accumulates
  * d/dOmega_m into omega_cot[0] (device float, caller zeroes) -- the force VJP walked by
- * force_adj (nbody/integrator.py:620-686). */
+ * force_adj (nbody/integrator.py:620-686).  Each call takes a workspace of at least the size
+ * its own query returns (pm_gravity_workspace / pm_gravity_vjp_workspace); the reverse pass
+ * needs more (it keeps the forward meshes and three cotangent meshes). */
 int pm_gravity_workspace(const pm_gravity_desc* desc, int64_t n, size_t* bytes);
+int pm_gravity_vjp_workspace(const pm_gravity_desc* desc, int64_t n, size_t* bytes);
 int pm_gravity(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                const pm_gravity_desc* desc, float* acc, void* workspace, size_t workspace_bytes,
                void* stream);

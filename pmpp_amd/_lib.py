@@ -2,7 +2,8 @@
 
 This mirrors the reference's native-library loader (``src/pmpp/distributed/cuda.py:136-230``:
 locate the shared object, load it with ``ctypes.CDLL``, check an ABI version, expose
-typed entry points) without the XLA-FFI registration, which lives in ``pmpp_amd.jax_ffi``.
+typed entry points).  The XLA-FFI registration the reference adds on top of this
+(``jax.ffi.register_ffi_target``) needs jaxlib, which this package does not depend on.
 
 There is no fallback: if the library is missing, every operator raises.
 """
@@ -29,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "pm_sort_cells_workspace", "pm_sort_cells",
     "pm_fft_workspace", "pm_rfft3d", "pm_irfft3d",
     "pm_spectral_workspace", "pm_spectral_force_multiply",
-    "pm_gravity_workspace", "pm_gravity", "pm_gravity_vjp",
+    "pm_gravity_workspace", "pm_gravity_vjp_workspace", "pm_gravity", "pm_gravity_vjp",
 )
 
 
@@ -90,6 +91,7 @@ _SIGNATURES = {
     "pm_spectral_force_multiply": ([_vp, _vp, ctypes.POINTER(GravityDesc), _i32, _i32, _vp, ctypes.c_size_t, _vp],
                                    ctypes.c_int),
     "pm_gravity_workspace": ([ctypes.POINTER(GravityDesc), _i64, _sz_p], ctypes.c_int),
+    "pm_gravity_vjp_workspace": ([ctypes.POINTER(GravityDesc), _i64, _sz_p], ctypes.c_int),
     "pm_gravity": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, ctypes.c_size_t, _vp],
                    ctypes.c_int),
     "pm_gravity_vjp": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],

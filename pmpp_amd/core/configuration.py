@@ -33,7 +33,7 @@ class Configuration:
     a_custom: Optional[Tuple[float, ...]] = None
     symp_splits: Tuple[Tuple[float, float], ...] = ((0, 0.5), (1, 0.5))
     lpt_order: int = 2
-    # cell-sort / CIC-tile granularity (cells per brick along each axis); None: (32, 16, 32)
+    # cell-sort / CIC-tile granularity (cells per brick along each axis); None: (16, 16, 32)
     brick: Optional[Tuple[int, int, int]] = None
     # the PM step is float32 with int16 anchors, as PM++'s defaults (configuration.py:183-185)
     float_dtype: str = "float32"
@@ -141,6 +141,7 @@ class Configuration:
 
 
 def default_brick(mesh_shape) -> Tuple[int, int, int]:
-    """(32, 16, 32)-cell bricks (clipped to the mesh): a 33x17x33 float LDS tile (74 KB, two
-    workgroups per CU), 128-byte rows along z, and 2^16 bricks on a 1024^3 mesh."""
-    return tuple(min(int(n), b) for n, b in zip(mesh_shape, (32, 16, 32)))
+    """(16, 16, 32)-cell bricks (clipped to the mesh): a 17x17x34 int64 LDS tile for the
+    fixed-point scatter (78.6 KB, two workgroups per CU), 128-byte rows along z, and 2^17
+    bricks on a 1024^3 mesh (two 9-bit radix passes)."""
+    return tuple(min(int(n), b) for n, b in zip(mesh_shape, (16, 16, 32)))

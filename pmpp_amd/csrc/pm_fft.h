@@ -275,8 +275,16 @@ struct ColPass {
   int64_t s_row, s_col, s_batch;
   int ncols;
   // index of the other two spectral axes for the Green multiply: for column c of batch b,
-  // ky = b*ky_b + c*ky_c, kz = b*kz_b + c*kz_c + kz0
+  // ky = b*ky_b + c*ky_c (then + ky_shift when above ky_split), kz = b*kz_b + c*kz_c + kz0
   int ky_b, ky_c, kz_b, kz_c, kz0;
+  // output strides (-1: same as the input's)
+  int64_t o_row = -1, o_batch = -1;
+  // band compaction of the transformed axis: only spectral rows |k| <= keep (k <= keep or
+  // k >= N - keep) are stored, packed as 0..keep, then N-keep..N-1 -> keep+1..2*keep.
+  // keep < 0: full rows.  compact_in selects the packed side: 1 = input (inverse passes
+  // expand the band to full rows, the rows outside it are zero), 0 = output (forward passes).
+  int keep = -1, compact_in = 0;
+  int ky_split = 0x7fffffff, ky_shift = 0;
 };
 
 enum ColMode { kColFwd = 0, kColInv = 1, kColInv2 = 2, kColGreen = 3 };
@@ -293,6 +301,11 @@ struct ZPass {
   float src_scale;
   float out_scale;  // C2R: output scale
   int deriv;        // C2R pre-op: multiply by i*negk_z[k] when nonzero
+  // band of stored spectrum columns: R2C stores kz < kz_lim (and the Nyquist column only when
+  // nyq != 0); C2R reads them and takes every other column, the Nyquist one included when
+  // nyq == 0, as zero
+  int kz_lim = 0x7fffffff;
+  int nyq = 1;
 };
 
 }  // namespace fft

@@ -88,35 +88,56 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
   const int p = lt / T;
   const int cg = tilex * T + col;
   const bool cok = cg < io.ncols;
-  const int64_t base = b * io.s_batch + (int64_t)cg * io.s_col;
+  // loads use a clamped column so they are unconditional (branch-free batches stay in flight)
+  const int cgl = cok ? cg : io.ncols - 1;
+  const int64_t lbase = b * io.s_batch + (int64_t)cgl * io.s_col;
+  const int64_t o_row = io.o_row >= 0 ? io.o_row : io.s_row;
+  const int64_t obase = b * (io.o_batch >= 0 ? io.o_batch : io.s_batch) + (int64_t)cg * io.s_col;
+  const int keep = io.keep;
+  // band compaction of the transformed axis (see ColPass::keep)
+  auto in_band = [&](int k) { return keep < 0 || k <= keep || k >= N - keep; };
+  auto packed = [&](int k) { return (keep < 0 || k <= keep) ? k : k - (N - 2 * keep - 1); };
   float* my = lds + half * CF::LDS_FLOATS;
   auto idx = [&](int row) { return row * T + col + (row / N2) * CF::PADK; };
 
   float2 v[E];
   if constexpr (MODE == kColFwd || MODE == kColGreen) {
 #pragma unroll
-    for (int s = 0; s < E; ++s)
-      v[s] = cok ? io.in[base + (int64_t)nrow<N>(p, s) * io.s_row] : make_float2(0.f, 0.f);
+    for (int s = 0; s < E; ++s) v[s] = io.in[lbase + (int64_t)nrow<N>(p, s) * io.s_row];
     forward_core<N>(v, my, p, idx, tw);
   } else {
+    if (keep >= 0 && io.compact_in) {  // packed band in, zero outside it
 #pragma unroll
-    for (int s = 0; s < E; ++s)
-      v[s] = cok ? io.in[base + (int64_t)xrow<N>(p, s) * io.s_row] : make_float2(0.f, 0.f);
+      for (int s = 0; s < E; ++s) {
+        int k = xrow<N>(p, s);
+        v[s] = io.in[lbase + (int64_t)(in_band(k) ? packed(k) : 0) * io.s_row];
+      }
+#pragma unroll
+      for (int s = 0; s < E; ++s)
+        if (!in_band(xrow<N>(p, s))) v[s] = make_float2(0.f, 0.f);
+    } else {
+#pragma unroll
+      for (int s = 0; s < E; ++s) v[s] = io.in[lbase + (int64_t)xrow<N>(p, s) * io.s_row];
+    }
   }
 
   if constexpr (MODE == kColFwd) {
     if (cok) {
 #pragma unroll
-      for (int s = 0; s < E; ++s) io.out0[base + (int64_t)xrow<N>(p, s) * io.s_row] = v[s];
+      for (int s = 0; s < E; ++s) {
+        int k = xrow<N>(p, s);
+        if (in_band(k)) io.out0[obase + (int64_t)packed(k) * o_row] = v[s];  // forward: packed output
+      }
     }
     return;
   } else {
     if constexpr (MODE == kColGreen) {
       // mask x Poisson symbol (gravity.py:174-196, 263-287); the second half also applies
       // D_x = i*negk_x (configuration.py:365-376)
-      int ky = (int)(b * io.ky_b) + cg * io.ky_c;
-      int kz = (int)(b * io.kz_b) + cg * io.kz_c + io.kz0;
-      float kyv = cok ? tb.k[1][ky] : 0.f, kzv = cok ? tb.k[2][kz] : 0.f;
+      int ky = (int)(b * io.ky_b) + cgl * io.ky_c;
+      ky += ky > io.ky_split ? io.ky_shift : 0;
+      int kz = (int)(b * io.kz_b) + cgl * io.kz_c + io.kz0;
+      float kyv = tb.k[1][ky], kzv = tb.k[2][kz];
       bool keep_yz = cok && tb.mask[1][ky] != 0.f && tb.mask[2][kz] != 0.f;
 #pragma unroll
       for (int s = 0; s < E; ++s) {
@@ -137,7 +158,7 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
     float2* out = (half == 1) ? io.out1 : io.out0;
     if (cok) {
 #pragma unroll
-      for (int s = 0; s < E; ++s) out[base + (int64_t)nrow<N>(p, s) * io.s_row] = v[s];
+      for (int s = 0; s < E; ++s) out[obase + (int64_t)nrow<N>(p, s) * o_row] = v[s];
     }
   }
 }
@@ -148,7 +169,10 @@ struct ZCfg {
   static constexpr int P = Plan<M>::P;
   static constexpr int LINES = 256 / P;
   static constexpr int THREADS = 256;
-  static constexpr int LSTR = 2 * (M + 2);  // floats per line slot (holds M+1 float2)
+  // floats per line slot: holds M+1 float2, and LSTR = P (mod 64) so the 64/P lines of a wave
+  // start on disjoint groups of the 64 LDS banks (the exchange rows of one line are P apart)
+  static constexpr int LSTR_MIN = 2 * (M + 2);
+  static constexpr int LSTR = ((LSTR_MIN + 63) / 64) * 64 + (P % 64);
   static constexpr int LDS_FLOATS = LINES * LSTR;
 };
 
@@ -167,15 +191,14 @@ __global__ void __launch_bounds__(256) k_zpass(ZPass zp, SpecTables tb, const fl
   auto idx = [&](int row) { return line * ZC::LSTR + row + row / N2; };
   float2 v[E];
 
+  const int64_t lgl = ok ? lg : zp.nlines - 1;  // clamped line: unconditional, batched loads
   if constexpr (!C2R) {
-    const float* src = zp.rin + lg * zp.pitch_r;
+    const float* src = zp.rin + lgl * zp.pitch_r;
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-      int m = nrow<M>(p, s);
-      float2 r = ok ? *reinterpret_cast<const float2*>(src + 2 * m) : make_float2(0.f, 0.f);
-      if (ok) r = make_float2((r.x + zp.src_shift) * zp.src_scale, (r.y + zp.src_shift) * zp.src_scale);
-      v[s] = r;
-    }
+    for (int s = 0; s < E; ++s) v[s] = *reinterpret_cast<const float2*>(src + 2 * nrow<M>(p, s));
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+      v[s] = make_float2((v[s].x + zp.src_shift) * zp.src_scale, (v[s].y + zp.src_shift) * zp.src_scale);
     forward_core<M>(v, lds, p, idx, tw);
 #pragma unroll
     for (int s = 0; s < E; ++s) nat[xrow<M>(p, s)] = v[s];
@@ -190,26 +213,31 @@ __global__ void __launch_bounds__(256) k_zpass(ZPass zp, SpecTables tb, const fl
       float2 fe = c_scale(c_add(a, bc), 0.5f);
       float2 fo = make_float2(0.5f * d.y, -0.5f * d.x);  // -i (a - bc) / 2
       float2 x = c_add(fe, c_mul(table_tw<2 * M, false>(tw, k), fo));
-      if (ok) dst[k] = x;
+      if (ok && k < zp.kz_lim) dst[k] = x;
     }
-    if (ok && p == 0) {
+    if (ok && p == 0 && zp.nyq) {
       float2 z0 = nat[0];
       zp.cnyq[lg * zp.nyq_stride] = make_float2(z0.x - z0.y, 0.f);
     }
   } else {
-    const float2* srcm = zp.cmain + lg * zp.pitch_c;
+    const float2* srcm = zp.cmain + lgl * zp.pitch_c;
+    float2 xn = zp.cnyq[lgl * zp.nyq_stride];  // cnyq is a valid address even when nyq == 0
+    const int klim = zp.kz_lim < M ? zp.kz_lim : M;
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
-      int k = p + P * j;
-      float2 x = ok ? srcm[k] : make_float2(0.f, 0.f);
-      if (zp.deriv) x = c_imul(tb.negk[2][k], x);
-      nat[k] = x;
+    for (int j = 0; j < E; ++j)  // P*j < klim is uniform: columns past the band cost no load
+      v[j] = (P * j < klim) ? srcm[min(p + P * j, klim - 1)] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+      if (p + P * j >= klim) v[j] = make_float2(0.f, 0.f);
+    if (!zp.nyq) xn = make_float2(0.f, 0.f);
+    if (zp.deriv) {
+#pragma unroll
+      for (int j = 0; j < E; ++j) v[j] = c_imul(tb.negk[2][p + P * j], v[j]);
+      xn = c_imul(tb.negk[2][M], xn);
     }
-    if (p == 0) {
-      float2 x = ok ? zp.cnyq[lg * zp.nyq_stride] : make_float2(0.f, 0.f);
-      if (zp.deriv) x = c_imul(tb.negk[2][M], x);
-      nat[M] = x;
-    }
+#pragma unroll
+    for (int j = 0; j < E; ++j) nat[p + P * j] = v[j];
+    if (p == 0) nat[M] = xn;
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < E; ++s) {

@@ -107,7 +107,13 @@ __global__ void __launch_bounds__(kBlock) k_gather3_sorted(const int16_t* __rest
 // shells of the 7 lower neighbours into each brick's lower faces (pull, deterministic).
 // Gather: one workgroup per brick stages each force component's extended tile in LDS and
 // interpolates its particles from there.
-constexpr int kTileMaxFloats = 19100;  // (32+1)*(16+1)*(32+2) = 19074 fits (76.3 KB: 2 workgroups/CU)
+constexpr int kTileMaxFloats = 19100;  // gather f32 tile: (32+1)*(16+1)*(32+2) = 19074 fits
+constexpr int kTileMaxWords = 9900;    // scatter int64 tile: (16+1)*(16+1)*(32+2) = 9826 (78.6 KB: 2 WG/CU)
+// Scatter accumulation is exact 64-bit fixed point (2^-32 units): gfx950's LDS integer atomics run
+// an order of magnitude faster than its LDS float atomics, and integer sums are order independent,
+// so the density is deterministic; each cell is rounded to float32 once when stored.
+constexpr float kFix = 4294967296.0f;         // 2^32
+constexpr float kUnfix = 2.3283064365386963e-10f;  // 2^-32
 
 struct Bricks {
   int n[3];   // mesh shape
@@ -135,8 +141,9 @@ inline Bricks make_bricks(const int32_t* mesh, const int32_t* brick) {
 inline bool tiled_ok(const int32_t* mesh, const int32_t* brick) {
   for (int a = 0; a < 3; ++a)
     if (brick[a] < 1 || mesh[a] % brick[a] != 0) return false;
-  // LDS tile rows have a z pitch of b2 + 2 floats
-  return (int64_t)(brick[0] + 1) * (brick[1] + 1) * (brick[2] + 2) <= kTileMaxFloats;
+  // LDS tile rows have a z pitch of b2 + 2 words
+  const int64_t words = (int64_t)(brick[0] + 1) * (brick[1] + 1) * (brick[2] + 2);
+  return words <= kTileMaxFloats && words <= kTileMaxWords;
 }
 
 inline bool is_brick(const Bricks& k, int x, int y, int z) { return k.b[0] == x && k.b[1] == y && k.b[2] == z; }
@@ -195,6 +202,18 @@ __device__ __forceinline__ void base_frac(const int16_t* __restrict__ pmid, cons
   }
 }
 
+// Tile cell of a particle's base (corner 0) cell.  The particle was binned by the sort with
+// the brick of this same base cell (pm_sort.hip cell_key: the same float32 division, floor and
+// periodic wrap), so the local index lies inside the brick; the clamp only keeps the LDS
+// access in bounds should that invariant ever be broken.
+template <int BX, int BY, int BZ>
+__device__ __forceinline__ int local_cell(const BrickShape<BX, BY, BZ>& B, const int bs[3], const int o[3]) {
+  int l0 = min(max(bs[0] - o[0], 0), B.b0 - 1);
+  int l1 = min(max(bs[1] - o[1], 0), B.b1 - 1);
+  int l2 = min(max(bs[2] - o[2], 0), B.b2 - 1);
+  return B.cell(l0, l1, l2);
+}
+
 // CIC weight of corner c from the fractional offsets, in the reference's product order
 __device__ __forceinline__ float frac_weight(const float f[3], int c) {
   float wx = (c & 1) ? 1.f - fabsf(f[0] - 1.f) : 1.f - fabsf(f[0]);
@@ -207,14 +226,14 @@ template <int BX, int BY, int BZ>
 __global__ void __launch_bounds__(kTileThreads) k_scatter_tiles(
     const int16_t* __restrict__ pmid, const float* __restrict__ disp, const int32_t* __restrict__ bin_off,
     float val, Geom g, Bricks k, float* __restrict__ mesh, float* __restrict__ shell) {
-  __shared__ __align__(16) float tile[kTileMaxFloats];
+  __shared__ __align__(16) unsigned long long tile[kTileMaxWords];
   const BrickShape<BX, BY, BZ> B(k);
   const int64_t bid = blockIdx.x;
   int bc[3];
   brick_coords(k, bid, bc);
   const int o[3] = {bc[0] * B.b0, bc[1] * B.b1, bc[2] * B.b2};
   const int tf = B.tfloats();
-  for (int i = threadIdx.x; i < tf; i += kTileThreads) tile[i] = 0.f;
+  for (int i = threadIdx.x; i < tf; i += kTileThreads) tile[i] = 0ull;
   __syncthreads();
   const int64_t j0 = bin_off[bid], j1 = bin_off[bid + 1];
   constexpr int PPT = 4;
@@ -223,30 +242,19 @@ __global__ void __launch_bounds__(kTileThreads) k_scatter_tiles(
     float fr[PPT][3];
     bool act[PPT];
 #pragma unroll
-    for (int q = 0; q < PPT; ++q) {  // all loads of the chunk first
+    for (int q = 0; q < PPT; ++q) {  // all loads of the chunk first (branch-free, clamped index)
       int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
       act[q] = j < j1;
-      if (act[q]) base_frac(pmid, disp, j, g.a1, k.n, bs[q], fr[q]);
+      base_frac(pmid, disp, act[q] ? j : j1 - 1, g.a1, k.n, bs[q], fr[q]);
     }
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
       if (!act[q]) continue;
-      int l0 = bs[q][0] - o[0], l1 = bs[q][1] - o[1], l2 = bs[q][2] - o[2];
-      bool inside = l0 >= 0 && l0 < B.b0 && l1 >= 0 && l1 < B.b1 && l2 >= 0 && l2 < B.b2;
-      if (inside) {
-        const int c0i = B.cell(l0, l1, l2);
+      const int c0i = local_cell(B, bs[q], o);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) atomicAdd(&tile[c0i + B.corner_off(c)], val * frac_weight(fr[q], c));
-      } else {  // cannot happen when the key and the CIC agree; stay correct regardless
-        int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
-        AxisCic ax[3];
-        particle_cic(g, pmid, disp, j, ax);
-        for (int c = 0; c < 8; ++c) {
-          int64_t flat;
-          corner_index(g, ax, c, flat);
-          atomic_add_f32(mesh + flat, val * corner_weight(ax, c));
-        }
-      }
+      for (int c = 0; c < 8; ++c)
+        atomicAdd(&tile[c0i + B.corner_off(c)],
+                  (unsigned long long)__float2ll_rn(val * frac_weight(fr[q], c) * kFix));
     }
   }
   __syncthreads();
@@ -257,17 +265,18 @@ __global__ void __launch_bounds__(kTileThreads) k_scatter_tiles(
     for (int it = threadIdx.x; it < ITEMS; it += kTileThreads) {
       int r = it / C4, c4 = it - r * C4;
       int x = r / BY, y = r - x * BY;
-      const float* src = &tile[B.cell(x, y, 4 * c4)];
-      float2 lo = *reinterpret_cast<const float2*>(src), hi = *reinterpret_cast<const float2*>(src + 2);
+      const long long* src = reinterpret_cast<const long long*>(&tile[B.cell(x, y, 4 * c4)]);
+      longlong2 lo = *reinterpret_cast<const longlong2*>(src), hi = *reinterpret_cast<const longlong2*>(src + 2);
       *reinterpret_cast<float4*>(&mesh[((int64_t)(o[0] + x) * k.n[1] + (o[1] + y)) * k.n[2] + o[2] + 4 * c4]) =
-          make_float4(lo.x, lo.y, hi.x, hi.y);
+          make_float4((float)lo.x * kUnfix, (float)lo.y * kUnfix, (float)hi.x * kUnfix, (float)hi.y * kUnfix);
     }
   } else {
     const int rows = B.b0 * B.b1;
     for (int i = threadIdx.x; i < rows * B.b2; i += kTileThreads) {
       int r = i / B.b2, z = i - r * B.b2;
       int x = r / B.b1, y = r - x * B.b1;
-      mesh[((int64_t)(o[0] + x) * k.n[1] + (o[1] + y)) * k.n[2] + (o[2] + z)] = tile[B.cell(x, y, z)];
+      mesh[((int64_t)(o[0] + x) * k.n[1] + (o[1] + y)) * k.n[2] + (o[2] + z)] =
+          (float)(long long)tile[B.cell(x, y, z)] * kUnfix;
     }
   }
   // upper shell: face x = b0 (t1*t2), face y = b1 (b0*t2), face z = b2 (b0*b1)
@@ -291,7 +300,7 @@ __global__ void __launch_bounds__(kTileThreads) k_scatter_tiles(
       x = r / B.b1;
       y = r - x * B.b1;
     }
-    sh[i] = tile[B.cell(x, y, z)];
+    sh[i] = (float)(long long)tile[B.cell(x, y, z)] * kUnfix;
   }
 }
 
@@ -337,17 +346,31 @@ __global__ void __launch_bounds__(kTileThreads) k_scatter_halo(Bricks k, const f
       x = 1 + r / (B.b1 - 1);
       y = 1 + r - (x - 1) * (B.b1 - 1);
     }
-    float acc = 0.f;
+    // all 7 candidate shell values are loaded unconditionally so the loads overlap; a neighbour
+    // that does not touch this cell re-reads the first one that does (same cache line) and
+    // counts zero
+    const int dp = x == 0 ? 1 : (y == 0 ? 2 : 4);
+    auto shell_at = [&](int d) {
+      int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
+      int sx = x + dx * B.b0, sy = y + dy * B.b1, sz = z + dz * B.b2;
+      int si = (sx == B.b0) ? sy * t2 + sz : (sy == B.b1) ? fxs + sx * t2 + sz : fxs + fys + sx * B.b1 + sy;
+      return nbr[d] + si;
+    };
+    const int64_t prim = shell_at(dp);
+    float sv[8];
 #pragma unroll
     for (int d = 1; d < 8; ++d) {
       int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-      if ((dx && x != 0) || (dy && y != 0) || (dz && z != 0)) continue;
-      int sx = x + dx * B.b0, sy = y + dy * B.b1, sz = z + dz * B.b2;
-      int si = (sx == B.b0) ? sy * t2 + sz : (sy == B.b1) ? fxs + sx * t2 + sz : fxs + fys + sx * B.b1 + sy;
-      acc += shell[nbr[d] + si];
+      bool use = !((dx && x != 0) || (dy && y != 0) || (dz && z != 0));
+      float val = shell[use ? shell_at(d) : prim];
+      sv[d] = use ? val : 0.f;
     }
     int64_t flat = ((int64_t)(o[0] + x) * k.n[1] + (o[1] + y)) * k.n[2] + (o[2] + z);
-    mesh[flat] += acc;
+    float m = mesh[flat];
+    float acc = 0.f;
+#pragma unroll
+    for (int d = 1; d < 8; ++d) acc += sv[d];
+    mesh[flat] = m + acc;
   }
 }
 
@@ -359,21 +382,39 @@ __device__ __forceinline__ void load_tile(float* tile, const float* __restrict__
     constexpr int C4 = BZ / 4;
     constexpr int ROWS = (BX + 1) * (BY + 1);
     constexpr int ITEMS = ROWS * C4;
-    constexpr int ROUND = 4;  // float4 loads in flight per thread per round (64 B/lane)
+    // float4 loads in flight per thread per round: the whole tile in one round when that
+    // takes at most 6 per thread (5 for a 16x16x32 brick)
+    constexpr int ONE = (ITEMS + kTileThreads - 1) / kTileThreads;
+    constexpr int ROUND = ONE <= 6 ? ONE : 4;
+    // z == b2 column (first cell of the next brick along z, wrapped): issued first, stored last
+    constexpr int NZ = (ROWS + kTileThreads - 1) / kTileThreads;
+    float zc[NZ];
+    {
+      int gz = o[2] + BZ;
+      gz -= gz >= k.n[2] ? k.n[2] : 0;
+#pragma unroll
+      for (int u = 0; u < NZ; ++u) {
+        int r = min((int)threadIdx.x + u * kTileThreads, ROWS - 1);
+        int x = r / (BY + 1), y = r - x * (BY + 1);
+        int gx = o[0] + x, gy = o[1] + y;
+        gx -= gx >= k.n[0] ? k.n[0] : 0;
+        gy -= gy >= k.n[1] ? k.n[1] : 0;
+        zc[u] = F[((int64_t)gx * k.n[1] + gy) * k.n[2] + gz];
+      }
+    }
 #pragma unroll 1
     for (int base = 0; base < ITEMS; base += ROUND * kTileThreads) {
       float4 v[ROUND];
 #pragma unroll
-      for (int u = 0; u < ROUND; ++u) {  // all row loads of the round before any LDS store
-        int it = base + threadIdx.x + u * kTileThreads;
-        if (it < ITEMS) {
-          int r = it / C4, c4 = it - r * C4;
-          int x = r / (BY + 1), y = r - x * (BY + 1);
-          int gx = o[0] + x, gy = o[1] + y;
-          gx -= gx >= k.n[0] ? k.n[0] : 0;
-          gy -= gy >= k.n[1] ? k.n[1] : 0;
-          v[u] = *reinterpret_cast<const float4*>(&F[((int64_t)gx * k.n[1] + gy) * k.n[2] + o[2] + 4 * c4]);
-        }
+      for (int u = 0; u < ROUND; ++u) {  // all row loads of the round before any LDS store:
+        // branch-free (clamped index) so the compiler keeps every load in flight
+        int it = min(base + (int)threadIdx.x + u * kTileThreads, ITEMS - 1);
+        int r = it / C4, c4 = it - r * C4;
+        int x = r / (BY + 1), y = r - x * (BY + 1);
+        int gx = o[0] + x, gy = o[1] + y;
+        gx -= gx >= k.n[0] ? k.n[0] : 0;
+        gy -= gy >= k.n[1] ? k.n[1] : 0;
+        v[u] = *reinterpret_cast<const float4*>(&F[((int64_t)gx * k.n[1] + gy) * k.n[2] + o[2] + 4 * c4]);
       }
 #pragma unroll
       for (int u = 0; u < ROUND; ++u) {
@@ -386,16 +427,10 @@ __device__ __forceinline__ void load_tile(float* tile, const float* __restrict__
         }
       }
     }
-    // z == b2 column (first cell of the next brick along z, wrapped)
-    int gz = o[2] + BZ;
-    gz -= gz >= k.n[2] ? k.n[2] : 0;
-#pragma unroll 1
-    for (int r = threadIdx.x; r < ROWS; r += kTileThreads) {
-      int x = r / (BY + 1), y = r - x * (BY + 1);
-      int gx = o[0] + x, gy = o[1] + y;
-      gx -= gx >= k.n[0] ? k.n[0] : 0;
-      gy -= gy >= k.n[1] ? k.n[1] : 0;
-      tile[r * (BZ + 2) + BZ] = F[((int64_t)gx * k.n[1] + gy) * k.n[2] + gz];
+#pragma unroll
+    for (int u = 0; u < NZ; ++u) {
+      int r = (int)threadIdx.x + u * kTileThreads;
+      if (r < ROWS) tile[r * (BZ + 2) + BZ] = zc[u];
     }
   } else {
     const int t1 = B.t1(), t2 = B.t2();
@@ -404,20 +439,17 @@ __device__ __forceinline__ void load_tile(float* tile, const float* __restrict__
       float v[kBatch];
       int dst[kBatch];
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {  // issue kBatch independent loads, then store
-        int i = i0 + u * kTileThreads + threadIdx.x;
-        v[u] = 0.f;
-        dst[u] = -1;
-        if (i < tvol) {
-          int x = i / tyz, rem = i - x * tyz;
-          int y = rem / t2, z = rem - y * t2;
-          int gx = o[0] + x, gy = o[1] + y, gz = o[2] + z;
-          gx -= gx >= k.n[0] ? k.n[0] : 0;
-          gy -= gy >= k.n[1] ? k.n[1] : 0;
-          gz -= gz >= k.n[2] ? k.n[2] : 0;
-          v[u] = F[((int64_t)gx * k.n[1] + gy) * k.n[2] + gz];
-          dst[u] = B.cell(x, y, z);
-        }
+      for (int u = 0; u < kBatch; ++u) {  // issue kBatch independent (branch-free) loads, then store
+        int i0u = i0 + u * kTileThreads + (int)threadIdx.x;
+        int i = min(i0u, tvol - 1);
+        int x = i / tyz, rem = i - x * tyz;
+        int y = rem / t2, z = rem - y * t2;
+        int gx = o[0] + x, gy = o[1] + y, gz = o[2] + z;
+        gx -= gx >= k.n[0] ? k.n[0] : 0;
+        gy -= gy >= k.n[1] ? k.n[1] : 0;
+        gz -= gz >= k.n[2] ? k.n[2] : 0;
+        v[u] = F[((int64_t)gx * k.n[1] + gy) * k.n[2] + gz];
+        dst[u] = i0u < tvol ? B.cell(x, y, z) : -1;
       }
 #pragma unroll
       for (int u = 0; u < kBatch; ++u)
@@ -426,80 +458,66 @@ __device__ __forceinline__ void load_tile(float* tile, const float* __restrict__
   }
 }
 
-constexpr int kGatherPPT = 4;  // particles per thread per chunk
+// Particles of one brick processed per tile staging: their three interpolated components
+// wait in LDS, so each component tile is read from HBM once per brick unless the brick holds
+// more than kGatherCap particles (rare: ~2x the mean occupancy at 1/8 particle per cell).
+constexpr int kGatherCap = 2048;
+constexpr int kGatherPPT = kGatherCap / kTileThreads;
+constexpr int kGatherBatch = 2;  // particles whose loads are in flight together
 
 template <int BX, int BY, int BZ>
-__global__ void __launch_bounds__(kTileThreads, 4) k_gather3_tiles(
+__global__ void __launch_bounds__(kTileThreads) k_gather3_tiles(
     const int16_t* __restrict__ pmid, const float* __restrict__ disp, const int32_t* __restrict__ bin_off,
     const int32_t* __restrict__ perm, Geom g, Bricks k, const float* __restrict__ f0,
     const float* __restrict__ f1, const float* __restrict__ f2, float* __restrict__ acc) {
-  __shared__ __align__(16) float tile[kTileMaxFloats];
+  constexpr int kLds = BX ? (BX + 1) * (BY + 1) * (BZ + 2) : kTileMaxFloats;
+  __shared__ __align__(16) float tile[kLds];
+  __shared__ float res[3][kGatherCap];
   const BrickShape<BX, BY, BZ> B(k);
   const int64_t bid = blockIdx.x;
   int bc[3];
   brick_coords(k, bid, bc);
   const int o[3] = {bc[0] * B.b0, bc[1] * B.b1, bc[2] * B.b2};
   const int64_t j0 = bin_off[bid], j1 = bin_off[bid + 1];
-  for (int64_t c0 = j0; c0 < j1; c0 += (int64_t)kTileThreads * kGatherPPT) {
-    int cell0[kGatherPPT];  // LDS cell of corner 0; -1 outside the tile, -2 inactive
-    float w[kGatherPPT][8];                 // corner weights, computed once for the 3 components
-    float r[kGatherPPT][3];
-#pragma unroll
-    for (int q = 0; q < kGatherPPT; ++q) {
-      int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
-      cell0[q] = -2;
-      r[q][0] = r[q][1] = r[q][2] = 0.f;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) w[q][c] = 0.f;
-      if (j < j1) {
-        int bs[3];
-        float fr[3];
-        base_frac(pmid, disp, j, g.a1, k.n, bs, fr);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) w[q][c] = frac_weight(fr, c);
-        int l0 = bs[0] - o[0], l1 = bs[1] - o[1], l2 = bs[2] - o[2];
-        bool in = l0 >= 0 && l0 < B.b0 && l1 >= 0 && l1 < B.b1 && l2 >= 0 && l2 < B.b2;
-        cell0[q] = in ? B.cell(l0, l1, l2) : -1;
-      }
-    }
-    // one force component: stage its tile, interpolate this thread's particles
-    auto component = [&](const float* __restrict__ F, int comp) {
-      __syncthreads();
+  for (int64_t s0 = j0; s0 < j1; s0 += kGatherCap) {
+    const int64_t s1 = s0 + kGatherCap < j1 ? s0 + kGatherCap : j1;
+#pragma unroll 1
+    for (int comp = 0; comp < 3; ++comp) {
+      const float* __restrict__ F = comp == 0 ? f0 : (comp == 1 ? f1 : f2);
+      __syncthreads();  // the previous component's interpolation is done with the tile
       load_tile<BX, BY, BZ>(tile, F, B, k, o);
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < kGatherPPT; ++q) {
-        if (cell0[q] == -2) continue;
-        float s = 0.f;
-        if (cell0[q] >= 0) {
+      for (int h = 0; h < kGatherPPT; h += kGatherBatch) {
+      int bs[kGatherBatch][3];
+      float fr[kGatherBatch][3];
 #pragma unroll
-          for (int c = 0; c < 8; ++c) s = s + tile[cell0[q] + B.corner_off(c)] * w[q][c];
-        } else {
-          int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
-          AxisCic ax[3];
-          particle_cic(g, pmid, disp, j, ax);
-          for (int c = 0; c < 8; ++c) {
-            int64_t flat;
-            corner_index(g, ax, c, flat);
-            s = s + F[flat] * corner_weight(ax, c);
-          }
-        }
-        if (comp == 0) r[q][0] = s;
-        if (comp == 1) r[q][1] = s;
-        if (comp == 2) r[q][2] = s;
+      for (int u = 0; u < kGatherBatch; ++u) {  // the batch's particle loads first (clamped, branch-free)
+        int64_t j = s0 + threadIdx.x + (int64_t)(h + u) * kTileThreads;
+        base_frac(pmid, disp, j < s1 ? j : s1 - 1, g.a1, k.n, bs[u], fr[u]);
       }
-    };
-    component(f0, 0);
-    component(f1, 1);
-    component(f2, 2);
+#pragma unroll
+      for (int u = 0; u < kGatherBatch; ++u) {
+        const int q = u;
+        const int jl = threadIdx.x + (h + u) * kTileThreads;
+        if (s0 + jl >= s1) continue;
+        const int c0i = local_cell(B, bs[q], o);
+        float sum = 0.f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) sum = sum + tile[c0i + B.corner_off(c)] * frac_weight(fr[q], c);
+        res[comp][jl] = sum;
+      }
+      }
+    }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < kGatherPPT; ++q) {
-      int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
-      if (j < j1) {
-        int64_t i = perm[j];
-        acc[i * 3 + 0] = r[q][0];
-        acc[i * 3 + 1] = r[q][1];
-        acc[i * 3 + 2] = r[q][2];
+      const int jl = threadIdx.x + q * kTileThreads;
+      if (s0 + jl < s1) {
+        int64_t i = perm[s0 + jl];
+        acc[i * 3 + 0] = res[0][jl];
+        acc[i * 3 + 1] = res[1][jl];
+        acc[i * 3 + 2] = res[2][jl];
       }
     }
   }
@@ -624,12 +642,33 @@ __global__ void __launch_bounds__(kBlock) k_source_bwd(float* __restrict__ src_c
 
 struct Layout {
   int nx, ny, nz, m;
-  int64_t pitch;  // complex row pitch (float2), >= m+1, multiple of 16
+  int64_t pitch;  // complex row pitch (float2) of a full half spectrum, >= m+1, multiple of 16
   int64_t nreal;  // nx*ny*nz
   int64_t ncplx;  // nx*ny*pitch
+  // Band of the force pipeline.  With the particle-Nyquist filter active every mode with
+  // |k_i| > pi/l_p (1 + 8 eps) is zero after the Green multiply, so the passes after the z
+  // r2c keep only kz < nkz and, after the y forward pass, only the 2*ky_keep+1 rows with
+  // |ky| <= ky_keep (packed).  Unfiltered axes keep everything (nkz = m+1, ky_keep = -1).
+  int nkz;         // stored spectrum columns (kz = 0..nkz-1); nkz = m+1 includes Nyquist
+  int ky_keep;     // packed y band half-width, -1 = full rows
+  int nky;         // rows after the y forward pass (2*ky_keep+1 or ny)
+  int64_t pitch1;  // row pitch of the banded spectra, multiple of 16
+  int64_t s1;      // nx*ny*pitch1:  full-y banded spectrum
+  int64_t s2;      // nx*nky*pitch1: packed-y banded spectrum
 };
 
-Layout make_layout(const int32_t* s) {
+// Largest |m| on an axis of n points whose wavenumber 2 pi m / (n cell) can pass the
+// particle-Nyquist mask |k| <= pi/l_p (1 + 8 eps_f32) (core/utils.py:33-37).  The slack of
+// 1e-4 makes the band a superset of the device-side float32 mask, which still decides
+// every mode inside the band.
+int band_half_width(int n, double cell, double spacing) {
+  double kmax = 3.14159265358979323846 / spacing * (1.0 + 8.0 * 1.1920928955078125e-07) * (1.0 + 1e-4);
+  double dk = 2.0 * 3.14159265358979323846 / ((double)n * cell);
+  return (int)std::floor(kmax / dk);
+}
+
+Layout make_layout(const pm_gravity_desc* d) {
+  const int32_t* s = d->mesh_shape;
   Layout L;
   L.nx = s[0];
   L.ny = s[1];
@@ -638,6 +677,21 @@ Layout make_layout(const int32_t* s) {
   L.pitch = (int64_t)align_up((size_t)L.m + 1, 16);
   L.nreal = (int64_t)L.nx * L.ny * L.nz;
   L.ncplx = (int64_t)L.nx * L.ny * L.pitch;
+  L.nkz = L.m + 1;
+  L.ky_keep = -1;
+  L.nky = L.ny;
+  if (d->mask_active) {
+    int kz = band_half_width(L.nz, d->cell_size, d->ptcl_spacing);
+    if (kz < L.m) L.nkz = kz + 1;
+    int ky = band_half_width(L.ny, d->cell_size, d->ptcl_spacing);
+    if (2 * ky + 1 < L.ny) {
+      L.ky_keep = ky;
+      L.nky = 2 * ky + 1;
+    }
+  }
+  L.pitch1 = (int64_t)align_up((size_t)L.nkz, 16);
+  L.s1 = (int64_t)L.nx * L.ny * L.pitch1;
+  L.s2 = (int64_t)L.nx * L.nky * L.pitch1;
   return L;
 }
 
@@ -666,6 +720,8 @@ struct Buffers {
   float2* ca;
   float2* cb;
   float2* cc;
+  float2* cd;  // packed-y spectra (only when the y band is packed)
+  float2* ce;
   // reverse only: F_x (the forward keeps rho in r0), three mesh cotangents
   float* g0;
   float* g1;
@@ -692,9 +748,18 @@ bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const in
   b.r0 = ws.take<float>(L.nreal);
   b.r1 = ws.take<float>(L.nreal);
   b.r2 = ws.take<float>(L.nreal);
-  b.ca = ws.take<float2>(L.ncplx);
-  b.cb = ws.take<float2>(L.ncplx);
-  b.cc = ws.take<float2>(L.ncplx);
+  // banded force pipeline: ca = A (s1), cb = C (s2), cc = E (s1), and when the y band is
+  // packed cd = B (s2) and ce = F (s1); the reverse pass also runs full-spectrum FFTs in
+  // ca, cb and cc
+  b.ca = ws.take<float2>(reverse ? L.ncplx : L.s1);
+  b.cb = ws.take<float2>(reverse ? L.ncplx : L.s2);
+  b.cc = ws.take<float2>(reverse ? L.ncplx : L.s1);
+  b.cd = nullptr;
+  b.ce = nullptr;
+  if (L.ky_keep >= 0) {
+    b.cd = ws.take<float2>(L.s2);
+    b.ce = ws.take<float2>(L.s1);
+  }
   if (reverse) {
     b.g0 = ws.take<float>(L.nreal);
     b.g1 = ws.take<float>(L.nreal);
@@ -706,7 +771,7 @@ bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const in
 }
 
 size_t workspace_bytes(const pm_gravity_desc* d, int64_t n, bool reverse) {
-  Layout L = make_layout(d->mesh_shape);
+  Layout L = make_layout(d);
   Arena probe(reinterpret_cast<void*>(uintptr_t(256)), SIZE_MAX / 2);
   Buffers b;
   carve(probe, L, n, d->mesh_shape, d->brick, reverse, b);
@@ -755,11 +820,11 @@ int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, 
   if (b.bins) {
     Bricks k = make_bricks(d->mesh_shape, d->brick);
     const unsigned nbins = (unsigned)((int64_t)k.nb[0] * k.nb[1] * k.nb[2]);
-    if (is_brick(k, 32, 16, 32)) {
-      hipLaunchKernelGGL((k_scatter_tiles<32, 16, 32>), dim3(nbins), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
+    if (is_brick(k, 16, 16, 32)) {
+      hipLaunchKernelGGL((k_scatter_tiles<16, 16, 32>), dim3(nbins), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
                          b.bins, d->particle_value, g, k, b.r0, b.shell);
       PM_LAUNCHED("k_scatter_tiles", s);
-      hipLaunchKernelGGL((k_scatter_halo<32, 16, 32>), dim3(nbins), dim3(kTileThreads), 0, s, k, b.shell, b.r0);
+      hipLaunchKernelGGL((k_scatter_halo<16, 16, 32>), dim3(nbins), dim3(kTileThreads), 0, s, k, b.shell, b.r0);
       PM_LAUNCHED("k_scatter_halo", s);
     } else {
       hipLaunchKernelGGL((k_scatter_tiles<0, 0, 0>), dim3(nbins), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
@@ -787,8 +852,8 @@ int gather_stage(const int16_t* pmid_s, const float* disp_s, const uint8_t* vali
   if (b.bins) {
     Bricks k = make_bricks(d->mesh_shape, d->brick);
     const int64_t nbins = (int64_t)k.nb[0] * k.nb[1] * k.nb[2];
-    if (is_brick(k, 32, 16, 32))
-      hipLaunchKernelGGL((k_gather3_tiles<32, 16, 32>), dim3((unsigned)nbins), dim3(kTileThreads), 0, s, pmid_s,
+    if (is_brick(k, 16, 16, 32))
+      hipLaunchKernelGGL((k_gather3_tiles<16, 16, 32>), dim3((unsigned)nbins), dim3(kTileThreads), 0, s, pmid_s,
                          disp_s, b.bins, b.perm, g, k, fx, fy, fz, acc);
     else
       hipLaunchKernelGGL((k_gather3_tiles<0, 0, 0>), dim3((unsigned)nbins), dim3(kTileThreads), 0, s, pmid_s,
@@ -812,49 +877,73 @@ int force_mesh_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b, cons
                      hipStream_t s, bool keep_density) {
   using namespace fft;
   const float coef = 1.5f * d->omega_m;  // 1.5 * Omega_m in float32 (gravity.py:630)
+  const bool zband = L.nkz <= L.m;      // Nyquist column not stored
+  const bool ypack = L.ky_keep >= 0;
+  float2* A = b.ca;                      // z r2c output, then chi (y inverse of psi)
+  float2* B = ypack ? b.cd : b.ca;       // y forward output, then psi
+  float2* C = b.cb;                      // D_x psi
+  float2* E = b.cc;                      // D_y chi
+  float2* F = ypack ? b.ce : b.cb;       // y inverse of D_x psi
+  const int64_t p1 = L.pitch1;
   ZPass z1{};
   z1.rin = b.r0;
-  z1.cmain = b.ca;
-  z1.cnyq = b.ca + L.m;
+  z1.cmain = A;
+  z1.cnyq = zband ? A : A + L.m;
   z1.pitch_r = L.nz;
-  z1.pitch_c = L.pitch;
-  z1.nyq_stride = L.pitch;
+  z1.pitch_c = p1;
+  z1.nyq_stride = p1;
   z1.nlines = (int64_t)L.nx * L.ny;
   z1.src_shift = -1.f;
   z1.src_scale = coef;
+  z1.kz_lim = L.nkz;
+  z1.nyq = zband ? 0 : 1;
   PM_TRY(launch_z(L.nz, false, z1, tb, s));
-  const int ncols = L.m + 1;
-  ColPass yf{b.ca, b.ca, nullptr, L.pitch, 1, (int64_t)L.ny * L.pitch, ncols, 0, 0, 0, 0, 0};
+  const int ncols = L.nkz;
+  // y forward, packing the |ky| band when the filter allows it
+  ColPass yf{A, B, nullptr, p1, 1, (int64_t)L.ny * p1, ncols, 0, 0, 0, 0, 0};
+  yf.o_row = p1;
+  yf.o_batch = (int64_t)L.nky * p1;
+  yf.keep = L.ky_keep;
   PM_TRY(launch_cols(L.ny, kColFwd, yf, L.nx, tb, 1, s));
-  // x: forward, Green, two inverses; columns are kz, batches are ky
-  ColPass xg{b.ca, b.ca, b.cb, (int64_t)L.ny * L.pitch, 1, L.pitch, ncols, 1, 0, 0, 1, 0};
-  PM_TRY(launch_cols(L.nx, kColGreen, xg, L.ny, tb, 0, s));
-  ColPass yi2{b.ca, b.ca, b.cc, L.pitch, 1, (int64_t)L.ny * L.pitch, ncols, 0, 0, 0, 0, 0};
+  // x: forward, Green, two inverses; columns are kz, batches are the (packed) ky rows
+  ColPass xg{B, B, C, (int64_t)L.nky * p1, 1, p1, ncols, 1, 0, 0, 1, 0};
+  if (ypack) {
+    xg.ky_split = L.ky_keep;
+    xg.ky_shift = L.ny - L.nky;
+  }
+  PM_TRY(launch_cols(L.nx, kColGreen, xg, L.nky, tb, 0, s));
+  // y inverses expand the band back to full rows
+  ColPass yi2{B, A, E, p1, 1, (int64_t)L.nky * p1, ncols, 0, 0, 0, 0, 0};
+  yi2.o_row = p1;
+  yi2.o_batch = (int64_t)L.ny * p1;
+  yi2.keep = L.ky_keep;
+  yi2.compact_in = 1;
   PM_TRY(launch_cols(L.ny, kColInv2, yi2, L.nx, tb, 1, s));
-  ColPass yi{b.cb, b.cb, nullptr, L.pitch, 1, (int64_t)L.ny * L.pitch, ncols, 0, 0, 0, 0, 0};
+  ColPass yi{C, F, nullptr, p1, 1, (int64_t)L.nky * p1, ncols, 0, 0, 0, 0, 0};
+  yi.o_row = p1;
+  yi.o_batch = (int64_t)L.ny * p1;
+  yi.keep = L.ky_keep;
+  yi.compact_in = 1;
   PM_TRY(launch_cols(L.ny, kColInv, yi, L.nx, tb, 1, s));
   const float norm = (float)(2.0 / ((double)L.nx * L.ny * L.nz));
   ZPass zc{};
   zc.pitch_r = L.nz;
-  zc.pitch_c = L.pitch;
-  zc.nyq_stride = L.pitch;
+  zc.pitch_c = p1;
+  zc.nyq_stride = p1;
   zc.nlines = (int64_t)L.nx * L.ny;
   zc.out_scale = norm;
+  zc.kz_lim = L.nkz;
+  zc.nyq = zband ? 0 : 1;
   float* fx = keep_density ? b.g0 : b.r0;  // reverse keeps rho for the Omega_m cotangent
-  zc.cmain = b.cb;
-  zc.cnyq = b.cb + L.m;
-  zc.rout = fx;
-  zc.deriv = 0;
-  PM_TRY(launch_z(L.nz, true, zc, tb, s));
-  zc.cmain = b.cc;
-  zc.cnyq = b.cc + L.m;
-  zc.rout = b.r1;
-  PM_TRY(launch_z(L.nz, true, zc, tb, s));
-  zc.cmain = b.ca;
-  zc.cnyq = b.ca + L.m;
-  zc.rout = b.r2;
-  zc.deriv = 1;
-  PM_TRY(launch_z(L.nz, true, zc, tb, s));
+  const float2* srcs[3] = {F, E, A};
+  float* dsts[3] = {fx, b.r1, b.r2};
+  for (int c = 0; c < 3; ++c) {
+    zc.cmain = const_cast<float2*>(srcs[c]);
+    zc.cnyq = zband ? zc.cmain : zc.cmain + L.m;
+    zc.rout = dsts[c];
+    zc.deriv = c == 2;  // D_z on the last component
+    PM_TRY(launch_z(L.nz, true, zc, tb, s));
+  }
   return PM_SUCCESS;
 }
 
@@ -865,7 +954,7 @@ int prepare(const pm_gravity_desc* d, int64_t n, void* workspace, size_t workspa
   size_t need = workspace_bytes(d, n, reverse);
   PM_RETURN_IF(workspace == nullptr || workspace_bytes_ < need, PM_ERROR_WORKSPACE,
                "gravity workspace too small: need " + std::to_string(need) + " bytes");
-  L = make_layout(d->mesh_shape);
+  L = make_layout(d);
   PM_RETURN_IF(!carve(ws, L, n, d->mesh_shape, d->brick, reverse, b), PM_ERROR_WORKSPACE,
                "workspace carve failed");
   PM_TRY(fft::launch_spec_tables(b.tables, d->mesh_shape, d->cell_size, d->ptcl_spacing, d->mask_active, s));
@@ -885,9 +974,15 @@ int pm_gravity_workspace(const pm_gravity_desc* desc, int64_t n, size_t* bytes) 
   int st = grav::check_desc(desc, n);
   if (st) return st;
   PM_RETURN_IF(bytes == nullptr, PM_ERROR_INVALID_ARGUMENT, "bytes is NULL");
-  size_t f = grav::workspace_bytes(desc, n, false);
-  size_t r = grav::workspace_bytes(desc, n, true);
-  *bytes = std::max(f, r);
+  *bytes = grav::workspace_bytes(desc, n, false);
+  return PM_SUCCESS;
+}
+
+int pm_gravity_vjp_workspace(const pm_gravity_desc* desc, int64_t n, size_t* bytes) {
+  int st = grav::check_desc(desc, n);
+  if (st) return st;
+  PM_RETURN_IF(bytes == nullptr, PM_ERROR_INVALID_ARGUMENT, "bytes is NULL");
+  *bytes = grav::workspace_bytes(desc, n, true);
   return PM_SUCCESS;
 }
 

@@ -29,6 +29,8 @@ struct KeyGeom {
   int n[3];
   int b[3];
   int nb[3];
+  int nshift[3];  // log2(n) when n is a power of two, else -1
+  int bshift[3];  // log2(b) when b is a power of two, else -1
   float cell;
   int sentinel;
 };
@@ -43,7 +45,8 @@ __device__ __forceinline__ int cell_key(const KeyGeom& g, const int16_t* __restr
     // containing cell: the CIC base index with offset 0 (cic/pallas.py:227-236)
     float t = disp[i * 3 + a] / g.cell;
     int base = (int)pmid[i * 3 + a] + (int)floorf(t);
-    c[a] = pos_mod(base, g.n[a]) / g.b[a];
+    int cell = g.nshift[a] >= 0 ? (base & (g.n[a] - 1)) : pos_mod(base, g.n[a]);
+    c[a] = g.bshift[a] >= 0 ? (cell >> g.bshift[a]) : cell / g.b[a];
   }
   return (c[0] * g.nb[1] + c[1]) * g.nb[2] + c[2];
 }
@@ -61,13 +64,24 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   __syncthreads();
   const int64_t seg = (int64_t)blockIdx.x * kWavesPerBlock + w;
   if (seg < nseg) {
+    // equal digits of a 64-item group are counted by their lowest lane (ballot match), so the
+    // wave-private row h[w] needs no atomics and nearly sorted keys do not serialise on one bank
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int64_t start = seg * kSeg;
     for (int it = 0; it < kIters; ++it) {
       int64_t i = start + it * kWave + lane;
-      if (i < n) {
-        int key = FIRST ? cell_key(g, pmid, disp, valid, i) : keys[i];
-        atomicAdd(&h[w][(key >> shift) & (radix - 1)], 1);
+      bool active = i < n;
+      int key = 0;
+      if (active) key = FIRST ? cell_key(g, pmid, disp, valid, i) : keys[i];
+      int d = (key >> shift) & (radix - 1);
+      uint64_t peers = __ballot(active);
+      for (int bit = 0; bit < dbits; ++bit) {
+        bool set = (d >> bit) & 1;
+        uint64_t m = __ballot(set);
+        peers &= set ? m : ~m;
       }
+      if (active && (peers & lt_mask) == 0) h[w][d] += __popcll(peers);
+      __builtin_amdgcn_wave_barrier();
     }
   }
   __syncthreads();
@@ -216,10 +230,18 @@ int scan_exclusive(const int32_t* in, int32_t* out, int64_t len, int32_t* partia
 KeyGeom make_key_geom(const int32_t* mesh_shape, float cell, const int32_t* brick) {
   KeyGeom g;
   int64_t nbins = 1;
+  auto log2_or_neg = [](int v) {
+    if (v <= 0 || (v & (v - 1))) return -1;
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+  };
   for (int a = 0; a < 3; ++a) {
     g.n[a] = mesh_shape[a];
     g.b[a] = brick[a];
     g.nb[a] = (int)ceil_div(mesh_shape[a], brick[a]);
+    g.nshift[a] = log2_or_neg(mesh_shape[a]);
+    g.bshift[a] = log2_or_neg(brick[a]);
     nbins *= g.nb[a];
   }
   g.cell = cell;

@@ -47,6 +47,9 @@ def test_queries_validate_without_a_gpu():
     d = gravity_desc(Configuration(1.0, (16, 16, 16), mesh_shape=2), 0.3)
     assert lib.pm_gravity_workspace(d, 4096, ctypes.byref(out)) == _lib.PM_SUCCESS
     assert out.value > 3 * 4 * 32 ** 3
+    fwd = out.value
+    assert lib.pm_gravity_vjp_workspace(d, 4096, ctypes.byref(out)) == _lib.PM_SUCCESS
+    assert out.value > fwd + 4 * 4 * 32 ** 3  # plus the forward meshes kept and 3 cotangent meshes
     with pytest.raises(NotImplementedError, match="powers of two"):
         _lib.call("pm_gravity_workspace", gravity_desc(Configuration(1.0, (12, 12, 12), mesh_shape=1), 0.3),
                   10, ctypes.byref(out))

@@ -51,16 +51,30 @@ def test_force_matches_float64_oracle(device, ng, ratio, box, scale, valid_frac)
         assert np.all(acc[ptcl.unused_index.cpu().numpy()] == 0)
 
 
-@pytest.mark.parametrize("brick", [(4, 2, 8), (1, 1, 1), (16, 16, 16), (32, 1, 2), (3, 3, 3), (32, 32, 32)])
+@pytest.mark.parametrize("brick", [(4, 2, 8), (1, 1, 1), (16, 16, 16), (32, 1, 2), (16, 16, 32), (3, 3, 3),
+                                   (32, 32, 32)])
 def test_force_is_independent_of_the_brick_tiling(device, brick):
     # (3,3,3) does not divide the mesh and (32,32,32) overflows the LDS tile: both take the
-    # untiled sorted path; the others exercise periodic brick neighbours including nb = 1
+    # untiled sorted path; the others exercise periodic brick neighbours including nb = 1,
+    # and (16,16,32) the compile-time-shape kernels
     conf = Configuration(2.0, (16, 16, 16), mesh_shape=2, brick=brick)
     ptcl = _ptcl(conf, 1.0, 3, device, valid_frac=0.9)
     acc = gravity(1.0, ptcl, 0.3, conf).cpu().numpy()
     ref, _ = _oracle_acc(conf, ptcl, 0.3)
     assert np.linalg.norm(acc - ref) / np.linalg.norm(ref) < 1e-5
     assert np.all(acc[ptcl.unused_index.cpu().numpy()] == 0)
+
+
+@pytest.mark.parametrize("grid,ratio", [((8, 16, 4), 2), ((16, 4, 8), 4), ((4, 8, 1), 2), ((2, 4, 8), 2)])
+def test_banded_spectrum_on_non_cubic_meshes(device, grid, ratio):
+    # the particle-Nyquist band differs per axis: packed y rows and kz columns of different
+    # widths, a z axis of 2 points (band = DC only) and an x axis of 4 points
+    conf = Configuration(1.5, grid, mesh_shape=ratio)
+    ptcl = _ptcl(conf, 0.6, sum(grid), device)
+    acc = gravity(1.0, ptcl, 0.3, conf).cpu().numpy()
+    ref, masks = _oracle_acc(conf, ptcl, 0.3)
+    assert len(masks) == 3
+    assert np.linalg.norm(acc - ref) / np.linalg.norm(ref) < 1e-5
 
 
 def test_clustered_particles_exceed_one_gather_chunk(device):

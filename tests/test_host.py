@@ -12,7 +12,7 @@ def test_configuration_geometry_and_validation():
     assert conf.mesh_shape == (1024,) * 3
     assert conf.cell_size == pytest.approx(1000 / 1024)
     assert conf.mesh_size == 1024 ** 3 and conf.ptcl_num == 512 ** 3
-    assert conf.brick == (32, 16, 32)
+    assert conf.brick == (16, 16, 32)
     with pytest.raises(ValueError, match="cannot be smaller"):
         Configuration(1.0, (8, 8, 8), mesh_shape=(4, 8, 8))
     with pytest.raises(ValueError, match="aspect ratios"):
