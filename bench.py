@@ -117,17 +117,18 @@ def strict_step_bytes(n_p, L):
 def measured_traffic(kernel, config):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary, if any.
 
-    ``profiles/traffic_<config>.txt`` holds lines ``<kernel> <FETCH KiB> <WRITE KiB> <factor>``
-    (per dispatch, averaged; factor = copy-kernel calibration of the counters).
+    ``profiles/traffic_<config>.txt`` holds lines ``<name> <fetch KiB> <fetch factor> <write KiB>
+    <write factor>`` per dispatch, where each factor is what the counter under-reads a plain
+    1 GiB device copy by (bytes moved = KiB * 1024 * factor).
     """
     path = os.path.join(ROOT, "profiles", f"traffic_{config}.txt")
     try:
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 4 and parts[0] == kernel:
-                    fetch, write, factor = (float(v) for v in parts[1:])
-                    return round((fetch + write) * 1024 * factor)
+                if len(parts) == 5 and parts[0] == kernel:
+                    fk, ff, wk, wf = (float(v) for v in parts[1:])
+                    return round((fk * ff + wk * wf) * 1024)
     except (OSError, ValueError):
         return None
     return None

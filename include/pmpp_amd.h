@@ -169,6 +169,26 @@ int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid,
                    const pm_gravity_desc* desc, const float* acc_cot, float* disp_cot,
                    float* omega_cot, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------ N-body updates ----
+ * Elementwise stages of the symplectic integrator on (n,3) float32 arrays (16-byte aligned):
+ * pm_kick   vel  += factor * acc    replaces kick              (nbody/integrator.py:512-538)
+ * pm_drift  disp += factor * vel    replaces drift             (nbody/integrator.py:203-237)
+ * Their reverse passes reconstruct the input state from the output and pull cotangents:
+ * pm_kick_adj   vel -= factor*acc;  acc_cot += factor*vel_cot;  *dot = sum(vel_cot*acc)
+ *               replaces kick_adj                               (nbody/integrator.py:541-582)
+ * pm_drift_adj  disp -= factor*vel; vel_cot += factor*disp_cot; *dot = sum(disp_cot*vel)
+ *               replaces drift_adj_from_output                  (nbody/integrator.py:429-474)
+ * *dot (a device double) is the factor's cotangent, which the caller chains into the
+ * cosmology cotangent; it is reduced deterministically in a workspace of
+ * pm_nbody_adj_workspace bytes.  acc_cot may be NULL (not accumulated). */
+int pm_kick(float* vel, const float* acc, int64_t n, float factor, void* stream);
+int pm_drift(float* disp, const float* vel, int64_t n, float factor, void* stream);
+int pm_nbody_adj_workspace(size_t* bytes);
+int pm_kick_adj(float* vel, const float* acc, const float* vel_cot, float* acc_cot, int64_t n, float factor,
+                double* dot, void* workspace, size_t workspace_bytes, void* stream);
+int pm_drift_adj(float* disp, const float* vel, const float* disp_cot, float* vel_cot, int64_t n, float factor,
+                 double* dot, void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

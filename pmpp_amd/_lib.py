@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "pm_fft_workspace", "pm_rfft3d", "pm_irfft3d",
     "pm_spectral_workspace", "pm_spectral_force_multiply",
     "pm_gravity_workspace", "pm_gravity_vjp_workspace", "pm_gravity", "pm_gravity_vjp",
+    "pm_kick", "pm_drift", "pm_nbody_adj_workspace", "pm_kick_adj", "pm_drift_adj",
 )
 
 
@@ -96,6 +97,11 @@ _SIGNATURES = {
                    ctypes.c_int),
     "pm_gravity_vjp": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
                        ctypes.c_int),
+    "pm_kick": ([_vp, _vp, _i64, _f32, _vp], ctypes.c_int),
+    "pm_drift": ([_vp, _vp, _i64, _f32, _vp], ctypes.c_int),
+    "pm_nbody_adj_workspace": ([_sz_p], ctypes.c_int),
+    "pm_kick_adj": ([_vp, _vp, _vp, _vp, _i64, _f32, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
+    "pm_drift_adj": ([_vp, _vp, _vp, _vp, _i64, _f32, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
 }
 
 _LIB = None

@@ -34,6 +34,12 @@ int fail(int code, const std::string& msg);
     if (cond) return ::pm::fail((code), (msg));                                        \
   } while (0)
 
+#define PM_TRY(expr)     \
+  do {                   \
+    int _st = (expr);    \
+    if (_st) return _st; \
+  } while (0)
+
 int ensure_device_tables();  // twiddle tables for the current device
 
 // Optional per-kernel timing (pm_timing_* in the ABI): when enabled on the calling host
@@ -41,7 +47,7 @@ int ensure_device_tables();  // twiddle tables for the current device
 void timing_mark(const char* name, hipStream_t s);
 #define PM_MARK(name, stream) ::pm::timing_mark((name), (stream))
 
-// launch check + timing mark (name must be a string literal)
+// launch check + timing mark (name must have static storage: the timer keeps the pointer)
 #define PM_LAUNCHED(what, stream) \
   do {                            \
     PM_LAUNCH_CHECK(what);        \

@@ -801,12 +801,6 @@ int fill(float* p, int64_t n, hipStream_t s) {
   return PM_SUCCESS;
 }
 
-#define PM_TRY(expr)        \
-  do {                      \
-    int _st = (expr);       \
-    if (_st) return _st;    \
-  } while (0)
-
 // Sort, payload, density.  Leaves rho in b.r0.
 int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                   const pm_gravity_desc* d, const Layout& L, Buffers& b, Arena& ws, hipStream_t s) {

@@ -34,6 +34,14 @@ class LCDM:
     n_s: float = 0.96
     sigma8: float = 0.8
     T_cmb: float = 2.7255
+    # curvature and w0-wa dark energy of the background (cosmology/models.py:16-52)
+    Omega_k: float = 0.0
+    w_0: float = -1.0
+    w_a: float = 0.0
+
+    @property
+    def Omega_de(self) -> float:
+        return 1.0 - self.Omega_m - self.Omega_k
 
 
 def transfer_eh_nowiggle(k_hmpc, c: LCDM):
