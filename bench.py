@@ -51,8 +51,8 @@ def kernel_step_bytes(name, L, n_p, brick):
 
     R = real float32 mesh; S1 / S2 = the complex64 spectrum band the force passes keep
     (kz < nkz; all y rows / the packed |ky| band), pitch excluded; n_p particles of 18 B
-    (int16 pmid + float32 disp).  Brick shells are the upper faces of every brick that
-    the tiled scatter hands to the halo fold; tile reads of the gather count the mesh once.
+    (int16 pmid + float32 disp).  The tiled scatter hands the upper x/y planes of each brick
+    column to the fold; tile reads of the gather count the mesh once.
     """
     R = 4 * L["nreal"]
     Ch = 8 * L["nx"] * L["ny"] * (L["nz"] // 2 + 1)
@@ -60,8 +60,9 @@ def kernel_step_bytes(name, L, n_p, brick):
     S2 = 8 * L["nx"] * L["nky"] * L["nkz"]  # banded spectrum, packed y band
     passes, nb = sort_passes((L["nx"], L["ny"], L["nz"]), brick)
     b0, b1, b2 = brick
-    shell = 4 * nb * ((b1 + 1) * (b2 + 1) + b0 * (b2 + 1) + b0 * b1)
-    faces = 8 * nb * (b1 * b2 + (b0 - 1) * b2 + (b0 - 1) * (b1 - 1))  # lower faces read + written
+    nb0, nb1 = L["nx"] // b0, L["ny"] // b1
+    planes = 4 * nb0 * nb1 * (b1 + 1 + b0) * L["nz"]  # upper x/y planes of every brick column
+    fold = 8 * (nb0 * L["ny"] + (L["nx"] - nb0) * nb1) * L["nz"]  # lower x/y plane cells read + written
     table = {
         "fft_z_r2c": R + S1,
         "fft_cols_fwd": S1 + S2,
@@ -73,9 +74,9 @@ def kernel_step_bytes(name, L, n_p, brick):
         "k_digit_scatter": (26 + 16 * (passes - 1)) * n_p,
         "k_bin_bounds": 4 * n_p + 4 * (nb + 2),
         "k_payload": (4 + 18 + 18 + 1) * n_p,
-        "k_scatter_tiles": 18 * n_p + R + shell,
-        "k_scatter_halo": shell + faces,
-        "k_gather3_tiles": (18 + 4 + 12) * n_p + 3 * R,
+        "k_scatter_columns": 18 * n_p + R + planes,
+        "k_fold_xy": planes + fold,
+        "k_gather_columns": (18 + 4 + 12) * n_p + 3 * R,
         "k_fill": R,
         "k_scatter_sorted": 19 * n_p + R,
         "k_gather3_sorted": (19 + 4 + 12) * n_p + 3 * R,

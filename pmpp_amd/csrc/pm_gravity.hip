@@ -98,17 +98,22 @@ __global__ void __launch_bounds__(kBlock) k_gather3_sorted(const int16_t* __rest
 // ----------------------------------------------------------------------------------------
 // Brick-tiled CIC on sorted particles.  The mesh is tiled by bricks of (BX,BY,BZ) cells; the
 // cell sort groups particles by the brick of their base (corner-0) cell, so every corner a
-// particle touches lies in its brick's tile extended by one cell on the upper side.
+// particle touches lies in its brick's tile extended by one cell on the upper side.  One
+// workgroup walks a column of bricks (fixed x and y brick coordinates) along z.
 //
-// Scatter (owner computes): one workgroup per brick accumulates its particles into an LDS
-// tile of (BX+1)(BY+1)(BZ+1) floats with LDS atomics, then stores the BX*BY*BZ owned cells
-// with plain stores (every mesh cell is written exactly once; no zero fill, no global
-// atomics) and the upper halo shell into a per-brick shell buffer.  A second pass folds the
-// shells of the 7 lower neighbours into each brick's lower faces (pull, deterministic).
-// Gather: one workgroup per brick stages each force component's extended tile in LDS and
-// interpolates its particles from there.
+// Scatter (owner computes): the column's bricks are accumulated one after the other into an
+// LDS tile of (BX+1)(BY+1)(BZ+1) int64 fixed-point cells with LDS atomics.  The tile's upper z
+// plane belongs to the next brick of the column and is carried to it inside LDS (the first
+// brick's lower plane waits for the last brick's carry), so the z halo never leaves the CU.
+// Owned cells are stored once with plain stores (no zero fill, no global atomics); the upper
+// x and y planes go to z-contiguous plane buffers (XS, YS) that k_fold_xy adds into the lower
+// x/y planes of the neighbouring columns (pull, fixed order: deterministic).
+// Gather: the column is walked downward in z.  Each force component's tile rows z in [0, BZ)
+// are read as aligned rows; the z = BZ plane is the lower plane of the brick above, kept in
+// LDS from the previous step, so no row costs a second cache line.
 constexpr int kTileMaxFloats = 19100;  // gather f32 tile: (32+1)*(16+1)*(32+2) = 19074 fits
 constexpr int kTileMaxWords = 9900;    // scatter int64 tile: (16+1)*(16+1)*(32+2) = 9826 (78.6 KB: 2 WG/CU)
+constexpr int kPlaneMax = 1100;        // (b0+1)*(b1+1) cells of one z plane of a tile
 // Scatter accumulation is exact 64-bit fixed point (2^-32 units): gfx950's LDS integer atomics run
 // an order of magnitude faster than its LDS float atomics, and integer sums are order independent,
 // so the density is deterministic; each cell is rounded to float32 once when stored.
@@ -119,8 +124,6 @@ struct Bricks {
   int n[3];   // mesh shape
   int b[3];   // brick shape
   int nb[3];  // bricks per axis
-  int t[3];   // tile shape b+1
-  int fx, fy, fz, sh;  // shell face sizes and total
 };
 
 inline Bricks make_bricks(const int32_t* mesh, const int32_t* brick) {
@@ -129,12 +132,7 @@ inline Bricks make_bricks(const int32_t* mesh, const int32_t* brick) {
     k.n[a] = mesh[a];
     k.b[a] = brick[a];
     k.nb[a] = mesh[a] / brick[a];
-    k.t[a] = brick[a] + 1;
   }
-  k.fx = (k.b[1] + 1) * (k.b[2] + 1);
-  k.fy = k.b[0] * (k.b[2] + 1);
-  k.fz = k.b[0] * k.b[1];
-  k.sh = k.fx + k.fy + k.fz;
   return k;
 }
 
@@ -142,23 +140,24 @@ inline bool tiled_ok(const int32_t* mesh, const int32_t* brick) {
   for (int a = 0; a < 3; ++a)
     if (brick[a] < 1 || mesh[a] % brick[a] != 0) return false;
   // LDS tile rows have a z pitch of b2 + 2 words
-  const int64_t words = (int64_t)(brick[0] + 1) * (brick[1] + 1) * (brick[2] + 2);
-  return words <= kTileMaxFloats && words <= kTileMaxWords;
+  const int64_t plane = (int64_t)(brick[0] + 1) * (brick[1] + 1);
+  const int64_t words = plane * (brick[2] + 2);
+  return words <= kTileMaxFloats && words <= kTileMaxWords && plane <= kPlaneMax;
 }
 
 inline bool is_brick(const Bricks& k, int x, int y, int z) { return k.b[0] == x && k.b[1] == y && k.b[2] == z; }
 
-// index of an upper-shell cell (x == bx or y == by or z == bz) in a brick's shell block
-__device__ __forceinline__ int shell_index(const Bricks& k, int x, int y, int z) {
-  if (x == k.b[0]) return y * (k.b[2] + 1) + z;
-  if (y == k.b[1]) return k.fx + x * (k.b[2] + 1) + z;
-  return k.fx + k.fy + x * k.b[1] + y;
-}
+// Upper-plane buffers of the tiled scatter, z-contiguous:
+//   XS[(col * (b1+1) + ly) * nz + z]  the x == b0 plane of column col (ly in [0, b1])
+//   YS[(col * b0 + lx) * nz + z]      the y == b1 plane of column col (lx in [0, b0))
+inline int64_t xs_floats(const Bricks& k) { return (int64_t)k.nb[0] * k.nb[1] * (k.b[1] + 1) * k.n[2]; }
+inline int64_t ys_floats(const Bricks& k) { return (int64_t)k.nb[0] * k.nb[1] * k.b[0] * k.n[2]; }
 
-__device__ __forceinline__ void brick_coords(const Bricks& k, int64_t bid, int c[3]) {
-  c[2] = (int)(bid % k.nb[2]);
-  c[1] = (int)((bid / k.nb[2]) % k.nb[1]);
-  c[0] = (int)(bid / ((int64_t)k.nb[2] * k.nb[1]));
+// Workgroups are dispatched round-robin over the 8 XCDs; logical column q runs on workgroup
+// (q % g8) * 8 + q / g8 so neighbouring columns (which share halo rows) share an L2.
+__device__ __forceinline__ int column_of(int L, int G) {
+  if (G % 8 != 0) return L;
+  return (L % 8) * (G / 8) + L / 8;
 }
 
 constexpr int kTileThreads = 512;
@@ -169,7 +168,8 @@ constexpr int kBatch = 8;  // independent loads kept in flight per thread in the
 //
 // LDS tile layout: tile[(x * t1 + y) * pitch + z], z in [0, b2], pitch = b2 + 2, so every
 // corner of a particle is its corner-0 address plus a compile-time offset, and rows are
-// 8-byte aligned for float2 LDS stores when b2 is even.
+// 8-byte aligned for float2 LDS stores when b2 is even.  The spare word z = b2 + 1 of each
+// scatter row holds the z carry.
 template <int BX, int BY, int BZ>
 struct BrickShape {
   int b0, b1, b2;
@@ -222,199 +222,197 @@ __device__ __forceinline__ float frac_weight(const float f[3], int c) {
   return wx * wy * wz;
 }
 
+__device__ __forceinline__ float unfix(unsigned long long v) { return (float)(long long)v * kUnfix; }
+
 template <int BX, int BY, int BZ>
-__global__ void __launch_bounds__(kTileThreads) k_scatter_tiles(
+__global__ void __launch_bounds__(kTileThreads) k_scatter_columns(
     const int16_t* __restrict__ pmid, const float* __restrict__ disp, const int32_t* __restrict__ bin_off,
-    float val, Geom g, Bricks k, float* __restrict__ mesh, float* __restrict__ shell) {
+    float val, Geom g, Bricks k, float* __restrict__ mesh, float* __restrict__ xs, float* __restrict__ ys,
+    unsigned long long* __restrict__ first) {
   __shared__ __align__(16) unsigned long long tile[kTileMaxWords];
   const BrickShape<BX, BY, BZ> B(k);
-  const int64_t bid = blockIdx.x;
-  int bc[3];
-  brick_coords(k, bid, bc);
-  const int o[3] = {bc[0] * B.b0, bc[1] * B.b1, bc[2] * B.b2};
-  const int tf = B.tfloats();
+  const int col = column_of(blockIdx.x, gridDim.x);
+  const int cx = col / k.nb[1], cy = col - cx * k.nb[1];
+  const int P = B.pitch(), rows = B.rows(), carry = B.b2 + 1, tf = B.tfloats();
+  const int64_t n1 = k.n[1], n2 = k.n[2];
   for (int i = threadIdx.x; i < tf; i += kTileThreads) tile[i] = 0ull;
   __syncthreads();
-  const int64_t j0 = bin_off[bid], j1 = bin_off[bid + 1];
-  constexpr int PPT = 4;
-  for (int64_t c0 = j0; c0 < j1; c0 += (int64_t)kTileThreads * PPT) {
-    int bs[PPT][3];
-    float fr[PPT][3];
-    bool act[PPT];
+  for (int bz = 0; bz < k.nb[2]; ++bz) {
+    const int o[3] = {cx * B.b0, cy * B.b1, bz * B.b2};
+    const int64_t bid = (int64_t)col * k.nb[2] + bz;
+    const int64_t j0 = bin_off[bid], j1 = bin_off[bid + 1];
+    constexpr int PPT = 4;
+    for (int64_t c0 = j0; c0 < j1; c0 += (int64_t)kTileThreads * PPT) {
+      int bs[PPT][3];
+      float fr[PPT][3];
+      bool act[PPT];
 #pragma unroll
-    for (int q = 0; q < PPT; ++q) {  // all loads of the chunk first (branch-free, clamped index)
-      int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
-      act[q] = j < j1;
-      base_frac(pmid, disp, act[q] ? j : j1 - 1, g.a1, k.n, bs[q], fr[q]);
-    }
+      for (int q = 0; q < PPT; ++q) {  // all loads of the chunk first (branch-free, clamped index)
+        int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
+        act[q] = j < j1;
+        base_frac(pmid, disp, act[q] ? j : j1 - 1, g.a1, k.n, bs[q], fr[q]);
+      }
 #pragma unroll
-    for (int q = 0; q < PPT; ++q) {
-      if (!act[q]) continue;
-      const int c0i = local_cell(B, bs[q], o);
+      for (int q = 0; q < PPT; ++q) {
+        if (!act[q]) continue;
+        const int c0i = local_cell(B, bs[q], o);
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
-        atomicAdd(&tile[c0i + B.corner_off(c)],
-                  (unsigned long long)__float2ll_rn(val * frac_weight(fr[q], c) * kFix));
-    }
-  }
-  __syncthreads();
-  // owned cells x < b0, y < b1, z < b2: main rows, moved as float4 when aligned
-  if constexpr (BrickShape<BX, BY, BZ>::kVec) {
-    constexpr int C4 = BZ / 4;
-    constexpr int ITEMS = BX * BY * C4;
-    for (int it = threadIdx.x; it < ITEMS; it += kTileThreads) {
-      int r = it / C4, c4 = it - r * C4;
-      int x = r / BY, y = r - x * BY;
-      const long long* src = reinterpret_cast<const long long*>(&tile[B.cell(x, y, 4 * c4)]);
-      longlong2 lo = *reinterpret_cast<const longlong2*>(src), hi = *reinterpret_cast<const longlong2*>(src + 2);
-      *reinterpret_cast<float4*>(&mesh[((int64_t)(o[0] + x) * k.n[1] + (o[1] + y)) * k.n[2] + o[2] + 4 * c4]) =
-          make_float4((float)lo.x * kUnfix, (float)lo.y * kUnfix, (float)hi.x * kUnfix, (float)hi.y * kUnfix);
-    }
-  } else {
-    const int rows = B.b0 * B.b1;
-    for (int i = threadIdx.x; i < rows * B.b2; i += kTileThreads) {
-      int r = i / B.b2, z = i - r * B.b2;
-      int x = r / B.b1, y = r - x * B.b1;
-      mesh[((int64_t)(o[0] + x) * k.n[1] + (o[1] + y)) * k.n[2] + (o[2] + z)] =
-          (float)(long long)tile[B.cell(x, y, z)] * kUnfix;
-    }
-  }
-  // upper shell: face x = b0 (t1*t2), face y = b1 (b0*t2), face z = b2 (b0*b1)
-  float* sh = shell + bid * k.sh;
-  const int t2 = B.t2();
-  const int fx = B.t1() * t2, fy = B.b0 * t2, fz = B.b0 * B.b1;
-  for (int i = threadIdx.x; i < fx + fy + fz; i += kTileThreads) {
-    int x, y, z;
-    if (i < fx) {
-      x = B.b0;
-      y = i / t2;
-      z = i - y * t2;
-    } else if (i < fx + fy) {
-      int r = i - fx;
-      y = B.b1;
-      x = r / t2;
-      z = r - x * t2;
-    } else {
-      int r = i - fx - fy;
-      z = B.b2;
-      x = r / B.b1;
-      y = r - x * B.b1;
-    }
-    sh[i] = (float)(long long)tile[B.cell(x, y, z)] * kUnfix;
-  }
-}
-
-// Fold the upper shells of the 7 lower neighbours into this brick's lower faces.
-template <int BX, int BY, int BZ>
-__global__ void __launch_bounds__(kTileThreads) k_scatter_halo(Bricks k, const float* __restrict__ shell,
-                                                                float* __restrict__ mesh) {
-  const BrickShape<BX, BY, BZ> B(k);
-  const int64_t bid = blockIdx.x;
-  int bc[3];
-  brick_coords(k, bid, bc);
-  const int o[3] = {bc[0] * B.b0, bc[1] * B.b1, bc[2] * B.b2};
-  int64_t nbr[8];  // shell block of the lower neighbour bid - (dx,dy,dz)
-#pragma unroll
-  for (int d = 1; d < 8; ++d) {
-    int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-    int sb0 = bc[0] - dx, sb1 = bc[1] - dy, sb2 = bc[2] - dz;
-    sb0 += sb0 < 0 ? k.nb[0] : 0;
-    sb1 += sb1 < 0 ? k.nb[1] : 0;
-    sb2 += sb2 < 0 ? k.nb[2] : 0;
-    nbr[d] = (((int64_t)sb0 * k.nb[1] + sb1) * k.nb[2] + sb2) * k.sh;
-  }
-  const int t2 = B.t2();
-  const int fxs = B.t1() * t2, fys = B.b0 * t2;
-  // lower-face cells: x==0 (b1*b2), then y==0 & x>0 ((b0-1)*b2), then z==0 & x>0 & y>0
-  const int f0 = B.b1 * B.b2;
-  const int f1 = (B.b0 - 1) * B.b2;
-  const int f2 = (B.b0 - 1) * (B.b1 - 1);
-  for (int i = threadIdx.x; i < f0 + f1 + f2; i += kTileThreads) {
-    int x, y, z;
-    if (i < f0) {
-      x = 0;
-      y = i / B.b2;
-      z = i - y * B.b2;
-    } else if (i < f0 + f1) {
-      int r = i - f0;
-      y = 0;
-      x = 1 + r / B.b2;
-      z = r - (x - 1) * B.b2;
-    } else {
-      int r = i - f0 - f1;
-      z = 0;
-      x = 1 + r / (B.b1 - 1);
-      y = 1 + r - (x - 1) * (B.b1 - 1);
-    }
-    // all 7 candidate shell values are loaded unconditionally so the loads overlap; a neighbour
-    // that does not touch this cell re-reads the first one that does (same cache line) and
-    // counts zero
-    const int dp = x == 0 ? 1 : (y == 0 ? 2 : 4);
-    auto shell_at = [&](int d) {
-      int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-      int sx = x + dx * B.b0, sy = y + dy * B.b1, sz = z + dz * B.b2;
-      int si = (sx == B.b0) ? sy * t2 + sz : (sy == B.b1) ? fxs + sx * t2 + sz : fxs + fys + sx * B.b1 + sy;
-      return nbr[d] + si;
-    };
-    const int64_t prim = shell_at(dp);
-    float sv[8];
-#pragma unroll
-    for (int d = 1; d < 8; ++d) {
-      int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-      bool use = !((dx && x != 0) || (dy && y != 0) || (dz && z != 0));
-      float val = shell[use ? shell_at(d) : prim];
-      sv[d] = use ? val : 0.f;
-    }
-    int64_t flat = ((int64_t)(o[0] + x) * k.n[1] + (o[1] + y)) * k.n[2] + (o[2] + z);
-    float m = mesh[flat];
-    float acc = 0.f;
-#pragma unroll
-    for (int d = 1; d < 8; ++d) acc += sv[d];
-    mesh[flat] = m + acc;
-  }
-}
-
-// Stage one force component's extended tile (b+1)^3 in LDS.
-template <int BX, int BY, int BZ>
-__device__ __forceinline__ void load_tile(float* tile, const float* __restrict__ F, const BrickShape<BX, BY, BZ>& B,
-                                          const Bricks& k, const int o[3]) {
-  if constexpr (BrickShape<BX, BY, BZ>::kVec) {
-    constexpr int C4 = BZ / 4;
-    constexpr int ROWS = (BX + 1) * (BY + 1);
-    constexpr int ITEMS = ROWS * C4;
-    // float4 loads in flight per thread per round: the whole tile in one round when that
-    // takes at most 6 per thread (5 for a 16x16x32 brick)
-    constexpr int ONE = (ITEMS + kTileThreads - 1) / kTileThreads;
-    constexpr int ROUND = ONE <= 6 ? ONE : 4;
-    // z == b2 column (first cell of the next brick along z, wrapped): issued first, stored last
-    constexpr int NZ = (ROWS + kTileThreads - 1) / kTileThreads;
-    float zc[NZ];
-    {
-      int gz = o[2] + BZ;
-      gz -= gz >= k.n[2] ? k.n[2] : 0;
-#pragma unroll
-      for (int u = 0; u < NZ; ++u) {
-        int r = min((int)threadIdx.x + u * kTileThreads, ROWS - 1);
-        int x = r / (BY + 1), y = r - x * (BY + 1);
-        int gx = o[0] + x, gy = o[1] + y;
-        gx -= gx >= k.n[0] ? k.n[0] : 0;
-        gy -= gy >= k.n[1] ? k.n[1] : 0;
-        zc[u] = F[((int64_t)gx * k.n[1] + gy) * k.n[2] + gz];
+        for (int c = 0; c < 8; ++c)
+          atomicAdd(&tile[c0i + B.corner_off(c)],
+                    (unsigned long long)__float2ll_rn(val * frac_weight(fr[q], c) * kFix));
       }
     }
+    __syncthreads();
+    // z carry: the plane z = b2 is the next brick's z = 0 plane
+    for (int r = threadIdx.x; r < rows; r += kTileThreads) {
+      unsigned long long* row = tile + r * P;
+      const unsigned long long cin = row[carry];
+      row[carry] = row[B.b2];
+      row[0] += cin;
+      if (bz == 0) first[(int64_t)col * rows + r] = row[0];  // completed after the last brick
+    }
+    __syncthreads();
+    // owned cells x < b0, y < b1, z < b2, then the x == b0 and y == b1 planes (z < b2)
+    if constexpr (BrickShape<BX, BY, BZ>::kVec) {
+      constexpr int C4 = BZ / 4;
+      constexpr int OWN = BX * BY * C4, XP = (BY + 1) * C4, YP = BX * C4;
+      for (int it = threadIdx.x; it < OWN + XP + YP; it += kTileThreads) {
+        int x, y, c4;
+        float* dst;
+        if (it < OWN) {
+          int r = it / C4;
+          c4 = it - r * C4;
+          x = r / BY;
+          y = r - x * BY;
+          dst = &mesh[((int64_t)(o[0] + x) * n1 + (o[1] + y)) * n2 + o[2] + 4 * c4];
+        } else if (it < OWN + XP) {
+          int r = (it - OWN) / C4;
+          c4 = (it - OWN) - r * C4;
+          x = BX;
+          y = r;
+          dst = &xs[((int64_t)col * (BY + 1) + y) * n2 + o[2] + 4 * c4];
+        } else {
+          int r = (it - OWN - XP) / C4;
+          c4 = (it - OWN - XP) - r * C4;
+          x = r;
+          y = BY;
+          dst = &ys[((int64_t)col * BX + x) * n2 + o[2] + 4 * c4];
+        }
+        const unsigned long long* src = &tile[B.cell(x, y, 4 * c4)];
+        longlong2 lo = *reinterpret_cast<const longlong2*>(src), hi = *reinterpret_cast<const longlong2*>(src + 2);
+        *reinterpret_cast<float4*>(dst) = make_float4((float)lo.x * kUnfix, (float)lo.y * kUnfix,
+                                                      (float)hi.x * kUnfix, (float)hi.y * kUnfix);
+      }
+    } else {
+      const int own = B.b0 * B.b1 * B.b2, xp = B.t1() * B.b2, yp = B.b0 * B.b2;
+      for (int i = threadIdx.x; i < own + xp + yp; i += kTileThreads) {
+        if (i < own) {
+          int r = i / B.b2, z = i - r * B.b2;
+          int x = r / B.b1, y = r - x * B.b1;
+          mesh[((int64_t)(o[0] + x) * n1 + (o[1] + y)) * n2 + (o[2] + z)] = unfix(tile[B.cell(x, y, z)]);
+        } else if (i < own + xp) {
+          int r = (i - own) / B.b2, z = (i - own) - r * B.b2;
+          xs[((int64_t)col * B.t1() + r) * n2 + o[2] + z] = unfix(tile[B.cell(B.b0, r, z)]);
+        } else {
+          int r = (i - own - xp) / B.b2, z = (i - own - xp) - r * B.b2;
+          ys[((int64_t)col * B.b0 + r) * n2 + o[2] + z] = unfix(tile[B.cell(r, B.b1, z)]);
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < tf; i += kTileThreads)
+      if (i % P != carry) tile[i] = 0ull;
+    __syncthreads();
+  }
+  // the first brick's z = 0 plane: its own part plus the carry of the last brick (periodic)
+  for (int r = threadIdx.x; r < rows; r += kTileThreads) {
+    const int x = r / B.t1(), y = r - x * B.t1();
+    const float v = unfix(first[(int64_t)col * rows + r] + tile[r * P + carry]);
+    if (x < B.b0 && y < B.b1)
+      mesh[((int64_t)(cx * B.b0 + x) * n1 + (cy * B.b1 + y)) * n2] = v;
+    else if (x == B.b0)
+      xs[((int64_t)col * B.t1() + y) * n2] = v;
+    else
+      ys[((int64_t)col * B.b0 + x) * n2] = v;
+  }
+}
+
+// Add the upper x/y planes of the neighbouring columns into the lower x/y planes of each
+// column.  A cell with lx = X % b0 and ly = Y % b1 receives
+//   lx == 0: XS of column (cx-1, cy) row ly; and when ly == 0 also XS of (cx-1, cy-1) row b1
+//   ly == 0: YS of column (cx, cy-1) row lx
+// (columns periodic).  One thread per z element, z fastest: every access is coalesced.
+__global__ void __launch_bounds__(kBlock) k_fold_xy(Bricks k, const float* __restrict__ xs,
+                                                    const float* __restrict__ ys, float* __restrict__ mesh) {
+  const int b0 = k.b[0], b1 = k.b[1];
+  const int nb0 = k.nb[0], nb1 = k.nb[1];
+  const int64_t nx = k.n[0], ny = k.n[1], nz = k.n[2];
+  const int64_t nxp = (int64_t)nb0 * ny * nz;        // x-plane cells
+  const int64_t nyp = (nx - nb0) * nb1 * nz;         // y-plane cells with lx != 0
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nxp + nyp;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t X, Y, Z;
+    float add;
+    if (i < nxp) {
+      Z = i % nz;
+      int64_t t = i / nz;
+      Y = t % ny;
+      const int cxp = (int)(t / ny);
+      X = (int64_t)cxp * b0;
+      const int cy = (int)(Y / b1), ly = (int)(Y - (int64_t)cy * b1);
+      const int sx = cxp == 0 ? nb0 - 1 : cxp - 1;
+      add = xs[(((int64_t)sx * nb1 + cy) * (b1 + 1) + ly) * nz + Z];
+      if (ly == 0) {
+        const int sy = cy == 0 ? nb1 - 1 : cy - 1;
+        add += xs[(((int64_t)sx * nb1 + sy) * (b1 + 1) + b1) * nz + Z];
+        add += ys[(((int64_t)cxp * nb1 + sy) * b0) * nz + Z];
+      }
+    } else {
+      const int64_t j = i - nxp;
+      Z = j % nz;
+      int64_t t = j / nz;
+      const int64_t xr = t % (nx - nb0);
+      const int cyp = (int)(t / (nx - nb0));
+      const int cx = (int)(xr / (b0 - 1)), lx = 1 + (int)(xr - (int64_t)cx * (b0 - 1));
+      X = (int64_t)cx * b0 + lx;
+      Y = (int64_t)cyp * b1;
+      const int sy = cyp == 0 ? nb1 - 1 : cyp - 1;
+      add = ys[(((int64_t)cx * nb1 + sy) * b0 + lx) * nz + Z];
+    }
+    const int64_t flat = (X * ny + Y) * nz + Z;
+    mesh[flat] = mesh[flat] + add;
+  }
+}
+
+// Particles of one brick processed per tile staging: their three interpolated components
+// wait in LDS, so each component tile is read from HBM once per brick unless the brick holds
+// more than kGatherCap particles (rare: ~2x the mean occupancy at 1/8 particle per cell).
+constexpr int kGatherCap = 2048;
+constexpr int kGatherPPT = kGatherCap / kTileThreads;
+constexpr int kGatherBatch = 2;  // particles whose loads are in flight together
+
+// Stage rows z in [0, b2) of one component's tile for brick (cx, cy, bz) (rows wrap in x, y).
+template <int BX, int BY, int BZ>
+__device__ __forceinline__ void stage_rows(float* tile, const float* __restrict__ F, const BrickShape<BX, BY, BZ>& B,
+                                           const Bricks& k, int cx, int cy, int bz) {
+  const int ox = cx * B.b0, oy = cy * B.b1, oz = bz * B.b2;
+  if constexpr (BrickShape<BX, BY, BZ>::kVec) {
+    constexpr int C4 = BZ / 4;
+    constexpr int ITEMS = (BX + 1) * (BY + 1) * C4;
+    constexpr int ONE = (ITEMS + kTileThreads - 1) / kTileThreads;
+    constexpr int ROUND = ONE <= 6 ? ONE : 4;
 #pragma unroll 1
     for (int base = 0; base < ITEMS; base += ROUND * kTileThreads) {
       float4 v[ROUND];
 #pragma unroll
-      for (int u = 0; u < ROUND; ++u) {  // all row loads of the round before any LDS store:
-        // branch-free (clamped index) so the compiler keeps every load in flight
+      for (int u = 0; u < ROUND; ++u) {  // unconditional (clamped) loads: the round stays in flight
         int it = min(base + (int)threadIdx.x + u * kTileThreads, ITEMS - 1);
         int r = it / C4, c4 = it - r * C4;
         int x = r / (BY + 1), y = r - x * (BY + 1);
-        int gx = o[0] + x, gy = o[1] + y;
+        int gx = ox + x, gy = oy + y;
         gx -= gx >= k.n[0] ? k.n[0] : 0;
         gy -= gy >= k.n[1] ? k.n[1] : 0;
-        v[u] = *reinterpret_cast<const float4*>(&F[((int64_t)gx * k.n[1] + gy) * k.n[2] + o[2] + 4 * c4]);
+        v[u] = *reinterpret_cast<const float4*>(&F[((int64_t)gx * k.n[1] + gy) * k.n[2] + oz + 4 * c4]);
       }
 #pragma unroll
       for (int u = 0; u < ROUND; ++u) {
@@ -427,29 +425,23 @@ __device__ __forceinline__ void load_tile(float* tile, const float* __restrict__
         }
       }
     }
-#pragma unroll
-    for (int u = 0; u < NZ; ++u) {
-      int r = (int)threadIdx.x + u * kTileThreads;
-      if (r < ROWS) tile[r * (BZ + 2) + BZ] = zc[u];
-    }
   } else {
-    const int t1 = B.t1(), t2 = B.t2();
-    const int tyz = t1 * t2, tvol = B.t0() * tyz;
-    for (int i0 = 0; i0 < tvol; i0 += kTileThreads * kBatch) {
+    const int t1 = B.t1(), b2 = B.b2, P = B.pitch();
+    const int items = B.rows() * b2;
+    for (int i0 = 0; i0 < items; i0 += kTileThreads * kBatch) {
       float v[kBatch];
       int dst[kBatch];
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {  // issue kBatch independent (branch-free) loads, then store
-        int i0u = i0 + u * kTileThreads + (int)threadIdx.x;
-        int i = min(i0u, tvol - 1);
-        int x = i / tyz, rem = i - x * tyz;
-        int y = rem / t2, z = rem - y * t2;
-        int gx = o[0] + x, gy = o[1] + y, gz = o[2] + z;
+      for (int u = 0; u < kBatch; ++u) {
+        int iu = i0 + u * kTileThreads + (int)threadIdx.x;
+        int i = min(iu, items - 1);
+        int r = i / b2, z = i - r * b2;
+        int x = r / t1, y = r - x * t1;
+        int gx = ox + x, gy = oy + y;
         gx -= gx >= k.n[0] ? k.n[0] : 0;
         gy -= gy >= k.n[1] ? k.n[1] : 0;
-        gz -= gz >= k.n[2] ? k.n[2] : 0;
-        v[u] = F[((int64_t)gx * k.n[1] + gy) * k.n[2] + gz];
-        dst[u] = i0u < tvol ? B.cell(x, y, z) : -1;
+        v[u] = F[((int64_t)gx * k.n[1] + gy) * k.n[2] + oz + z];
+        dst[u] = iu < items ? r * P + z : -1;
       }
 #pragma unroll
       for (int u = 0; u < kBatch; ++u)
@@ -458,66 +450,91 @@ __device__ __forceinline__ void load_tile(float* tile, const float* __restrict__
   }
 }
 
-// Particles of one brick processed per tile staging: their three interpolated components
-// wait in LDS, so each component tile is read from HBM once per brick unless the brick holds
-// more than kGatherCap particles (rare: ~2x the mean occupancy at 1/8 particle per cell).
-constexpr int kGatherCap = 2048;
-constexpr int kGatherPPT = kGatherCap / kTileThreads;
-constexpr int kGatherBatch = 2;  // particles whose loads are in flight together
+// The z = 0 plane of brick (cx, cy, bz) of one component, read directly (rows wrap in x, y).
+template <int BX, int BY, int BZ>
+__device__ __forceinline__ void load_plane(float* plane, const float* __restrict__ F, const BrickShape<BX, BY, BZ>& B,
+                                           const Bricks& k, int cx, int cy, int bz) {
+  const int t1 = B.t1();
+  for (int r = threadIdx.x; r < B.rows(); r += kTileThreads) {
+    int x = r / t1, y = r - x * t1;
+    int gx = cx * B.b0 + x, gy = cy * B.b1 + y;
+    gx -= gx >= k.n[0] ? k.n[0] : 0;
+    gy -= gy >= k.n[1] ? k.n[1] : 0;
+    plane[r] = F[((int64_t)gx * k.n[1] + gy) * k.n[2] + (int64_t)bz * B.b2];
+  }
+}
 
 template <int BX, int BY, int BZ>
-__global__ void __launch_bounds__(kTileThreads) k_gather3_tiles(
+__global__ void __launch_bounds__(kTileThreads) k_gather_columns(
     const int16_t* __restrict__ pmid, const float* __restrict__ disp, const int32_t* __restrict__ bin_off,
     const int32_t* __restrict__ perm, Geom g, Bricks k, const float* __restrict__ f0,
     const float* __restrict__ f1, const float* __restrict__ f2, float* __restrict__ acc) {
   constexpr int kLds = BX ? (BX + 1) * (BY + 1) * (BZ + 2) : kTileMaxFloats;
+  constexpr int kPl = BX ? (BX + 1) * (BY + 1) : kPlaneMax;
   __shared__ __align__(16) float tile[kLds];
   __shared__ float res[3][kGatherCap];
+  __shared__ float plane[2][3][kPl];  // [parity of bz][component]: lower z plane of brick bz
   const BrickShape<BX, BY, BZ> B(k);
-  const int64_t bid = blockIdx.x;
-  int bc[3];
-  brick_coords(k, bid, bc);
-  const int o[3] = {bc[0] * B.b0, bc[1] * B.b1, bc[2] * B.b2};
-  const int64_t j0 = bin_off[bid], j1 = bin_off[bid + 1];
-  for (int64_t s0 = j0; s0 < j1; s0 += kGatherCap) {
-    const int64_t s1 = s0 + kGatherCap < j1 ? s0 + kGatherCap : j1;
+  const int col = column_of(blockIdx.x, gridDim.x);
+  const int cx = col / k.nb[1], cy = col - cx * k.nb[1];
+  const int rows = B.rows(), P = B.pitch(), nb2 = k.nb[2];
+  // the brick above the top one is brick 0 (periodic)
+  load_plane(plane[nb2 & 1][0], f0, B, k, cx, cy, 0);
+  load_plane(plane[nb2 & 1][1], f1, B, k, cx, cy, 0);
+  load_plane(plane[nb2 & 1][2], f2, B, k, cx, cy, 0);
+  for (int bz = nb2 - 1; bz >= 0; --bz) {
+    const int o[3] = {cx * B.b0, cy * B.b1, bz * B.b2};
+    const int64_t bid = (int64_t)col * nb2 + bz;
+    const int64_t j0 = bin_off[bid], j1 = bin_off[bid + 1];
+    const int up = (bz + 1) & 1, me = bz & 1;
+    if (j0 == j1) {  // no particles: only this brick's lower planes are needed (by the brick below)
+      __syncthreads();
+      load_plane(plane[me][0], f0, B, k, cx, cy, bz);
+      load_plane(plane[me][1], f1, B, k, cx, cy, bz);
+      load_plane(plane[me][2], f2, B, k, cx, cy, bz);
+      continue;
+    }
+    for (int64_t s0 = j0; s0 < j1; s0 += kGatherCap) {
+      const int64_t s1 = s0 + kGatherCap < j1 ? s0 + kGatherCap : j1;
 #pragma unroll 1
-    for (int comp = 0; comp < 3; ++comp) {
-      const float* __restrict__ F = comp == 0 ? f0 : (comp == 1 ? f1 : f2);
-      __syncthreads();  // the previous component's interpolation is done with the tile
-      load_tile<BX, BY, BZ>(tile, F, B, k, o);
+      for (int comp = 0; comp < 3; ++comp) {
+        const float* __restrict__ F = comp == 0 ? f0 : (comp == 1 ? f1 : f2);
+        __syncthreads();  // the previous component's interpolation is done with the tile
+        stage_rows<BX, BY, BZ>(tile, F, B, k, cx, cy, bz);
+        for (int r = threadIdx.x; r < rows; r += kTileThreads) tile[r * P + B.b2] = plane[up][comp][r];
+        __syncthreads();
+        for (int r = threadIdx.x; r < rows; r += kTileThreads) plane[me][comp][r] = tile[r * P];
+#pragma unroll
+        for (int h = 0; h < kGatherPPT; h += kGatherBatch) {
+          int bs[kGatherBatch][3];
+          float fr[kGatherBatch][3];
+#pragma unroll
+          for (int u = 0; u < kGatherBatch; ++u) {  // the batch's particle loads first (clamped)
+            int64_t j = s0 + threadIdx.x + (int64_t)(h + u) * kTileThreads;
+            base_frac(pmid, disp, j < s1 ? j : s1 - 1, g.a1, k.n, bs[u], fr[u]);
+          }
+#pragma unroll
+          for (int u = 0; u < kGatherBatch; ++u) {
+            const int jl = threadIdx.x + (h + u) * kTileThreads;
+            if (s0 + jl >= s1) continue;
+            const int c0i = local_cell(B, bs[u], o);
+            float sum = 0.f;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) sum = sum + tile[c0i + B.corner_off(c)] * frac_weight(fr[u], c);
+            res[comp][jl] = sum;
+          }
+        }
+      }
       __syncthreads();
 #pragma unroll
-      for (int h = 0; h < kGatherPPT; h += kGatherBatch) {
-      int bs[kGatherBatch][3];
-      float fr[kGatherBatch][3];
-#pragma unroll
-      for (int u = 0; u < kGatherBatch; ++u) {  // the batch's particle loads first (clamped, branch-free)
-        int64_t j = s0 + threadIdx.x + (int64_t)(h + u) * kTileThreads;
-        base_frac(pmid, disp, j < s1 ? j : s1 - 1, g.a1, k.n, bs[u], fr[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < kGatherBatch; ++u) {
-        const int q = u;
-        const int jl = threadIdx.x + (h + u) * kTileThreads;
-        if (s0 + jl >= s1) continue;
-        const int c0i = local_cell(B, bs[q], o);
-        float sum = 0.f;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) sum = sum + tile[c0i + B.corner_off(c)] * frac_weight(fr[q], c);
-        res[comp][jl] = sum;
-      }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kGatherPPT; ++q) {
-      const int jl = threadIdx.x + q * kTileThreads;
-      if (s0 + jl < s1) {
-        int64_t i = perm[s0 + jl];
-        acc[i * 3 + 0] = res[0][jl];
-        acc[i * 3 + 1] = res[1][jl];
-        acc[i * 3 + 2] = res[2][jl];
+      for (int q = 0; q < kGatherPPT; ++q) {
+        const int jl = threadIdx.x + q * kTileThreads;
+        if (s0 + jl < s1) {
+          int64_t i = perm[s0 + jl];
+          acc[i * 3 + 0] = res[0][jl];
+          acc[i * 3 + 1] = res[1][jl];
+          acc[i * 3 + 2] = res[2][jl];
+        }
       }
     }
   }
@@ -713,7 +730,9 @@ struct Buffers {
   uint8_t* valid_s;
   float* tables;
   int32_t* bins;   // brick offsets of the sorted particles (tiled path)
-  float* shell;    // per-brick upper halo shells (tiled scatter)
+  float* xs;       // upper x / y planes of every brick column (tiled scatter)
+  float* ys;
+  unsigned long long* first;  // first brick's lower z plane per column (tiled scatter)
   float* r0;  // density, then F_x
   float* r1;  // F_y
   float* r2;  // F_z
@@ -733,12 +752,15 @@ struct Buffers {
 bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const int32_t* brick, bool reverse,
            Buffers& b) {
   b.bins = nullptr;
-  b.shell = nullptr;
+  b.xs = b.ys = nullptr;
+  b.first = nullptr;
   if (tiled_ok(shape, brick)) {
     Bricks k = make_bricks(shape, brick);
-    int64_t nbins = (int64_t)k.nb[0] * k.nb[1] * k.nb[2];
-    b.bins = ws.take<int32_t>(nbins + 2);
-    b.shell = ws.take<float>(nbins * k.sh);
+    int64_t ncols = (int64_t)k.nb[0] * k.nb[1];
+    b.bins = ws.take<int32_t>(ncols * k.nb[2] + 2);
+    b.xs = ws.take<float>(xs_floats(k));
+    b.ys = ws.take<float>(ys_floats(k));
+    b.first = ws.take<unsigned long long>(ncols * (k.b[0] + 1) * (k.b[1] + 1));
   }
   b.perm = ws.take<int32_t>(n);
   b.pmid_s = ws.take<int16_t>(3 * n);
@@ -813,20 +835,18 @@ int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, 
   }
   if (b.bins) {
     Bricks k = make_bricks(d->mesh_shape, d->brick);
-    const unsigned nbins = (unsigned)((int64_t)k.nb[0] * k.nb[1] * k.nb[2]);
-    if (is_brick(k, 16, 16, 32)) {
-      hipLaunchKernelGGL((k_scatter_tiles<16, 16, 32>), dim3(nbins), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
-                         b.bins, d->particle_value, g, k, b.r0, b.shell);
-      PM_LAUNCHED("k_scatter_tiles", s);
-      hipLaunchKernelGGL((k_scatter_halo<16, 16, 32>), dim3(nbins), dim3(kTileThreads), 0, s, k, b.shell, b.r0);
-      PM_LAUNCHED("k_scatter_halo", s);
-    } else {
-      hipLaunchKernelGGL((k_scatter_tiles<0, 0, 0>), dim3(nbins), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
-                         b.bins, d->particle_value, g, k, b.r0, b.shell);
-      PM_LAUNCHED("k_scatter_tiles", s);
-      hipLaunchKernelGGL((k_scatter_halo<0, 0, 0>), dim3(nbins), dim3(kTileThreads), 0, s, k, b.shell, b.r0);
-      PM_LAUNCHED("k_scatter_halo", s);
-    }
+    const unsigned ncols = (unsigned)((int64_t)k.nb[0] * k.nb[1]);
+    if (is_brick(k, 16, 16, 32))
+      hipLaunchKernelGGL((k_scatter_columns<16, 16, 32>), dim3(ncols), dim3(kTileThreads), 0, s, b.pmid_s,
+                         b.disp_s, b.bins, d->particle_value, g, k, b.r0, b.xs, b.ys, b.first);
+    else
+      hipLaunchKernelGGL((k_scatter_columns<0, 0, 0>), dim3(ncols), dim3(kTileThreads), 0, s, b.pmid_s,
+                         b.disp_s, b.bins, d->particle_value, g, k, b.r0, b.xs, b.ys, b.first);
+    PM_LAUNCHED("k_scatter_columns", s);
+    const int64_t plane_cells = ((int64_t)k.nb[0] * k.n[1] + (int64_t)(k.n[0] - k.nb[0]) * k.nb[1]) * k.n[2];
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(plane_cells, kBlock), 8192));
+    hipLaunchKernelGGL(k_fold_xy, dim3(blocks), dim3(kBlock), 0, s, k, b.xs, b.ys, b.r0);
+    PM_LAUNCHED("k_fold_xy", s);
     return PM_SUCCESS;
   }
   PM_TRY(fill(b.r0, L.nreal, s));
@@ -846,13 +866,14 @@ int gather_stage(const int16_t* pmid_s, const float* disp_s, const uint8_t* vali
   if (b.bins) {
     Bricks k = make_bricks(d->mesh_shape, d->brick);
     const int64_t nbins = (int64_t)k.nb[0] * k.nb[1] * k.nb[2];
+    const unsigned ncols = (unsigned)((int64_t)k.nb[0] * k.nb[1]);
     if (is_brick(k, 16, 16, 32))
-      hipLaunchKernelGGL((k_gather3_tiles<16, 16, 32>), dim3((unsigned)nbins), dim3(kTileThreads), 0, s, pmid_s,
-                         disp_s, b.bins, b.perm, g, k, fx, fy, fz, acc);
+      hipLaunchKernelGGL((k_gather_columns<16, 16, 32>), dim3(ncols), dim3(kTileThreads), 0, s, pmid_s, disp_s,
+                         b.bins, b.perm, g, k, fx, fy, fz, acc);
     else
-      hipLaunchKernelGGL((k_gather3_tiles<0, 0, 0>), dim3((unsigned)nbins), dim3(kTileThreads), 0, s, pmid_s,
-                         disp_s, b.bins, b.perm, g, k, fx, fy, fz, acc);
-    PM_LAUNCHED("k_gather3_tiles", s);
+      hipLaunchKernelGGL((k_gather_columns<0, 0, 0>), dim3(ncols), dim3(kTileThreads), 0, s, pmid_s, disp_s,
+                         b.bins, b.perm, g, k, fx, fy, fz, acc);
+    PM_LAUNCHED("k_gather_columns", s);
     if (valid) {
       unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), 4096);
       hipLaunchKernelGGL(k_zero_invalid, dim3(blocks), dim3(kBlock), 0, s, b.perm, b.bins, nbins, n, acc);
