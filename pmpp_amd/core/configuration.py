@@ -30,6 +30,7 @@ class Configuration:
     a_start: float = 1 / 64
     a_stop: float = 1.0
     a_nbody_maxstep: float = 1 / 64
+    a_lpt_maxstep: float = 1 / 128
     a_custom: Optional[Tuple[float, ...]] = None
     symp_splits: Tuple[Tuple[float, float], ...] = ((0, 0.5), (1, 0.5))
     lpt_order: int = 2
@@ -138,6 +139,23 @@ class Configuration:
         if self.a_custom is not None:
             return np.asarray(self.a_custom, dtype=np.float64)
         return np.linspace(self.a_start, self.a_stop, num=1 + self.a_nbody_num, dtype=np.float64)
+
+    @property
+    def a_lpt_num(self) -> int:
+        return math.ceil(self.a_start / self.a_lpt_maxstep)
+
+    @property
+    def a_lpt_step(self) -> float:
+        return self.a_start / self.a_lpt_num
+
+    @property
+    def a_lpt(self) -> np.ndarray:
+        return np.linspace(0.0, self.a_start, num=self.a_lpt_num + 1, dtype=np.float64)
+
+    @property
+    def growth_a(self) -> np.ndarray:
+        """Nodes of the growth table: LPT steps then N-body steps (configuration.py:558-563)."""
+        return np.concatenate((self.a_lpt, self.a_nbody[1:]))
 
 
 def default_brick(mesh_shape) -> Tuple[int, int, int]:

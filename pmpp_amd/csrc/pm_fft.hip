@@ -54,36 +54,37 @@ namespace fft {
 namespace {
 
 // ----------------------------------------------------------- column passes ----
-// T adjacent columns per workgroup half; two-output passes use two halves, each of which
-// loads the same input tile (the second read is served by L2), runs the forward transform
-// and inverts one of the two outputs, so no thread keeps both spectra live.
+// T adjacent columns per workgroup.  Two-output passes run two workgroups per tile (HALVES),
+// each of which loads the same input tile (the second read is served by L2: the pair runs
+// on one XCD at nearly the same time), runs the forward transform and inverts one of the
+// two outputs, so no thread keeps both spectra live; their outputs must not alias the input.
 template <int N, int T, int HALVES>
 struct ColCfg {
   static constexpr int P = Plan<N>::P;
-  static constexpr int HALF_THREADS = T * P;
-  static constexpr int THREADS = HALF_THREADS * HALVES;
+  static constexpr int THREADS = T * P;
   static constexpr int PADK = T;  // per-k1-block LDS pad: spreads the strided rows over banks
   static constexpr int LDS_FLOATS = N * T + Plan<N>::N1 * PADK;
 };
 
 // Column tiles are visited in an XCD-aware order: workgroups are dispatched round-robin
-// over the 8 XCDs, so logical tile q is given to workgroup (q % g8) * 8 + q / g8 ... i.e.
-// consecutive logical tiles (which share cache lines when T*8 < 128 bytes) run on the same
-// XCD at nearly the same time.
+// over the 8 XCDs, so logical item q is given to workgroup (q % g8) * 8 + q / g8 ... i.e.
+// consecutive logical items (the two halves of a tile, and neighbouring tiles, which share
+// cache lines when T*8 < 128 bytes) run on the same XCD at nearly the same time.
 template <int N, int T, int MODE, int HALVES>
 __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
     k_cols(ColPass io, SpecTables tb, int dax, const float2* __restrict__ tw, int gx, int gy, int g8) {
   using PL = Plan<N>;
   using CF = ColCfg<N, T, HALVES>;
   constexpr int E = PL::E, N2 = PL::N2;
-  __shared__ float lds[HALVES * CF::LDS_FLOATS];
+  __shared__ float lds[CF::LDS_FLOATS];
   const int L = blockIdx.x;
   const int q = (L % 8) * g8 + L / 8;
-  const int tilex = q % gx;
-  const int64_t b = q / gx;
+  const int half = HALVES > 1 ? q % HALVES : 0;
+  const int tile = HALVES > 1 ? q / HALVES : q;
+  const int tilex = tile % gx;
+  const int64_t b = tile / gx;
   if (b >= gy) return;  // padding workgroup of the XCD remap: uniform exit before any barrier
-  const int half = HALVES > 1 ? (int)(threadIdx.x / CF::HALF_THREADS) : 0;
-  const int lt = threadIdx.x % CF::HALF_THREADS;
+  const int lt = threadIdx.x;
   const int col = lt % T;
   const int p = lt / T;
   const int cg = tilex * T + col;
@@ -97,7 +98,7 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
   // band compaction of the transformed axis (see ColPass::keep)
   auto in_band = [&](int k) { return keep < 0 || k <= keep || k >= N - keep; };
   auto packed = [&](int k) { return (keep < 0 || k <= keep) ? k : k - (N - 2 * keep - 1); };
-  float* my = lds + half * CF::LDS_FLOATS;
+  float* my = lds;
   auto idx = [&](int row) { return row * T + col + (row / N2) * CF::PADK; };
 
   float2 v[E];
@@ -167,8 +168,10 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
 template <int M>
 struct ZCfg {
   static constexpr int P = Plan<M>::P;
-  static constexpr int LINES = 256 / P;
-  static constexpr int THREADS = 256;
+  // long lines fill LDS (one slot per line): smaller workgroups let four of them share a CU
+  // and interleave their load / transform / store phases
+  static constexpr int THREADS = M >= 256 ? 128 : 256;
+  static constexpr int LINES = THREADS / P;
   // floats per line slot: holds M+1 float2, and LSTR = P (mod 64) so the 64/P lines of a wave
   // start on disjoint groups of the 64 LDS banks (the exchange rows of one line are P apart)
   static constexpr int LSTR_MIN = 2 * (M + 2);
@@ -177,7 +180,7 @@ struct ZCfg {
 };
 
 template <int M, bool C2R>
-__global__ void __launch_bounds__(256) k_zpass(ZPass zp, SpecTables tb, const float2* __restrict__ tw) {
+__global__ void __launch_bounds__(ZCfg<M>::THREADS) k_zpass(ZPass zp, SpecTables tb, const float2* __restrict__ tw) {
   using PL = Plan<M>;
   using ZC = ZCfg<M>;
   constexpr int P = PL::P, E = PL::E, N2 = PL::N2;
@@ -325,7 +328,9 @@ int launch_cols_cfg(const ColPass& cp, int batches, const SpecTables& tb, int da
   PM_RETURN_IF(tw == nullptr, PM_ERROR_HIP, "twiddle table not initialised");
   const int gx = (int)ceil_div(cp.ncols, T);
   const int64_t tiles = (int64_t)gx * batches;
-  const int64_t g = ceil_div(tiles, 8) * 8;
+  const int64_t g = ceil_div(tiles * HALVES, 8) * 8;
+  PM_RETURN_IF(HALVES > 1 && (cp.in == cp.out0 || cp.in == cp.out1), PM_ERROR_INVALID_ARGUMENT,
+               "two-output FFT passes need outputs distinct from their input");
   PM_RETURN_IF(g > INT32_MAX, PM_ERROR_UNSUPPORTED, "too many FFT column tiles");
   hipLaunchKernelGGL((k_cols<N, T, MODE, HALVES>), dim3((unsigned)g), dim3(CF::THREADS), 0, s, cp, tb, dax, tw,
                      gx, batches, (int)(g / 8));
@@ -337,8 +342,8 @@ int launch_cols_cfg(const ColPass& cp, int batches, const SpecTables& tb, int da
 template <int N>
 int launch_cols_n(int mode, const ColPass& cp, int batches, const SpecTables& tb, int dax,
                   hipStream_t s) {
-  constexpr int T1 = (N >= 2048) ? 8 : 16;  // one-output passes: 128-byte row segments
-  constexpr int T2 = (N >= 2048) ? 4 : 8;   // two-output passes: two halves of T2 columns
+  constexpr int T1 = (N >= 512) ? 8 : 16;  // one-output passes: 64 or 128-byte row segments
+  constexpr int T2 = (N >= 2048) ? 4 : 8;   // two-output passes: two workgroups of T2 columns per tile
   switch (mode) {
     case kColFwd: return launch_cols_cfg<N, T1, kColFwd, 1>(cp, batches, tb, dax, s, "fft_cols_fwd");
     case kColInv: return launch_cols_cfg<N, T1, kColInv, 1>(cp, batches, tb, dax, s, "fft_cols_inv");

@@ -32,23 +32,6 @@ __global__ void k_fill(float* __restrict__ p, int64_t n, float v) {
     p[i] = v;
 }
 
-__global__ void __launch_bounds__(kBlock) k_payload(const int32_t* __restrict__ perm,
-                                                    const int16_t* __restrict__ pmid,
-                                                    const float* __restrict__ disp,
-                                                    const uint8_t* __restrict__ valid, int64_t n,
-                                                    int16_t* __restrict__ pmid_s, float* __restrict__ disp_s,
-                                                    uint8_t* __restrict__ valid_s) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int64_t i = perm[j];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    pmid_s[j * 3 + a] = pmid[i * 3 + a];
-    disp_s[j * 3 + a] = disp[i * 3 + a];
-  }
-  valid_s[j] = valid ? valid[i] : (uint8_t)1;
-}
-
 __global__ void __launch_bounds__(kBlock) k_scatter_sorted(const int16_t* __restrict__ pmid,
                                                            const float* __restrict__ disp,
                                                            const uint8_t* __restrict__ valid, int64_t n,
@@ -202,6 +185,40 @@ __device__ __forceinline__ void base_frac(const int16_t* __restrict__ pmid, cons
   }
 }
 
+// Raw particle record (int16 anchor, float32 displacement) held in registers.
+struct PRaw {
+  int p[3];
+  float d[3];
+};
+
+// Loads particles start + threadIdx.x + q * kTileThreads (q < PPT), clamped to end - 1 so every
+// load is unconditional (callers make sure end > start of the particle range).
+template <int PPT>
+__device__ __forceinline__ void load_raw(PRaw* raw, const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+                                         int64_t start, int64_t end) {
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    int64_t j = start + threadIdx.x + (int64_t)q * kTileThreads;
+    j = j < end ? j : end - 1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      raw[q].p[a] = pmid[j * 3 + a];
+      raw[q].d[a] = disp[j * 3 + a];
+    }
+  }
+}
+
+// base_frac on a raw record (same arithmetic)
+__device__ __forceinline__ void raw_base_frac(const PRaw& r, float a1, const int n[3], int base[3], float f[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float t = r.d[a] / a1;
+    float lf = floorf(t);
+    base[a] = pos_mod(r.p[a] + (int)lf, n[a]);
+    f[a] = t - lf;
+  }
+}
+
 // Tile cell of a particle's base (corner 0) cell.  The particle was binned by the sort with
 // the brick of this same base cell (pm_sort.hip cell_key: the same float32 division, floor and
 // periodic wrap), so the local index lies inside the brick; the clamp only keeps the LDS
@@ -236,30 +253,36 @@ __global__ void __launch_bounds__(kTileThreads) k_scatter_columns(
   const int P = B.pitch(), rows = B.rows(), carry = B.b2 + 1, tf = B.tfloats();
   const int64_t n1 = k.n[1], n2 = k.n[2];
   for (int i = threadIdx.x; i < tf; i += kTileThreads) tile[i] = 0ull;
+  // Particles stream through the column in chunks of kTileThreads * PPT in index order (the
+  // bricks' bins are contiguous).  The raw loads of the next chunk are issued before the
+  // current one is accumulated, so they are in flight across the flush between bricks.
+  constexpr int PPT = 4;
+  constexpr int64_t CH = (int64_t)kTileThreads * PPT;
+  const int64_t col_begin = bin_off[(int64_t)col * k.nb[2]], col_end = bin_off[(int64_t)(col + 1) * k.nb[2]];
+  const bool any = col_end > col_begin;
+  PRaw raw[PPT];
+  if (any) load_raw<PPT>(raw, pmid, disp, col_begin, col_end);
   __syncthreads();
   for (int bz = 0; bz < k.nb[2]; ++bz) {
     const int o[3] = {cx * B.b0, cy * B.b1, bz * B.b2};
     const int64_t bid = (int64_t)col * k.nb[2] + bz;
     const int64_t j0 = bin_off[bid], j1 = bin_off[bid + 1];
-    constexpr int PPT = 4;
-    for (int64_t c0 = j0; c0 < j1; c0 += (int64_t)kTileThreads * PPT) {
-      int bs[PPT][3];
-      float fr[PPT][3];
-      bool act[PPT];
+    for (int64_t c0 = j0; c0 < j1; c0 += CH) {
+      PRaw cur[PPT];
 #pragma unroll
-      for (int q = 0; q < PPT; ++q) {  // all loads of the chunk first (branch-free, clamped index)
-        int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
-        act[q] = j < j1;
-        base_frac(pmid, disp, act[q] ? j : j1 - 1, g.a1, k.n, bs[q], fr[q]);
-      }
+      for (int q = 0; q < PPT; ++q) cur[q] = raw[q];
+      load_raw<PPT>(raw, pmid, disp, c0 + CH < j1 ? c0 + CH : j1, col_end);  // next chunk
 #pragma unroll
       for (int q = 0; q < PPT; ++q) {
-        if (!act[q]) continue;
-        const int c0i = local_cell(B, bs[q], o);
+        int64_t j = c0 + threadIdx.x + (int64_t)q * kTileThreads;
+        if (j >= j1) continue;
+        int bs[3];
+        float fr[3];
+        raw_base_frac(cur[q], g.a1, k.n, bs, fr);
+        const int c0i = local_cell(B, bs, o);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
-          atomicAdd(&tile[c0i + B.corner_off(c)],
-                    (unsigned long long)__float2ll_rn(val * frac_weight(fr[q], c) * kFix));
+          atomicAdd(&tile[c0i + B.corner_off(c)], (unsigned long long)__float2ll_rn(val * frac_weight(fr, c) * kFix));
       }
     }
     __syncthreads();
@@ -729,7 +752,8 @@ struct Buffers {
   float* disp_s;
   uint8_t* valid_s;
   float* tables;
-  int32_t* bins;   // brick offsets of the sorted particles (tiled path)
+  int32_t* bins;   // brick offsets of the bucketed particles
+  bool tiled;      // brick-tiled scatter/gather (the brick divides the mesh and fits LDS)
   float* xs;       // upper x / y planes of every brick column (tiled scatter)
   float* ys;
   unsigned long long* first;  // first brick's lower z plane per column (tiled scatter)
@@ -751,13 +775,15 @@ struct Buffers {
 
 bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const int32_t* brick, bool reverse,
            Buffers& b) {
-  b.bins = nullptr;
   b.xs = b.ys = nullptr;
   b.first = nullptr;
-  if (tiled_ok(shape, brick)) {
+  int64_t nbins = 1;
+  for (int a = 0; a < 3; ++a) nbins *= ceil_div(shape[a], brick[a]);
+  b.bins = ws.take<int32_t>(nbins + 2);
+  b.tiled = tiled_ok(shape, brick);
+  if (b.tiled) {
     Bricks k = make_bricks(shape, brick);
     int64_t ncols = (int64_t)k.nb[0] * k.nb[1];
-    b.bins = ws.take<int32_t>(ncols * k.nb[2] + 2);
     b.xs = ws.take<float>(xs_floats(k));
     b.ys = ws.take<float>(ys_floats(k));
     b.first = ws.take<unsigned long long>(ncols * (k.b[0] + 1) * (k.b[1] + 1));
@@ -770,18 +796,13 @@ bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const in
   b.r0 = ws.take<float>(L.nreal);
   b.r1 = ws.take<float>(L.nreal);
   b.r2 = ws.take<float>(L.nreal);
-  // banded force pipeline: ca = A (s1), cb = C (s2), cc = E (s1), and when the y band is
-  // packed cd = B (s2) and ce = F (s1); the reverse pass also runs full-spectrum FFTs in
-  // ca, cb and cc
+  // banded force pipeline (see force_mesh_stage): s1-sized ca, cc, ce and s2-sized cb, cd;
+  // the reverse pass also runs full-spectrum FFTs in ca, cb and cc
   b.ca = ws.take<float2>(reverse ? L.ncplx : L.s1);
   b.cb = ws.take<float2>(reverse ? L.ncplx : L.s2);
   b.cc = ws.take<float2>(reverse ? L.ncplx : L.s1);
-  b.cd = nullptr;
-  b.ce = nullptr;
-  if (L.ky_keep >= 0) {
-    b.cd = ws.take<float2>(L.s2);
-    b.ce = ws.take<float2>(L.s1);
-  }
+  b.cd = ws.take<float2>(L.s2);
+  b.ce = ws.take<float2>(L.s1);
   if (reverse) {
     b.g0 = ws.take<float>(L.nreal);
     b.g1 = ws.take<float>(L.nreal);
@@ -798,7 +819,7 @@ size_t workspace_bytes(const pm_gravity_desc* d, int64_t n, bool reverse) {
   Buffers b;
   carve(probe, L, n, d->mesh_shape, d->brick, reverse, b);
   // the sort's temporaries are carved after the step buffers
-  return probe.used + 256 + sort::workspace_bytes(n, d->mesh_shape, d->brick) + 256;
+  return probe.used + 256 + sort::bucket_workspace_bytes(d->mesh_shape, d->brick) + 256;
 }
 
 Geom step_geom(const pm_gravity_desc* d) {
@@ -823,17 +844,13 @@ int fill(float* p, int64_t n, hipStream_t s) {
   return PM_SUCCESS;
 }
 
-// Sort, payload, density.  Leaves rho in b.r0.
+// Bucketing by brick (with the payload gathered into bucket order), density.  Leaves rho in b.r0.
 int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                   const pm_gravity_desc* d, const Layout& L, Buffers& b, Arena& ws, hipStream_t s) {
   const Geom g = step_geom(d);
-  PM_TRY(sort::sort_cells(pmid, disp, valid, n, d->mesh_shape, d->cell_size, d->brick, b.perm, b.bins, ws, s));
-  if (n > 0) {
-    hipLaunchKernelGGL(k_payload, dim3(grid_for(n)), dim3(kBlock), 0, s, b.perm, pmid, disp, valid, n, b.pmid_s,
-                       b.disp_s, b.valid_s);
-    PM_LAUNCHED("k_payload", s);
-  }
-  if (b.bins) {
+  PM_TRY(sort::bucket_cells(pmid, disp, valid, n, d->mesh_shape, d->cell_size, d->brick, b.perm, b.bins, b.pmid_s,
+                            b.disp_s, b.valid_s, ws, s));
+  if (b.tiled) {
     Bricks k = make_bricks(d->mesh_shape, d->brick);
     const unsigned ncols = (unsigned)((int64_t)k.nb[0] * k.nb[1]);
     if (is_brick(k, 16, 16, 32))
@@ -863,7 +880,7 @@ int gather_stage(const int16_t* pmid_s, const float* disp_s, const uint8_t* vali
                  const float* fz, float* acc, hipStream_t s) {
   if (n == 0) return PM_SUCCESS;
   const Geom g = step_geom(d);
-  if (b.bins) {
+  if (b.tiled) {
     Bricks k = make_bricks(d->mesh_shape, d->brick);
     const int64_t nbins = (int64_t)k.nb[0] * k.nb[1] * k.nb[2];
     const unsigned ncols = (unsigned)((int64_t)k.nb[0] * k.nb[1]);
@@ -894,11 +911,17 @@ int force_mesh_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b, cons
   const float coef = 1.5f * d->omega_m;  // 1.5 * Omega_m in float32 (gravity.py:630)
   const bool zband = L.nkz <= L.m;      // Nyquist column not stored
   const bool ypack = L.ky_keep >= 0;
-  float2* A = b.ca;                      // z r2c output, then chi (y inverse of psi)
-  float2* B = ypack ? b.cd : b.ca;       // y forward output, then psi
-  float2* C = b.cb;                      // D_x psi
-  float2* E = b.cc;                      // D_y chi
-  float2* F = ypack ? b.ce : b.cb;       // y inverse of D_x psi
+  // every pass writes a buffer other than the one it reads (the two-output passes need it):
+  //   A (s1) z r2c output, later chi = y inverse of psi (D_z applied by the last z c2r)
+  //   B (s2) y forward output (packed |ky| band)
+  //   G (s1) psi = x inverse of the Green-multiplied spectrum, later F_x's y inverse
+  //   C (s2) D_x psi
+  //   E (s1) D_y chi
+  float2* A = b.ca;
+  float2* B = b.cd;
+  float2* G = b.cc;
+  float2* C = b.cb;
+  float2* E = b.ce;
   const int64_t p1 = L.pitch1;
   ZPass z1{};
   z1.rin = b.r0;
@@ -921,20 +944,20 @@ int force_mesh_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b, cons
   yf.keep = L.ky_keep;
   PM_TRY(launch_cols(L.ny, kColFwd, yf, L.nx, tb, 1, s));
   // x: forward, Green, two inverses; columns are kz, batches are the (packed) ky rows
-  ColPass xg{B, B, C, (int64_t)L.nky * p1, 1, p1, ncols, 1, 0, 0, 1, 0};
+  ColPass xg{B, G, C, (int64_t)L.nky * p1, 1, p1, ncols, 1, 0, 0, 1, 0};
   if (ypack) {
     xg.ky_split = L.ky_keep;
     xg.ky_shift = L.ny - L.nky;
   }
   PM_TRY(launch_cols(L.nx, kColGreen, xg, L.nky, tb, 0, s));
   // y inverses expand the band back to full rows
-  ColPass yi2{B, A, E, p1, 1, (int64_t)L.nky * p1, ncols, 0, 0, 0, 0, 0};
+  ColPass yi2{G, A, E, p1, 1, (int64_t)L.nky * p1, ncols, 0, 0, 0, 0, 0};
   yi2.o_row = p1;
   yi2.o_batch = (int64_t)L.ny * p1;
   yi2.keep = L.ky_keep;
   yi2.compact_in = 1;
   PM_TRY(launch_cols(L.ny, kColInv2, yi2, L.nx, tb, 1, s));
-  ColPass yi{C, F, nullptr, p1, 1, (int64_t)L.nky * p1, ncols, 0, 0, 0, 0, 0};
+  ColPass yi{C, G, nullptr, p1, 1, (int64_t)L.nky * p1, ncols, 0, 0, 0, 0, 0};
   yi.o_row = p1;
   yi.o_batch = (int64_t)L.ny * p1;
   yi.keep = L.ky_keep;
@@ -950,7 +973,7 @@ int force_mesh_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b, cons
   zc.kz_lim = L.nkz;
   zc.nyq = zband ? 0 : 1;
   float* fx = keep_density ? b.g0 : b.r0;  // reverse keeps rho for the Omega_m cotangent
-  const float2* srcs[3] = {F, E, A};
+  const float2* srcs[3] = {G, E, A};
   float* dsts[3] = {fx, b.r1, b.r2};
   for (int c = 0; c < 3; ++c) {
     zc.cmain = const_cast<float2*>(srcs[c]);

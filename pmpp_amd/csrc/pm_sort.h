@@ -12,5 +12,12 @@ int sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int
                const int32_t* mesh_shape, float cell_size, const int32_t* brick, int32_t* perm,
                int32_t* bin_offsets, Arena& ws, hipStream_t s);
 
+// Unstable bucketing by the same key (gravity step): bin_offsets (num_bricks + 2 entries, the
+// last bucket holding invalid particles), perm and the payload gathered into bucket order.
+size_t bucket_workspace_bytes(const int32_t* mesh_shape, const int32_t* brick);
+int bucket_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n, const int32_t* mesh_shape,
+                 float cell_size, const int32_t* brick, int32_t* perm, int32_t* bin_offsets, int16_t* pmid_s,
+                 float* disp_s, uint8_t* valid_s, Arena& ws, hipStream_t s);
+
 }  // namespace sort
 }  // namespace pm

@@ -249,6 +249,80 @@ KeyGeom make_key_geom(const int32_t* mesh_shape, float cell, const int32_t* bric
   return g;
 }
 
+// ---- unstable bucketing by key (the gravity step's binning) ----
+// Each wave groups its lanes by key (leader loop over ballots) and issues one global atomic
+// per distinct key; nearly sorted inputs (particles in Lagrangian order) have few keys per
+// wave.  The order inside a bucket depends on atomic arrival order, which the tile kernels
+// do not see: the scatter sums in exact integer arithmetic and the gather is per particle.
+__device__ __forceinline__ int raw_cell_key(const KeyGeom& g, const int p[3], const float d[3], bool ok) {
+  if (!ok) return g.sentinel;
+  int c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float t = d[a] / g.cell;  // the same float32 arithmetic as cell_key and the CIC base cell
+    int base = p[a] + (int)floorf(t);
+    int cell = g.nshift[a] >= 0 ? (base & (g.n[a] - 1)) : pos_mod(base, g.n[a]);
+    c[a] = g.bshift[a] >= 0 ? (cell >> g.bshift[a]) : cell / g.b[a];
+  }
+  return (c[0] * g.nb[1] + c[1]) * g.nb[2] + c[2];
+}
+
+constexpr int kBucketBlock = 256;
+
+template <bool PLACE>
+__global__ void __launch_bounds__(kBucketBlock) k_bucket(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+                                                         const uint8_t* __restrict__ valid, KeyGeom g, int64_t n,
+                                                         int32_t* __restrict__ count, const int32_t* __restrict__ bins,
+                                                         int32_t* __restrict__ perm, int16_t* __restrict__ pmid_s,
+                                                         float* __restrict__ disp_s, uint8_t* __restrict__ valid_s) {
+  const int lane = threadIdx.x % kWave;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // the wave's first index: uniform over the wave, so the ballot loop below runs converged
+  for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; w0 < n; w0 += stride) {
+    const int64_t i = w0 + lane;
+    const bool act = i < n;
+    const int64_t ic = act ? i : n - 1;  // clamped: unconditional loads
+    int p[3];
+    float d[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      p[a] = pmid[ic * 3 + a];
+      d[a] = disp[ic * 3 + a];
+    }
+    const uint8_t v = valid ? valid[ic] : (uint8_t)1;
+    const int key = raw_cell_key(g, p, d, v != 0);
+    uint64_t live = __ballot(act);
+    int dest = 0;
+    while (live) {
+      const int leader = __ffsll((unsigned long long)live) - 1;
+      const int lk = __shfl(key, leader);
+      const uint64_t m = __ballot(act && key == lk) & live;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&count[lk], __popcll(m));
+      if (PLACE) {
+        base = __shfl(base, leader);
+        if ((m >> lane) & 1ull) dest = bins[lk] + base + __popcll(m & lt_mask);
+      }
+      live &= ~m;
+    }
+    if (PLACE && act) {
+      perm[dest] = (int32_t)i;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        pmid_s[(int64_t)dest * 3 + a] = (int16_t)p[a];
+        disp_s[(int64_t)dest * 3 + a] = d[a];
+      }
+      valid_s[dest] = v;
+    }
+  }
+}
+
+__global__ void k_zero_i32(int32_t* __restrict__ p, int64_t len) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
 int key_bits(int64_t max_key) {
   int b = 1;
   while ((int64_t(1) << b) <= max_key) ++b;
@@ -256,6 +330,42 @@ int key_bits(int64_t max_key) {
 }
 
 }  // namespace
+
+size_t bucket_workspace_bytes(const int32_t* mesh_shape, const int32_t* brick) {
+  int64_t nbins = 1;
+  for (int a = 0; a < 3; ++a) nbins *= ceil_div(mesh_shape[a], brick[a]);
+  const int64_t len = nbins + 2;
+  return align_up(sizeof(int32_t) * len, 256) + align_up(sizeof(int32_t) * (ceil_div(len, kScanChunk) + 1), 256) + 512;
+}
+
+int bucket_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n, const int32_t* mesh_shape,
+                 float cell_size, const int32_t* brick, int32_t* perm, int32_t* bin_offsets, int16_t* pmid_s,
+                 float* disp_s, uint8_t* valid_s, Arena& ws, hipStream_t s) {
+  KeyGeom g = make_key_geom(mesh_shape, cell_size, brick);
+  const int64_t len = (int64_t)g.sentinel + 2;  // bricks, the invalid bucket, and the end
+  int32_t* count = ws.take<int32_t>(len);
+  int32_t* partial = ws.take<int32_t>(ceil_div(len, kScanChunk) + 1);
+  PM_RETURN_IF(!ws.ok(), PM_ERROR_WORKSPACE, "bucket workspace too small");
+  const unsigned zb = (unsigned)std::min<int64_t>(ceil_div(len, 256), 1024);
+  hipLaunchKernelGGL(k_zero_i32, dim3(zb), dim3(256), 0, s, count, len);
+  PM_LAUNCHED("k_zero_i32", s);
+  const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBucketBlock), 8192));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_bucket<false>, dim3(nb), dim3(kBucketBlock), 0, s, pmid, disp, valid, g, n, count,
+                       (const int32_t*)nullptr, (int32_t*)nullptr, (int16_t*)nullptr, (float*)nullptr,
+                       (uint8_t*)nullptr);
+    PM_LAUNCHED("k_bucket_count", s);
+  }
+  PM_TRY(scan_exclusive(count, bin_offsets, len, partial, s));
+  hipLaunchKernelGGL(k_zero_i32, dim3(zb), dim3(256), 0, s, count, len);
+  PM_LAUNCHED("k_zero_i32", s);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_bucket<true>, dim3(nb), dim3(kBucketBlock), 0, s, pmid, disp, valid, g, n, count,
+                       (const int32_t*)bin_offsets, perm, pmid_s, disp_s, valid_s);
+    PM_LAUNCHED("k_bucket_place", s);
+  }
+  return PM_SUCCESS;
+}
 
 int check_sort_args(int64_t n, const int32_t* mesh_shape, const int32_t* brick) {
   PM_RETURN_IF(n < 0 || n > INT32_MAX, PM_ERROR_UNSUPPORTED, "particle count must be in [0, 2^31)");

@@ -83,3 +83,28 @@ def gravity(a, ptcl, cosmo, conf, correction=None):
     if not isinstance(omega, torch.Tensor):
         omega = torch.tensor(float(omega), dtype=torch.float32, device=ptcl.disp.device)
     return _GravityFn.apply(ptcl.disp, omega, ptcl.pmid.detach(), _valid_u8(ptcl), conf)
+
+
+def gravity_vjp(ptcl, cosmo, conf, acc_cot):
+    """(disp cotangent (N,3) tensor, Omega_m cotangent float) of ``gravity`` for ``acc_cot``.
+
+    The reverse of the force step without autograd bookkeeping (``pm_gravity_vjp``), as the
+    N-body reverse sweep calls it once per drift stage (integrator.py:620-686).
+    """
+    omega = cosmo.Omega_m if hasattr(cosmo, "Omega_m") else cosmo
+    dev = ptcl.disp.device
+    ensure_init(dev)
+    d = gravity_desc(conf, float(omega))
+    pm = as_contig(ptcl.pmid, torch.int16, "pmid")
+    dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
+    n = pm.shape[0]
+    nbytes = _lib.size_query("pm_gravity_vjp_workspace", d, n)
+    ws = workspace(nbytes, dev)
+    ac = as_contig(acc_cot.detach(), torch.float32, "acc_cot")
+    disp_cot = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    omega_cot = torch.zeros((), dtype=torch.float32, device=dev)
+    valid = _valid_u8(ptcl)
+    _lib.call("pm_gravity_vjp", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(ac), ptr(disp_cot), ptr(omega_cot),
+              ptr(ws), nbytes, stream_handle(dev))
+    return disp_cot, float(omega_cot.item())
+

@@ -39,3 +39,26 @@ def test_nbody_schedule():
     conf = Configuration(2.0, (8, 8, 8), a_start=1 / 64, a_stop=1.0, a_nbody_maxstep=(1 - 1 / 64) / 10)
     a = conf.a_nbody
     assert len(a) == 11 and a[0] == pytest.approx(1 / 64) and a[-1] == pytest.approx(1.0)
+
+
+def test_step_factors_match_the_oracle():
+    # host float64 growth table (Dormand-Prince) vs the oracle's (scipy DOP853), same nodes
+    import numpy as np
+    from oracle import cosmology as oc
+    from pmpp_amd import Configuration
+    from pmpp_amd.initial_conditions import LCDM
+    from pmpp_amd.nbody import integrator
+
+    conf = Configuration(2.0, (8, 8, 8), mesh_shape=2)
+    for cos in (LCDM(), LCDM(Omega_m=0.25, Omega_k=0.05, w_0=-0.9, w_a=0.1)):
+        kw = dict(Ok=cos.Omega_k, w0=cos.w_0, wa=cos.w_a)
+        t = oc.growth_table(conf.growth_a, cos.Omega_m, conf.a_lpt_step, **kw)
+        for a0, a1 in zip(conf.a_nbody[:-1], conf.a_nbody[1:]):
+            am = 0.5 * (a0 + a1)
+            d = oc.drift_factor(am, a0, a1, t, conf.growth_a, cos.Omega_m, **kw)
+            k = oc.kick_factor(a0, a0, am, t, conf.growth_a, cos.Omega_m, **kw)
+            assert integrator.drift_factor(am, a0, a1, cos, conf) == pytest.approx(d, rel=1e-9)
+            assert integrator.kick_factor(a0, a0, am, cos, conf) == pytest.approx(k, rel=1e-9)
+    assert np.isclose(integrator._factor_dOm("kick_factor", 0.5, 0.5, 0.55, LCDM(), conf),
+                      (integrator.kick_factor(0.5, 0.5, 0.55, LCDM(Omega_m=0.30001), conf)
+                       - integrator.kick_factor(0.5, 0.5, 0.55, LCDM(Omega_m=0.29999), conf)) / 2e-5, rtol=1e-5)
