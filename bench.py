@@ -37,13 +37,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def sort_passes(mesh, brick):
-    """Radix passes of the brick-key sort (digits of at most 10 bits; no validity mask)."""
+def num_bricks(mesh, brick):
     nb = 1
     for m, b in zip(mesh, brick):
-        nb *= m // b
-    bits = max(1, (nb - 1).bit_length())
-    return -(-bits // 10), nb
+        nb *= -(-m // b)
+    return nb
 
 
 def kernel_step_bytes(name, L, n_p, brick):
@@ -58,7 +56,7 @@ def kernel_step_bytes(name, L, n_p, brick):
     Ch = 8 * L["nx"] * L["ny"] * (L["nz"] // 2 + 1)
     S1 = 8 * L["nx"] * L["ny"] * L["nkz"]  # banded spectrum, full y rows
     S2 = 8 * L["nx"] * L["nky"] * L["nkz"]  # banded spectrum, packed y band
-    passes, nb = sort_passes((L["nx"], L["ny"], L["nz"]), brick)
+    nb = num_bricks((L["nx"], L["ny"], L["nz"]), brick)
     b0, b1, b2 = brick
     nb0, nb1 = L["nx"] // b0, L["ny"] // b1
     planes = 4 * nb0 * nb1 * (b1 + 1 + b0) * L["nz"]  # upper x/y planes of every brick column
@@ -70,10 +68,10 @@ def kernel_step_bytes(name, L, n_p, brick):
         "fft_cols_inv2": S2 + 2 * S1,
         "fft_cols_inv": S2 + S1,
         "fft_z_c2r": 3 * (S1 + R),
-        "k_digit_hist": (18 + 4 * (passes - 1)) * n_p,
-        "k_digit_scatter": (26 + 16 * (passes - 1)) * n_p,
-        "k_bin_bounds": 4 * n_p + 4 * (nb + 2),
-        "k_payload": (4 + 18 + 18 + 1) * n_p,
+        "k_bucket_count": 18 * n_p,
+        "k_bucket_place": (18 + 4 + 18 + 1) * n_p,
+        "k_zero_i32": 4 * (nb + 2),
+        "scan": 8 * (nb + 2),
         "k_scatter_columns": 18 * n_p + R + planes,
         "k_fold_xy": planes + fold,
         "k_gather_columns": (18 + 4 + 12) * n_p + 3 * R,

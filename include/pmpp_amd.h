@@ -169,6 +169,26 @@ int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid,
                    const pm_gravity_desc* desc, const float* acc_cot, float* disp_cot,
                    float* omega_cot, void* workspace, size_t workspace_bytes, void* stream);
 
+/* The force step with a learned correction (nbody/gravity.py:419-430, 635-640 with
+ * apply_potential_correction, corrections/core.py:286-334): rho_add (the real mesh, Nx*Ny*Nz
+ * float32, C order) is added to the CIC density before S = 1.5*Omega_m*(rho + rho_add - 1),
+ * so the correction enters the Poisson solve as a source residual (a potential residual
+ * phi_add corresponds to rho_add = -laplacian(phi_add) / (1.5 Omega_m)) instead of costing
+ * an extra irfftn + rfftn per force.  rho_add = NULL is pm_gravity.  The VJP also writes
+ * rho_cot = dL/d rho_add (the real mesh, overwritten) when rho_cot is not NULL; workspaces
+ * are those of pm_gravity_workspace / pm_gravity_vjp_workspace.  pm_gravity_density writes
+ * the step's own CIC density (the mesh the correction network reads) and stops there. */
+int pm_gravity_density(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                       const pm_gravity_desc* desc, float* rho, void* workspace, size_t workspace_bytes,
+                       void* stream);
+int pm_gravity_corrected(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                         const pm_gravity_desc* desc, const float* rho_add, float* acc, void* workspace,
+                         size_t workspace_bytes, void* stream);
+int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                             const pm_gravity_desc* desc, const float* rho_add, const float* acc_cot,
+                             float* disp_cot, float* omega_cot, float* rho_cot, void* workspace,
+                             size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------ N-body updates ----
  * Elementwise stages of the symplectic integrator on (n,3) float32 arrays (16-byte aligned):
  * pm_kick   vel  += factor * acc    replaces kick              (nbody/integrator.py:512-538)
@@ -188,6 +208,73 @@ int pm_kick_adj(float* vel, const float* acc, const float* vel_cot, float* acc_c
                 double* dot, void* workspace, size_t workspace_bytes, void* stream);
 int pm_drift_adj(float* disp, const float* vel, const float* disp_cot, float* vel_cot, int64_t n, float factor,
                  double* dot, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------ power spectrum ----
+ * Integer-shell sums of an rfft half spectrum (nx, ny, nz/2+1) complex64: the shell
+ * reduction of analysis/power_spectrum.py:108-143 (_shell_reduce_local) and :145-176 (its
+ * cross variant).  sums[0:num_shells] += sum over modes of the shell of
+ * mult * |a|^2 * (sinc sinc sinc)^(-2 mas_power); with spec_b also sums[num_shells:2n] for
+ * |b|^2 and sums[2n:3n] for Re(a conj b).  density != 0 treats the inputs as densities
+ * (delta_k = rho_k / mean, density_to_pk :348-374).  sums (float64, device) is overwritten;
+ * the caller divides by the shell mode counts and multiplies by cell^3 / N_m (:251-256). */
+int pm_power_spectrum(const float* spec_a, const float* spec_b, const int32_t shape[3], int32_t mas_power,
+                      int32_t num_shells, int32_t density, double* sums, void* stream);
+
+/* ------------------------------------------------------------ conv3d (learned correction) ----
+ * Periodic 3x3x3 convolutions of channels-last bf16 meshes on the matrix cores (BF16 MFMA,
+ * float32 accumulation), 16 output channels, for the mesh-CNN potential correction
+ * (corrections/mesh_cnn.py:54-105, its Conv3D layers with periodic padding).  Mesh axes are
+ * multiples of 8; x / w / res bf16, activations [X][Y][Z][C].
+ * pm_conv3d_fwd        pre = conv(x; w[27][16][cin]) + bias (+ res); act = gelu(pre) or pre;
+ *                      pre (float32) and act (bf16) are each optional outputs.
+ * pm_conv3d_bwd_data   dx = conv^T(dz; w) (+ add), times gelu'(gact) when gact is given
+ *                      (16 -> 16 layers).
+ * pm_conv3d_bwd_weight dw_db = [dW (27*16*cin floats), db (16 floats)] for pre-activation
+ *                      gradient dz and layer input x; workspace from the query. */
+int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, const void* res, int32_t gelu,
+                  float* pre, void* act, const int32_t shape[3], void* stream);
+int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const float* gact, float* dx,
+                       const int32_t shape[3], void* stream);
+int pm_conv3d_bwd_weight_workspace(const int32_t shape[3], int32_t cin, size_t* bytes);
+int pm_conv3d_bwd_weight(const float* dz, const void* x, int32_t cin, const int32_t shape[3], float* dw_db,
+                         void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------ learned correction ----
+ * The elementwise stages around the conv stack of the mesh CNN correction
+ * (corrections/mesh_cnn.py:54-105), its training loss and its optimizer
+ * (corrections/common.py:136-174).  With c = 1.5 Omega_m and S = c (rho - 1):
+ * pm_correction_features  sigma = sqrt(mean S^2 + 1e-6) and the gain
+ *                         g = max_residual tanh(gain_params . [a, Omega_m, sigma8, 1]) into
+ *                         stats[0..2] (device floats); x0 = bf16 [S/sigma, a, Omega_m, sigma8,
+ *                         0, 0, 0, 0] per cell (n x 8 bf16, 16-byte aligned).
+ * pm_correction_residual  rho_add = g (rho - 1) + max_residual tanh(out[.,0]) sigma / c, out the
+ *                         output layer's float32 pre-activations (n x 16).
+ * pm_correction_residual_bwd  the output layer's pre-activation gradient dz_out (n x 16,
+ *                         channel 0 live) and the gain parameters' gradient (4 floats) from
+ *                         rho_cot.
+ * pm_force_loss           loss[0] = sum |acc - target|^2 / sum |target|^2 (device doubles,
+ *                         loss[1] = 1 / sum |target|^2), acc_cot = d loss / d acc if not NULL.
+ * pm_adamax_step          clip_by_global_norm(clip_norm) then adamax(learning_rate, b1, b2, eps)
+ *                         on n float32 parameters, skipped when the gradient norm is not
+ *                         finite (the apply_if_finite wrapper); count (device int32) is the
+ *                         step counter; params_bf16 (optional) receives the bf16 mirror.
+ * pm_to_bf16              dst = bf16(src).
+ * Reductions are deterministic (fixed-order float64 partials); the workspace holds them. */
+int pm_correction_workspace(size_t* bytes);
+int pm_correction_features(const float* rho, int64_t n, float omega_m, float a, float sigma8, float max_residual,
+                           const float* gain_params, void* x0, float* stats, void* workspace,
+                           size_t workspace_bytes, void* stream);
+int pm_correction_residual(const float* out, const float* rho, int64_t n, float omega_m, float max_residual,
+                           const float* stats, float* rho_add, void* stream);
+int pm_correction_residual_bwd(const float* rho_cot, const float* out, const float* rho, int64_t n, float omega_m,
+                               float a, float sigma8, float max_residual, const float* stats, float* dz_out,
+                               float* gain_grad, void* workspace, size_t workspace_bytes, void* stream);
+int pm_force_loss(const float* acc, const float* target, int64_t n, double* loss, float* acc_cot, void* workspace,
+                  size_t workspace_bytes, void* stream);
+int pm_adamax_step(float* params, const float* grads, float* mu, float* nu, int32_t* count, int64_t n,
+                   float learning_rate, float b1, float b2, float eps, float clip_norm, void* params_bf16,
+                   void* workspace, size_t workspace_bytes, void* stream);
+int pm_to_bf16(const float* src, int64_t n, void* dst, void* stream);
 
 #ifdef __cplusplus
 }

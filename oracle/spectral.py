@@ -103,29 +103,39 @@ def default_value(n, mesh_shape, ptcl_num, valid=None):
     return v
 
 
-def gravity(pmid, disp, omega_m, mesh_shape, cell_size, ptcl_num, masks=(), valid=None):
-    """Accelerations (N,3) of the single-device PM force (gravity.py:786-814), float64."""
+def gravity(pmid, disp, omega_m, mesh_shape, cell_size, ptcl_num, masks=(), valid=None, rho_add=None):
+    """Accelerations (N,3) of the single-device PM force (gravity.py:786-814), float64.
+
+    ``rho_add`` (a real mesh) is added to the CIC density before the source is formed: the
+    learned correction applied as a source residual (the potential-residual hook of
+    corrections/core.py:286-334 moved in front of the Poisson solve)."""
     n = np.asarray(pmid).shape[0]
     val = default_value(n, mesh_shape, ptcl_num, valid)
     dens = scatter(pmid, disp, val, np.zeros(tuple(mesh_shape)), cell_size, mesh_shape, valid=valid)
+    if rho_add is not None:
+        dens = dens + np.asarray(rho_add, dtype=np.float64)
     comps = force_meshes(dens, omega_m, mesh_shape, cell_size, masks)
     acc = np.stack([gather(pmid, disp, c, cell_size, mesh_shape, valid=valid) for c in comps], axis=-1)
     return acc
 
 
-def gravity_vjp(pmid, disp, omega_m, acc_cot, mesh_shape, cell_size, ptcl_num, masks=(), valid=None):
+def gravity_vjp(pmid, disp, omega_m, acc_cot, mesh_shape, cell_size, ptcl_num, masks=(), valid=None,
+                rho_add=None):
     """Reverse of ``gravity`` w.r.t. ``disp`` and ``omega_m`` (call stack (2) of the force VJP).
 
     Order of the pullbacks: gather^T per component (disp cotangent + mesh cotangent), the
     irfftn transpose, the -ik_i multiply transpose, the self-adjoint Poisson symbol
     (gravity.py:307-318), the mask, the rfftn transpose, the 1.5 Omega_m scale, and the
     scatter^T.  Implemented through the exact linear adjoints of numpy's real FFTs.
-    Returns ``(disp_cot (N,3), omega_m_cot)``.
+    Returns ``(disp_cot (N,3), omega_m_cot)``, and ``rho_cot`` (the density cotangent, which
+    is also that of ``rho_add``) as a third item when ``rho_add`` is given.
     """
     mesh_shape = tuple(mesh_shape)
     n = np.asarray(pmid).shape[0]
     val = default_value(n, mesh_shape, ptcl_num, valid)
     dens = scatter(pmid, disp, val, np.zeros(mesh_shape), cell_size, mesh_shape, valid=valid)
+    if rho_add is not None:
+        dens = dens + np.asarray(rho_add, dtype=np.float64)
     comps = force_meshes(dens, omega_m, mesh_shape, cell_size, masks)
     acc_cot = np.asarray(acc_cot, dtype=np.float64)
     disp_cot = np.zeros((n, 3))
@@ -150,4 +160,6 @@ def gravity_vjp(pmid, disp, omega_m, acc_cot, mesh_shape, cell_size, ptcl_num, m
     dens_cot = src_cot * (1.5 * float(omega_m))
     dc, _ = scatter_adj(pmid, disp, val, dens_cot, cell_size, mesh_shape, valid=valid)
     disp_cot += dc
+    if rho_add is not None:
+        return disp_cot, omega_cot, dens_cot
     return disp_cot, omega_cot

@@ -31,7 +31,12 @@ EXPORTED_SYMBOLS = (
     "pm_fft_workspace", "pm_rfft3d", "pm_irfft3d",
     "pm_spectral_workspace", "pm_spectral_force_multiply",
     "pm_gravity_workspace", "pm_gravity_vjp_workspace", "pm_gravity", "pm_gravity_vjp",
+    "pm_gravity_density", "pm_gravity_corrected", "pm_gravity_corrected_vjp",
     "pm_kick", "pm_drift", "pm_nbody_adj_workspace", "pm_kick_adj", "pm_drift_adj",
+    "pm_power_spectrum",
+    "pm_conv3d_fwd", "pm_conv3d_bwd_data", "pm_conv3d_bwd_weight_workspace", "pm_conv3d_bwd_weight",
+    "pm_correction_workspace", "pm_correction_features", "pm_correction_residual", "pm_correction_residual_bwd",
+    "pm_force_loss", "pm_adamax_step", "pm_to_bf16",
 )
 
 
@@ -97,11 +102,32 @@ _SIGNATURES = {
                    ctypes.c_int),
     "pm_gravity_vjp": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
                        ctypes.c_int),
+    "pm_gravity_density": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, ctypes.c_size_t, _vp],
+                           ctypes.c_int),
+    "pm_gravity_corrected": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, ctypes.c_size_t,
+                              _vp], ctypes.c_int),
+    "pm_gravity_corrected_vjp": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, _vp, _vp, _vp,
+                                  ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_kick": ([_vp, _vp, _i64, _f32, _vp], ctypes.c_int),
     "pm_drift": ([_vp, _vp, _i64, _f32, _vp], ctypes.c_int),
     "pm_nbody_adj_workspace": ([_sz_p], ctypes.c_int),
     "pm_kick_adj": ([_vp, _vp, _vp, _vp, _i64, _f32, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_drift_adj": ([_vp, _vp, _vp, _vp, _i64, _f32, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
+    "pm_power_spectrum": ([_vp, _vp, _i32x3, _i32, _i32, _i32, _vp, _vp], ctypes.c_int),
+    "pm_conv3d_fwd": ([_vp, _i32, _vp, _vp, _vp, _i32, _vp, _vp, _i32x3, _vp], ctypes.c_int),
+    "pm_conv3d_bwd_data": ([_vp, _vp, _vp, _vp, _vp, _i32x3, _vp], ctypes.c_int),
+    "pm_conv3d_bwd_weight_workspace": ([_i32x3, _i32, _sz_p], ctypes.c_int),
+    "pm_conv3d_bwd_weight": ([_vp, _vp, _i32, _i32x3, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
+    "pm_correction_workspace": ([_sz_p], ctypes.c_int),
+    "pm_correction_features": ([_vp, _i64, _f32, _f32, _f32, _f32, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
+                               ctypes.c_int),
+    "pm_correction_residual": ([_vp, _vp, _i64, _f32, _f32, _vp, _vp, _vp], ctypes.c_int),
+    "pm_correction_residual_bwd": ([_vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _vp, _vp, _vp, _vp,
+                                    ctypes.c_size_t, _vp], ctypes.c_int),
+    "pm_force_loss": ([_vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
+    "pm_adamax_step": ([_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _f32, _vp, _vp, ctypes.c_size_t,
+                        _vp], ctypes.c_int),
+    "pm_to_bf16": ([_vp, _i64, _vp, _vp], ctypes.c_int),
 }
 
 _LIB = None

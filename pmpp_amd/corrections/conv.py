@@ -1,0 +1,64 @@
+"""Thin wrappers of the BF16-MFMA periodic 3x3x3 convolution kernels (``pm_conv3d_*``).
+
+Activations are channels-last torch tensors (X, Y, Z, C); inputs to a layer are bf16 with 8 or
+16 channels, weights bf16 (27, 16, C_in) with tap = (dx+1)*9 + (dy+1)*3 + (dz+1), biases
+float32 (16,).  Mesh axes must be multiples of 8.  The layers are those of the reference's
+periodic Conv3D stack (corrections/mesh_cnn.py:54-105), with float32 accumulation.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from .. import _lib
+from .._torch import ensure_init, ptr, require_device, stream_handle, workspace
+
+
+def _shape(x):
+    if x.ndim != 4:
+        raise ValueError("conv3d takes channels-last (X, Y, Z, C) meshes")
+    return tuple(int(s) for s in x.shape[:3])
+
+
+def conv3d(x, w, b, res=None, gelu=False, want_pre=True, want_act=True):
+    """(pre, act): pre = conv(x) + b (+ res) float32, act = gelu(pre) (or pre) bf16."""
+    require_device(x, "x")
+    if x.dtype != torch.bfloat16 or w.dtype != torch.bfloat16:
+        raise RuntimeError("conv3d takes bf16 activations and weights")
+    ensure_init(x.device)
+    shape = _shape(x)
+    cin = int(x.shape[3])
+    pre = torch.empty(shape + (16,), dtype=torch.float32, device=x.device) if want_pre else None
+    act = torch.empty(shape + (16,), dtype=torch.bfloat16, device=x.device) if want_act else None
+    _lib.call("pm_conv3d_fwd", ptr(x.contiguous()), cin, ptr(w.contiguous()), ptr(b.contiguous()),
+              ptr(res.contiguous()) if res is not None else None, int(gelu), ptr(pre), ptr(act), _lib.int3(shape),
+              stream_handle(x.device))
+    return pre, act
+
+
+def conv3d_bwd_data(dz, w, add=None, gact=None):
+    """dx = conv^T(dz; w) (+ add), times gelu'(gact) if given (16 -> 16 layers)."""
+    shape = _shape(dz)
+    dx = torch.empty(shape + (16,), dtype=torch.float32, device=dz.device)
+    _lib.call("pm_conv3d_bwd_data", ptr(dz.contiguous()), ptr(w.contiguous()),
+              ptr(add.contiguous()) if add is not None else None, ptr(gact.contiguous()) if gact is not None else None,
+              ptr(dx), _lib.int3(shape), stream_handle(dz.device))
+    return dx
+
+
+def conv3d_bwd_weight(dz, x, out=None):
+    """(dW (27, 16, C_in) float32, db (16,) float32) of a layer with input x.
+
+    ``out`` (optional) is a contiguous float32 tensor of 27*16*C_in + 16 elements that receives
+    [dW, db] in place (a slice of a flat gradient vector)."""
+    shape = _shape(dz)
+    cin = int(x.shape[3])
+    nbytes = _lib.size_query("pm_conv3d_bwd_weight_workspace", _lib.int3(shape), cin)
+    ws = workspace(nbytes, dz.device)
+    if out is None:
+        out = torch.empty(27 * 16 * cin + 16, dtype=torch.float32, device=dz.device)
+    elif out.numel() != 27 * 16 * cin + 16 or not out.is_contiguous() or out.dtype != torch.float32:
+        raise ValueError("out must be a contiguous float32 tensor of 27*16*C_in + 16 elements")
+    _lib.call("pm_conv3d_bwd_weight", ptr(dz.contiguous()), ptr(x.contiguous()), cin, _lib.int3(shape), ptr(out),
+              ptr(ws), nbytes, stream_handle(dz.device))
+    return out[:-16].view(27, 16, cin), out[-16:]

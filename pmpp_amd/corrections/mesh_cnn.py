@@ -1,0 +1,249 @@
+"""Mesh-CNN correction of the PM force on the HIP library (PM++ ``corrections/mesh_cnn.py``).
+
+The network is the reference's periodic residual CNN (``mesh_cnn.py:54-105``): a 3^3 stem
+convolution with GELU, ``depth - 1`` residual blocks (conv, GELU, conv, + input, GELU), a
+zero-initialised 3^3 output convolution bounded by ``max_residual * tanh``, and a
+conditioning-only "direct gain" ``max_residual * tanh(Linear([a, Omega_m, sigma8]))``.  The
+MI355X build differs in three deliberate ways:
+
+* hidden width is 16 channels (one MFMA tile of output channels) and activations are bf16
+  with float32 accumulation (``pm_conv3d_*``);
+* the input channels are [S / sigma, a, Omega_m, sigma8] padded to 8 -- the potential channel
+  of the reference needs an extra irfftn per force and is left out;
+* the residual is applied to the source of the Poisson solve, not to the potential:
+  ``S' = (1 + g) S + max_residual tanh(o) sigma`` enters ``pm_gravity_corrected`` as the
+  density residual ``rho_add = (S' - S) / (1.5 Omega_m)``.  The reference's ``g phi`` term is
+  exactly ``g S`` through the linear Poisson solve; the convolutional term is learned in
+  source units instead of potential units.
+
+Parameters live in one flat float32 device vector (with a bf16 mirror the matrix cores read),
+laid out layer by layer as ``[W (27*16*C_in), b (16)]`` at 64-element-aligned offsets, then the
+four gain parameters.  The backward pass (``residual_backward``) returns the gradient in the
+same layout for the optimizer (``training.py``).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import torch
+
+from .. import _lib
+from .._torch import ensure_init, ptr, require_device, stream_handle, workspace
+from .conv import conv3d, conv3d_bwd_data, conv3d_bwd_weight
+
+CHANNELS = 16
+IN_CHANNELS = 8
+LIVE_IN_CHANNELS = 4  # S / sigma, a, Omega_m, sigma8
+TAPS = 27
+
+
+def _align(n: int, a: int = 64) -> int:
+    return (n + a - 1) // a * a
+
+
+@dataclasses.dataclass(frozen=True)
+class Layer:
+    name: str
+    cin: int
+    offset: int  # first weight element; the bias follows the 27*16*cin weights
+
+    @property
+    def nweights(self) -> int:
+        return TAPS * CHANNELS * self.cin
+
+    @property
+    def size(self) -> int:
+        return self.nweights + CHANNELS
+
+
+def param_layout(depth: int):
+    """(conv layers, gain offset, total size) of the flat parameter vector."""
+    if depth < 1:
+        raise ValueError("depth must be >= 1")
+    layers: List[Layer] = []
+    off = 0
+
+    def add(name, cin):
+        nonlocal off
+        layers.append(Layer(name, cin, off))
+        off = _align(off + TAPS * CHANNELS * cin + CHANNELS)
+
+    add("stem", IN_CHANNELS)
+    for b in range(depth - 1):
+        add(f"block_{b}_conv1", CHANNELS)
+        add(f"block_{b}_conv2", CHANNELS)
+    add("out", CHANNELS)
+    gain = off
+    return layers, gain, _align(gain + 4)
+
+
+@dataclasses.dataclass
+class MeshCNNPotentialCorrection:
+    """Trainable mesh correction (``mesh_cnn.py:136-160``): parameters plus hyperparameters."""
+
+    params: torch.Tensor
+    depth: int = 2
+    max_residual: float = 0.1
+    output_init_scale: float = 1e-2
+    sigma8_value: float = 0.8
+    params_bf16: Optional[torch.Tensor] = None
+
+    def __post_init__(self):
+        require_device(self.params, "params")
+        layers, gain, total = param_layout(self.depth)
+        if self.params.dtype != torch.float32 or self.params.numel() != total or not self.params.is_contiguous():
+            raise ValueError(f"params must be a contiguous float32 vector of {total} elements")
+        self.layers = layers
+        self.gain_offset = gain
+        if self.params_bf16 is None:
+            self.params_bf16 = torch.empty(total, dtype=torch.bfloat16, device=self.params.device)
+            self.sync_bf16()
+
+    def sync_bf16(self):
+        """Refresh the bf16 mirror after the float32 parameters changed outside the optimizer."""
+        ensure_init(self.params.device)
+        _lib.call("pm_to_bf16", ptr(self.params), self.params.numel(), ptr(self.params_bf16),
+                  stream_handle(self.params.device))
+
+    # views into the flat vectors
+    def weight(self, layer: Layer) -> torch.Tensor:
+        return self.params_bf16[layer.offset:layer.offset + layer.nweights]
+
+    def bias(self, layer: Layer) -> torch.Tensor:
+        return self.params[layer.offset + layer.nweights:layer.offset + layer.size]
+
+    @property
+    def gain_params(self) -> torch.Tensor:
+        return self.params[self.gain_offset:self.gain_offset + 4]
+
+
+def _truncated_lecun(gen, fan_in, n):
+    # VarianceScaling(1, "fan_in", "truncated_normal"): a normal truncated at two standard
+    # deviations, rescaled so that the truncated distribution has variance 1 / fan_in
+    std = math.sqrt(1.0 / fan_in) / 0.87962566103423978
+    t = torch.empty(n, dtype=torch.float32)
+    torch.nn.init.trunc_normal_(t, mean=0.0, std=std, a=-2 * std, b=2 * std, generator=gen)
+    return t
+
+
+def initial_params(seed=0, depth=2, output_init_scale=1e-2) -> torch.Tensor:
+    """The flat float32 initial parameters on the host (see ``init_mesh_cnn_potential_correction``)."""
+    layers, gain, total = param_layout(depth)
+    gen = torch.Generator().manual_seed(int(seed))
+    host = torch.zeros(total, dtype=torch.float32)
+    for layer in layers:
+        if layer.name == "out":
+            continue
+        live = LIVE_IN_CHANNELS if layer.name == "stem" else CHANNELS
+        w = _truncated_lecun(gen, TAPS * live, TAPS * CHANNELS * layer.cin).view(TAPS, CHANNELS, layer.cin)
+        w[:, :, live:] = 0.0
+        host[layer.offset:layer.offset + layer.nweights] = w.reshape(-1)
+    host[gain:gain + 3] = torch.randn(3, generator=gen) * output_init_scale
+    return host
+
+
+def init_mesh_cnn_potential_correction(seed=0, depth=2, max_residual=0.1, output_init_scale=1e-2, sigma8_value=0.8,
+                                       device="cuda"):
+    """A freshly initialised correction (``mesh_cnn.py:163-214``): LeCun truncated-normal
+    convolutions with zero biases, a zero output layer and a small random gain layer, so the
+    initial correction is exactly zero."""
+    host = initial_params(seed, depth, output_init_scale)
+    return MeshCNNPotentialCorrection(params=host.to(device), depth=depth, max_residual=max_residual,
+                                      output_init_scale=output_init_scale, sigma8_value=sigma8_value)
+
+
+@dataclasses.dataclass
+class Tape:
+    """What the backward pass needs from a forward evaluation."""
+
+    rho: torch.Tensor
+    stats: torch.Tensor
+    x0: torch.Tensor
+    pres: list  # float32 pre-activations of stem and block layers
+    acts: list  # bf16 activations
+    out: torch.Tensor
+    a: float
+    omega_m: float
+
+
+def _workspace(dev):
+    return workspace(_lib.size_query("pm_correction_workspace"), dev)
+
+
+def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=False):
+    """Density residual ``rho_add`` of the correction for the density mesh ``rho`` (X, Y, Z).
+
+    Returns ``(rho_add, tape)``; ``tape`` is None unless ``keep`` (then it holds the
+    activations for ``residual_backward``)."""
+    require_device(rho, "rho")
+    if rho.dtype != torch.float32 or rho.ndim != 3:
+        raise ValueError("rho must be a float32 (X, Y, Z) mesh")
+    dev = rho.device
+    ensure_init(dev)
+    rho = rho.contiguous()
+    shape = tuple(rho.shape)
+    n = rho.numel()
+    a, om = float(a), float(omega_m)
+    ws = _workspace(dev)
+    stats = torch.empty(4, dtype=torch.float32, device=dev)
+    x0 = torch.empty(shape + (IN_CHANNELS,), dtype=torch.bfloat16, device=dev)
+    s = stream_handle(dev)
+    _lib.call("pm_correction_features", ptr(rho), n, om, a, float(model.sigma8_value), float(model.max_residual),
+              ptr(model.gain_params), ptr(x0), ptr(stats), ptr(ws), ws.numel(), s)
+    layers = model.layers
+    pres, acts = [], []
+    pre, x = conv3d(x0, model.weight(layers[0]), model.bias(layers[0]), gelu=True, want_pre=keep)
+    pres.append(pre)
+    acts.append(x)
+    for i in range(model.depth - 1):
+        l1, l2 = layers[1 + 2 * i], layers[2 + 2 * i]
+        p1, h = conv3d(x, model.weight(l1), model.bias(l1), gelu=True, want_pre=keep)
+        p2, x = conv3d(h, model.weight(l2), model.bias(l2), res=x, gelu=True, want_pre=keep)
+        pres += [p1, p2]
+        acts += [h, x]
+    out, _ = conv3d(x, model.weight(layers[-1]), model.bias(layers[-1]), gelu=False, want_pre=True, want_act=False)
+    rho_add = torch.empty_like(rho)
+    _lib.call("pm_correction_residual", ptr(out), ptr(rho), n, om, float(model.max_residual), ptr(stats),
+              ptr(rho_add), s)
+    tape = Tape(rho, stats, x0, pres, acts, out, a, om) if keep else None
+    return rho_add, tape
+
+
+def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, grads=None):
+    """Gradient of ``<rho_cot, rho_add>`` with respect to the parameters (flat float32, the
+    parameter layout); written into ``grads`` when given."""
+    dev = tape.rho.device
+    n = tape.rho.numel()
+    shape = tuple(tape.rho.shape)
+    if grads is None:
+        grads = torch.zeros_like(model.params)
+    rc = rho_cot.contiguous()
+    if rc.dtype != torch.float32 or tuple(rc.shape) != shape:
+        raise ValueError("rho_cot must be a float32 mesh shaped like rho")
+    ws = _workspace(dev)
+    dz = torch.empty(shape + (CHANNELS,), dtype=torch.float32, device=dev)
+    g = model.gain_offset
+    _lib.call("pm_correction_residual_bwd", ptr(rc), ptr(tape.out), ptr(tape.rho), n, tape.omega_m, tape.a,
+              float(model.sigma8_value), float(model.max_residual), ptr(tape.stats), ptr(dz), ptr(grads[g:g + 4]),
+              ptr(ws), ws.numel(), stream_handle(dev))
+    layers = model.layers
+
+    def wgrad(layer, dzl, x):
+        conv3d_bwd_weight(dzl, x, out=grads[layer.offset:layer.offset + layer.size])
+
+    # output layer: input acts[-1] = gelu(pres[-1])
+    wgrad(layers[-1], dz, tape.acts[-1])
+    dz = conv3d_bwd_data(dz, model.weight(layers[-1]), gact=tape.pres[-1])
+    for i in reversed(range(model.depth - 1)):
+        l1, l2 = layers[1 + 2 * i], layers[2 + 2 * i]
+        x_in, pre_in = tape.acts[2 * i], tape.pres[2 * i]
+        h, pre1 = tape.acts[2 * i + 1], tape.pres[2 * i + 1]
+        wgrad(l2, dz, h)
+        dz1 = conv3d_bwd_data(dz, model.weight(l2), gact=pre1)
+        wgrad(l1, dz1, x_in)
+        dz = conv3d_bwd_data(dz1, model.weight(l1), add=dz, gact=pre_in)
+    wgrad(layers[0], dz, tape.x0)
+    return grads

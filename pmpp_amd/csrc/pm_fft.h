@@ -264,7 +264,26 @@ struct SpecTables {
   const float* k[3];     // angular wavenumbers
   const float* mask[3];  // particle-Nyquist masks (0/1)
   const float* negk[3];  // -k with the Nyquist derivative zeroed
+  // the scalars the tables are made from, for passes that evaluate them in registers
+  double period;    // 2 pi / cell
+  float klim;       // particle-Nyquist limit pi/l_p (1 + 8 eps_f32), rounded as the reference
+  float nyq;        // mesh Nyquist pi / cell
+  float nyq_eps;    // its zero-derivative tolerance (pi / cell) eps_f32
+  int mask_active;
 };
+
+// Angular wavenumber of index i on an axis of n points (numerics/fft.py:30-41): the double
+// product m * (1/n) * (2 pi / cell) rounded once to float; `half` is the rfft axis.
+__device__ __forceinline__ float spec_k(int i, int n, bool half, double period) {
+  const int m = half ? i : (i < (n + 1) / 2 ? i : i - n);
+  return (float)((double)m * (1.0 / (double)n) * period);
+}
+__device__ __forceinline__ bool spec_keep(float k, const SpecTables& tb) {
+  return !tb.mask_active || fabsf(k) <= tb.klim;
+}
+__device__ __forceinline__ float spec_negk(float k, const SpecTables& tb) {
+  return (fabsf(fabsf(k) - tb.nyq) <= tb.nyq_eps) ? 0.f : -k;
+}
 
 // Column pass: FFT along a strided axis over tiles of T adjacent columns (unit stride).
 // element (row r, column c, batch b) lives at base[b*s_batch + r*s_row + c*s_col].
@@ -306,6 +325,8 @@ struct ZPass {
   // nyq == 0, as zero
   int kz_lim = 0x7fffffff;
   int nyq = 1;
+  // R2C: optional second real input (same layout) added before the pre-op
+  const float* rin_add = nullptr;
 };
 
 }  // namespace fft

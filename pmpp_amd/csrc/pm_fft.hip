@@ -140,14 +140,17 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
       int kz = (int)(b * io.kz_b) + cgl * io.kz_c + io.kz0;
       float kyv = tb.k[1][ky], kzv = tb.k[2][kz];
       bool keep_yz = cok && tb.mask[1][ky] != 0.f && tb.mask[2][kz] != 0.f;
+      // k_x, its mask and -k_x are evaluated in registers (the same expressions as the
+      // tables): 3*E table loads per thread would be hoisted above the transform and stay live
 #pragma unroll
       for (int s = 0; s < E; ++s) {
         int kx = xrow<N>(p, s);
-        float kxv = tb.k[0][kx];
+        float kxv = spec_k(kx, N, false, tb.period);
         float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-        bool keep = keep_yz && tb.mask[0][kx] != 0.f && k2 != 0.f;
-        float2 phi = keep ? make_float2(-v[s].x / k2, -v[s].y / k2) : make_float2(0.f, 0.f);
-        v[s] = (half == 1) ? c_imul(tb.negk[0][kx], phi) : phi;
+        bool keep = keep_yz && spec_keep(kxv, tb) && k2 != 0.f;
+        float g = keep ? -1.f / k2 : 0.f;  // one division per mode, shared by both parts
+        float2 phi = make_float2(v[s].x * g, v[s].y * g);
+        v[s] = (half == 1) ? c_imul(spec_negk(kxv, tb), phi) : phi;
       }
     } else if constexpr (MODE == kColInv2) {
       if (half == 1) {
@@ -199,6 +202,14 @@ __global__ void __launch_bounds__(ZCfg<M>::THREADS) k_zpass(ZPass zp, SpecTables
     const float* src = zp.rin + lgl * zp.pitch_r;
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = *reinterpret_cast<const float2*>(src + 2 * nrow<M>(p, s));
+    if (zp.rin_add) {  // uniform: the added field's loads join the same batch
+      const float* src2 = zp.rin_add + lgl * zp.pitch_r;
+#pragma unroll
+      for (int s = 0; s < E; ++s) {
+        const float2 w = *reinterpret_cast<const float2*>(src2 + 2 * nrow<M>(p, s));
+        v[s] = make_float2(v[s].x + w.x, v[s].y + w.y);
+      }
+    }
 #pragma unroll
     for (int s = 0; s < E; ++s)
       v[s] = make_float2((v[s].x + zp.src_shift) * zp.src_scale, (v[s].y + zp.src_shift) * zp.src_scale);
@@ -273,25 +284,16 @@ __global__ void __launch_bounds__(ZCfg<M>::THREADS) k_zpass(ZPass zp, SpecTables
 // float (numerics/fft.py:30-41), the particle-Nyquist masks (core/utils.py:33-37) and
 // negk with the derivative zeroed at the mesh Nyquist (configuration.py:369-376), all
 // decided in float32 like the reference.
-__global__ void k_spec_tables(float* out, int n0, int n1, int n2, double period, float k_nyq_ptcl,
-                              int mask_active, float nyq_mesh) {
+__global__ void k_spec_tables(float* out, int n0, int n1, int n2, SpecTables sc) {
   int n[3] = {n0, n1, n2};
   int len[3] = {n0, n1, n2 / 2 + 1};
   int off = 0;
-  const float eps32 = 1.1920928955078125e-07f;
   for (int a = 0; a < 3; ++a) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len[a]; i += gridDim.x * blockDim.x) {
-      int m = (a == 2) ? i : (i < (n[a] + 1) / 2 ? i : i - n[a]);
-      double k64 = (double)m * (1.0 / (double)n[a]) * period;
-      float k = (float)k64;
-      float* kk = out + off;
-      float* mk = out + off + len[a];
-      float* nk = out + off + 2 * len[a];
-      kk[i] = k;
-      float limit = k_nyq_ptcl + k_nyq_ptcl * (8.0f * eps32);
-      mk[i] = (!mask_active || fabsf(k) <= limit) ? 1.f : 0.f;
-      float eps = nyq_mesh * eps32;
-      nk[i] = (fabsf(fabsf(k) - nyq_mesh) <= eps) ? 0.f : -k;
+      const float k = spec_k(i, n[a], a == 2, sc.period);
+      out[off + i] = k;
+      out[off + len[a] + i] = spec_keep(k, sc) ? 1.f : 0.f;
+      out[off + 2 * len[a] + i] = spec_negk(k, sc);
     }
     off += 3 * len[a];
   }
@@ -409,8 +411,22 @@ size_t spec_table_floats(const int32_t* shape) {
   return 3 * ((size_t)shape[0] + shape[1] + (shape[2] / 2 + 1));
 }
 
-SpecTables spec_tables_view(const float* base, const int32_t* shape) {
-  SpecTables tb;
+SpecTables spec_scalars(double cell_size, double ptcl_spacing, int mask_active) {
+  SpecTables tb{};
+  const float eps32 = 1.1920928955078125e-07f;
+  tb.period = 2.0 * 3.14159265358979323846 / cell_size;
+  const float k_nyq_ptcl = (float)(3.14159265358979323846 / ptcl_spacing);
+  // two float32 roundings as the reference (volatile: no contraction into an fma)
+  volatile float tol = k_nyq_ptcl * (8.0f * eps32);
+  tb.klim = k_nyq_ptcl + tol;
+  tb.nyq = (float)(3.14159265358979323846 / cell_size);
+  tb.nyq_eps = tb.nyq * eps32;
+  tb.mask_active = mask_active;
+  return tb;
+}
+
+SpecTables spec_tables_view(const float* base, const int32_t* shape, const SpecTables& scalars) {
+  SpecTables tb = scalars;
   int len[3] = {shape[0], shape[1], shape[2] / 2 + 1};
   size_t off = 0;
   for (int a = 0; a < 3; ++a) {
@@ -422,13 +438,8 @@ SpecTables spec_tables_view(const float* base, const int32_t* shape) {
   return tb;
 }
 
-int launch_spec_tables(float* out, const int32_t* shape, double cell_size, double ptcl_spacing,
-                       int mask_active, hipStream_t s) {
-  double period = 2.0 * 3.14159265358979323846 / cell_size;
-  float k_nyq_ptcl = (float)(3.14159265358979323846 / ptcl_spacing);
-  float nyq_mesh = (float)(3.14159265358979323846 / cell_size);
-  hipLaunchKernelGGL(k_spec_tables, dim3(8), dim3(256), 0, s, out, shape[0], shape[1], shape[2], period,
-                     k_nyq_ptcl, mask_active, nyq_mesh);
+int launch_spec_tables(float* out, const int32_t* shape, const SpecTables& scalars, hipStream_t s) {
+  hipLaunchKernelGGL(k_spec_tables, dim3(8), dim3(256), 0, s, out, shape[0], shape[1], shape[2], scalars);
   PM_LAUNCHED("k_spec_tables", s);
   return PM_SUCCESS;
 }

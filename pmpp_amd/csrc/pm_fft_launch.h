@@ -13,9 +13,9 @@ int launch_z(int nz, bool c2r, const ZPass& zp, const SpecTables& tb, hipStream_
 
 // Spectral tables: 3 * (nx + ny + nz/2+1) floats, laid out per axis as k | mask | negk.
 size_t spec_table_floats(const int32_t* shape);
-SpecTables spec_tables_view(const float* base, const int32_t* shape);
-int launch_spec_tables(float* out, const int32_t* shape, double cell_size, double ptcl_spacing,
-                       int mask_active, hipStream_t s);
+SpecTables spec_scalars(double cell_size, double ptcl_spacing, int mask_active);
+SpecTables spec_tables_view(const float* base, const int32_t* shape, const SpecTables& scalars);
+int launch_spec_tables(float* out, const int32_t* shape, const SpecTables& scalars, hipStream_t s);
 int launch_spectral_multiply(const float2* in, float2* out, const int32_t* shape, const SpecTables& tb,
                              int axis, int adjoint, hipStream_t s);
 

@@ -659,15 +659,21 @@ __global__ void k_adjoint_combine(float2* __restrict__ a, const float2* __restri
   }
 }
 
-// omega_cot += 1.5 * sum_x src_cot(x) * (rho(x) - 1); dens_cot = src_cot * scale, in place
+// omega_cot += 1.5 * sum_x src_cot(x) * (rho(x) [+ rho_add(x)] - 1); dens_cot = src_cot * scale,
+// in place, and copied to rho_cot when given (the cotangent of an added density)
 __global__ void __launch_bounds__(kBlock) k_source_bwd(float* __restrict__ src_cot,
-                                                       const float* __restrict__ rho, int64_t n,
+                                                       const float* __restrict__ rho,
+                                                       const float* __restrict__ rho_add,
+                                                       float* __restrict__ rho_cot, int64_t n,
                                                        float scale, float* __restrict__ omega_cot) {
   float acc = 0.f;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     float sc = src_cot[i];
-    acc += sc * (rho[i] - 1.f);
+    float r = rho[i];
+    if (rho_add) r += rho_add[i];
+    acc += sc * (r - 1.f);
     src_cot[i] = sc * scale;
+    if (rho_cot) rho_cot[i] = sc * scale;
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
   __shared__ float part[kBlock / 64];
@@ -906,7 +912,7 @@ int gather_stage(const int16_t* pmid_s, const float* disp_s, const uint8_t* vali
 
 // Density (b.r0) -> force meshes F_x (b.r0), F_y (b.r1), F_z (b.r2).
 int force_mesh_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b, const fft::SpecTables& tb,
-                     hipStream_t s, bool keep_density) {
+                     hipStream_t s, bool keep_density, const float* rho_add) {
   using namespace fft;
   const float coef = 1.5f * d->omega_m;  // 1.5 * Omega_m in float32 (gravity.py:630)
   const bool zband = L.nkz <= L.m;      // Nyquist column not stored
@@ -933,6 +939,7 @@ int force_mesh_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b, cons
   z1.nlines = (int64_t)L.nx * L.ny;
   z1.src_shift = -1.f;
   z1.src_scale = coef;
+  z1.rin_add = rho_add;  // learned density correction (null: none)
   z1.kz_lim = L.nkz;
   z1.nyq = zband ? 0 : 1;
   PM_TRY(launch_z(L.nz, false, z1, tb, s));
@@ -995,8 +1002,9 @@ int prepare(const pm_gravity_desc* d, int64_t n, void* workspace, size_t workspa
   L = make_layout(d);
   PM_RETURN_IF(!carve(ws, L, n, d->mesh_shape, d->brick, reverse, b), PM_ERROR_WORKSPACE,
                "workspace carve failed");
-  PM_TRY(fft::launch_spec_tables(b.tables, d->mesh_shape, d->cell_size, d->ptcl_spacing, d->mask_active, s));
-  tb = fft::spec_tables_view(b.tables, d->mesh_shape);
+  const fft::SpecTables sc = fft::spec_scalars(d->cell_size, d->ptcl_spacing, d->mask_active);
+  PM_TRY(fft::launch_spec_tables(b.tables, d->mesh_shape, sc, s));
+  tb = fft::spec_tables_view(b.tables, d->mesh_shape, sc);
   return PM_SUCCESS;
 }
 
@@ -1027,6 +1035,12 @@ int pm_gravity_vjp_workspace(const pm_gravity_desc* desc, int64_t n, size_t* byt
 int pm_gravity(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                const pm_gravity_desc* desc, float* acc, void* workspace, size_t workspace_bytes,
                void* stream) {
+  return pm_gravity_corrected(pmid, disp, valid, n, desc, nullptr, acc, workspace, workspace_bytes, stream);
+}
+
+int pm_gravity_corrected(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                         const pm_gravity_desc* desc, const float* rho_add, float* acc, void* workspace,
+                         size_t workspace_bytes, void* stream) {
   hipStream_t s = as_stream(stream);
   grav::Layout L;
   grav::Buffers b{};
@@ -1038,14 +1052,40 @@ int pm_gravity(const int16_t* pmid, const float* disp, const uint8_t* valid, int
                "pmid/disp/acc NULL");
   st = grav::density_stage(pmid, disp, valid, n, desc, L, b, ws, s);
   if (st) return st;
-  st = grav::force_mesh_stage(desc, L, b, tb, s, false);
+  st = grav::force_mesh_stage(desc, L, b, tb, s, false, rho_add);
   if (st) return st;
   return grav::gather_stage(b.pmid_s, b.disp_s, b.valid_s, valid, n, desc, b, b.r0, b.r1, b.r2, acc, s);
+}
+
+int pm_gravity_density(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                       const pm_gravity_desc* desc, float* rho, void* workspace, size_t workspace_bytes,
+                       void* stream) {
+  hipStream_t s = as_stream(stream);
+  grav::Layout L;
+  grav::Buffers b{};
+  Arena ws(workspace, workspace_bytes);
+  fft::SpecTables tb;
+  int st = grav::prepare(desc, n, workspace, workspace_bytes, false, L, b, ws, tb, s);
+  if (st) return st;
+  PM_RETURN_IF(rho == nullptr || (n > 0 && (pmid == nullptr || disp == nullptr)), PM_ERROR_INVALID_ARGUMENT,
+               "pmid/disp/rho NULL");
+  st = grav::density_stage(pmid, disp, valid, n, desc, L, b, ws, s);
+  if (st) return st;
+  PM_HIP_TRY(hipMemcpyAsync(rho, b.r0, sizeof(float) * L.nreal, hipMemcpyDeviceToDevice, s));
+  return PM_SUCCESS;
 }
 
 int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                    const pm_gravity_desc* desc, const float* acc_cot, float* disp_cot,
                    float* omega_cot, void* workspace, size_t workspace_bytes, void* stream) {
+  return pm_gravity_corrected_vjp(pmid, disp, valid, n, desc, nullptr, acc_cot, disp_cot, omega_cot, nullptr,
+                                  workspace, workspace_bytes, stream);
+}
+
+int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                             const pm_gravity_desc* desc, const float* rho_add, const float* acc_cot,
+                             float* disp_cot, float* omega_cot, float* rho_cot, void* workspace,
+                             size_t workspace_bytes, void* stream) {
   hipStream_t s = as_stream(stream);
   grav::Layout L;
   grav::Buffers b{};
@@ -1060,7 +1100,7 @@ int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid,
   // recompute the forward meshes: rho stays in r0, F_x goes to g0, F_y r1, F_z r2
   st = grav::density_stage(pmid, disp, valid, n, desc, L, b, ws, s);
   if (st) return st;
-  st = grav::force_mesh_stage(desc, L, b, tb, s, true);
+  st = grav::force_mesh_stage(desc, L, b, tb, s, true, rho_add);
   if (st) return st;
   // gather^T: acc_cot scattered into three cotangent meshes, displacement cotangent from F
   st = grav::fill(b.g1, L.nreal, s);
@@ -1096,7 +1136,8 @@ int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid,
   {
     const float coef = 1.5f * desc->omega_m;
     unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(L.nreal, 256 * 4), 4096);
-    hipLaunchKernelGGL(grav::k_source_bwd, dim3(blocks), dim3(256), 0, s, b.g1, b.r0, L.nreal, coef, omega_cot);
+    hipLaunchKernelGGL(grav::k_source_bwd, dim3(blocks), dim3(256), 0, s, b.g1, b.r0, rho_add, rho_cot, L.nreal, coef,
+                       omega_cot);
     PM_LAUNCHED("k_source_bwd", s);
   }
   if (n > 0) {
@@ -1131,9 +1172,10 @@ int pm_spectral_force_multiply(const float* in_complex, float* out_complex, cons
                "spectral workspace too small: need " + std::to_string(need) + " bytes");
   hipStream_t s = as_stream(stream);
   float* tables = static_cast<float*>(workspace);
-  st = fft::launch_spec_tables(tables, desc->mesh_shape, desc->cell_size, desc->ptcl_spacing, desc->mask_active, s);
+  const fft::SpecTables sc = fft::spec_scalars(desc->cell_size, desc->ptcl_spacing, desc->mask_active);
+  st = fft::launch_spec_tables(tables, desc->mesh_shape, sc, s);
   if (st) return st;
-  fft::SpecTables tb = fft::spec_tables_view(tables, desc->mesh_shape);
+  fft::SpecTables tb = fft::spec_tables_view(tables, desc->mesh_shape, sc);
   return fft::launch_spectral_multiply(reinterpret_cast<const float2*>(in_complex),
                                        reinterpret_cast<float2*>(out_complex), desc->mesh_shape, tb, axis,
                                        adjoint, s);

@@ -73,12 +73,19 @@ def gravity(a, ptcl, cosmo, conf, correction=None):
     """Gravitational accelerations of particles in [H_0^2] (gravity.py:786-814).
 
     ``cosmo`` needs an ``Omega_m`` (float or tensor; a tensor with ``requires_grad`` gets a
-    cotangent).  ``a`` is unused by the uncorrected force, as in the reference.
+    cotangent).  ``a`` is unused by the uncorrected force, as in the reference.  A
+    ``MeshCNNPotentialCorrection`` adds its density residual before the Poisson solve
+    (gravity.py:635-640 with corrections/core.py:286-334); that path has no reverse mode
+    through the network's inputs and refuses ``requires_grad`` inputs.
     """
-    del a
     if correction is not None:
-        # the learned potential correction (corrections/core.py:286-334) is not built yet
-        raise NotImplementedError("gravity(correction=...) is not supported by the HIP force step yet")
+        from ..corrections.training import corrected_gravity
+
+        om = cosmo.Omega_m if hasattr(cosmo, "Omega_m") else cosmo
+        if ptcl.disp.requires_grad or (isinstance(om, torch.Tensor) and om.requires_grad):
+            raise NotImplementedError("gradients through the corrected force's network inputs are not implemented")
+        return corrected_gravity(a, ptcl, cosmo, conf, correction)
+    del a
     omega = cosmo.Omega_m if hasattr(cosmo, "Omega_m") else cosmo
     if not isinstance(omega, torch.Tensor):
         omega = torch.tensor(float(omega), dtype=torch.float32, device=ptcl.disp.device)

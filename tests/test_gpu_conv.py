@@ -1,0 +1,80 @@
+"""BF16-MFMA periodic conv3d kernels vs a plain PyTorch float32 reference (CPU).
+
+Inputs are bf16-representable, so products are exact and only float32 summation order
+differs: tolerance 1e-4 relative to the output scale.
+"""
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as Fn
+
+from pmpp_amd.corrections import conv3d, conv3d_bwd_data, conv3d_bwd_weight
+
+pytestmark = pytest.mark.gpu
+
+
+def _ref_conv(x, w, b):
+    """x (X,Y,Z,C) float32, w (27,16,C) -> (X,Y,Z,16) periodic conv, float32 on the CPU."""
+    xt = x.permute(3, 0, 1, 2)[None]
+    wt = w.reshape(3, 3, 3, 16, -1).permute(3, 4, 0, 1, 2)  # (co, ci, kx, ky, kz), tap = (dx+1)*9+...
+    xp = Fn.pad(xt, (1, 1, 1, 1, 1, 1), mode="circular")
+    y = Fn.conv3d(xp, wt)  # cross-correlation: out[v] = sum w[off] x[v + off]
+    return y[0].permute(1, 2, 3, 0) + b
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+@pytest.mark.parametrize("cin,shape,gelu", [(8, (8, 8, 8), False), (16, (16, 8, 24), True), (8, (32, 16, 16), True)])
+def test_forward_matches_torch(device, cin, shape, gelu):
+    g = torch.Generator().manual_seed(cin + sum(shape))
+    x = _bf(torch.randn(shape + (cin,), generator=g))
+    w = _bf(torch.randn(27, 16, cin, generator=g) * 0.2)
+    b = torch.randn(16, generator=g)
+    res = _bf(torch.randn(shape + (16,), generator=g))
+    pre, act = conv3d(x.to(device, torch.bfloat16), w.to(device, torch.bfloat16), b.to(device),
+                      res=res.to(device, torch.bfloat16), gelu=gelu)
+    ref = _ref_conv(x, w, b) + res
+    scale = ref.abs().max().item()
+    np.testing.assert_allclose(pre.cpu().numpy(), ref.numpy(), atol=1e-4 * scale)
+    r_act = Fn.gelu(ref, approximate="tanh") if gelu else ref
+    np.testing.assert_allclose(act.float().cpu().numpy(), _bf(r_act).numpy(), atol=1e-2 * scale)
+
+
+def test_backward_data_and_weight_match_autograd(device):
+    shape = (16, 16, 8)
+    g = torch.Generator().manual_seed(3)
+    x = _bf(torch.randn(shape + (16,), generator=g))
+    w = _bf(torch.randn(27, 16, 16, generator=g) * 0.2)
+    dz = _bf(torch.randn(shape + (16,), generator=g))
+    xr = x.clone().requires_grad_(True)
+    wr = w.clone().requires_grad_(True)
+    br = torch.zeros(16, requires_grad=True)
+    y = _ref_conv(xr, wr, br)
+    y.backward(dz)
+    dx = conv3d_bwd_data(dz.to(device), w.to(device, torch.bfloat16))
+    np.testing.assert_allclose(dx.cpu().numpy(), xr.grad.numpy(), atol=1e-4 * xr.grad.abs().max().item())
+    dw, db = conv3d_bwd_weight(dz.to(device), x.to(device, torch.bfloat16))
+    np.testing.assert_allclose(dw.cpu().numpy(), wr.grad.numpy(), atol=1e-4 * wr.grad.abs().max().item())
+    np.testing.assert_allclose(db.cpu().numpy(), br.grad.numpy(), atol=1e-4 * br.grad.abs().max().item())
+    # the fused gelu' and residual terms of the data gradient
+    gact = torch.randn(shape + (16,), generator=g)
+    add = torch.randn(shape + (16,), generator=g)
+    dx2 = conv3d_bwd_data(dz.to(device), w.to(device, torch.bfloat16), add=add.to(device), gact=gact.to(device))
+    t = torch.tanh(0.7978845608028654 * (gact + 0.044715 * gact ** 3))
+    gg = 0.5 * (1 + t) + 0.5 * gact * (1 - t * t) * 0.7978845608028654 * (1 + 3 * 0.044715 * gact ** 2)
+    np.testing.assert_allclose(dx2.cpu().numpy(), ((xr.grad + add) * gg).numpy(), rtol=1e-4, atol=1e-4)
+
+
+def test_weight_gradient_cin8(device):
+    shape = (8, 16, 16)
+    g = torch.Generator().manual_seed(4)
+    x = _bf(torch.randn(shape + (8,), generator=g))
+    w = _bf(torch.randn(27, 16, 8, generator=g) * 0.2)
+    dz = _bf(torch.randn(shape + (16,), generator=g))
+    wr = w.clone().requires_grad_(True)
+    _ref_conv(x, wr, torch.zeros(16)).backward(dz)
+    dw, db = conv3d_bwd_weight(dz.to(device), x.to(device, torch.bfloat16))
+    np.testing.assert_allclose(dw.cpu().numpy(), wr.grad.numpy(), atol=1e-4 * wr.grad.abs().max().item())

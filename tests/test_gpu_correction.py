@@ -1,0 +1,262 @@
+"""The learned force correction: corrected force step vs the float64 oracle, the network and
+its backward vs a float32 PyTorch restatement, the loss and optimizer kernels, and training on
+coarse/fine force pairs.
+
+Tolerances: the corrected force and its adjoint carry the force step's float32 bar (1e-5
+relative field error vs float64).  The network runs bf16 activations with float32
+accumulation, so it is compared with a float32 PyTorch network fed the same bf16-rounded
+inputs, weights and activations: 2e-2 relative (norm) on the residual and on each layer's
+gradient, which the backward's bf16 rounding of the pre-activation gradients sets.
+"""
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from oracle import cic, spectral
+from pmpp_amd import Configuration, Particles, gravity
+from pmpp_amd.corrections import (build_correction_optimizer, density, evaluate_residual, force_loss,
+                                  gravity_with_residual, gravity_with_residual_vjp,
+                                  init_mesh_cnn_potential_correction, residual_backward, train_step)
+from pmpp_amd.initial_conditions import LCDM, particles_2lpt
+
+pytestmark = pytest.mark.gpu
+
+
+def _ptcl(conf, disp_scale, seed, device):
+    base = Particles.gen_grid(conf, device=device)
+    rng = np.random.default_rng(seed)
+    d = base.disp.cpu().numpy() + rng.normal(size=base.disp.shape) * disp_scale * conf.ptcl_spacing
+    return base.replace(disp=torch.as_tensor(d.astype(np.float32), device=device))
+
+
+def _np(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def _masks(conf):
+    return spectral.particle_nyquist_masks(conf.mesh_shape, conf.ptcl_grid_shape, conf.ptcl_spacing,
+                                           conf.cell_size)
+
+
+@pytest.mark.parametrize("ng,ratio,box", [(16, 1, 32.0), (8, 2, 16.0), (16, 2, 48.0)])
+def test_corrected_force_and_adjoint_match_oracle(device, ng, ratio, box):
+    conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio)
+    ptcl = _ptcl(conf, 0.5, ng + ratio, device)
+    rng = np.random.default_rng(7)
+    rho_add = (0.2 * rng.normal(size=conf.mesh_shape)).astype(np.float32)
+    ra = torch.from_numpy(rho_add).to(device)
+    acc = _np(gravity_with_residual(ptcl, 0.3, conf, ra))
+    pm, dp = ptcl.pmid.cpu().numpy().astype(np.int64), _np(ptcl.disp)
+    args = (pm, dp, 0.3)
+    ref = spectral.gravity(*args, conf.mesh_shape, conf.cell_size, conf.ptcl_num, _masks(conf), rho_add=rho_add)
+    assert np.linalg.norm(acc - ref) / np.linalg.norm(ref) < 1e-5
+    # no residual is the uncorrected step, bit for bit (same kernels, the added load skipped)
+    np.testing.assert_array_equal(_np(gravity_with_residual(ptcl, 0.3, conf, None)),
+                                  _np(gravity(1.0, ptcl, 0.3, conf)))
+    ac = rng.normal(size=acc.shape).astype(np.float32)
+    dc, oc, rc = gravity_with_residual_vjp(ptcl, 0.3, conf, ra, torch.from_numpy(ac).to(device))
+    dc0, oc0, rc0 = spectral.gravity_vjp(*args, ac, conf.mesh_shape, conf.cell_size, conf.ptcl_num, _masks(conf),
+                                         rho_add=rho_add)
+    assert np.linalg.norm(_np(rc) - rc0) / np.linalg.norm(rc0) < 1e-5
+    assert np.linalg.norm(_np(dc) - dc0) / np.linalg.norm(dc0) < 1e-4
+    assert float(oc) == pytest.approx(oc0, rel=1e-4)
+
+
+def test_density_is_the_step_cic_mesh(device):
+    conf = Configuration(2.0, (16, 16, 16), mesh_shape=2)
+    ptcl = _ptcl(conf, 0.8, 3, device)
+    rho = _np(density(ptcl, 0.3, conf))
+    pm, dp = ptcl.pmid.cpu().numpy().astype(np.int64), _np(ptcl.disp)
+    val = spectral.default_value(len(pm), conf.mesh_shape, conf.ptcl_num)
+    ref = cic.scatter(pm, dp, val, np.zeros(conf.mesh_shape), conf.cell_size, conf.mesh_shape)
+    np.testing.assert_allclose(rho, ref, atol=2e-6 * np.abs(ref).max())
+    assert rho.sum() == pytest.approx(np.prod(conf.mesh_shape), rel=1e-6)
+
+
+def _randomised(model, seed, out_scale=0.05):
+    """Non-zero output and gain layers so that every parameter carries gradient."""
+    g = torch.Generator().manual_seed(seed)
+    p = model.params.cpu()
+    out = model.layers[-1]
+    p[out.offset:out.offset + out.size] = torch.randn(out.size, generator=g) * out_scale
+    p[model.gain_offset:model.gain_offset + 4] = torch.randn(4, generator=g) * 0.3
+    for layer in model.layers[:-1]:  # non-zero biases
+        p[layer.offset + layer.nweights:layer.offset + layer.size] = torch.randn(16, generator=g) * 0.1
+    model.params.copy_(p.to(model.params.device))
+    model.sync_bf16()
+    return model
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+def _torch_conv(x, w, b, cin):
+    # x (X,Y,Z,C) float32; w flat bf16 [tap][cout][cin] with tap = (dx+1)*9 + (dy+1)*3 + (dz+1)
+    k = w.to(torch.float32).view(3, 3, 3, 16, cin).permute(3, 4, 0, 1, 2)
+    xc = x.permute(3, 0, 1, 2)[None]
+    xc = F.pad(xc, (1, 1, 1, 1, 1, 1), mode="circular")
+    return F.conv3d(xc, k, b)[0].permute(1, 2, 3, 0)
+
+
+def _torch_residual(model, rho, a, om, params):
+    """float32 restatement of the network with the kernels' bf16 roundings (differentiable in
+    the float32 parameter vector ``params``)."""
+    r, s8 = model.max_residual, model.sigma8_value
+    c = torch.tensor(1.5 * om, dtype=torch.float32)
+    S = (rho - 1.0) * c
+    sigma = torch.sqrt((S.double() ** 2).mean().float() + 1e-6)
+    ones = torch.ones_like(rho)
+    x0 = torch.stack([S / sigma, a * ones, om * ones, s8 * ones] + [0 * ones] * 4, -1)
+    x0 = _bf(x0)
+
+    def layer(i):
+        L = model.layers[i]
+        return _bf(params[L.offset:L.offset + L.nweights]), params[L.offset + L.nweights:L.offset + L.size]
+
+    gelu = lambda z: F.gelu(z, approximate="tanh")  # noqa: E731
+    w, b = layer(0)
+    x = _bf(gelu(_torch_conv(x0, w, b, 8)))
+    for i in range(model.depth - 1):
+        w1, b1 = layer(1 + 2 * i)
+        w2, b2 = layer(2 + 2 * i)
+        h = _bf(gelu(_torch_conv(x, w1, b1, 16)))
+        x = _bf(gelu(_torch_conv(h, w2, b2, 16) + x))
+    w, b = layer(len(model.layers) - 1)
+    out = _torch_conv(x, w, b, 16)[..., 0]
+    gp = params[model.gain_offset:model.gain_offset + 4]
+    g = r * torch.tanh(gp[0] * a + gp[1] * om + gp[2] * s8 + gp[3])
+    return g * (rho - 1.0) + r * torch.tanh(out) * sigma / c
+
+
+@pytest.mark.parametrize("depth,shape", [(1, (8, 8, 8)), (2, (16, 8, 24)), (3, (16, 16, 16))])
+def test_network_and_backward_match_float32_torch(device, depth, shape):
+    model = _randomised(init_mesh_cnn_potential_correction(seed=depth, depth=depth, device=device), depth)
+    rng = np.random.default_rng(depth)
+    rho = torch.from_numpy((1 + 0.5 * rng.normal(size=shape)).astype(np.float32)).to(device)
+    a, om = 0.7, 0.31
+    rho_add, tape = evaluate_residual(model, rho, a, om, keep=True)
+    params = model.params.detach().cpu().clone().requires_grad_(True)
+    ref = _torch_residual(model, rho.cpu(), a, om, params)
+    got = rho_add.cpu()
+    assert torch.linalg.norm(got - ref.detach()) / torch.linalg.norm(ref.detach()) < 2e-2
+    rho_cot = torch.from_numpy(rng.normal(size=shape).astype(np.float32))
+    grads = residual_backward(model, tape, rho_cot.to(device)).cpu()
+    (ref * rho_cot).sum().backward()
+    gref = params.grad
+    for L in model.layers:
+        sl = slice(L.offset, L.offset + L.size)
+        err = torch.linalg.norm(grads[sl] - gref[sl]) / torch.linalg.norm(gref[sl])
+        assert err < 2e-2, (L.name, float(err))
+    gs = slice(model.gain_offset, model.gain_offset + 4)
+    np.testing.assert_allclose(grads[gs].numpy(), gref[gs].numpy(), rtol=2e-2, atol=1e-6)
+    # the layout's padding never receives gradient
+    live = torch.zeros(model.params.numel(), dtype=torch.bool)
+    for L in model.layers:
+        live[L.offset:L.offset + L.size] = True
+    live[gs] = True
+    assert torch.all(grads[~live] == 0)
+
+
+def test_zero_initialised_correction_is_exactly_the_plain_force(device):
+    conf = Configuration(2.0, (16, 16, 16), mesh_shape=1)
+    ptcl = _ptcl(conf, 0.7, 5, device)
+    model = init_mesh_cnn_potential_correction(seed=0, device=device)
+    with torch.no_grad():
+        model.params[model.gain_offset:model.gain_offset + 4] = 0  # gain 0, output layer 0
+    model.sync_bf16()
+    acc = gravity(1.0, ptcl, 0.3, conf, correction=model)
+    np.testing.assert_array_equal(_np(acc), _np(gravity(1.0, ptcl, 0.3, conf)))
+
+
+def test_force_loss_and_its_gradient(device):
+    rng = np.random.default_rng(2)
+    acc = rng.normal(size=(1000, 3)).astype(np.float32)
+    tgt = rng.normal(size=(1000, 3)).astype(np.float32)
+    loss, cot = force_loss(torch.from_numpy(acc).to(device), torch.from_numpy(tgt).to(device))
+    a64, t64 = acc.astype(np.float64), tgt.astype(np.float64)
+    ref = np.sum((a64 - t64) ** 2) / np.sum(t64 ** 2)
+    assert float(loss[0]) == pytest.approx(ref, rel=1e-12)
+    np.testing.assert_allclose(_np(cot), 2 * (a64 - t64) / np.sum(t64 ** 2), rtol=1e-6)
+    # the loss of the target itself is zero
+    loss0, _ = force_loss(torch.from_numpy(tgt).to(device), torch.from_numpy(tgt).to(device), want_grad=False)
+    assert float(loss0[0]) == 0.0
+
+
+def test_adamax_first_step_clipping_and_non_finite_skip(device):
+    model = init_mesh_cnn_potential_correction(seed=1, device=device)
+    p0 = model.params.clone()
+    n = p0.numel()
+    rng = np.random.default_rng(3)
+    g = torch.from_numpy(rng.normal(size=n).astype(np.float32)).to(device)
+    opt = build_correction_optimizer(1e-3, gradient_clip_norm=1e9)
+    st = opt.init(model.params)
+    opt.step(model, g, st)
+    # first Adamax step: m_hat = g, u = |g| + eps, so the move is lr g / (|g| + eps)
+    np.testing.assert_allclose(_np(model.params - p0), _np(-1e-3 * g / (g.abs() + 1e-8)), rtol=1e-4, atol=2e-8)
+    assert int(st.count.item()) == 1
+    np.testing.assert_array_equal(_np(model.params_bf16.float()), _np(model.params.to(torch.bfloat16).float()))
+    # clipping: the clipped gradient g c / |g| enters the moments
+    model2 = init_mesh_cnn_potential_correction(seed=1, device=device)
+    opt2 = build_correction_optimizer(1e-3, gradient_clip_norm=1.0)
+    st2 = opt2.init(model2.params)
+    opt2.step(model2, g, st2)
+    gc = g / torch.linalg.norm(g.double()).float()
+    np.testing.assert_allclose(_np(st2.mu), _np(0.1 * gc), rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(_np(st2.nu), _np(gc.abs() + 1e-8), rtol=1e-6)
+    # a non-finite gradient leaves parameters, moments and the counter untouched
+    before = [t.clone() for t in (model2.params, st2.mu, st2.nu)]
+    bad = g.clone()
+    bad[5] = float("nan")
+    opt2.step(model2, bad, st2)
+    for t0, t1 in zip(before, (model2.params, st2.mu, st2.nu)):
+        assert torch.equal(t0, t1)
+    assert int(st2.count.item()) == 1
+    # second step against a float64 restatement of clip + adamax
+    g2 = torch.from_numpy(rng.normal(size=n).astype(np.float32)).to(device)
+    p, m, u = _np(model2.params), _np(st2.mu), _np(st2.nu)
+    gg = _np(g2)
+    gg = gg / np.linalg.norm(gg) if np.linalg.norm(gg) >= 1.0 else gg
+    m = 0.9 * m + 0.1 * gg
+    u = np.maximum(0.999 * u, np.abs(gg) + 1e-8)
+    p = p - 1e-3 * (m / (1 - 0.9 ** 2)) / u
+    opt2.step(model2, g2, st2)
+    np.testing.assert_allclose(_np(model2.params), p, rtol=1e-5, atol=1e-9)
+
+
+def _pair(conf, conf_fine, cosmo, seed, device):
+    ptcl = particles_2lpt(conf, cosmo, seed=seed, device=device)
+    fine = Particles.from_pos(conf_fine, ptcl.pos())
+    target = gravity(1.0, fine, cosmo.Omega_m, conf_fine)
+    return ptcl, target
+
+
+def test_training_on_fine_mesh_forces_lowers_held_out_error(device):
+    # 32^3 particles in a 64 Mpc/h box at a = 1 (2LPT), coarse mesh 32^3, target mesh 128^3
+    cosmo = LCDM()
+    conf = Configuration(2.0, (32, 32, 32), mesh_shape=1)
+    conf_fine = Configuration(2.0, (32, 32, 32), mesh_shape=4)
+    train = [_pair(conf, conf_fine, cosmo, s, device) for s in range(3)]
+    held, held_t = _pair(conf, conf_fine, cosmo, 11, device)
+    model = init_mesh_cnn_potential_correction(seed=0, depth=2, device=device)
+    opt = build_correction_optimizer(3e-3)
+    st = opt.init(model.params)
+
+    def held_loss():
+        acc = gravity(1.0, held, cosmo.Omega_m, conf, correction=model)
+        return float(force_loss(acc, held_t, want_grad=False)[0][0])
+
+    before = held_loss()
+    plain = float(force_loss(gravity(1.0, held, cosmo.Omega_m, conf), held_t, want_grad=False)[0][0])
+    assert before == pytest.approx(plain, rel=1e-3)  # zero output layer: only the small gain acts
+    losses = []
+    for it in range(60):
+        ptcl, tgt = train[it % len(train)]
+        losses.append(float(train_step(model, opt, st, ptcl, 1.0, cosmo, conf, tgt)))
+    after = held_loss()
+    print(f"held-out relative force error^2: plain {plain:.4f} -> corrected {after:.4f}; "
+          f"train first {losses[0]:.4f} last {losses[-1]:.4f}")
+    assert np.all(np.isfinite(losses))
+    assert after < 0.8 * plain

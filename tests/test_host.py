@@ -62,3 +62,35 @@ def test_step_factors_match_the_oracle():
     assert np.isclose(integrator._factor_dOm("kick_factor", 0.5, 0.5, 0.55, LCDM(), conf),
                       (integrator.kick_factor(0.5, 0.5, 0.55, LCDM(Omega_m=0.30001), conf)
                        - integrator.kick_factor(0.5, 0.5, 0.55, LCDM(Omega_m=0.29999), conf)) / 2e-5, rtol=1e-5)
+
+
+def test_correction_parameter_layout_and_initialisation():
+    import math
+
+    import torch
+
+    from pmpp_amd.corrections.mesh_cnn import initial_params, param_layout
+
+    for depth in (1, 2, 4):
+        layers, gain, total = param_layout(depth)
+        assert [L.name for L in layers][0] == "stem" and layers[-1].name == "out"
+        assert len(layers) == 2 * depth
+        ends = [L.offset + L.size for L in layers]
+        # 64-element alignment (16-byte aligned bf16 weights and float32 biases), no overlap
+        assert all(L.offset % 64 == 0 for L in layers) and gain % 64 == 0
+        assert all(e <= nxt.offset for e, nxt in zip(ends, layers[1:])) and ends[-1] <= gain
+        assert total >= gain + 4
+    layers, gain, total = param_layout(2)
+    p = initial_params(seed=3, depth=2)
+    assert p.shape == (total,)
+    out = layers[-1]
+    assert torch.all(p[out.offset:out.offset + out.size] == 0)  # zero output layer: zero residual
+    stem = p[:layers[0].nweights].view(27, 16, 8)
+    assert torch.all(stem[:, :, 4:] == 0)  # padding input channels
+    std = math.sqrt(1 / (27 * 4)) / 0.87962566103423978
+    assert stem[:, :, :4].abs().max() <= 2 * std + 1e-7
+    assert abs(float(stem[:, :, :4].std()) - math.sqrt(1 / (27 * 4))) < 0.1 * math.sqrt(1 / (27 * 4))
+    conv1 = p[layers[1].offset:layers[1].offset + layers[1].nweights]
+    assert abs(float(conv1.std()) - math.sqrt(1 / (27 * 16))) < 0.05 * math.sqrt(1 / (27 * 16))
+    assert torch.all(p[layers[0].nweights:layers[0].size] == 0)  # zero biases
+    assert 0 < float(p[gain:gain + 3].abs().max()) < 0.05 and p[gain + 3] == 0

@@ -101,3 +101,37 @@ def test_force_is_linear_in_omega_m_and_vjp_matches_finite_differences():
         fd = np.sum((spectral.gravity(q, dp, 0.3, ms, cell, ng ** 3, masks)
                      - spectral.gravity(q, dm, 0.3, ms, cell, ng ** 3, masks)) * ac) / (2 * eps)
         assert fd == pytest.approx(dc[idx], rel=1e-6, abs=1e-10)
+
+
+def test_density_residual_plane_wave_and_its_cotangent():
+    # particles on a uniform lattice exert no force, so the force of an added density
+    # rho_add = A cos(k0 x) / (1.5 Omega_m) is the plane-wave closed form at the particle sites
+    ng, ratio, cell, om, A = 4, 2, 0.5, 0.3, 0.01
+    ms = (8, 8, 8)
+    q = _lattice(ng, ratio)
+    x = np.arange(8) * cell
+    k0 = 2 * np.pi / (8 * cell)
+    rho_add = (A * np.cos(k0 * x) / (1.5 * om))[:, None, None] * np.ones(ms)
+    acc = spectral.gravity(q, np.zeros(q.shape), om, ms, cell, ng ** 3, rho_add=rho_add)
+    xp = q[:, 0] * cell  # pmid counts mesh cells here (mesh_shape = ratio)
+    np.testing.assert_allclose(acc[:, 0], -A * np.sin(k0 * xp) / k0, atol=1e-15)
+    np.testing.assert_allclose(acc[:, 1:], 0, atol=1e-15)
+    # the force is affine in rho_add: <acc_cot, J v> == <rho_cot, v> for the density cotangent
+    rng = np.random.default_rng(9)
+    disp = rng.uniform(-0.4, 0.4, q.shape)
+    masks = spectral.particle_nyquist_masks(ms, (ng,) * 3, 1.0, cell)
+    base = rng.normal(scale=0.1, size=ms)
+    v = rng.normal(size=ms)
+    ac = rng.normal(size=q.shape)
+    f0 = spectral.gravity(q, disp, om, ms, cell, ng ** 3, masks, rho_add=base)
+    f1 = spectral.gravity(q, disp, om, ms, cell, ng ** 3, masks, rho_add=base + v)
+    dc, oc, rc = spectral.gravity_vjp(q, disp, om, ac, ms, cell, ng ** 3, masks, rho_add=base)
+    assert np.sum(ac * (f1 - f0)) == pytest.approx(np.sum(rc * v), rel=1e-10)
+    eps = 1e-6
+    for idx in [(3, 0), (21, 2)]:  # the gather derivative now sees the residual's force too
+        dp = disp.copy(); dp[idx] += eps
+        dm = disp.copy(); dm[idx] -= eps
+        fd = np.sum((spectral.gravity(q, dp, om, ms, cell, ng ** 3, masks, rho_add=base)
+                     - spectral.gravity(q, dm, om, ms, cell, ng ** 3, masks, rho_add=base)) * ac) / (2 * eps)
+        assert fd == pytest.approx(dc[idx], rel=1e-6, abs=1e-10)
+    assert oc == pytest.approx(np.sum(f0 * ac) / om, rel=1e-10)
