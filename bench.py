@@ -143,7 +143,7 @@ def cpu_baseline(dev, omega_m):
 
     ng, ratio, box = 128, 2, 250.0  # same construction and density as the 512^3/1000 Mpc/h workload
     conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio)
-    ptcl = particles_2lpt(conf, LCDM(), seed=0, jitter_seed=1, device=dev)
+    ptcl = particles_2lpt(conf, LCDM(), seed=0, jitter_seed=1, device=dev, a=1.0)
     pm = ptcl.pmid.cpu().numpy()
     dp = ptcl.disp.cpu().numpy()
     cores = len(os.sched_getaffinity(0))
@@ -214,7 +214,7 @@ def main():
     cosmo = LCDM()
     conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio)
     log(f"[bench] building 2LPT input: {ng}^3 particles, {ng * ratio}^3 mesh, {box:g} Mpc/h")
-    ptcl = particles_2lpt(conf, cosmo, seed=0, jitter_seed=1, device=dev)
+    ptcl = particles_2lpt(conf, cosmo, seed=0, jitter_seed=1, device=dev, a=1.0)
     torch.cuda.synchronize()
     ensure_init(dev)
     d = gravity_desc(conf, cosmo.Omega_m)

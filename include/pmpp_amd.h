@@ -219,6 +219,13 @@ int pm_drift_adj(float* disp, const float* vel, const float* disp_cot, float* ve
  * the caller divides by the shell mode counts and multiplies by cell^3 / N_m (:251-256). */
 int pm_power_spectrum(const float* spec_a, const float* spec_b, const int32_t shape[3], int32_t mas_power,
                       int32_t num_shells, int32_t density, double* sums, void* stream);
+/* Reverse of pm_power_spectrum (auto spectrum): grad_spec = coef[shell] * window * s^2 * spec
+ * (s = N / Re spec[0] for a density, else 1), so that with coef[s] = 2 cell^3 dL/dP_s / nmodes_s
+ * irfftn(grad_spec) = dL/d(mesh) -- the adjoint the reference takes by autodiff through
+ * analysis/power_spectrum.py:196-260.  For a density the constant term of the mean is left
+ * out (it carries no particle gradient). */
+int pm_power_spectrum_bwd(const float* spec, const int32_t shape[3], int32_t mas_power, int32_t num_shells,
+                          int32_t density, const float* shell_coef, float* grad_spec, void* stream);
 
 /* ------------------------------------------------------------ conv3d (learned correction) ----
  * Periodic 3x3x3 convolutions of channels-last bf16 meshes on the matrix cores (BF16 MFMA,
@@ -275,6 +282,33 @@ int pm_adamax_step(float* params, const float* grads, float* mu, float* nu, int3
                    float learning_rate, float b1, float b2, float eps, float clip_norm, void* params_bf16,
                    void* workspace, size_t workspace_bytes, void* stream);
 int pm_to_bf16(const float* src, int64_t n, void* dst, void* stream);
+
+/* ------------------------------------------------------------ initial conditions ----
+ * Spectral and real-space steps of the IC pipeline on the particle grid (n0, n1, n2), spectra
+ * in the rfftn layout (n0, n1, n2/2+1) complex; ktab[a] are the float32 angular wavenumbers
+ * of axis a (fftfreq * 2 pi / spacing, rfftfreq for the last axis: n0, n1, n2/2+1 entries).
+ * pm_ic_linear_modes   spec *= scale sqrt(coef (k/k_pivot)^n_s T(k)^2), T linearly
+ *                      interpolated from (table_k, table_T) and clamped, k = 0 -> 0:
+ *                      linear_modes (initial_conditions/modes.py:399-454, boltzmann.py:428-482)
+ * pm_ic_spectral_op    out = symbol(k) in: op 0 scale; 1 -1/k^2 (k = 0 -> 0,
+ *                      laplace_transposed, gravity.py:263-287); 2 -k_i k_j with the Nyquist
+ *                      wavenumber zeroed for i != j (_strain, lpt.py:17-43); 3 -i k_i with the
+ *                      Nyquist zeroed (neg_grad, gravity.py:355-391).  A wavenumber is the
+ *                      Nyquist when | |k| - nyquist[a] | <= tol[a].
+ * pm_ic_product_acc    acc = (init ? 0 : acc) + coef a b elementwise (b = NULL: + coef a); the
+ *                      2LPT source (_L, lpt.py:239-302) and the sums of its adjoint
+ * pm_ic_lpt_accumulate disp[:, axis] += D grad, vel[:, axis] += V grad (vel may be NULL) for
+ *                      particles in grid order (lpt.py:419-436) */
+int pm_ic_linear_modes(float* spec, const int32_t shape[3], const float* const ktab[3], const float* table_k,
+                       const float* table_T, int32_t table_n, float coef, float k_pivot, float n_s, float scale,
+                       void* stream);
+int pm_ic_spectral_op(const float* in, float* out, const int32_t shape[3], const float* const ktab[3], int32_t op,
+                      int32_t axis_i, int32_t axis_j, float scale, const float nyquist[3], const float tol[3],
+                      void* stream);
+int pm_ic_product_acc(float* acc, const float* a, const float* b, int64_t n, float coef, int32_t init,
+                      void* stream);
+int pm_ic_lpt_accumulate(float* disp, float* vel, const float* grad, int64_t n, int32_t axis, float D, float V,
+                         void* stream);
 
 #ifdef __cplusplus
 }

@@ -33,10 +33,11 @@ EXPORTED_SYMBOLS = (
     "pm_gravity_workspace", "pm_gravity_vjp_workspace", "pm_gravity", "pm_gravity_vjp",
     "pm_gravity_density", "pm_gravity_corrected", "pm_gravity_corrected_vjp",
     "pm_kick", "pm_drift", "pm_nbody_adj_workspace", "pm_kick_adj", "pm_drift_adj",
-    "pm_power_spectrum",
+    "pm_power_spectrum", "pm_power_spectrum_bwd",
     "pm_conv3d_fwd", "pm_conv3d_bwd_data", "pm_conv3d_bwd_weight_workspace", "pm_conv3d_bwd_weight",
     "pm_correction_workspace", "pm_correction_features", "pm_correction_residual", "pm_correction_residual_bwd",
     "pm_force_loss", "pm_adamax_step", "pm_to_bf16",
+    "pm_ic_linear_modes", "pm_ic_spectral_op", "pm_ic_product_acc", "pm_ic_lpt_accumulate",
 )
 
 
@@ -114,6 +115,7 @@ _SIGNATURES = {
     "pm_kick_adj": ([_vp, _vp, _vp, _vp, _i64, _f32, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_drift_adj": ([_vp, _vp, _vp, _vp, _i64, _f32, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_power_spectrum": ([_vp, _vp, _i32x3, _i32, _i32, _i32, _vp, _vp], ctypes.c_int),
+    "pm_power_spectrum_bwd": ([_vp, _i32x3, _i32, _i32, _i32, _vp, _vp, _vp], ctypes.c_int),
     "pm_conv3d_fwd": ([_vp, _i32, _vp, _vp, _vp, _i32, _vp, _vp, _i32x3, _vp], ctypes.c_int),
     "pm_conv3d_bwd_data": ([_vp, _vp, _vp, _vp, _vp, _i32x3, _vp], ctypes.c_int),
     "pm_conv3d_bwd_weight_workspace": ([_i32x3, _i32, _sz_p], ctypes.c_int),
@@ -128,6 +130,10 @@ _SIGNATURES = {
     "pm_adamax_step": ([_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _f32, _vp, _vp, ctypes.c_size_t,
                         _vp], ctypes.c_int),
     "pm_to_bf16": ([_vp, _i64, _vp, _vp], ctypes.c_int),
+    "pm_ic_linear_modes": ([_vp, _i32x3, _vp, _vp, _vp, _i32, _f32, _f32, _f32, _f32, _vp], ctypes.c_int),
+    "pm_ic_spectral_op": ([_vp, _vp, _i32x3, _vp, _i32, _i32, _i32, _f32, _vp, _vp, _vp], ctypes.c_int),
+    "pm_ic_product_acc": ([_vp, _vp, _vp, _i64, _f32, _i32, _vp], ctypes.c_int),
+    "pm_ic_lpt_accumulate": ([_vp, _vp, _vp, _i64, _i32, _f32, _f32, _vp], ctypes.c_int),
 }
 
 _LIB = None

@@ -130,3 +130,39 @@ def particles_to_pk(ptcl, conf, mas="CIC"):
 
     mesh = scatter(ptcl, conf)
     return density_to_pk(mesh, conf, mas=mas)
+
+
+def _pk_vjp(mesh, conf, pk_cot, mas, density):
+    require_device(mesh, "mesh")
+    if mesh.ndim != 3 or mesh.dtype != torch.float32:
+        raise NotImplementedError("power spectra take (X, Y, Z) float32 meshes")
+    from ..numerics import irfftn
+
+    dev = mesh.device
+    ensure_init(dev)
+    shape = tuple(mesh.shape)
+    ns = num_shells(shape)
+    _, nmodes = shell_statistics(shape, float(conf.cell_size))
+    cot = np.asarray(pk_cot.detach().cpu() if isinstance(pk_cot, torch.Tensor) else pk_cot, dtype=np.float64)
+    n = min(len(nmodes), ns - 1)
+    if cot.shape != (n,):
+        raise ValueError(f"pk_cot must have {n} entries (shells 1..{n})")
+    coef = np.zeros(ns, dtype=np.float64)
+    coef[1:1 + n] = 2.0 * float(conf.cell_size) ** 3 * cot / nmodes[:n]
+    coef_d = torch.from_numpy(coef.astype(np.float32)).to(dev)
+    spec = rfftn(mesh.contiguous())
+    grad = torch.empty_like(spec)
+    _lib.call("pm_power_spectrum_bwd", ptr(torch.view_as_real(spec)), _lib.int3(shape), _mas(mas), ns, int(density),
+              ptr(coef_d), ptr(torch.view_as_real(grad)), stream_handle(dev))
+    return irfftn(grad, s=shape)
+
+
+def delta_to_pk_vjp(delta, conf, pk_cot, mas="CIC"):
+    """d/d(delta) of <pk_cot, P(k)> for ``delta_to_pk`` (the reverse of the shell sums)."""
+    return _pk_vjp(delta, conf, pk_cot, mas, density=False)
+
+
+def density_to_pk_vjp(density, conf, pk_cot, mas="CIC"):
+    """d/d(density) of <pk_cot, P(k)> for ``density_to_pk``, up to a constant field (the
+    derivative through the mean density, which moves no particle)."""
+    return _pk_vjp(density, conf, pk_cot, mas, density=True)

@@ -17,7 +17,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB_NAME = "libpmpp_amd.so"
 SOURCES = ["pm_cic.hip", "pm_sort.hip", "pm_fft.hip", "pm_gravity.hip", "pm_nbody.hip", "pm_analysis.hip", "pm_conv.hip",
-           "pm_correction.hip"]
+           "pm_correction.hip", "pm_ic.hip"]
 
 ARCH = os.environ.get("PMPP_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -171,3 +171,58 @@ def train_step(model, optimizer, state, ptcl, a, cosmo, conf, target, grads=None
     grads = residual_backward(model, tape, rho_cot, grads)
     optimizer.step(model, grads, state)
     return loss[0]
+
+
+def fine_run_pairs(cosmo, conf, conf_fine, seeds, every=1, device="cuda"):
+    """Training pairs generated by the method: snapshots of fine-mesh N-body runs (one per
+    seed, 2LPT initial conditions at ``conf_fine.a_start``), each re-anchored on the coarse
+    mesh, with the fine-mesh force at that state as the target.  Returns
+    ``[(a, coarse_particles, target_acc)]``; ``every`` keeps every n-th step."""
+    from ..initial_conditions import linear_modes, lpt, white_noise
+    from ..nbody.particles import Particles
+    from ..nbody.solver import nbody
+
+    pairs = []
+    for seed in seeds:
+        modes = linear_modes(white_noise(seed, conf_fine, device=device), cosmo, conf_fine)
+        state = lpt(modes, cosmo, conf_fine)
+        steps = []
+
+        def observe(a, p):
+            steps.append((a, p))
+
+        nbody(state, cosmo, conf_fine, observer=observe)
+        for i, (a, p) in enumerate(steps):
+            if i % every:
+                continue
+            coarse = Particles.from_pos(conf, p.pos(dtype=torch.float64), vel=p.vel)
+            pairs.append((a, coarse, p.acc.clone()))
+    return pairs
+
+
+def compare_runs(model, cosmo, conf, conf_fine, seed, device="cuda"):
+    """The judge of the correction: from one seed's initial conditions, the final density of
+    the plain coarse run, the corrected coarse run and the fine run, each measured on the fine
+    mesh.  Returns a dict with ``k``, ``pk_plain``, ``pk_corrected``, ``pk_fine`` and the
+    cross-correlations ``r_plain``, ``r_corrected`` against the fine run (torch, device)."""
+    from ..analysis import density_to_cross_correlation, density_to_pk
+    from ..cic import scatter
+    from ..initial_conditions import linear_modes, lpt, white_noise
+    from ..nbody.particles import Particles
+    from ..nbody.solver import nbody
+
+    modes = linear_modes(white_noise(seed, conf, device=device), cosmo, conf)
+    plain = nbody(lpt(modes, cosmo, conf), cosmo, conf, observer=lambda a, p: None)
+    corr = nbody(lpt(modes, cosmo, conf), cosmo, conf, correction=model)
+    fine = nbody(lpt(modes, cosmo, conf_fine), cosmo, conf_fine, observer=lambda a, p: None)
+
+    def dens(p):
+        q = Particles.from_pos(conf_fine, p.pos(dtype=torch.float64))
+        return scatter(q, conf_fine)
+
+    d_plain, d_corr, d_fine = dens(plain), dens(corr), dens(fine)
+    k, pk_fine, _ = density_to_pk(d_fine, conf_fine)
+    _, r_plain, _, pk_plain, _, _ = density_to_cross_correlation(d_plain, d_fine, conf_fine)
+    _, r_corr, _, pk_corr, _, _ = density_to_cross_correlation(d_corr, d_fine, conf_fine)
+    return {"k": k, "pk_plain": pk_plain, "pk_corrected": pk_corr, "pk_fine": pk_fine, "r_plain": r_plain,
+            "r_corrected": r_corr}

@@ -1,3 +1,4 @@
-"""Background expansion and linear growth on the host (float64), for the N-body factors."""
+"""Background expansion, linear growth and the linear power spectrum on the host (float64)."""
 
 from .background import E2, H_deriv, Omega_m_a, GrowthTable  # noqa: F401
+from .models import LCDM, SimpleLCDM  # noqa: F401

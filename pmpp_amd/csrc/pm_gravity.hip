@@ -854,8 +854,15 @@ int fill(float* p, int64_t n, hipStream_t s) {
 int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                   const pm_gravity_desc* d, const Layout& L, Buffers& b, Arena& ws, hipStream_t s) {
   const Geom g = step_geom(d);
-  PM_TRY(sort::bucket_cells(pmid, disp, valid, n, d->mesh_shape, d->cell_size, d->brick, b.perm, b.bins, b.pmid_s,
-                            b.disp_s, b.valid_s, ws, s));
+  // the Lagrangian grid the particle count implies (mesh * cell / spacing per axis)
+  int32_t grid[3] = {0, 0, 0};
+  if (d->ptcl_spacing > 0.f)
+    for (int a = 0; a < 3; ++a) {
+      const double v = (double)d->mesh_shape[a] * d->cell_size / d->ptcl_spacing;
+      grid[a] = (v >= 1.0 && v < 1e9) ? (int32_t)lrint(v) : 0;
+    }
+  PM_TRY(sort::bucket_cells(pmid, disp, valid, n, d->mesh_shape, d->cell_size, d->brick, grid, b.perm, b.bins,
+                            b.pmid_s, b.disp_s, b.valid_s, ws, s));
   if (b.tiled) {
     Bricks k = make_bricks(d->mesh_shape, d->brick);
     const unsigned ncols = (unsigned)((int64_t)k.nb[0] * k.nb[1]);

@@ -267,55 +267,135 @@ __device__ __forceinline__ int raw_cell_key(const KeyGeom& g, const int p[3], co
   return (c[0] * g.nb[1] + c[1]) * g.nb[2] + c[2];
 }
 
-constexpr int kBucketBlock = 256;
+// Bucketing in chunks of 4096 particles per workgroup: keys are aggregated in an LDS hash
+// table (4096 slots, so the linear probe always ends), then one global atomic per distinct key
+// and chunk reserves that key's range, and the chunk's particles of one key are written
+// contiguously by one workgroup.  When the particles are a whole Lagrangian grid in C order
+// (the usual PM state), a chunk is an 8 x 8 x 64 block of that grid, which falls into a few
+// bricks, so a chunk needs only a handful of global atomics and its writes fill cache lines;
+// otherwise chunks are contiguous index ranges.  The mapping only affects speed: every
+// partition of the particles gives a valid (unstable) bucketing.
+constexpr int kBkThreads = 256;
+constexpr int kBkChunk = 4096;
+constexpr int kBkPer = kBkChunk / kBkThreads;  // 16 particles per thread
+constexpr int kBkSlots = 4096;                // >= particles per chunk
+constexpr int kRankBits = 12;
+
+struct ChunkMap {
+  int64_t n;
+  int64_t nchunks;
+  int tiled;     // 1: 8 x 8 x 64 blocks of a (g0, g1, g2) particle grid
+  int g1, g2;    // grid extents (tiled)
+  int nby, nbz;  // blocks along y and z (tiled)
+};
+
+// particle index of slot e of thread t in chunk c
+__device__ __forceinline__ int64_t chunk_index(const ChunkMap& m, int64_t c, int e, int t) {
+  if (!m.tiled) return c * kBkChunk + (int64_t)e * kBkThreads + t;
+  const int bz = (int)(c % m.nbz);
+  const int64_t r = c / m.nbz;
+  const int by = (int)(r % m.nby);
+  const int bx = (int)(r / m.nby);
+  const int row = (t >> 6) + 4 * e;  // 64 rows of 64 particles; lane = z offset
+  const int ix = bx * 8 + (row >> 3), iy = by * 8 + (row & 7);
+  return ((int64_t)ix * m.g1 + iy) * m.g2 + (int64_t)bz * 64 + (t & 63);
+}
+
+__device__ __forceinline__ unsigned slot_hash(int key) { return ((unsigned)key * 2654435761u) >> (32 - 12); }
 
 template <bool PLACE>
-__global__ void __launch_bounds__(kBucketBlock) k_bucket(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
-                                                         const uint8_t* __restrict__ valid, KeyGeom g, int64_t n,
-                                                         int32_t* __restrict__ count, const int32_t* __restrict__ bins,
-                                                         int32_t* __restrict__ perm, int16_t* __restrict__ pmid_s,
-                                                         float* __restrict__ disp_s, uint8_t* __restrict__ valid_s) {
-  const int lane = threadIdx.x % kWave;
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  // the wave's first index: uniform over the wave, so the ballot loop below runs converged
-  for (int64_t w0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; w0 < n; w0 += stride) {
-    const int64_t i = w0 + lane;
-    const bool act = i < n;
-    const int64_t ic = act ? i : n - 1;  // clamped: unconditional loads
-    int p[3];
-    float d[3];
+__global__ void __launch_bounds__(kBkThreads) k_bucket(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
+                                                       const uint8_t* __restrict__ valid, KeyGeom g, ChunkMap m,
+                                                       int32_t* __restrict__ count, const int32_t* __restrict__ bins,
+                                                       int32_t* __restrict__ perm, int16_t* __restrict__ pmid_s,
+                                                       float* __restrict__ disp_s, uint8_t* __restrict__ valid_s) {
+  __shared__ int s_key[kBkSlots];
+  __shared__ int s_cnt[kBkSlots];  // count per slot, then (PLACE) the slot's first destination
+  const int t = threadIdx.x;
+  for (int h = t; h < kBkSlots; h += kBkThreads) {
+    s_key[h] = -1;
+    s_cnt[h] = 0;
+  }
+  __syncthreads();
+  for (int64_t c = blockIdx.x; c < m.nchunks; c += gridDim.x) {
+    int code[kBkPer];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      p[a] = pmid[ic * 3 + a];
-      d[a] = disp[ic * 3 + a];
-    }
-    const uint8_t v = valid ? valid[ic] : (uint8_t)1;
-    const int key = raw_cell_key(g, p, d, v != 0);
-    uint64_t live = __ballot(act);
-    int dest = 0;
-    while (live) {
-      const int leader = __ffsll((unsigned long long)live) - 1;
-      const int lk = __shfl(key, leader);
-      const uint64_t m = __ballot(act && key == lk) & live;
-      int base = 0;
-      if (lane == leader) base = atomicAdd(&count[lk], __popcll(m));
-      if (PLACE) {
-        base = __shfl(base, leader);
-        if ((m >> lane) & 1ull) dest = bins[lk] + base + __popcll(m & lt_mask);
-      }
-      live &= ~m;
-    }
-    if (PLACE && act) {
-      perm[dest] = (int32_t)i;
+    for (int e = 0; e < kBkPer; ++e) {
+      const int64_t i = chunk_index(m, c, e, t);
+      const bool act = i < m.n;
+      const int64_t ic = act ? i : m.n - 1;  // clamped: unconditional loads
+      int p[3];
+      float d[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        pmid_s[(int64_t)dest * 3 + a] = (int16_t)p[a];
-        disp_s[(int64_t)dest * 3 + a] = d[a];
+        p[a] = pmid[ic * 3 + a];
+        d[a] = disp[ic * 3 + a];
       }
-      valid_s[dest] = v;
+      const uint8_t v = valid ? valid[ic] : (uint8_t)1;
+      const int key = raw_cell_key(g, p, d, v != 0);
+      code[e] = -1;
+      if (act) {
+        unsigned h = slot_hash(key);
+        for (;;) {
+          const int prev = atomicCAS(&s_key[h], -1, key);
+          if (prev == -1 || prev == key) break;
+          h = (h + 1) & (kBkSlots - 1);
+        }
+        const int r = atomicAdd(&s_cnt[h], 1);
+        code[e] = ((int)h << kRankBits) | r;
+      }
     }
+    __syncthreads();
+    for (int h = t; h < kBkSlots; h += kBkThreads) {
+      const int k = s_key[h];
+      if (k >= 0) {
+        const int cnt = s_cnt[h];
+        if (PLACE)
+          s_cnt[h] = bins[k] + atomicAdd(&count[k], cnt);
+        else
+          atomicAdd(&count[k], cnt);
+      }
+    }
+    __syncthreads();
+    if (PLACE) {
+#pragma unroll 4
+      for (int e = 0; e < kBkPer; ++e) {
+        if (code[e] < 0) continue;
+        const int64_t i = chunk_index(m, c, e, t);  // reloaded: the chunk is still in L2
+        const int dest = s_cnt[code[e] >> kRankBits] + (code[e] & ((1 << kRankBits) - 1));
+        perm[dest] = (int32_t)i;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          pmid_s[(int64_t)dest * 3 + a] = pmid[i * 3 + a];
+          disp_s[(int64_t)dest * 3 + a] = disp[i * 3 + a];
+        }
+        valid_s[dest] = valid ? valid[i] : (uint8_t)1;
+      }
+    }
+    __syncthreads();
+    for (int h = t; h < kBkSlots; h += kBkThreads) {
+      s_key[h] = -1;
+      s_cnt[h] = 0;
+    }
+    __syncthreads();
   }
+}
+
+ChunkMap make_chunk_map(int64_t n, const int32_t* grid) {
+  ChunkMap m{};
+  m.n = n;
+  if (grid && (int64_t)grid[0] * grid[1] * grid[2] == n && grid[0] % 8 == 0 && grid[1] % 8 == 0 &&
+      grid[2] % 64 == 0) {
+    m.tiled = 1;
+    m.g1 = grid[1];
+    m.g2 = grid[2];
+    m.nby = grid[1] / 8;
+    m.nbz = grid[2] / 64;
+    m.nchunks = (int64_t)(grid[0] / 8) * m.nby * m.nbz;
+  } else {
+    m.nchunks = ceil_div(n, kBkChunk);
+  }
+  return m;
 }
 
 __global__ void k_zero_i32(int32_t* __restrict__ p, int64_t len) {
@@ -339,8 +419,8 @@ size_t bucket_workspace_bytes(const int32_t* mesh_shape, const int32_t* brick) {
 }
 
 int bucket_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n, const int32_t* mesh_shape,
-                 float cell_size, const int32_t* brick, int32_t* perm, int32_t* bin_offsets, int16_t* pmid_s,
-                 float* disp_s, uint8_t* valid_s, Arena& ws, hipStream_t s) {
+                 float cell_size, const int32_t* brick, const int32_t* ptcl_grid, int32_t* perm, int32_t* bin_offsets,
+                 int16_t* pmid_s, float* disp_s, uint8_t* valid_s, Arena& ws, hipStream_t s) {
   KeyGeom g = make_key_geom(mesh_shape, cell_size, brick);
   const int64_t len = (int64_t)g.sentinel + 2;  // bricks, the invalid bucket, and the end
   int32_t* count = ws.take<int32_t>(len);
@@ -349,9 +429,10 @@ int bucket_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, i
   const unsigned zb = (unsigned)std::min<int64_t>(ceil_div(len, 256), 1024);
   hipLaunchKernelGGL(k_zero_i32, dim3(zb), dim3(256), 0, s, count, len);
   PM_LAUNCHED("k_zero_i32", s);
-  const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBucketBlock), 8192));
+  const ChunkMap m = make_chunk_map(n, ptcl_grid);
+  const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(m.nchunks, 4096));
   if (n > 0) {
-    hipLaunchKernelGGL(k_bucket<false>, dim3(nb), dim3(kBucketBlock), 0, s, pmid, disp, valid, g, n, count,
+    hipLaunchKernelGGL(k_bucket<false>, dim3(nb), dim3(kBkThreads), 0, s, pmid, disp, valid, g, m, count,
                        (const int32_t*)nullptr, (int32_t*)nullptr, (int16_t*)nullptr, (float*)nullptr,
                        (uint8_t*)nullptr);
     PM_LAUNCHED("k_bucket_count", s);
@@ -360,7 +441,7 @@ int bucket_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, i
   hipLaunchKernelGGL(k_zero_i32, dim3(zb), dim3(256), 0, s, count, len);
   PM_LAUNCHED("k_zero_i32", s);
   if (n > 0) {
-    hipLaunchKernelGGL(k_bucket<true>, dim3(nb), dim3(kBucketBlock), 0, s, pmid, disp, valid, g, n, count,
+    hipLaunchKernelGGL(k_bucket<true>, dim3(nb), dim3(kBkThreads), 0, s, pmid, disp, valid, g, m, count,
                        (const int32_t*)bin_offsets, perm, pmid_s, disp_s, valid_s);
     PM_LAUNCHED("k_bucket_place", s);
   }

@@ -29,11 +29,15 @@ def _schedule(conf, reverse):
     return a[::-1] if reverse else a
 
 
-def _run(ptcl, cosmo, conf, reverse=False):
+def _run(ptcl, cosmo, conf, reverse=False, correction=None, observer=None):
     a = _schedule(conf, reverse)
-    ptcl = nbody_init(float(a[0]), ptcl, cosmo, conf)
+    ptcl = nbody_init(float(a[0]), ptcl, cosmo, conf, correction=correction)
+    if observer is not None:
+        observer(float(a[0]), ptcl)
     for a0, a1 in zip(a[:-1], a[1:]):
-        ptcl = nbody_step(float(a0), float(a1), ptcl, cosmo, conf)
+        ptcl = nbody_step(float(a0), float(a1), ptcl, cosmo, conf, correction=correction)
+        if observer is not None:
+            observer(float(a1), ptcl)
     return ptcl
 
 
@@ -85,14 +89,23 @@ class _NBodyFn(torch.autograd.Function):
         return cin.disp, cin.vel, om, None, None, None, None
 
 
-def nbody(ptcl, cosmo, conf, reverse=False):
+def nbody(ptcl, cosmo, conf, reverse=False, correction=None, observer=None):
     """Integrate ``ptcl`` over ``conf.a_nbody`` with KDK steps; returns the final particles.
 
     ``ptcl.vel`` must be set (zeros for a cold start).  Differentiable with respect to
     ``ptcl.disp``, ``ptcl.vel`` and a tensor ``cosmo.Omega_m`` through the reverse sweep.
+    With a ``correction`` (a learned force correction, ``pmpp_amd.corrections``) the run is
+    forward only.  ``observer(a, ptcl)`` (optional) sees the state after the initial force and
+    after every step (the reference's observers, solver.py:21-63).
     """
     if ptcl.vel is None:
         raise ValueError("nbody needs particle velocities (ptcl.vel)")
+    if correction is not None or observer is not None:
+        om = cosmo.Omega_m
+        if ptcl.disp.requires_grad or ptcl.vel.requires_grad or (isinstance(om, torch.Tensor) and om.requires_grad):
+            raise NotImplementedError("reverse mode with a correction or an observer is not implemented")
+        with torch.no_grad():
+            return _run(ptcl, cosmo, conf, reverse, correction=correction, observer=observer)
     om = cosmo.Omega_m
     om_t = om if isinstance(om, torch.Tensor) else torch.tensor(float(om), dtype=torch.float64,
                                                                 device=ptcl.disp.device)
