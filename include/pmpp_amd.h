@@ -259,6 +259,12 @@ int pm_conv3d_bwd_weight(const float* dz, const void* x, int32_t cin, const int3
  * pm_correction_residual_bwd  the output layer's pre-activation gradient dz_out (n x 16,
  *                         channel 0 live) and the gain parameters' gradient (4 floats) from
  *                         rho_cot.
+ * pm_correction_input_bwd the cotangents through the network's inputs: rho_bar (n floats,
+ *                         overwritten) = d<rho_cot, rho_add>/d rho and omega_cot[0] += its
+ *                         Omega_m derivative, from dx0 (n x 16 float32, the gradient of the
+ *                         network input; channels 0 and 2 are read), the forward's out and
+ *                         stats.  With the scatter adjoint of rho_bar this is reverse mode of
+ *                         the corrected force through the density. 
  * pm_force_loss           loss[0] = sum |acc - target|^2 / sum |target|^2 (device doubles,
  *                         loss[1] = 1 / sum |target|^2), acc_cot = d loss / d acc if not NULL.
  * pm_adamax_step          clip_by_global_norm(clip_norm) then adamax(learning_rate, b1, b2, eps)
@@ -276,6 +282,9 @@ int pm_correction_residual(const float* out, const float* rho, int64_t n, float 
 int pm_correction_residual_bwd(const float* rho_cot, const float* out, const float* rho, int64_t n, float omega_m,
                                float a, float sigma8, float max_residual, const float* stats, float* dz_out,
                                float* gain_grad, void* workspace, size_t workspace_bytes, void* stream);
+int pm_correction_input_bwd(const float* rho_cot, const float* out, const float* rho, const float* dx0, int64_t n,
+                            float omega_m, float max_residual, const float* stats, const float* gain_params,
+                            float* rho_bar, float* omega_cot, void* workspace, size_t workspace_bytes, void* stream);
 int pm_force_loss(const float* acc, const float* target, int64_t n, double* loss, float* acc_cot, void* workspace,
                   size_t workspace_bytes, void* stream);
 int pm_adamax_step(float* params, const float* grads, float* mu, float* nu, int32_t* count, int64_t n,

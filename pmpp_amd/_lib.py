@@ -36,7 +36,7 @@ EXPORTED_SYMBOLS = (
     "pm_power_spectrum", "pm_power_spectrum_bwd",
     "pm_conv3d_fwd", "pm_conv3d_bwd_data", "pm_conv3d_bwd_weight_workspace", "pm_conv3d_bwd_weight",
     "pm_correction_workspace", "pm_correction_features", "pm_correction_residual", "pm_correction_residual_bwd",
-    "pm_force_loss", "pm_adamax_step", "pm_to_bf16",
+    "pm_correction_input_bwd", "pm_force_loss", "pm_adamax_step", "pm_to_bf16",
     "pm_ic_linear_modes", "pm_ic_spectral_op", "pm_ic_product_acc", "pm_ic_lpt_accumulate",
 )
 
@@ -126,6 +126,8 @@ _SIGNATURES = {
     "pm_correction_residual": ([_vp, _vp, _i64, _f32, _f32, _vp, _vp, _vp], ctypes.c_int),
     "pm_correction_residual_bwd": ([_vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _vp, _vp, _vp, _vp,
                                     ctypes.c_size_t, _vp], ctypes.c_int),
+    "pm_correction_input_bwd": ([_vp, _vp, _vp, _vp, _i64, _f32, _f32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t,
+                                 _vp], ctypes.c_int),
     "pm_force_loss": ([_vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_adamax_step": ([_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _f32, _vp, _vp, ctypes.c_size_t,
                         _vp], ctypes.c_int),

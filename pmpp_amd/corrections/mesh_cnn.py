@@ -212,9 +212,20 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     return rho_add, tape
 
 
-def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, grads=None):
+def _stem_weight_16(model):
+    """The stem's bf16 weights [27][16][8] zero-padded to 16 input channels (for its data
+    gradient, which the 16 -> 16 kernel computes)."""
+    w = model.weight(model.layers[0]).view(TAPS, CHANNELS, IN_CHANNELS)
+    out = torch.zeros((TAPS, CHANNELS, CHANNELS), dtype=torch.bfloat16, device=w.device)
+    out[:, :, :IN_CHANNELS] = w
+    return out
+
+
+def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, grads=None, inputs=False):
     """Gradient of ``<rho_cot, rho_add>`` with respect to the parameters (flat float32, the
-    parameter layout); written into ``grads`` when given."""
+    parameter layout); written into ``grads`` when given.  With ``inputs`` also the cotangents
+    through the network's inputs: returns ``(grads, rho_bar, omega_bar)`` with ``rho_bar`` the
+    density cotangent (mesh) and ``omega_bar`` a device float (``pm_correction_input_bwd``)."""
     dev = tape.rho.device
     n = tape.rho.numel()
     shape = tuple(tape.rho.shape)
@@ -246,4 +257,12 @@ def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, gr
         wgrad(l1, dz1, x_in)
         dz = conv3d_bwd_data(dz1, model.weight(l1), add=dz, gact=pre_in)
     wgrad(layers[0], dz, tape.x0)
-    return grads
+    if not inputs:
+        return grads
+    dx0 = conv3d_bwd_data(dz, _stem_weight_16(model))
+    rho_bar = torch.empty_like(tape.rho)
+    omega_bar = torch.zeros((), dtype=torch.float32, device=dev)
+    _lib.call("pm_correction_input_bwd", ptr(rc), ptr(tape.out), ptr(tape.rho), ptr(dx0), n, tape.omega_m,
+              float(model.max_residual), ptr(tape.stats), ptr(model.gain_params), ptr(rho_bar), ptr(omega_bar),
+              ptr(ws), ws.numel(), stream_handle(dev))
+    return grads, rho_bar, omega_bar

@@ -146,6 +146,25 @@ def corrected_gravity(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrecti
     return gravity_with_residual(ptcl, cosmo, conf, rho_add)
 
 
+def corrected_gravity_vjp(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrection, acc_cot):
+    """(disp_cot, omega_cot float) of ``corrected_gravity``: the force step's adjoint at the
+    corrected source (``pm_gravity_corrected_vjp``) plus the path through the network's inputs
+    (``pm_correction_input_bwd``), whose density cotangent reaches the particles through the
+    CIC scatter adjoint."""
+    from ..cic.kernels import hip_scatter_bwd
+
+    om = _omega(cosmo)
+    rho = density(ptcl, cosmo, conf)
+    rho_add, tape = evaluate_residual(correction, rho, a, om, keep=True)
+    disp_cot, omega_cot, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
+    _, rho_bar, omega_bar = residual_backward(correction, tape, rho_cot, inputs=True)
+    valid = _valid_u8(ptcl)
+    dc, _ = hip_scatter_bwd(ptcl.pmid, ptcl.disp.detach(), conf.mesh_size / conf.ptcl_num, rho_bar, offset=0.0,
+                            particle_cell_size=conf.cell_size, cell_size=None, global_shape=tuple(conf.mesh_shape),
+                            valid_mask=None if valid is None else valid.bool())
+    return disp_cot + dc, float(omega_cot.item()) + float(omega_bar.item())
+
+
 def force_loss(acc, target, want_grad=True):
     """(loss, acc_cot): loss a device float64 tensor [|acc - t|^2 / |t|^2, 1 / |t|^2]."""
     dev = acc.device

@@ -134,6 +134,86 @@ __global__ void k_gain_bwd(const double* __restrict__ partials, int nb, float r,
   dlin[3] = dp;
 }
 
+// ------------------------------------------------------ input-path reverse ----
+// The density and Omega_m cotangents of rho_add through the network's inputs.  With
+// T = tanh(o), S = c (rho - 1), x0_0 = S / sigma, x0_2 = Omega_m and dx0 the gradient of the
+// network input (channels 0 and 2 used):
+//   sigma_bar = (r / c) sum rho_cot T - sum dx0_0 S / sigma^2
+//   S_bar     = dx0_0 / sigma + sigma_bar S / (N sigma)
+//   rho_bar   = g rho_cot + c S_bar
+//   Om_bar    = 1.5 sum S_bar (rho - 1) + sum dx0_2 - 1.5 (r sigma / c^2) sum rho_cot T
+//               + r (1 - tanh_g^2) w_1 sum rho_cot (rho - 1)
+// Pass 1 reduces the four sums, pass 2 writes rho_bar and reduces sum S_bar (rho - 1).
+__global__ void __launch_bounds__(kBlock) k_input_sums(const float* __restrict__ rho_cot,
+                                                       const float* __restrict__ out,
+                                                       const float* __restrict__ rho,
+                                                       const float4* __restrict__ dx0, int64_t n, float c,
+                                                       double* __restrict__ partials) {
+  __shared__ double lds[kBlock / 64];
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const float rc = rho_cot[i];
+    const float d = rho[i] - 1.f;
+    const float4 g = dx0[i * 4];  // channels 0..3 of the input gradient
+    s1 += (double)(rc * tanhf(out[i * 16]));
+    s2 += (double)(g.x * (d * c));
+    s3 += (double)g.z;
+    s4 += (double)(rc * d);
+  }
+  const double t1 = block_sum(s1, lds);
+  const double t2 = block_sum(s2, lds);
+  const double t3 = block_sum(s3, lds);
+  const double t4 = block_sum(s4, lds);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = t1;
+    partials[kMaxBlocks + blockIdx.x] = t2;
+    partials[2 * kMaxBlocks + blockIdx.x] = t3;
+    partials[3 * kMaxBlocks + blockIdx.x] = t4;
+  }
+}
+
+// scal[0] = sigma_bar, scal[1] = the Omega_m terms known after pass 1
+__global__ void k_input_scalars(const double* __restrict__ partials, int nb, float c, float r,
+                                const float* __restrict__ stats, const float* __restrict__ lin, float* __restrict__ scal) {
+  if (threadIdx.x != 0) return;
+  const double p1 = sum_partials(partials, nb);
+  const double p2 = sum_partials(partials + kMaxBlocks, nb);
+  const double p3 = sum_partials(partials + 2 * kMaxBlocks, nb);
+  const double p4 = sum_partials(partials + 3 * kMaxBlocks, nb);
+  const double sigma = stats[0];
+  const double th = stats[2];
+  scal[0] = (float)((double)r / c * p1 - p2 / (sigma * sigma));
+  scal[1] = (float)(p3 - 1.5 * (double)r * sigma / ((double)c * c) * p1 + (double)r * (1.0 - th * th) * lin[1] * p4);
+}
+
+__global__ void __launch_bounds__(kBlock) k_input_bwd(const float* __restrict__ rho_cot,
+                                                      const float* __restrict__ rho,
+                                                      const float4* __restrict__ dx0, int64_t n, float c,
+                                                      const float* __restrict__ stats,
+                                                      const float* __restrict__ scal, float* __restrict__ rho_bar,
+                                                      double* __restrict__ partials) {
+  __shared__ double lds[kBlock / 64];
+  const float sigma = stats[0], g = stats[1];
+  const float sb = scal[0] / ((float)n * sigma);
+  const float inv = 1.f / sigma;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const float d = rho[i] - 1.f;
+    const float S = d * c;
+    const float Sbar = dx0[i * 4].x * inv + sb * S;
+    rho_bar[i] = g * rho_cot[i] + c * Sbar;
+    acc += (double)(Sbar * d);
+  }
+  const double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void k_input_omega(const double* __restrict__ partials, int nb, const float* __restrict__ scal,
+                              float* __restrict__ omega_cot) {
+  if (threadIdx.x != 0) return;
+  omega_cot[0] += (float)(1.5 * sum_partials(partials, nb) + (double)scal[1]);
+}
+
 // ---------------------------------------------------------------- loss ----
 // loss = sum |acc - target|^2 / sum |target|^2 over 3n floats; acc_cot = d loss / d acc
 __global__ void __launch_bounds__(kBlock) k_loss_partials(const float* __restrict__ acc,
@@ -232,7 +312,7 @@ __global__ void __launch_bounds__(kBlock) k_to_bf16(const float* __restrict__ sr
 using namespace pm;
 
 namespace {
-constexpr size_t kPartialBytes = sizeof(double) * 2 * corr::kMaxBlocks;
+constexpr size_t kPartialBytes = sizeof(double) * 4 * corr::kMaxBlocks + 256;
 
 int check_mesh(int64_t n) {
   PM_RETURN_IF(n < 1, PM_ERROR_INVALID_ARGUMENT, "mesh must have at least one cell");
@@ -304,6 +384,35 @@ int pm_correction_residual_bwd(const float* rho_cot, const float* out, const flo
   hipLaunchKernelGGL(corr::k_gain_bwd, dim3(1), dim3(64), 0, s, part, (int)nb, max_residual, stats, a, omega_m,
                      sigma8, gain_grad);
   PM_LAUNCHED("k_gain_bwd", s);
+  return PM_SUCCESS;
+}
+
+int pm_correction_input_bwd(const float* rho_cot, const float* out, const float* rho, const float* dx0, int64_t n,
+                            float omega_m, float max_residual, const float* stats, const float* gain_params,
+                            float* rho_bar, float* omega_cot, void* workspace, size_t workspace_bytes, void* stream) {
+  PM_TRY(check_mesh(n));
+  PM_RETURN_IF(rho_cot == nullptr || out == nullptr || rho == nullptr || dx0 == nullptr || stats == nullptr ||
+                   gain_params == nullptr || rho_bar == nullptr || omega_cot == nullptr,
+               PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF(reinterpret_cast<uintptr_t>(dx0) % 16 != 0, PM_ERROR_INVALID_ARGUMENT, "dx0 must be 16-byte aligned");
+  PM_RETURN_IF(workspace == nullptr || workspace_bytes < kPartialBytes, PM_ERROR_WORKSPACE,
+               "correction workspace too small");
+  hipStream_t s = as_stream(stream);
+  const float c = 1.5f * omega_m;
+  double* part = static_cast<double*>(workspace);
+  float* scal = reinterpret_cast<float*>(static_cast<char*>(workspace) + sizeof(double) * 4 * corr::kMaxBlocks);
+  const unsigned nb = corr::grid_for(n);
+  const float4* dx4 = reinterpret_cast<const float4*>(dx0);
+  hipLaunchKernelGGL(corr::k_input_sums, dim3(nb), dim3(corr::kBlock), 0, s, rho_cot, out, rho, dx4, n, c, part);
+  PM_LAUNCHED("k_input_sums", s);
+  hipLaunchKernelGGL(corr::k_input_scalars, dim3(1), dim3(64), 0, s, part, (int)nb, c, max_residual, stats,
+                     gain_params, scal);
+  PM_LAUNCHED("k_input_scalars", s);
+  hipLaunchKernelGGL(corr::k_input_bwd, dim3(nb), dim3(corr::kBlock), 0, s, rho_cot, rho, dx4, n, c, stats, scal,
+                     rho_bar, part);
+  PM_LAUNCHED("k_input_bwd", s);
+  hipLaunchKernelGGL(corr::k_input_omega, dim3(1), dim3(64), 0, s, part, (int)nb, scal, omega_cot);
+  PM_LAUNCHED("k_input_omega", s);
   return PM_SUCCESS;
 }
 

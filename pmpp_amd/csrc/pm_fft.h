@@ -306,74 +306,6 @@ struct ColPass {
   int ky_split = 0x7fffffff, ky_shift = 0;
 };
 
-// ------------------------------------------------------------ Stockham line core ----
-// Self-sorting radix-E FFT of a length-N line held by P = N / E threads, thread t keeping
-// elements t + P s (s < E) in registers -- the same natural layout in and out, so loads and
-// stores of consecutive threads are consecutive.  Stage q has radix R = min(E, N / Ns)
-// (Ns = product of the earlier radices): butterfly b (thread b % P, E / R per thread) takes
-// elements b + r N / R, multiplies by W_{Ns R}^{(b mod Ns) r}, does an R-point register DFT
-// and sends output r to (b / Ns) Ns R + (b mod Ns) + r Ns.  Every stage but the last goes
-// through the line's LDS planes (re, im, padded one float per 32); the last stage's outputs
-// are already the thread's natural elements.  Small E keeps a thread at 8 complex values.
-struct SkPad {
-  __device__ __forceinline__ int operator()(int i) const { return i + (i >> 5); }
-};
-
-template <int N>
-constexpr int sk_e() { return N >= 8 ? 8 : N; }
-
-template <int N>
-constexpr int sk_plane() { return N + (N >> 5) + 1; }  // floats per plane (padded)
-
-template <int N, int E, int NS, bool INV>
-__device__ __forceinline__ void sk_stages(float2* v, float* re, float* im, int t, const float2* __restrict__ tw) {
-  constexpr int P = N / E;
-  constexpr int R = (N / NS < E) ? N / NS : E;
-  constexpr int U = E / R;
-  constexpr bool LAST = NS * R == N;
-  SkPad pad;
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    float2 x[R];
-    const int b = t + P * u;
-    const int bm = b & (NS - 1);
-#pragma unroll
-    for (int r = 0; r < R; ++r) x[r] = v[u + U * r];
-    if constexpr (NS > 1) {
-#pragma unroll
-      for (int r = 1; r < R; ++r) x[r] = c_mul(x[r], table_tw<NS * R, INV>(tw, bm * r));
-    }
-    dft<R, INV>(x);
-    if constexpr (LAST) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[u + U * r] = x[r];
-    } else {
-      const int base = (b / NS) * NS * R + bm;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int y = pad(base + r * NS);
-        re[y] = x[r].x;
-        im[y] = x[r].y;
-      }
-    }
-  }
-  if constexpr (!LAST) {
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-      const int e = pad(t + P * s);
-      v[s] = make_float2(re[e], im[e]);
-    }
-    __syncthreads();
-    sk_stages<N, E, NS * R, INV>(v, re, im, t, tw);
-  }
-}
-
-template <int N, bool INV>
-__device__ __forceinline__ void sk_fft(float2* v, float* re, float* im, int t, const float2* __restrict__ tw) {
-  sk_stages<N, sk_e<N>(), 1, INV>(v, re, im, t, tw);
-}
-
 enum ColMode { kColFwd = 0, kColInv = 1, kColInv2 = 2, kColGreen = 3 };
 
 // Line pass along the contiguous last axis (real <-> half spectrum, split main + Nyquist).

@@ -279,115 +279,6 @@ __global__ void __launch_bounds__(ZCfg<M>::THREADS) k_zpass(ZPass zp, SpecTables
   }
 }
 
-// Stockham variant of the z pass: each line is held by P = M / 8 threads with 8 complex
-// values each (natural layout, see sk_fft), so a thread needs ~50 registers instead of the
-// two-stage core's ~160 and a CU keeps 8 waves per SIMD in flight.  Same contract as k_zpass.
-template <int M>
-struct ZSk {
-  static constexpr int E = sk_e<M>();
-  static constexpr int P = M / E;
-  static constexpr int THREADS = P >= 256 ? P : 256;
-  static constexpr int LINES = THREADS / P;
-  static constexpr int PLANE = sk_plane<M>();
-  static constexpr int LSTR = 2 * PLANE + 2;  // two planes (>= M + 1 float2 for the pre/post step)
-};
-
-template <int M, bool C2R>
-__global__ void __launch_bounds__(ZSk<M>::THREADS) k_zpass_sk(ZPass zp, SpecTables tb, const float2* __restrict__ tw) {
-  using ZC = ZSk<M>;
-  constexpr int P = ZC::P, E = ZC::E;
-  __shared__ float lds[ZC::LINES * ZC::LSTR];
-  const int t = threadIdx.x % P;
-  const int line = threadIdx.x / P;
-  const int64_t lg = (int64_t)blockIdx.x * ZC::LINES + line;
-  const bool ok = lg < zp.nlines;
-  float* slot = lds + line * ZC::LSTR;
-  float* re = slot;
-  float* im = slot + ZC::PLANE;
-  float2* nat = reinterpret_cast<float2*>(slot);
-  float2 v[E];
-  const int64_t lgl = ok ? lg : zp.nlines - 1;  // clamped line: unconditional, batched loads
-  if constexpr (!C2R) {
-    const float* src = zp.rin + lgl * zp.pitch_r;
-#pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = *reinterpret_cast<const float2*>(src + 2 * (t + P * q));
-    if (zp.rin_add) {
-      const float* src2 = zp.rin_add + lgl * zp.pitch_r;
-#pragma unroll
-      for (int q = 0; q < E; ++q) {
-        const float2 w = *reinterpret_cast<const float2*>(src2 + 2 * (t + P * q));
-        v[q] = make_float2(v[q].x + w.x, v[q].y + w.y);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < E; ++q)
-      v[q] = make_float2((v[q].x + zp.src_shift) * zp.src_scale, (v[q].y + zp.src_shift) * zp.src_scale);
-    sk_fft<M, false>(v, re, im, t, tw);
-    __syncthreads();  // the core's last reads of the planes are done before nat overwrites them
-#pragma unroll
-    for (int q = 0; q < E; ++q) nat[t + P * q] = v[q];
-    __syncthreads();
-    float2* dst = zp.cmain + lg * zp.pitch_c;
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-      const int k = t + P * q;
-      const float2 a = nat[k];
-      const float2 bc = c_conj(nat[(M - k) & (M - 1)]);
-      const float2 d = c_sub(a, bc);
-      const float2 fe = c_scale(c_add(a, bc), 0.5f);
-      const float2 fo = make_float2(0.5f * d.y, -0.5f * d.x);  // -i (a - bc) / 2
-      const float2 x = c_add(fe, c_mul(table_tw<2 * M, false>(tw, k), fo));
-      if (ok && k < zp.kz_lim) dst[k] = x;
-    }
-    if (ok && t == 0 && zp.nyq) {
-      const float2 z0 = nat[0];
-      zp.cnyq[lg * zp.nyq_stride] = make_float2(z0.x - z0.y, 0.f);
-    }
-  } else {
-    const float2* srcm = zp.cmain + lgl * zp.pitch_c;
-    float2 xn = zp.cnyq[lgl * zp.nyq_stride];  // cnyq is a valid address even when nyq == 0
-    const int klim = zp.kz_lim < M ? zp.kz_lim : M;
-#pragma unroll
-    for (int q = 0; q < E; ++q)  // P*q < klim is uniform: columns past the band cost no load
-      v[q] = (P * q < klim) ? srcm[min(t + P * q, klim - 1)] : make_float2(0.f, 0.f);
-#pragma unroll
-    for (int q = 0; q < E; ++q)
-      if (t + P * q >= klim) v[q] = make_float2(0.f, 0.f);
-    if (!zp.nyq) xn = make_float2(0.f, 0.f);
-    if (zp.deriv) {
-#pragma unroll
-      for (int q = 0; q < E; ++q) v[q] = c_imul(tb.negk[2][t + P * q], v[q]);
-      xn = c_imul(tb.negk[2][M], xn);
-    }
-#pragma unroll
-    for (int q = 0; q < E; ++q) nat[t + P * q] = v[q];
-    if (t == 0) nat[M] = xn;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-      const int k = t + P * q;
-      float2 xk = nat[k];
-      float2 xm = nat[M - k];
-      if (k == 0) {  // irfft uses only the real parts of the DC and Nyquist bins
-        xk.y = 0.f;
-        xm.y = 0.f;
-      }
-      const float2 e = c_scale(c_add(xk, c_conj(xm)), 0.5f);
-      const float2 o = c_scale(c_mulconj(c_sub(xk, c_conj(xm)), table_tw<2 * M, false>(tw, k)), 0.5f);
-      v[q] = make_float2(e.x - o.y, e.y + o.x);  // e + i o
-    }
-    __syncthreads();
-    sk_fft<M, true>(v, re, im, t, tw);
-    if (ok) {
-      float* dst = zp.rout + lg * zp.pitch_r;
-#pragma unroll
-      for (int q = 0; q < E; ++q)
-        *reinterpret_cast<float2*>(dst + 2 * (t + P * q)) =
-            make_float2(v[q].x * zp.out_scale, v[q].y * zp.out_scale);
-    }
-  }
-}
-
 // ------------------------------------------------------------ spectral tables ----
 // Angular wavenumbers k = m * (1/n) * (2 pi / cell) evaluated in double and rounded to
 // float (numerics/fft.py:30-41), the particle-Nyquist masks (core/utils.py:33-37) and
@@ -466,22 +357,10 @@ int launch_cols_n(int mode, const ColPass& cp, int batches, const SpecTables& tb
 
 template <int M>
 int launch_z_n(bool c2r, const ZPass& zp, const SpecTables& tb, hipStream_t s) {
+  using ZC = ZCfg<M>;
   const float2* tw = twiddle_table();
   PM_RETURN_IF(tw == nullptr, PM_ERROR_HIP, "twiddle table not initialised");
   if (zp.nlines == 0) return PM_SUCCESS;
-  if constexpr (M >= 64) {  // long lines: the Stockham core (8 values per thread)
-    using ZC = ZSk<M>;
-    dim3 grid((unsigned)ceil_div(zp.nlines, ZC::LINES));
-    if (c2r) {
-      hipLaunchKernelGGL((k_zpass_sk<M, true>), grid, dim3(ZC::THREADS), 0, s, zp, tb, tw);
-      PM_LAUNCHED("fft_z_c2r", s);
-    } else {
-      hipLaunchKernelGGL((k_zpass_sk<M, false>), grid, dim3(ZC::THREADS), 0, s, zp, tb, tw);
-      PM_LAUNCHED("fft_z_r2c", s);
-    }
-    return PM_SUCCESS;
-  }
-  using ZC = ZCfg<M>;
   dim3 grid((unsigned)ceil_div(zp.nlines, ZC::LINES));
   if (c2r) {
     hipLaunchKernelGGL((k_zpass<M, true>), grid, dim3(ZC::THREADS), 0, s, zp, tb, tw);

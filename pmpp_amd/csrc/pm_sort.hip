@@ -267,24 +267,24 @@ __device__ __forceinline__ int raw_cell_key(const KeyGeom& g, const int p[3], co
   return (c[0] * g.nb[1] + c[1]) * g.nb[2] + c[2];
 }
 
-// Bucketing in chunks of 4096 particles per workgroup: keys are aggregated in an LDS hash
-// table (4096 slots, so the linear probe always ends), then one global atomic per distinct key
+// Bucketing in chunks of 2048 particles per workgroup: keys are aggregated in an LDS hash
+// table (2048 slots, so the linear probe always ends), then one global atomic per distinct key
 // and chunk reserves that key's range, and the chunk's particles of one key are written
 // contiguously by one workgroup.  When the particles are a whole Lagrangian grid in C order
-// (the usual PM state), a chunk is an 8 x 8 x 64 block of that grid, which falls into a few
+// (the usual PM state), a chunk is a 4 x 8 x 64 block of that grid, which falls into a few
 // bricks, so a chunk needs only a handful of global atomics and its writes fill cache lines;
 // otherwise chunks are contiguous index ranges.  The mapping only affects speed: every
 // partition of the particles gives a valid (unstable) bucketing.
 constexpr int kBkThreads = 256;
-constexpr int kBkChunk = 4096;
-constexpr int kBkPer = kBkChunk / kBkThreads;  // 16 particles per thread
-constexpr int kBkSlots = 4096;                // >= particles per chunk
-constexpr int kRankBits = 12;
+constexpr int kBkChunk = 2048;
+constexpr int kBkPer = kBkChunk / kBkThreads;  // 8 particles per thread
+constexpr int kBkSlots = 2048;                // >= particles per chunk
+constexpr int kRankBits = 11;
 
 struct ChunkMap {
   int64_t n;
   int64_t nchunks;
-  int tiled;     // 1: 8 x 8 x 64 blocks of a (g0, g1, g2) particle grid
+  int tiled;     // 1: 4 x 8 x 64 blocks of a (g0, g1, g2) particle grid
   int g1, g2;    // grid extents (tiled)
   int nby, nbz;  // blocks along y and z (tiled)
 };
@@ -296,12 +296,12 @@ __device__ __forceinline__ int64_t chunk_index(const ChunkMap& m, int64_t c, int
   const int64_t r = c / m.nbz;
   const int by = (int)(r % m.nby);
   const int bx = (int)(r / m.nby);
-  const int row = (t >> 6) + 4 * e;  // 64 rows of 64 particles; lane = z offset
-  const int ix = bx * 8 + (row >> 3), iy = by * 8 + (row & 7);
+  const int row = (t >> 6) + 4 * e;  // 32 rows of 64 particles; lane = z offset
+  const int ix = bx * 4 + (row >> 3), iy = by * 8 + (row & 7);
   return ((int64_t)ix * m.g1 + iy) * m.g2 + (int64_t)bz * 64 + (t & 63);
 }
 
-__device__ __forceinline__ unsigned slot_hash(int key) { return ((unsigned)key * 2654435761u) >> (32 - 12); }
+__device__ __forceinline__ unsigned slot_hash(int key) { return ((unsigned)key * 2654435761u) >> (32 - 11); }
 
 template <bool PLACE>
 __global__ void __launch_bounds__(kBkThreads) k_bucket(const int16_t* __restrict__ pmid, const float* __restrict__ disp,
@@ -319,6 +319,10 @@ __global__ void __launch_bounds__(kBkThreads) k_bucket(const int16_t* __restrict
   __syncthreads();
   for (int64_t c = blockIdx.x; c < m.nchunks; c += gridDim.x) {
     int code[kBkPer];
+    // the placing pass keeps each particle's record in registers between its two phases
+    // (pmid packed as 2 x 16 bits + 16 bits with the valid flag, disp as 3 floats)
+    uint32_t pk0[PLACE ? kBkPer : 1], pk1[PLACE ? kBkPer : 1];
+    float pd[PLACE ? kBkPer : 1][3];
 #pragma unroll
     for (int e = 0; e < kBkPer; ++e) {
       const int64_t i = chunk_index(m, c, e, t);
@@ -333,6 +337,12 @@ __global__ void __launch_bounds__(kBkThreads) k_bucket(const int16_t* __restrict
       }
       const uint8_t v = valid ? valid[ic] : (uint8_t)1;
       const int key = raw_cell_key(g, p, d, v != 0);
+      if constexpr (PLACE) {
+        pk0[e] = ((uint32_t)p[0] & 0xffffu) | ((uint32_t)p[1] << 16);
+        pk1[e] = ((uint32_t)p[2] & 0xffffu) | ((uint32_t)v << 16);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) pd[e][a] = d[a];
+      }
       code[e] = -1;
       if (act) {
         unsigned h = slot_hash(key);
@@ -357,19 +367,19 @@ __global__ void __launch_bounds__(kBkThreads) k_bucket(const int16_t* __restrict
       }
     }
     __syncthreads();
-    if (PLACE) {
-#pragma unroll 4
+    if constexpr (PLACE) {
+#pragma unroll
       for (int e = 0; e < kBkPer; ++e) {
         if (code[e] < 0) continue;
-        const int64_t i = chunk_index(m, c, e, t);  // reloaded: the chunk is still in L2
+        const int64_t i = chunk_index(m, c, e, t);
         const int dest = s_cnt[code[e] >> kRankBits] + (code[e] & ((1 << kRankBits) - 1));
         perm[dest] = (int32_t)i;
+        pmid_s[(int64_t)dest * 3 + 0] = (int16_t)(pk0[e] & 0xffffu);
+        pmid_s[(int64_t)dest * 3 + 1] = (int16_t)(pk0[e] >> 16);
+        pmid_s[(int64_t)dest * 3 + 2] = (int16_t)(pk1[e] & 0xffffu);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          pmid_s[(int64_t)dest * 3 + a] = pmid[i * 3 + a];
-          disp_s[(int64_t)dest * 3 + a] = disp[i * 3 + a];
-        }
-        valid_s[dest] = valid ? valid[i] : (uint8_t)1;
+        for (int a = 0; a < 3; ++a) disp_s[(int64_t)dest * 3 + a] = pd[e][a];
+        valid_s[dest] = (uint8_t)(pk1[e] >> 16);
       }
     }
     __syncthreads();
@@ -384,14 +394,14 @@ __global__ void __launch_bounds__(kBkThreads) k_bucket(const int16_t* __restrict
 ChunkMap make_chunk_map(int64_t n, const int32_t* grid) {
   ChunkMap m{};
   m.n = n;
-  if (grid && (int64_t)grid[0] * grid[1] * grid[2] == n && grid[0] % 8 == 0 && grid[1] % 8 == 0 &&
+  if (grid && (int64_t)grid[0] * grid[1] * grid[2] == n && grid[0] % 4 == 0 && grid[1] % 8 == 0 &&
       grid[2] % 64 == 0) {
     m.tiled = 1;
     m.g1 = grid[1];
     m.g2 = grid[2];
     m.nby = grid[1] / 8;
     m.nbz = grid[2] / 64;
-    m.nchunks = (int64_t)(grid[0] / 8) * m.nby * m.nbz;
+    m.nchunks = (int64_t)(grid[0] / 4) * m.nby * m.nbz;
   } else {
     m.nchunks = ceil_div(n, kBkChunk);
   }

@@ -69,22 +69,39 @@ class _GravityFn(torch.autograd.Function):
         return disp_cot, omega_cot.to(omega_m.dtype).reshape(omega_m.shape), None, None, None
 
 
+class _CorrectedGravityFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, disp, omega_m, ptcl, conf, correction, a):
+        from ..corrections.training import corrected_gravity
+
+        p = ptcl.replace(disp=disp.detach())
+        ctx.state = (p, conf, correction, a, omega_m)
+        return corrected_gravity(a, p, float(omega_m.detach()), conf, correction)
+
+    @staticmethod
+    def backward(ctx, acc_cot):
+        from ..corrections.training import corrected_gravity_vjp
+
+        p, conf, correction, a, omega_m = ctx.state
+        d_disp, d_om = corrected_gravity_vjp(a, p, float(omega_m.detach()), conf, correction, acc_cot)
+        om = torch.as_tensor(d_om, dtype=omega_m.dtype, device=omega_m.device).reshape(omega_m.shape)
+        return d_disp, om, None, None, None, None
+
+
 def gravity(a, ptcl, cosmo, conf, correction=None):
     """Gravitational accelerations of particles in [H_0^2] (gravity.py:786-814).
 
     ``cosmo`` needs an ``Omega_m`` (float or tensor; a tensor with ``requires_grad`` gets a
     cotangent).  ``a`` is unused by the uncorrected force, as in the reference.  A
     ``MeshCNNPotentialCorrection`` adds its density residual before the Poisson solve
-    (gravity.py:635-640 with corrections/core.py:286-334); that path has no reverse mode
-    through the network's inputs and refuses ``requires_grad`` inputs.
+    (gravity.py:635-640 with corrections/core.py:286-334); its reverse runs through the network
+    and its density input (``corrected_gravity_vjp``).
     """
     if correction is not None:
-        from ..corrections.training import corrected_gravity
-
-        om = cosmo.Omega_m if hasattr(cosmo, "Omega_m") else cosmo
-        if ptcl.disp.requires_grad or (isinstance(om, torch.Tensor) and om.requires_grad):
-            raise NotImplementedError("gradients through the corrected force's network inputs are not implemented")
-        return corrected_gravity(a, ptcl, cosmo, conf, correction)
+        omega = cosmo.Omega_m if hasattr(cosmo, "Omega_m") else cosmo
+        if not isinstance(omega, torch.Tensor):
+            omega = torch.tensor(float(omega), dtype=torch.float32, device=ptcl.disp.device)
+        return _CorrectedGravityFn.apply(ptcl.disp, omega, ptcl, conf, correction, float(a))
     del a
     omega = cosmo.Omega_m if hasattr(cosmo, "Omega_m") else cosmo
     if not isinstance(omega, torch.Tensor):

@@ -187,17 +187,23 @@ def drift_adj_from_output(a_vel, a_prev, a_next, st: AdjointState, cosmo, conf):
     st.omega_cot += _factor_dOm("drift_factor", a_vel, a_prev, a_next, cosmo, conf) * st.take_dot()
 
 
-def force_adj(a, st: AdjointState, cosmo, conf):
-    """Pull acc_cot through the force at the current state (integrator.py:620-686)."""
-    from .gravity import gravity_vjp
+def force_adj(a, st: AdjointState, cosmo, conf, correction=None):
+    """Pull acc_cot through the force at the current state (integrator.py:620-686); with a
+    learned correction also through the network and its density input."""
+    if correction is None:
+        from .gravity import gravity_vjp
 
-    d_disp, d_om = gravity_vjp(st.particles(), cosmo, conf, st.acc_cot)
+        d_disp, d_om = gravity_vjp(st.particles(), cosmo, conf, st.acc_cot)
+    else:
+        from ..corrections.training import corrected_gravity_vjp
+
+        d_disp, d_om = corrected_gravity_vjp(a, st.particles(), cosmo, conf, correction, st.acc_cot)
     st.disp_cot += d_disp
     st.omega_cot += d_om
     st.acc_cot.zero_()
 
 
-def integrate_adj(a_prev, a_next, st: AdjointState, cosmo, conf):
+def integrate_adj(a_prev, a_next, st: AdjointState, cosmo, conf, correction=None):
     """Reverse one macro step from its output state (integrator.py:781-837)."""
     for stage in reversed(stage_schedule(a_prev, a_next, conf)):
         if stage[0] == "kick":
@@ -205,7 +211,7 @@ def integrate_adj(a_prev, a_next, st: AdjointState, cosmo, conf):
             kick_adj(a_acc, a_vel, a_vel_next, st, cosmo, conf)
             continue
         _, a_vel, a_disp, a_disp_next, a_acc_in = stage
-        force_adj(a_disp_next, st, cosmo, conf)
+        force_adj(a_disp_next, st, cosmo, conf, correction=correction)
         drift_adj_from_output(a_vel, a_disp, a_disp_next, st, cosmo, conf)
         # restore the acceleration the earlier kick consumed
-        st.acc = force(a_acc_in, st.particles(), cosmo, conf).acc.contiguous()
+        st.acc = force(a_acc_in, st.particles(), cosmo, conf, correction=correction).acc.contiguous()

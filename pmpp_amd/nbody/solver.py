@@ -41,7 +41,7 @@ def _run(ptcl, cosmo, conf, reverse=False, correction=None, observer=None):
     return ptcl
 
 
-def nbody_adj(ptcl, ptcl_cot, cosmo, conf, reverse=False):
+def nbody_adj(ptcl, ptcl_cot, cosmo, conf, reverse=False, correction=None):
     """Reverse sweep from the final state (solver.py:385-444).
 
     ``ptcl`` is the final state (disp, vel, acc), ``ptcl_cot`` a Particles-like object whose
@@ -52,9 +52,9 @@ def nbody_adj(ptcl, ptcl_cot, cosmo, conf, reverse=False):
     a = _schedule(conf, reverse)
     st = AdjointState(ptcl, ptcl_cot.disp, ptcl_cot.vel, ptcl_cot.acc)
     for a0, a1 in zip(a[:-1][::-1], a[1:][::-1]):
-        integrate_adj(float(a0), float(a1), st, cosmo, conf)
+        integrate_adj(float(a0), float(a1), st, cosmo, conf, correction=correction)
     # the initial force (computed by nbody_init before the first step) is pulled through last
-    force_adj(float(a[0]), st, cosmo, conf)
+    force_adj(float(a[0]), st, cosmo, conf, correction=correction)
     ptcl_in = st.particles()
     cot = ptcl.replace(disp=st.disp_cot, vel=st.vel_cot, acc=st.acc_cot)
     return ptcl_in, cot, st.omega_cot
@@ -70,23 +70,23 @@ class _Cosmo:
 
 class _NBodyFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, disp, vel, omega_m, ptcl, cosmo, conf, reverse):
+    def forward(ctx, disp, vel, omega_m, ptcl, cosmo, conf, reverse, correction):
         c = _Cosmo(cosmo, float(omega_m))
         p = ptcl.replace(disp=disp.detach().contiguous(), vel=vel.detach().contiguous())
-        out = _run(p, c, conf, reverse)
-        ctx.state = (out, c, conf, reverse, omega_m)
+        out = _run(p, c, conf, reverse, correction=correction)
+        ctx.state = (out, c, conf, reverse, omega_m, correction)
         return out.disp, out.vel, out.acc
 
     @staticmethod
     def backward(ctx, disp_cot, vel_cot, acc_cot):
-        out, c, conf, reverse, omega_m = ctx.state
+        out, c, conf, reverse, omega_m, correction = ctx.state
         zeros = torch.zeros_like(out.disp)
         cot = out.replace(disp=disp_cot if disp_cot is not None else zeros,
                           vel=vel_cot if vel_cot is not None else zeros,
                           acc=acc_cot if acc_cot is not None else zeros)
-        _, cin, om_cot = nbody_adj(out, cot, c, conf, reverse)
+        _, cin, om_cot = nbody_adj(out, cot, c, conf, reverse, correction=correction)
         om = torch.as_tensor(om_cot, dtype=omega_m.dtype, device=omega_m.device).reshape(omega_m.shape)
-        return cin.disp, cin.vel, om, None, None, None, None
+        return cin.disp, cin.vel, om, None, None, None, None, None
 
 
 def nbody(ptcl, cosmo, conf, reverse=False, correction=None, observer=None):
@@ -94,20 +94,21 @@ def nbody(ptcl, cosmo, conf, reverse=False, correction=None, observer=None):
 
     ``ptcl.vel`` must be set (zeros for a cold start).  Differentiable with respect to
     ``ptcl.disp``, ``ptcl.vel`` and a tensor ``cosmo.Omega_m`` through the reverse sweep.
-    With a ``correction`` (a learned force correction, ``pmpp_amd.corrections``) the run is
-    forward only.  ``observer(a, ptcl)`` (optional) sees the state after the initial force and
+    A ``correction`` (a learned force correction, ``pmpp_amd.corrections``) is applied at every
+    force; the reverse sweep then also runs through the network and its density input.
+    ``observer(a, ptcl)`` (optional, forward only) sees the state after the initial force and
     after every step (the reference's observers, solver.py:21-63).
     """
     if ptcl.vel is None:
         raise ValueError("nbody needs particle velocities (ptcl.vel)")
-    if correction is not None or observer is not None:
+    if observer is not None:
         om = cosmo.Omega_m
         if ptcl.disp.requires_grad or ptcl.vel.requires_grad or (isinstance(om, torch.Tensor) and om.requires_grad):
-            raise NotImplementedError("reverse mode with a correction or an observer is not implemented")
+            raise NotImplementedError("reverse mode with an observer is not implemented")
         with torch.no_grad():
             return _run(ptcl, cosmo, conf, reverse, correction=correction, observer=observer)
     om = cosmo.Omega_m
     om_t = om if isinstance(om, torch.Tensor) else torch.tensor(float(om), dtype=torch.float64,
                                                                 device=ptcl.disp.device)
-    disp, vel, acc = _NBodyFn.apply(ptcl.disp, ptcl.vel, om_t, ptcl, cosmo, conf, reverse)
+    disp, vel, acc = _NBodyFn.apply(ptcl.disp, ptcl.vel, om_t, ptcl, cosmo, conf, reverse, correction)
     return ptcl.replace(disp=disp, vel=vel, acc=acc)

@@ -103,9 +103,10 @@ def _torch_conv(x, w, b, cin):
 
 def _torch_residual(model, rho, a, om, params):
     """float32 restatement of the network with the kernels' bf16 roundings (differentiable in
-    the float32 parameter vector ``params``)."""
+    the float32 parameter vector ``params``, in ``rho`` and in a tensor ``om``)."""
     r, s8 = model.max_residual, model.sigma8_value
-    c = torch.tensor(1.5 * om, dtype=torch.float32)
+    om = torch.as_tensor(om, dtype=torch.float32)
+    c = 1.5 * om
     S = (rho - 1.0) * c
     sigma = torch.sqrt((S.double() ** 2).mean().float() + 1e-6)
     ones = torch.ones_like(rho)
@@ -302,3 +303,71 @@ def test_correction_trained_on_fine_runs_moves_pk_towards_the_fine_run(device):
     assert trained_loss < 0.95 * plain_loss
     assert err_corr < 0.8 * err_plain
     assert r_corr > r_plain - 0.03  # the phases stay as close to the fine run's
+
+
+def test_corrected_force_reverse_through_the_network_inputs(device):
+    # disp and Omega_m cotangents of the corrected force: the HIP chain (force-step adjoint at the
+    # corrected source, network backward to its input, input-path kernel, scatter adjoint)
+    # against the float64 oracle force adjoint plus float32 torch autograd of the network
+    # (straight-through bf16 roundings) and the oracle scatter adjoint
+    from pmpp_amd.corrections.training import corrected_gravity_vjp, density
+
+    conf = Configuration(2.0, (16, 16, 16), mesh_shape=1)
+    ptcl = _ptcl(conf, 0.6, 21, device)
+    model = _randomised(init_mesh_cnn_potential_correction(seed=4, depth=2, device=device), 4, out_scale=0.2)
+    om, a = 0.3, 0.8
+    rng = np.random.default_rng(12)
+    ac = rng.normal(size=(conf.ptcl_num, 3)).astype(np.float32)
+    dc, oc = corrected_gravity_vjp(a, ptcl, om, conf, model, torch.from_numpy(ac).to(device))
+    # reference
+    rho = density(ptcl, om, conf).cpu()
+    params = model.params.detach().cpu()
+    rho_t = rho.clone().requires_grad_(True)
+    om_t = torch.tensor(om, dtype=torch.float32, requires_grad=True)
+    rho_add = _torch_residual(model, rho_t, a, om_t, params)
+    pm, dp = ptcl.pmid.cpu().numpy().astype(np.int64), _np(ptcl.disp)
+    dc0, oc0, rc0 = spectral.gravity_vjp(pm, dp, om, ac, conf.mesh_shape, conf.cell_size, conf.ptcl_num,
+                                         _masks(conf), rho_add=_np(rho_add))
+    (rho_add * torch.from_numpy(rc0.astype(np.float32))).sum().backward()
+    val = spectral.default_value(len(pm), conf.mesh_shape, conf.ptcl_num)
+    dsc, _ = cic.scatter_adj(pm, dp, val, rho_t.grad.numpy().astype(np.float64), conf.cell_size, conf.mesh_shape)
+    ref = dc0 + dsc
+    err = np.linalg.norm(_np(dc) - ref) / np.linalg.norm(ref)
+    assert err < 2e-2, err
+    assert np.linalg.norm(dsc) > 0.05 * np.linalg.norm(dc0)  # the input path is not negligible here
+    assert oc == pytest.approx(oc0 + float(om_t.grad), rel=2e-2)
+    # and the autograd wrapper gives the same
+    disp = ptcl.disp.clone().requires_grad_(True)
+    omt = torch.tensor(om, dtype=torch.float32, device=device, requires_grad=True)
+
+    class _C:
+        Omega_m = omt
+
+    acc = gravity(a, ptcl.replace(disp=disp), _C(), conf, correction=model)
+    (acc * torch.from_numpy(ac).to(device)).sum().backward()
+    np.testing.assert_allclose(_np(disp.grad), _np(dc), rtol=1e-5, atol=1e-6 * np.abs(_np(dc)).max())
+
+
+def test_nbody_reverse_sweep_with_a_correction_matches_finite_differences(device):
+    # the N-body adjoint with the corrected force (per-step recomputation through the network)
+    # against central differences of the corrected forward run along one random direction;
+    # float32 forward, so eps and the tolerance are loose
+    from pmpp_amd.nbody.solver import nbody
+
+    conf = Configuration(4.0, (16, 16, 16), mesh_shape=1, a_start=0.5, a_nbody_maxstep=0.25)
+    cos = LCDM()
+    base = _ptcl(conf, 0.3, 31, device)
+    base = base.replace(vel=torch.zeros_like(base.disp))
+    model = _randomised(init_mesh_cnn_potential_correction(seed=5, depth=1, device=device), 5, out_scale=0.1)
+    rng = np.random.default_rng(3)
+    cot = torch.from_numpy(rng.normal(size=(conf.ptcl_num, 3)).astype(np.float32)).to(device)
+    disp = base.disp.clone().requires_grad_(True)
+    out = nbody(base.replace(disp=disp), cos, conf, correction=model)
+    (out.disp * cot).sum().backward()
+    v = torch.from_numpy(rng.normal(size=(conf.ptcl_num, 3)).astype(np.float32)).to(device)
+    eps = 2e-2
+    with torch.no_grad():
+        fp = nbody(base.replace(disp=base.disp + eps * v), cos, conf, correction=model).disp
+        fm = nbody(base.replace(disp=base.disp - eps * v), cos, conf, correction=model).disp
+    fd = float(((fp - fm) * cot).sum()) / (2 * eps)
+    assert float((disp.grad * v).sum()) == pytest.approx(fd, rel=5e-2)
