@@ -176,6 +176,68 @@ def cpu_baseline(dev, omega_m):
     }
 
 
+def nbody_run(args):
+    """The full single-GPU run of the 512^3 / 1024^3 / 1000 Mpc/h workload: 2LPT at a = 1/64,
+    20 KDK steps to a = 1 (plain), the reverse sweep of that run, the same run with the learned
+    correction (BF16 network, forward), the memory high-water mark, and one plain force step on
+    a 2048^3 mesh for comparison.  Prints one JSON line (not the headline metric)."""
+    import torch
+
+    from pmpp_amd import Configuration, Particles, gravity
+    from pmpp_amd.corrections import init_mesh_cnn_potential_correction
+    from pmpp_amd.initial_conditions import LCDM, linear_modes, lpt, white_noise
+    from pmpp_amd.nbody.solver import nbody, nbody_adj
+
+    dev = torch.device("cuda", 0)
+    ng, ratio, box = CONFIGS["c5"]
+    a0, nsteps = 1 / 64, args.run_steps
+    conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio, a_start=a0, a_nbody_maxstep=(1 - a0) / nsteps)
+    cosmo = LCDM()
+    log(f"[run] 2LPT at a = {a0:g}: {ng}^3 particles")
+    ptcl = lpt(linear_modes(white_noise(0, conf, device=dev), cosmo, conf), cosmo, conf)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    out, t_plain = timed(lambda: nbody(ptcl, cosmo, conf, observer=lambda a, p: None))
+    log(f"[run] plain: {nsteps} steps in {t_plain:.3f} s")
+    g = torch.Generator(device="cpu").manual_seed(0)
+    cot = out.replace(disp=torch.randn(out.disp.shape, generator=g).to(dev), vel=torch.zeros_like(out.vel),
+                      acc=torch.zeros_like(out.acc))
+    _, t_adj = timed(lambda: nbody_adj(out, cot, cosmo, conf))
+    log(f"[run] reverse sweep in {t_adj:.3f} s")
+    del cot
+    model = init_mesh_cnn_potential_correction(seed=0, depth=2, device=dev)
+    corr, t_corr = timed(lambda: nbody(ptcl, cosmo, conf, correction=model, observer=lambda a, p: None))
+    log(f"[run] corrected: {nsteps} steps in {t_corr:.3f} s")
+    peak = torch.cuda.max_memory_allocated() / 2 ** 30
+    fin = out.pos(dtype=torch.float64)
+    del out, corr, ptcl, model
+    torch.cuda.empty_cache()
+    conf2 = conf.replace(mesh_shape=4)
+    p2 = Particles.from_pos(conf2, fin)
+    del fin
+    gravity(1.0, p2, cosmo.Omega_m, conf2)  # warm (workspace allocation)
+    reps = 3
+    _, t2 = timed(lambda: [gravity(1.0, p2, cosmo.Omega_m, conf2) for _ in range(reps)])
+    res = {
+        "metric": "N-body run steps/s (512^3 particles, 1024^3 mesh, 1000 Mpc/h, 20 KDK steps)",
+        "steps_per_s_plain": nsteps / t_plain, "steps_per_s_reverse": nsteps / t_adj,
+        "steps_per_s_corrected": nsteps / t_corr, "memory_high_water_GiB": round(peak, 2),
+        "force_step_ms_2048_mesh": t2 / reps * 1e3, "n_gpus": 1, "dtype": "f32 (network bf16)",
+        "data": "synthetic (2LPT LCDM field from seeded white noise)",
+        "config": {"workload": "N-body run: 512^3 particles, 1024^3 mesh, 1000 Mpc/h, a = 1/64 -> 1",
+                   "steps": nsteps, "comparison_mesh": 2048},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +245,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nbody-run", action="store_true",
+                    help="time the full 20-step run (plain, reverse sweep, with the learned correction) and a "
+                         "plain 2048^3-mesh force step instead of the force-step benchmark")
+    ap.add_argument("--run-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (the known-size kernel that "
                          "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -194,6 +260,10 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.nbody_run:
+        nbody_run(args)
+        return
 
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")

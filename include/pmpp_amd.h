@@ -233,13 +233,14 @@ int pm_power_spectrum_bwd(const float* spec, const int32_t shape[3], int32_t mas
  * (corrections/mesh_cnn.py:54-105, its Conv3D layers with periodic padding).  Mesh axes are
  * multiples of 8; x / w / res bf16, activations [X][Y][Z][C].
  * pm_conv3d_fwd        pre = conv(x; w[27][16][cin]) + bias (+ res); act = gelu(pre) or pre;
- *                      pre (float32) and act (bf16) are each optional outputs.
+ *                      pre (float32, the first pre_channels = 1 or 16 channels per voxel) and
+ *                      act (bf16, 16 channels) are each optional outputs.
  * pm_conv3d_bwd_data   dx = conv^T(dz; w) (+ add), times gelu'(gact) when gact is given
  *                      (16 -> 16 layers).
  * pm_conv3d_bwd_weight dw_db = [dW (27*16*cin floats), db (16 floats)] for pre-activation
  *                      gradient dz and layer input x; workspace from the query. */
 int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, const void* res, int32_t gelu,
-                  float* pre, void* act, const int32_t shape[3], void* stream);
+                  float* pre, int32_t pre_channels, void* act, const int32_t shape[3], void* stream);
 int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const float* gact, float* dx,
                        const int32_t shape[3], void* stream);
 int pm_conv3d_bwd_weight_workspace(const int32_t shape[3], int32_t cin, size_t* bytes);
@@ -254,8 +255,8 @@ int pm_conv3d_bwd_weight(const float* dz, const void* x, int32_t cin, const int3
  *                         g = max_residual tanh(gain_params . [a, Omega_m, sigma8, 1]) into
  *                         stats[0..2] (device floats); x0 = bf16 [S/sigma, a, Omega_m, sigma8,
  *                         0, 0, 0, 0] per cell (n x 8 bf16, 16-byte aligned).
- * pm_correction_residual  rho_add = g (rho - 1) + max_residual tanh(out[.,0]) sigma / c, out the
- *                         output layer's float32 pre-activations (n x 16).
+ * pm_correction_residual  rho_add = g (rho - 1) + max_residual tanh(out) sigma / c, out the
+ *                         output layer's float32 pre-activation of channel 0 (n floats).
  * pm_correction_residual_bwd  the output layer's pre-activation gradient dz_out (n x 16,
  *                         channel 0 live) and the gain parameters' gradient (4 floats) from
  *                         rho_cot.

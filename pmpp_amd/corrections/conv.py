@@ -20,19 +20,22 @@ def _shape(x):
     return tuple(int(s) for s in x.shape[:3])
 
 
-def conv3d(x, w, b, res=None, gelu=False, want_pre=True, want_act=True):
-    """(pre, act): pre = conv(x) + b (+ res) float32, act = gelu(pre) (or pre) bf16."""
+def conv3d(x, w, b, res=None, gelu=False, want_pre=True, want_act=True, pre_channels=16):
+    """(pre, act): pre = conv(x) + b (+ res) float32 (its first ``pre_channels`` channels: 16 or
+    1), act = gelu(pre) (or pre) bf16."""
     require_device(x, "x")
     if x.dtype != torch.bfloat16 or w.dtype != torch.bfloat16:
         raise RuntimeError("conv3d takes bf16 activations and weights")
     ensure_init(x.device)
     shape = _shape(x)
     cin = int(x.shape[3])
-    pre = torch.empty(shape + (16,), dtype=torch.float32, device=x.device) if want_pre else None
+    if pre_channels not in (1, 16):
+        raise ValueError("pre_channels must be 1 or 16")
+    pre = torch.empty(shape + (pre_channels,), dtype=torch.float32, device=x.device) if want_pre else None
     act = torch.empty(shape + (16,), dtype=torch.bfloat16, device=x.device) if want_act else None
     _lib.call("pm_conv3d_fwd", ptr(x.contiguous()), cin, ptr(w.contiguous()), ptr(b.contiguous()),
-              ptr(res.contiguous()) if res is not None else None, int(gelu), ptr(pre), ptr(act), _lib.int3(shape),
-              stream_handle(x.device))
+              ptr(res.contiguous()) if res is not None else None, int(gelu), ptr(pre), int(pre_channels), ptr(act),
+              _lib.int3(shape), stream_handle(x.device))
     return pre, act
 
 

@@ -204,7 +204,8 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
         p2, x = conv3d(h, model.weight(l2), model.bias(l2), res=x, gelu=True, want_pre=keep)
         pres += [p1, p2]
         acts += [h, x]
-    out, _ = conv3d(x, model.weight(layers[-1]), model.bias(layers[-1]), gelu=False, want_pre=True, want_act=False)
+    out, _ = conv3d(x, model.weight(layers[-1]), model.bias(layers[-1]), gelu=False, want_pre=True, want_act=False,
+                    pre_channels=1)
     rho_add = torch.empty_like(rho)
     _lib.call("pm_correction_residual", ptr(out), ptr(rho), n, om, float(model.max_residual), ptr(stats),
               ptr(rho_add), s)

@@ -70,12 +70,13 @@ __device__ __forceinline__ void tile_voxel(int tile, int m, int& vx, int& vy, in
 }
 
 // Forward: pre = conv(x) + bias (+ res), act = gelu(pre) (or pre when !GELU), written as
-// float32 pre-activations (when pre != null) and bf16 activations (when act != null).
+// float32 pre-activations (when pre != null; the first pch of the 16 channels per voxel) and
+// bf16 activations (when act != null).
 template <int CIN, bool GELU>
 __global__ void __launch_bounds__(kThreads) k_conv_fwd(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
                                                        const float* __restrict__ bias, const __bf16* __restrict__ res,
                                                        float* __restrict__ pre, __bf16* __restrict__ act, int n0,
-                                                       int n1, int n2) {
+                                                       int n1, int n2, int pch) {
   constexpr int TPS = 32 / CIN;                       // taps per K step
   constexpr int STEPS = (kTaps + TPS - 1) / TPS;      // K steps (the last may be padded)
   __shared__ __align__(16) __bf16 xin[kH * kH * kH * CIN];
@@ -114,7 +115,7 @@ __global__ void __launch_bounds__(kThreads) k_conv_fwd(const __bf16* __restrict_
       const int64_t vox = ((int64_t)(ox + qx) * n1 + (oy + qy)) * n2 + (oz + qz);
       float z = c[i] + bias[m];
       if (res) z += (float)res[vox * kCout + m];
-      if (pre) pre[vox * kCout + m] = z;
+      if (pre && m < pch) pre[vox * pch + m] = z;
       if (act) act[vox * kCout + m] = (__bf16)(GELU ? gelu(z) : z);
     }
   }
@@ -304,8 +305,9 @@ constexpr int kWGrid = 1024;  // persistent workgroups of the weight-gradient pa
 extern "C" {
 
 int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, const void* res, int32_t gelu,
-                  float* pre, void* act, const int32_t shape[3], void* stream) {
+                  float* pre, int32_t pre_channels, void* act, const int32_t shape[3], void* stream) {
   PM_TRY(check_shape(shape));
+  PM_RETURN_IF(pre_channels != 1 && pre_channels != 16, PM_ERROR_INVALID_ARGUMENT, "pre_channels must be 1 or 16");
   PM_RETURN_IF(cin != 8 && cin != 16, PM_ERROR_UNSUPPORTED, "conv3d input channels must be 8 or 16");
   PM_TRY(check_ptr16(x, "x"));
   PM_RETURN_IF(w == nullptr || bias == nullptr, PM_ERROR_INVALID_ARGUMENT, "weights / bias NULL");
@@ -318,16 +320,16 @@ int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, 
   const dim3 grid(nblocks(shape)), block(conv::kThreads);
   if (cin == 8 && gelu)
     hipLaunchKernelGGL((conv::k_conv_fwd<8, true>), grid, block, 0, s, xb, wb, bias, rb, pre, ab, shape[0], shape[1],
-                       shape[2]);
+                       shape[2], (int)pre_channels);
   else if (cin == 8)
     hipLaunchKernelGGL((conv::k_conv_fwd<8, false>), grid, block, 0, s, xb, wb, bias, rb, pre, ab, shape[0], shape[1],
-                       shape[2]);
+                       shape[2], (int)pre_channels);
   else if (gelu)
     hipLaunchKernelGGL((conv::k_conv_fwd<16, true>), grid, block, 0, s, xb, wb, bias, rb, pre, ab, shape[0],
-                       shape[1], shape[2]);
+                       shape[1], shape[2], (int)pre_channels);
   else
     hipLaunchKernelGGL((conv::k_conv_fwd<16, false>), grid, block, 0, s, xb, wb, bias, rb, pre, ab, shape[0],
-                       shape[1], shape[2]);
+                       shape[1], shape[2], (int)pre_channels);
   PM_LAUNCHED("k_conv_fwd", s);
   return PM_SUCCESS;
 }

@@ -5,7 +5,7 @@
 //   S       = c (rho - 1),  c = 1.5 Omega_m                       (the source of the solve)
 //   sigma   = sqrt(mean(S^2) + 1e-6)                              (mesh_cnn.py:79-80)
 //   x0      = bf16 [S / sigma, a, Omega_m, sigma8, 0, 0, 0, 0]     (mesh_cnn.py:21-27)
-//   o       = conv stack(x0), channel 0 of the output layer
+//   o       = conv stack(x0), channel 0 of the output layer (stored alone, n floats)
 //   g       = r tanh(w . [a, Omega_m, sigma8] + b)                (the direct gain, :98-103)
 //   rho_add = g (rho - 1) + r tanh(o) sigma / c                   (:97, :104 moved to the source)
 // so that the solve sees S' = (1 + g) S + r tanh(o) sigma: the reference's potential residual
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(const float* __restrict__ o
   const float g = stats[1];
   const float amp = r * stats[0] / c;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    rho_add[i] = g * (rho[i] - 1.f) + amp * tanhf(out[i * 16]);
+    rho_add[i] = g * (rho[i] - 1.f) + amp * tanhf(out[i]);
 }
 
 // dz (the output layer's pre-activation gradient, 16 channels, channel 0 live) from rho_cot,
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_bwd(const float* __restrict
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const float rc = rho_cot[i];
-    const float t = tanhf(out[i * 16]);
+    const float t = tanhf(out[i]);
     acc += (double)(rc * (rho[i] - 1.f));
     const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
     dz[i * 4 + 0] = make_float4(rc * amp * (1.f - t * t), 0.f, 0.f, 0.f);
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(kBlock) k_input_sums(const float* __restrict__
     const float rc = rho_cot[i];
     const float d = rho[i] - 1.f;
     const float4 g = dx0[i * 4];  // channels 0..3 of the input gradient
-    s1 += (double)(rc * tanhf(out[i * 16]));
+    s1 += (double)(rc * tanhf(out[i]));
     s2 += (double)(g.x * (d * c));
     s3 += (double)g.z;
     s4 += (double)(rc * d);

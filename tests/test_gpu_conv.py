@@ -41,6 +41,11 @@ def test_forward_matches_torch(device, cin, shape, gelu):
     np.testing.assert_allclose(pre.cpu().numpy(), ref.numpy(), atol=1e-4 * scale)
     r_act = Fn.gelu(ref, approximate="tanh") if gelu else ref
     np.testing.assert_allclose(act.float().cpu().numpy(), _bf(r_act).numpy(), atol=1e-2 * scale)
+    # a single stored channel is channel 0 of the full output, bit for bit
+    pre1, _ = conv3d(x.to(device, torch.bfloat16), w.to(device, torch.bfloat16), b.to(device),
+                     res=res.to(device, torch.bfloat16), gelu=gelu, want_act=False, pre_channels=1)
+    assert pre1.shape == shape + (1,)
+    np.testing.assert_array_equal(pre1[..., 0].cpu().numpy(), pre[..., 0].cpu().numpy())
 
 
 def test_backward_data_and_weight_match_autograd(device):
