@@ -24,7 +24,7 @@ import torch
 
 from .. import _lib
 from .._torch import as_contig, ensure_init, ptr, stream_handle, workspace
-from ..nbody.gravity import gravity_desc, _valid_u8
+from ..nbody.gravity import _check_particles, _valid_u8, gravity_desc
 from .mesh_cnn import MeshCNNPotentialCorrection, evaluate_residual, residual_backward
 
 
@@ -86,7 +86,7 @@ def density(ptcl, cosmo, conf):
     d = gravity_desc(conf, _omega(cosmo))
     pm = as_contig(ptcl.pmid, torch.int16, "pmid")
     dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
-    n = pm.shape[0]
+    n = _check_particles(pm, dp)
     nbytes = _lib.size_query("pm_gravity_workspace", d, n)
     ws = workspace(nbytes, dev)
     rho = torch.empty(tuple(int(s) for s in conf.mesh_shape), dtype=torch.float32, device=dev)
@@ -110,7 +110,7 @@ def gravity_with_residual(ptcl, cosmo, conf, rho_add):
     d = gravity_desc(conf, _omega(cosmo))
     pm = as_contig(ptcl.pmid, torch.int16, "pmid")
     dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
-    n = pm.shape[0]
+    n = _check_particles(pm, dp)
     nbytes = _lib.size_query("pm_gravity_workspace", d, n)
     ws = workspace(nbytes, dev)
     acc = torch.empty((n, 3), dtype=torch.float32, device=dev)
@@ -126,7 +126,7 @@ def gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot):
     d = gravity_desc(conf, _omega(cosmo))
     pm = as_contig(ptcl.pmid, torch.int16, "pmid")
     dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
-    n = pm.shape[0]
+    n = _check_particles(pm, dp)
     nbytes = _lib.size_query("pm_gravity_vjp_workspace", d, n)
     ws = workspace(nbytes, dev)
     ac = as_contig(acc_cot.detach(), torch.float32, "acc_cot")

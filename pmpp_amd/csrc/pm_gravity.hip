@@ -245,8 +245,13 @@ template <int BX, int BY, int BZ>
 __global__ void __launch_bounds__(kTileThreads) k_scatter_columns(
     const int16_t* __restrict__ pmid, const float* __restrict__ disp, const int32_t* __restrict__ bin_off,
     float val, Geom g, Bricks k, float* __restrict__ mesh, float* __restrict__ xs, float* __restrict__ ys,
-    unsigned long long* __restrict__ first) {
+    unsigned long long* __restrict__ first, const float* __restrict__ vals, const float* __restrict__ vscale) {
   __shared__ __align__(16) unsigned long long tile[kTileMaxWords];
+  // per-particle values (sorted order) are brought to |v| <= 1 by vscale[0] before the fixed
+  // point and scaled back on output; a constant value is used as it is
+  const float vs = vals ? vscale[0] : 1.f;
+  const float us = vals ? kUnfix / vs : kUnfix;
+  auto unfix_s = [&](unsigned long long x) { return (float)(long long)x * us; };
   const BrickShape<BX, BY, BZ> B(k);
   const int col = column_of(blockIdx.x, gridDim.x);
   const int cx = col / k.nb[1], cy = col - cx * k.nb[1];
@@ -280,9 +285,10 @@ __global__ void __launch_bounds__(kTileThreads) k_scatter_columns(
         float fr[3];
         raw_base_frac(cur[q], g.a1, k.n, bs, fr);
         const int c0i = local_cell(B, bs, o);
+        const float v = vals ? vals[j] * vs : val;
 #pragma unroll
         for (int c = 0; c < 8; ++c)
-          atomicAdd(&tile[c0i + B.corner_off(c)], (unsigned long long)__float2ll_rn(val * frac_weight(fr, c) * kFix));
+          atomicAdd(&tile[c0i + B.corner_off(c)], (unsigned long long)__float2ll_rn(v * frac_weight(fr, c) * kFix));
       }
     }
     __syncthreads();
@@ -323,8 +329,8 @@ __global__ void __launch_bounds__(kTileThreads) k_scatter_columns(
         }
         const unsigned long long* src = &tile[B.cell(x, y, 4 * c4)];
         longlong2 lo = *reinterpret_cast<const longlong2*>(src), hi = *reinterpret_cast<const longlong2*>(src + 2);
-        *reinterpret_cast<float4*>(dst) = make_float4((float)lo.x * kUnfix, (float)lo.y * kUnfix,
-                                                      (float)hi.x * kUnfix, (float)hi.y * kUnfix);
+        *reinterpret_cast<float4*>(dst) =
+            make_float4((float)lo.x * us, (float)lo.y * us, (float)hi.x * us, (float)hi.y * us);
       }
     } else {
       const int own = B.b0 * B.b1 * B.b2, xp = B.t1() * B.b2, yp = B.b0 * B.b2;
@@ -332,13 +338,13 @@ __global__ void __launch_bounds__(kTileThreads) k_scatter_columns(
         if (i < own) {
           int r = i / B.b2, z = i - r * B.b2;
           int x = r / B.b1, y = r - x * B.b1;
-          mesh[((int64_t)(o[0] + x) * n1 + (o[1] + y)) * n2 + (o[2] + z)] = unfix(tile[B.cell(x, y, z)]);
+          mesh[((int64_t)(o[0] + x) * n1 + (o[1] + y)) * n2 + (o[2] + z)] = unfix_s(tile[B.cell(x, y, z)]);
         } else if (i < own + xp) {
           int r = (i - own) / B.b2, z = (i - own) - r * B.b2;
-          xs[((int64_t)col * B.t1() + r) * n2 + o[2] + z] = unfix(tile[B.cell(B.b0, r, z)]);
+          xs[((int64_t)col * B.t1() + r) * n2 + o[2] + z] = unfix_s(tile[B.cell(B.b0, r, z)]);
         } else {
           int r = (i - own - xp) / B.b2, z = (i - own - xp) - r * B.b2;
-          ys[((int64_t)col * B.b0 + r) * n2 + o[2] + z] = unfix(tile[B.cell(r, B.b1, z)]);
+          ys[((int64_t)col * B.b0 + r) * n2 + o[2] + z] = unfix_s(tile[B.cell(r, B.b1, z)]);
         }
       }
     }
@@ -350,7 +356,7 @@ __global__ void __launch_bounds__(kTileThreads) k_scatter_columns(
   // the first brick's z = 0 plane: its own part plus the carry of the last brick (periodic)
   for (int r = threadIdx.x; r < rows; r += kTileThreads) {
     const int x = r / B.t1(), y = r - x * B.t1();
-    const float v = unfix(first[(int64_t)col * rows + r] + tile[r * P + carry]);
+    const float v = unfix_s(first[(int64_t)col * rows + r] + tile[r * P + carry]);
     if (x < B.b0 && y < B.b1)
       mesh[((int64_t)(cx * B.b0 + x) * n1 + (cy * B.b1 + y)) * n2] = v;
     else if (x == B.b0)
@@ -577,6 +583,7 @@ __global__ void k_zero_invalid(const int32_t* __restrict__ perm, const int32_t* 
 
 // Reverse of the three gathers (gather^T = scatter of acc_cot, plus the displacement
 // cotangent from the mesh values), pallas.py:321-355 per component.
+template <bool SCATTER>
 __global__ void __launch_bounds__(kBlock) k_gather3_bwd_sorted(
     const int16_t* __restrict__ pmid, const float* __restrict__ disp, const uint8_t* __restrict__ valid,
     const int32_t* __restrict__ perm, int64_t n, Geom g, const float* __restrict__ fx,
@@ -597,9 +604,11 @@ __global__ void __launch_bounds__(kBlock) k_gather3_bwd_sorted(
       float w = corner_weight(ax, c);
       float gr[3];
       corner_grad(ax, c, gr);
-      atomic_add_f32(gx + flat, ac[0] * w);
-      atomic_add_f32(gy + flat, ac[1] * w);
-      atomic_add_f32(gz + flat, ac[2] * w);
+      if (SCATTER) {
+        atomic_add_f32(gx + flat, ac[0] * w);
+        atomic_add_f32(gy + flat, ac[1] * w);
+        atomic_add_f32(gz + flat, ac[2] * w);
+      }
       float inner = ac[0] * fx[flat] + ac[1] * fy[flat] + ac[2] * fz[flat];
 #pragma unroll
       for (int a = 0; a < 3; ++a) dc[a] = dc[a] + inner * gr[a];
@@ -607,6 +616,40 @@ __global__ void __launch_bounds__(kBlock) k_gather3_bwd_sorted(
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) disp_cot_s[j * 3 + a] = dc[a] / g.a1;
+}
+
+// Sorted, component-major copies of the acceleration cotangent (cot_s[c n + j] = acc_cot of
+// sorted particle j) for the tiled gather-adjoint scatter, and per-block maxima of |acc_cot|.
+__global__ void __launch_bounds__(kBlock) k_cot_sorted(const int32_t* __restrict__ perm,
+                                                       const float* __restrict__ acc_cot, int64_t n,
+                                                       float* __restrict__ cot_s, float* __restrict__ pmax) {
+  float mx = 0.f;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = perm[j];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float v = acc_cot[i * 3 + c];
+      cot_s[c * n + j] = v;
+      mx = fmaxf(mx, fabsf(v));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  __shared__ float part[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = 0.f;
+    for (int w = 0; w < kBlock / 64; ++w) m = fmaxf(m, part[w]);
+    pmax[blockIdx.x] = m;
+  }
+}
+
+// scale[0] = 1 / max |acc_cot| (1 when all are zero): the values enter the fixed point <= 1
+__global__ void k_cot_scale(const float* __restrict__ pmax, int nb, float* __restrict__ scale) {
+  if (threadIdx.x != 0) return;
+  float m = 0.f;
+  for (int i = 0; i < nb; ++i) m = fmaxf(m, pmax[i]);
+  scale[0] = m > 0.f ? 1.f / m : 1.f;
 }
 
 // Scatter adjoint for the density: disp_cot += v * sum_c dens_cot[c] grad_c / cell, then the
@@ -752,6 +795,8 @@ int check_desc(const pm_gravity_desc* d, int64_t n) {
   return sort::check_sort_args(n, d->mesh_shape, d->brick);
 }
 
+constexpr int kCotBlocks = 1024;  // blocks of k_cot_sorted (and slots of their maxima)
+
 struct Buffers {
   int32_t* perm;
   int16_t* pmid_s;
@@ -777,6 +822,8 @@ struct Buffers {
   float* g2;
   float* g3;
   float* disp_cot_s;
+  float* cot_s;      // sorted acc_cot, component-major (tiled reverse)
+  float* cot_pmax;   // per-block maxima, then the fixed-point scale
 };
 
 bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const int32_t* brick, bool reverse,
@@ -815,6 +862,8 @@ bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const in
     b.g2 = ws.take<float>(L.nreal);
     b.g3 = ws.take<float>(L.nreal);
     b.disp_cot_s = ws.take<float>(3 * n);
+    b.cot_s = b.tiled ? ws.take<float>(3 * n) : nullptr;
+    b.cot_pmax = b.tiled ? ws.take<float>(kCotBlocks + 1) : nullptr;
   }
   return ws.ok();
 }
@@ -851,6 +900,28 @@ int fill(float* p, int64_t n, hipStream_t s) {
 }
 
 // Bucketing by brick (with the payload gathered into bucket order), density.  Leaves rho in b.r0.
+// Tiled CIC scatter of the bucketed particles into mesh (whose every cell it writes): the
+// constant particle value, or per-particle values vals (sorted order) with vscale[0] * |vals|
+// <= 1; then the fold of the neighbouring columns' upper x/y planes.
+int scatter_tiled(const pm_gravity_desc* d, const Buffers& b, float* mesh, const float* vals, const float* vscale,
+                  hipStream_t s) {
+  const Geom g = step_geom(d);
+  Bricks k = make_bricks(d->mesh_shape, d->brick);
+  const unsigned ncols = (unsigned)((int64_t)k.nb[0] * k.nb[1]);
+  if (is_brick(k, 16, 16, 32))
+    hipLaunchKernelGGL((k_scatter_columns<16, 16, 32>), dim3(ncols), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
+                       b.bins, d->particle_value, g, k, mesh, b.xs, b.ys, b.first, vals, vscale);
+  else
+    hipLaunchKernelGGL((k_scatter_columns<0, 0, 0>), dim3(ncols), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
+                       b.bins, d->particle_value, g, k, mesh, b.xs, b.ys, b.first, vals, vscale);
+  PM_LAUNCHED("k_scatter_columns", s);
+  const int64_t plane_cells = ((int64_t)k.nb[0] * k.n[1] + (int64_t)(k.n[0] - k.nb[0]) * k.nb[1]) * k.n[2];
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(plane_cells, kBlock), 8192));
+  hipLaunchKernelGGL(k_fold_xy, dim3(blocks), dim3(kBlock), 0, s, k, b.xs, b.ys, mesh);
+  PM_LAUNCHED("k_fold_xy", s);
+  return PM_SUCCESS;
+}
+
 int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                   const pm_gravity_desc* d, const Layout& L, Buffers& b, Arena& ws, hipStream_t s) {
   const Geom g = step_geom(d);
@@ -863,22 +934,7 @@ int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, 
     }
   PM_TRY(sort::bucket_cells(pmid, disp, valid, n, d->mesh_shape, d->cell_size, d->brick, grid, b.perm, b.bins,
                             b.pmid_s, b.disp_s, b.valid_s, ws, s));
-  if (b.tiled) {
-    Bricks k = make_bricks(d->mesh_shape, d->brick);
-    const unsigned ncols = (unsigned)((int64_t)k.nb[0] * k.nb[1]);
-    if (is_brick(k, 16, 16, 32))
-      hipLaunchKernelGGL((k_scatter_columns<16, 16, 32>), dim3(ncols), dim3(kTileThreads), 0, s, b.pmid_s,
-                         b.disp_s, b.bins, d->particle_value, g, k, b.r0, b.xs, b.ys, b.first);
-    else
-      hipLaunchKernelGGL((k_scatter_columns<0, 0, 0>), dim3(ncols), dim3(kTileThreads), 0, s, b.pmid_s,
-                         b.disp_s, b.bins, d->particle_value, g, k, b.r0, b.xs, b.ys, b.first);
-    PM_LAUNCHED("k_scatter_columns", s);
-    const int64_t plane_cells = ((int64_t)k.nb[0] * k.n[1] + (int64_t)(k.n[0] - k.nb[0]) * k.nb[1]) * k.n[2];
-    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(plane_cells, kBlock), 8192));
-    hipLaunchKernelGGL(k_fold_xy, dim3(blocks), dim3(kBlock), 0, s, k, b.xs, b.ys, b.r0);
-    PM_LAUNCHED("k_fold_xy", s);
-    return PM_SUCCESS;
-  }
+  if (b.tiled) return scatter_tiled(d, b, b.r0, nullptr, nullptr, s);
   PM_TRY(fill(b.r0, L.nreal, s));
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter_sorted, dim3(grid_for(n)), dim3(kBlock), 0, s, b.pmid_s, b.disp_s, b.valid_s,
@@ -1109,18 +1165,40 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
   if (st) return st;
   st = grav::force_mesh_stage(desc, L, b, tb, s, true, rho_add);
   if (st) return st;
-  // gather^T: acc_cot scattered into three cotangent meshes, displacement cotangent from F
-  st = grav::fill(b.g1, L.nreal, s);
-  if (st) return st;
-  st = grav::fill(b.g2, L.nreal, s);
-  if (st) return st;
-  st = grav::fill(b.g3, L.nreal, s);
-  if (st) return st;
-  if (n > 0) {
-    hipLaunchKernelGGL(grav::k_gather3_bwd_sorted, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s, b.pmid_s,
-                       b.disp_s, b.valid_s, b.perm, n, g, b.g0, b.r1, b.r2, acc_cot, b.g1, b.g2, b.g3,
-                       b.disp_cot_s);
-    PM_LAUNCHED("k_gather3_bwd_sorted", s);
+  // gather^T: acc_cot scattered into three cotangent meshes, displacement cotangent from F.
+  // With brick tiling each component goes through the tiled scatter (sorted, scaled copies of
+  // acc_cot); otherwise through float atomics beside the displacement cotangent.
+  if (b.tiled) {
+    if (n > 0) {
+      hipLaunchKernelGGL(grav::k_gather3_bwd_sorted<false>, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s,
+                         b.pmid_s, b.disp_s, b.valid_s, b.perm, n, g, b.g0, b.r1, b.r2, acc_cot, nullptr, nullptr,
+                         nullptr, b.disp_cot_s);
+      PM_LAUNCHED("k_gather3_bwd_sorted", s);
+    }
+    const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, grav::kBlock), grav::kCotBlocks));
+    hipLaunchKernelGGL(grav::k_cot_sorted, dim3(nb), dim3(grav::kBlock), 0, s, b.perm, acc_cot, n, b.cot_s,
+                       b.cot_pmax);
+    PM_LAUNCHED("k_cot_sorted", s);
+    hipLaunchKernelGGL(grav::k_cot_scale, dim3(1), dim3(64), 0, s, b.cot_pmax, (int)nb, b.cot_pmax + grav::kCotBlocks);
+    PM_LAUNCHED("k_cot_scale", s);
+    float* gm[3] = {b.g1, b.g2, b.g3};
+    for (int c = 0; c < 3; ++c) {
+      st = grav::scatter_tiled(desc, b, gm[c], b.cot_s + (int64_t)c * n, b.cot_pmax + grav::kCotBlocks, s);
+      if (st) return st;
+    }
+  } else {
+    st = grav::fill(b.g1, L.nreal, s);
+    if (st) return st;
+    st = grav::fill(b.g2, L.nreal, s);
+    if (st) return st;
+    st = grav::fill(b.g3, L.nreal, s);
+    if (st) return st;
+    if (n > 0) {
+      hipLaunchKernelGGL(grav::k_gather3_bwd_sorted<true>, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s,
+                         b.pmid_s, b.disp_s, b.valid_s, b.perm, n, g, b.g0, b.r1, b.r2, acc_cot, b.g1, b.g2, b.g3,
+                         b.disp_cot_s);
+      PM_LAUNCHED("k_gather3_bwd_sorted", s);
+    }
   }
   // spectral adjoint: src_cot = irfftn(G * sum_i conj(D_i) rfftn(g_i))  (the symbol is
   // Hermitian, so the transpose of irfftn(H rfftn(.)) is irfftn(conj(H) rfftn(.)))

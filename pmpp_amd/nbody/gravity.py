@@ -29,9 +29,18 @@ def gravity_desc(conf, omega_m) -> _lib.GravityDesc:
     return d
 
 
+def _check_particles(pm, dp):
+    n = pm.shape[0]
+    if pm.ndim != 2 or pm.shape[1] != 3 or tuple(dp.shape) != (n, 3):
+        raise ValueError(f"pmid {tuple(pm.shape)} and disp {tuple(dp.shape)} must both be (N, 3)")
+    return n
+
+
 def _valid_u8(ptcl):
     if ptcl.unused_index is None:
         return None
+    if ptcl.unused_index.shape != (ptcl.pmid.shape[0],):
+        raise ValueError(f"unused_index has shape {tuple(ptcl.unused_index.shape)}, expected ({ptcl.pmid.shape[0]},)")
     return (~ptcl.unused_index.detach()).to(torch.uint8).contiguous()
 
 
@@ -43,7 +52,7 @@ class _GravityFn(torch.autograd.Function):
         d = gravity_desc(conf, omega_m.detach())
         pm = as_contig(pmid, torch.int16, "pmid")
         dp = as_contig(disp.detach(), torch.float32, "disp")
-        n = pm.shape[0]
+        n = _check_particles(pm, dp)
         nbytes = _lib.size_query("pm_gravity_workspace", d, n)
         ws = workspace(nbytes, dev)
         acc = torch.empty((n, 3), dtype=torch.float32, device=dev)
@@ -58,7 +67,7 @@ class _GravityFn(torch.autograd.Function):
         conf = ctx.conf
         dev = dp.device
         d = gravity_desc(conf, omega_m.detach())
-        n = pm.shape[0]
+        n = _check_particles(pm, dp)
         nbytes = _lib.size_query("pm_gravity_vjp_workspace", d, n)
         ws = workspace(nbytes, dev)
         ac = as_contig(acc_cot, torch.float32, "acc_cot")
@@ -121,7 +130,7 @@ def gravity_vjp(ptcl, cosmo, conf, acc_cot):
     d = gravity_desc(conf, float(omega))
     pm = as_contig(ptcl.pmid, torch.int16, "pmid")
     dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
-    n = pm.shape[0]
+    n = _check_particles(pm, dp)
     nbytes = _lib.size_query("pm_gravity_vjp_workspace", d, n)
     ws = workspace(nbytes, dev)
     ac = as_contig(acc_cot.detach(), torch.float32, "acc_cot")

@@ -144,3 +144,21 @@ def test_force_vjp_matches_oracle(device, ng, ratio):
     # gradients within 1e-3 (relative, field norm)
     assert np.linalg.norm(got - dref) / np.linalg.norm(dref) < 1e-3
     assert float(omega.grad) == pytest.approx(oref, rel=1e-3)
+
+
+def test_particle_subsets_empty_sets_and_all_invalid(device):
+    # a particle count that is not a whole lattice (odd, bucketed in plain index chunks), the
+    # empty set, and a set whose particles are all invalid
+    conf = Configuration(2.0, (16, 16, 16), mesh_shape=2)
+    full = _ptcl(conf, 0.8, 17, device)
+    idx = torch.from_numpy(np.random.default_rng(2).permutation(len(full))[:3001]).to(device)
+    sub = full.replace(pmid=full.pmid[idx], disp=full.disp[idx],
+                       unused_index=None if full.unused_index is None else full.unused_index[idx])
+    acc = gravity(1.0, sub, 0.3, conf).cpu().numpy()
+    ref, _ = _oracle_acc(conf, sub, 0.3)
+    assert np.linalg.norm(acc - ref) / np.linalg.norm(ref) < 1e-5
+    empty = full.replace(pmid=full.pmid[:0], disp=full.disp[:0],
+                         unused_index=None if full.unused_index is None else full.unused_index[:0])
+    assert tuple(gravity(1.0, empty, 0.3, conf).shape) == (0, 3)
+    dead = full.replace(unused_index=torch.ones(len(full), dtype=torch.bool, device=device))
+    assert torch.all(gravity(1.0, dead, 0.3, conf) == 0)
