@@ -238,7 +238,12 @@ int pm_power_spectrum_bwd(const float* spec, const int32_t shape[3], int32_t mas
  * pm_conv3d_bwd_data   dx = conv^T(dz; w) (+ add), times gelu'(gact) when gact is given
  *                      (16 -> 16 layers).
  * pm_conv3d_bwd_weight dw_db = [dW (27*16*cin floats), db (16 floats)] for pre-activation
- *                      gradient dz and layer input x; workspace from the query. */
+ *                      gradient dz and layer input x; workspace from the query.
+ * pm_conv3d_stem       act = gelu(conv(x0; w[27][16][8]) + bias) for the network input x0 that
+ *                      pm_correction_features would write from the density rho (float32
+ *                      mesh) and its stats, without writing x0: the constant input channels
+ *                      (a, Omega_m, sigma8) are folded into the bias (inference; training
+ *                      keeps x0 for the weight gradient). */
 int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, const void* res, int32_t gelu,
                   float* pre, int32_t pre_channels, void* act, const int32_t shape[3], void* stream);
 int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const float* gact, float* dx,
@@ -246,6 +251,8 @@ int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const f
 int pm_conv3d_bwd_weight_workspace(const int32_t shape[3], int32_t cin, size_t* bytes);
 int pm_conv3d_bwd_weight(const float* dz, const void* x, int32_t cin, const int32_t shape[3], float* dw_db,
                          void* workspace, size_t workspace_bytes, void* stream);
+int pm_conv3d_stem(const float* rho, const int32_t shape[3], float omega_m, float a, float sigma8,
+                   const float* stats, const void* w, const float* bias, void* act, void* stream);
 
 /* ------------------------------------------------------------ learned correction ----
  * The elementwise stages around the conv stack of the mesh CNN correction
@@ -254,7 +261,7 @@ int pm_conv3d_bwd_weight(const float* dz, const void* x, int32_t cin, const int3
  * pm_correction_features  sigma = sqrt(mean S^2 + 1e-6) and the gain
  *                         g = max_residual tanh(gain_params . [a, Omega_m, sigma8, 1]) into
  *                         stats[0..2] (device floats); x0 = bf16 [S/sigma, a, Omega_m, sigma8,
- *                         0, 0, 0, 0] per cell (n x 8 bf16, 16-byte aligned).
+ *                         0, 0, 0, 0] per cell (n x 8 bf16, 16-byte aligned; NULL: stats only).
  * pm_correction_residual  rho_add = g (rho - 1) + max_residual tanh(out) sigma / c, out the
  *                         output layer's float32 pre-activation of channel 0 (n floats).
  * pm_correction_residual_bwd  the output layer's pre-activation gradient dz_out (n x 16,

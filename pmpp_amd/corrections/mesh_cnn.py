@@ -189,13 +189,25 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     a, om = float(a), float(omega_m)
     ws = _workspace(dev)
     stats = torch.empty(4, dtype=torch.float32, device=dev)
-    x0 = torch.empty(shape + (IN_CHANNELS,), dtype=torch.bfloat16, device=dev)
     s = stream_handle(dev)
-    _lib.call("pm_correction_features", ptr(rho), n, om, a, float(model.sigma8_value), float(model.max_residual),
-              ptr(model.gain_params), ptr(x0), ptr(stats), ptr(ws), ws.numel(), s)
+    s8 = float(model.sigma8_value)
     layers = model.layers
     pres, acts = [], []
-    pre, x = conv3d(x0, model.weight(layers[0]), model.bias(layers[0]), gelu=True, want_pre=keep)
+    if keep:
+        # the backward's stem weight gradient needs the network input x0
+        x0 = torch.empty(shape + (IN_CHANNELS,), dtype=torch.bfloat16, device=dev)
+        _lib.call("pm_correction_features", ptr(rho), n, om, a, s8, float(model.max_residual),
+                  ptr(model.gain_params), ptr(x0), ptr(stats), ptr(ws), ws.numel(), s)
+        pre, x = conv3d(x0, model.weight(layers[0]), model.bias(layers[0]), gelu=True, want_pre=True)
+    else:
+        # inference: the stem reads the density directly (constant input channels folded
+        # into its bias), so x0 is never written
+        x0, pre = None, None
+        _lib.call("pm_correction_features", ptr(rho), n, om, a, s8, float(model.max_residual),
+                  ptr(model.gain_params), None, ptr(stats), ptr(ws), ws.numel(), s)
+        x = torch.empty(shape + (CHANNELS,), dtype=torch.bfloat16, device=dev)
+        _lib.call("pm_conv3d_stem", ptr(rho), _lib.int3(shape), om, a, s8, ptr(stats),
+                  ptr(model.weight(layers[0])), ptr(model.bias(layers[0])), ptr(x), s)
     pres.append(pre)
     acts.append(x)
     for i in range(model.depth - 1):

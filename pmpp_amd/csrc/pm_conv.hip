@@ -2,9 +2,9 @@
 // float32 accumulation): the learned potential correction's layers and their gradients.
 //
 // Implicit GEMM with M = output voxels, N = 16 output channels, K = 27 taps x CIN input
-// channels, on v_mfma_f32_16x16x32_bf16.  A workgroup owns an 8x8x8 block of output voxels;
-// its input block with a one-voxel periodic halo (10^3 x CIN bf16) and the weights are staged
-// in LDS, and each wave runs 16-voxel M tiles over the K steps (32 = 32/CIN taps x CIN).
+// channels, on v_mfma_f32_16x16x32_bf16.  Work is split into 8x8x8 blocks of output voxels;
+// a block's input with a one-voxel periodic halo (10^3 x CIN bf16) is staged in LDS, and each
+// wave runs 16-voxel M tiles over the K steps (32 = 32/CIN taps x CIN).
 // Lane layout of the MFMA (checked on the device by the conv tests): lane l feeds
 // A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j < 8, and receives
 // D[4 (l / 16) + i][l % 16], i < 4.
@@ -46,22 +46,6 @@ __device__ __forceinline__ float gelu_grad(float x) {
   return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * c * (1.f + 3.f * 0.044715f * x * x);
 }
 
-// Stage the (kH)^3 x CIN input block of output block (bx, by, bz) (periodic wrap) in LDS as
-// [x][y][z][CIN] bf16, 16-byte vectors.
-template <int CIN>
-__device__ __forceinline__ void stage_input(__bf16* lds, const __bf16* __restrict__ x, int n0, int n1, int n2,
-                                            int ox, int oy, int oz) {
-  constexpr int V = CIN / 8;  // 16-byte vectors per voxel
-  constexpr int ITEMS = kH * kH * kH * V;
-  for (int it = threadIdx.x; it < ITEMS; it += kThreads) {
-    const int v = it % V, vox = it / V;
-    const int z = vox % kH, y = (vox / kH) % kH, xx = vox / (kH * kH);
-    const int gx = wrap(ox + xx - 1, n0), gy = wrap(oy + y - 1, n1), gz = wrap(oz + z - 1, n2);
-    const uint4 val = *reinterpret_cast<const uint4*>(x + (((int64_t)gx * n1 + gy) * n2 + gz) * CIN + 8 * v);
-    *reinterpret_cast<uint4*>(lds + vox * CIN + 8 * v) = val;
-  }
-}
-
 // Voxel m (0..15) of M tile `tile`: tiles are 2 y-rows x 8 z of one x plane.
 __device__ __forceinline__ void tile_voxel(int tile, int m, int& vx, int& vy, int& vz) {
   vx = tile / 4;
@@ -69,55 +53,389 @@ __device__ __forceinline__ void tile_voxel(int tile, int m, int& vx, int& vy, in
   vz = m % 8;
 }
 
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+
+// GELU's tanh form (jax.nn.gelu's default approximate=True) as x * sigmoid(2u):
+// 0.5 x (1 + tanh(u)) = x / (1 + exp(-2u)), u = sqrt(2/pi) (x + 0.044715 x^3), with one
+// hardware exp2 and one reciprocal.  Saturates to x / inf -> -0 and x / 1 -> x.
+__device__ __forceinline__ float gelu_fast(float x) {
+  constexpr float k1 = -2.f * 0.7978845608028654f * 1.4426950408889634f;  // -2 sqrt(2/pi) log2(e)
+  constexpr float k3 = k1 * 0.044715f;
+  const float e = __builtin_amdgcn_exp2f(x * (k1 + k3 * (x * x)));
+  return x * __builtin_amdgcn_rcpf(1.f + e);
+}
+
+// ----------------------------------------------------------------------------------------
+// Forward convolution, walking x planes.
+//
+// The 27 taps are grouped by dx: within one dx the 9 (dy, dz) taps j = 3 dy' + dz' (dy', dz'
+// in 0..2) are spread over KS K steps of TPS taps (TPS = 32 / CIN; padded slots past j = 8 have
+// zero weights).  A wave owns the 2 y-rows x 8 z (16 voxels = one M tile) of every x plane
+// of the block.  Staged input plane p (p = 0..9) feeds output planes p (dx = -1), p - 1
+// (dx = 0) and p - 2 (dx = +1): the wave reads plane p's KS fragments from LDS once and
+// issues 3 KS MFMAs into three rotating accumulators, and reads plane p + 1 while those run.
+// That is KS LDS reads per M tile instead of one per K step, with the read latency covered
+// by the previous plane's matrix work.
+template <int CIN>
+struct PlaneTaps {
+  static constexpr int TPS = 32 / CIN;
+  static constexpr int KS = (9 + TPS - 1) / TPS;
+  // staged (y, z) offset of in-plane tap j
+  static constexpr int off2(int j) { return (j / 3) * kH + j % 3; }
+  // offset of slot t of step k relative to the step's slot 0 (padded slots: 0)
+  static constexpr int diff(int k, int t) { return k * TPS + t < 9 ? off2(k * TPS + t) - off2(k * TPS) : 0; }
+  // steps whose slots have the same relative offsets share a per-lane LDS base (a class)
+  int cls[KS] = {};
+  int rep[KS] = {};
+  int ncls = 0;
+  constexpr PlaneTaps() {
+    for (int k = 0; k < KS; ++k) {
+      int c = -1;
+      for (int q = 0; q < ncls && c < 0; ++q) {
+        bool same = true;
+        for (int t = 1; t < TPS; ++t) same = same && diff(k, t) == diff(rep[q], t);
+        if (same) c = q;
+      }
+      if (c < 0) {
+        c = ncls;
+        rep[ncls++] = k;
+      }
+      cls[k] = c;
+    }
+  }
+};
+
+template <int CIN, bool GELU, bool RES>
+struct PlaneWalk {
+  using T = PlaneTaps<CIN>;
+  static constexpr int KS = T::KS;
+  static constexpr T taps{};
+  // weights of (dx' = 0..2, step k): A[cout = m][k = 8 g + j]
+  bf16x8 wa[3 * KS];
+  float bsv[4];
+  int cbase[4];  // per-lane LDS element offset of each step class (half, lane voxel, slot)
+  bf16x8 frag[2][KS];
+  f32x4 acc[3];
+  const __bf16* xin;
+  // epilogue
+  int64_t el;     // lane's output voxel in plane 0 of the block
+  int64_t plane;  // voxels per x plane of the mesh
+  int g;
+  const __bf16* res;
+  float* pre;
+  __bf16* act;
+  int pch;
+
+  __device__ __forceinline__ void read_plane(int buf, int p) {
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+      frag[buf][k] = *reinterpret_cast<const bf16x8*>(
+          xin + cbase[taps.cls[k]] + (p * kH * kH + T::off2(k * T::TPS)) * 8);
+  }
+
+  __device__ __forceinline__ void finish(int t, const f32x4& c) {
+    // lane holds output channels 4g .. 4g+3 of its voxel in output plane t
+    const int64_t e = el + t * plane;
+    float zv[4];
+    bf16x4 r;
+    if constexpr (RES) r = *reinterpret_cast<const bf16x4*>(res + e * kCout + 4 * g);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      zv[i] = c[i] + bsv[i];
+      if constexpr (RES) zv[i] += (float)r[i];
+    }
+    if (pre) {
+      if (pch == kCout)
+        *reinterpret_cast<f32x4*>(pre + e * kCout + 4 * g) = f32x4{zv[0], zv[1], zv[2], zv[3]};
+      else if (g == 0)
+        pre[e] = zv[0];
+    }
+    if (act) {
+      bf16x4 o;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) o[i] = (__bf16)(GELU ? gelu_fast(zv[i]) : zv[i]);
+      *reinterpret_cast<bf16x4*>(act + e * kCout + 4 * g) = o;
+    }
+  }
+
+  // plane P: its fragments are in frag[P % 2]; plane P + 1 is read into the other buffer
+  template <int P>
+  __device__ __forceinline__ void walk() {
+    if constexpr (P + 1 < kH) read_plane((P + 1) % 2, P + 1);
+    if constexpr (P < kB) acc[P % 3] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      // output plane t = P - dx' takes dx' = P - t
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        constexpr int dummy = 0;
+        (void)dummy;
+        const int t = P - d;
+        if (t >= 0 && t < kB)
+          acc[(P - d + 3) % 3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[d * KS + k], frag[P % 2][k],
+                                                                       acc[(P - d + 3) % 3], 0, 0, 0);
+      }
+    }
+    if constexpr (P >= 2) finish(P - 2, acc[(P - 2) % 3]);
+    if constexpr (P + 1 < kH) walk<P + 1>();
+  }
+};
+
 // Forward: pre = conv(x) + bias (+ res), act = gelu(pre) (or pre when !GELU), written as
 // float32 pre-activations (when pre != null; the first pch of the 16 channels per voxel) and
 // bf16 activations (when act != null).
-template <int CIN, bool GELU>
-__global__ void __launch_bounds__(kThreads) k_conv_fwd(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
-                                                       const float* __restrict__ bias, const __bf16* __restrict__ res,
-                                                       float* __restrict__ pre, __bf16* __restrict__ act, int n0,
-                                                       int n1, int n2, int pch) {
-  constexpr int TPS = 32 / CIN;                       // taps per K step
-  constexpr int STEPS = (kTaps + TPS - 1) / TPS;      // K steps (the last may be padded)
-  __shared__ __align__(16) __bf16 xin[kH * kH * kH * CIN];
-  __shared__ __align__(16) __bf16 wl[STEPS * TPS * kCout * CIN];
-  const int nbz = n2 / kB, nby = n1 / kB;
-  const int blk = blockIdx.x;
-  const int bz = blk % nbz, by = (blk / nbz) % nby, bx = blk / (nbz * nby);
-  const int ox = bx * kB, oy = by * kB, oz = bz * kB;
-  stage_input<CIN>(xin, x, n0, n1, n2, ox, oy, oz);
-  for (int i = threadIdx.x; i < STEPS * TPS * kCout * CIN; i += kThreads)
-    wl[i] = i < kTaps * kCout * CIN ? w[i] : (__bf16)0.f;
-  __syncthreads();
+//
+// A persistent grid walks the 8^3 output blocks; gridDim.x is a multiple of 8 and workgroup
+// w (on XCD w % 8) takes block it * G + (w % 8) * (G / 8) + w / 8 of iteration it, so the
+// blocks of one iteration on one XCD are consecutive (z neighbours share their halo planes in
+// that XCD's L2).  The weights stay in registers as MFMA A fragments for the whole kernel
+// (D[cout][voxel] = W[cout][K] X[K][voxel]); the next block's input tile is loaded into
+// registers while the current block is computed from LDS, so the loads of one block are in
+// flight together and overlap the matrix work of the previous one.  The residual is read in
+// the epilogue (by then the prefetch has landed).  The staged tile is stored as one plane per
+// 8-channel half ([half][voxel][8]), so the 16-byte reads of an 8-lane group cover 128
+// contiguous bytes (no bank conflicts).  The global addresses of a block are the thread's
+// fixed offsets plus the block origin, except for blocks whose halo wraps around the mesh.
+template <int CIN, bool GELU, bool RES>
+__global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
+                                                          const float* __restrict__ bias,
+                                                          const __bf16* __restrict__ res, float* __restrict__ pre,
+                                                          __bf16* __restrict__ act, int n0, int n1, int n2, int pch) {
+  using Walk = PlaneWalk<CIN, GELU, RES>;
+  using Taps = PlaneTaps<CIN>;
+  constexpr int TPS = Taps::TPS, KS = Taps::KS;
+  constexpr int V = CIN / 8;        // 8-channel halves of a voxel (LDS planes)
+  constexpr int NS = kH * kH * kH;  // staged voxels
+  constexpr int ITEMS = NS * V;     // 16-byte vectors staged per block
+  constexpr int PER = (ITEMS + kThreads - 1) / kThreads;
+  constexpr Taps taps{};
+  static_assert(taps.ncls <= 4, "step classes");
+  __shared__ __align__(16) __bf16 xin[V * NS * 8];
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int m = lane % 16, g = lane / 16;
-  // K group g of a step: tap s*TPS + g / (CIN/8), channels 8*(g % (CIN/8)) .. +8
-  const int gt = g / (CIN / 8), gc = 8 * (g % (CIN / 8));
-  for (int tile = wave; tile < kVox / 16; tile += 4) {
-    int vx, vy, vz;
-    tile_voxel(tile, m, vx, vy, vz);
-    f32x4 c = {0.f, 0.f, 0.f, 0.f};
+  // K group g of a step: tap slot gt = g / V, channel half gh = g % V
+  const int gt = g / V, gh = g % V;
+  Walk wk;
 #pragma unroll
-    for (int s = 0; s < STEPS; ++s) {
-      const int tap = s * TPS + gt;
-      const int t = tap < kTaps ? tap : 0;  // padded taps read a valid voxel; their weights are 0
-      const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
-      const int lv = ((vx + 1 + dx) * kH + (vy + 1 + dy)) * kH + (vz + 1 + dz);
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(xin + lv * CIN + gc);
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(wl + (tap * kCout + m) * CIN + gc);
-      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-    }
-    // lane holds output channel m for voxels 4g .. 4g+3 of the tile
+  for (int d = 0; d < 3; ++d) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int qx, qy, qz;
-      tile_voxel(tile, 4 * g + i, qx, qy, qz);
-      const int64_t vox = ((int64_t)(ox + qx) * n1 + (oy + qy)) * n2 + (oz + qz);
-      float z = c[i] + bias[m];
-      if (res) z += (float)res[vox * kCout + m];
-      if (pre && m < pch) pre[vox * pch + m] = z;
-      if (act) act[vox * kCout + m] = (__bf16)(GELU ? gelu(z) : z);
+    for (int k = 0; k < KS; ++k) {
+      const int j = k * TPS + gt;
+      const int tap = d * 9 + (j < 9 ? j : 0);
+      bf16x8 v = *reinterpret_cast<const bf16x8*>(w + (tap * kCout + m) * CIN + 8 * gh);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) wk.wa[d * KS + k][i] = j < 9 ? v[i] : (__bf16)0.f;
     }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) wk.bsv[i] = bias[4 * g + i];
+  // weights and bias are in registers before the block loop starts (otherwise the wait for
+  // them at the loop head also waits for the prefetch in flight)
+  __builtin_amdgcn_s_waitcnt(0);
+  {
+    // this lane's output voxel: y = 2 wave + m / 8, z = m % 8 (staged corner of its taps)
+    const int corner = (2 * wave + m / 8) * kH + m % 8;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      int d = 0;
+#pragma unroll
+      for (int t = 1; t < TPS; ++t) d = gt == t ? Taps::diff(taps.rep[c < taps.ncls ? c : 0], t) : d;
+      wk.cbase[c] = (gh * NS + corner + d) * 8;
+    }
+  }
+  wk.xin = xin;
+  wk.plane = (int64_t)n1 * n2;
+  wk.g = g;
+  wk.res = res;
+  wk.pre = pre;
+  wk.act = act;
+  wk.pch = pch;
+  // this thread's staged vectors: element offset from the block's (-1, -1, -1) corner when
+  // the halo does not wrap (fits int32: the caller checks kH * n1 * n2 * CIN < 2^31)
+  typedef int Offsets __attribute__((ext_vector_type(PER)));
+  Offsets loc;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int it = min((int)threadIdx.x + u * kThreads, ITEMS - 1);
+    const int v = it % V, vox = it / V;
+    const int z = vox % kH, y = (vox / kH) % kH, xx = vox / (kH * kH);
+    loc[u] = ((xx * n1 + y) * n2 + z) * CIN + 8 * v;
+  }
+  const int nbz = n2 / kB, nby = n1 / kB;
+  const int nblk = (n0 / kB) * nby * nbz;  // < 2^31: the caller checks the mesh size
+  const int G = gridDim.x, per = G / 8, first = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  // iteration i loads block nxt into registers, computes block cur (staged in LDS by the
+  // previous iteration) while those loads are in flight, then stages nxt
+  typedef uint32_t Prefetch __attribute__((ext_vector_type(4 * PER)));  // a value, never a private array
+  int cur = -1, nxt = first;
+  for (;;) {
+    Prefetch pf;
+    if (nxt < nblk) {
+      const int bz = nxt % nbz, by = (nxt / nbz) % nby, bx = nxt / (nbz * nby);
+      const int ox = bx * kB, oy = by * kB, oz = bz * kB;
+      const bool inner = ox > 0 && ox + kB < n0 && oy > 0 && oy + kB < n1 && oz > 0 && oz + kB < n2;
+      if (inner) {
+        const __bf16* base = x + (((int64_t)(ox - 1) * n1 + (oy - 1)) * n2 + (oz - 1)) * CIN;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {  // unconditional (clamped) loads, all in flight together
+          const uint4 t = *reinterpret_cast<const uint4*>(base + loc[u]);
+          pf[4 * u] = t.x;
+          pf[4 * u + 1] = t.y;
+          pf[4 * u + 2] = t.z;
+          pf[4 * u + 3] = t.w;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+          const int it = min((int)threadIdx.x + u * kThreads, ITEMS - 1);
+          const int v = it % V, vox = it / V;
+          const int z = vox % kH, y = (vox / kH) % kH, xx = vox / (kH * kH);
+          const int gx = wrap(ox + xx - 1, n0), gy = wrap(oy + y - 1, n1), gz = wrap(oz + z - 1, n2);
+          const uint4 t = *reinterpret_cast<const uint4*>(x + (((int64_t)gx * n1 + gy) * n2 + gz) * CIN + 8 * v);
+          pf[4 * u] = t.x;
+          pf[4 * u + 1] = t.y;
+          pf[4 * u + 2] = t.z;
+          pf[4 * u + 3] = t.w;
+        }
+      }
+    }
+    if (cur >= 0) {
+      const int bz = cur % nbz, by = (cur / nbz) % nby, bx = cur / (nbz * nby);
+      const int64_t e0 = (((int64_t)bx * kB * n1 + by * kB) * n2 + bz * kB);  // block origin voxel
+      wk.el = e0 + (int64_t)(2 * wave + m / 8) * n2 + m % 8;
+      wk.read_plane(0, 0);
+      wk.template walk<0>();
+    }
+    if (nxt >= nblk) break;
+    __syncthreads();  // the MFMAs of block cur are done with the tile
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = threadIdx.x + u * kThreads;
+      if (i < ITEMS)
+        *reinterpret_cast<uint4*>(xin + ((i % V) * NS + i / V) * 8) =
+            make_uint4(pf[4 * u], pf[4 * u + 1], pf[4 * u + 2], pf[4 * u + 3]);
+    }
+    __syncthreads();
+    cur = nxt;
+    nxt += G;
+  }
+}
+
+// Inference stem straight from the density mesh: act = gelu(conv(x0; w) + bias) where x0 is
+// the 8-channel input [bf16((rho - 1) c / sigma), bf16(a), bf16(Omega_m), bf16(sigma8), 0 ...]
+// that pm_correction_features writes.  Channels 1..3 are the same at every voxel, so their
+// periodic convolution is sum_tap w[tap][cout][1..3] * value, a constant per output channel
+// that is folded into the bias; channels 4..7 are zero.  What remains is a one-channel
+// convolution whose 27 taps fit one K = 32 MFMA step (5 padded taps with zero weights): lane
+// (g, m) feeds taps 8g .. 8g+7 of voxel m, read as eight 2-byte LDS values.  The staged tile is
+// the bf16 source of the block's 10^3 halo, computed from float32 density on load with the
+// same arithmetic as the feature kernel.  Same persistent, prefetching block walk as
+// k_conv_fwd.
+__global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restrict__ rho, float c,
+                                                           const float* __restrict__ stats, float a, float om,
+                                                           float s8, const __bf16* __restrict__ w,
+                                                           const float* __restrict__ bias, __bf16* __restrict__ act,
+                                                           int n0, int n1, int n2) {
+  constexpr int NS = kH * kH * kH;
+  constexpr int PER = (NS + kThreads - 1) / kThreads;
+  constexpr int CIN = 8;  // channels of the stem's weight tensor
+  __shared__ __align__(16) __bf16 xin[PER * kThreads];
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int m = lane % 16, g = lane / 16;
+  // A fragment: channel-0 weights of taps 8g .. 8g+7 for output channel m
+  bf16x8 wa;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int tap = 8 * g + j;
+    wa[j] = tap < kTaps ? w[(tap * kCout + m) * CIN] : (__bf16)0.f;
+  }
+  // bias of output channels 4g .. 4g+3 with the constant channels folded in (fixed order)
+  const __bf16 ba = (__bf16)a, bo = (__bf16)om, bs = (__bf16)s8;
+  float bsv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int co = 4 * g + i;
+    float t = 0.f;
+    for (int tap = 0; tap < kTaps; ++tap) {
+      const __bf16* wt = w + (tap * kCout + co) * CIN;
+      t += (float)wt[1] * (float)ba + (float)wt[2] * (float)bo + (float)wt[3] * (float)bs;
+    }
+    bsv[i] = bias[co] + t;
+  }
+  const float inv = 1.f / stats[0];
+  __builtin_amdgcn_s_waitcnt(0);
+  // staged offsets of this lane's 8 taps (padded taps read tap 26's voxel; zero weights)
+  typedef int TapOffsets __attribute__((ext_vector_type(8)));
+  TapOffsets toff;
+  {
+    const int corner = (2 * wave + m / 8) * kH + m % 8;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int t = min(8 * g + j, kTaps - 1);
+      toff[j] = corner + ((t / 9) * kH + (t / 3) % 3) * kH + t % 3;
+    }
+  }
+  typedef int Offsets __attribute__((ext_vector_type(PER)));
+  Offsets loc;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int vox = min((int)threadIdx.x + u * kThreads, NS - 1);
+    const int z = vox % kH, y = (vox / kH) % kH, xx = vox / (kH * kH);
+    loc[u] = (xx * n1 + y) * n2 + z;
+  }
+  const int nbz = n2 / kB, nby = n1 / kB;
+  const int nblk = (n0 / kB) * nby * nbz;
+  const int G = gridDim.x, per = G / 8, first = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int64_t plane = (int64_t)n1 * n2;
+  typedef float Prefetch __attribute__((ext_vector_type(PER)));
+  int cur = -1, nxt = first;
+  for (;;) {
+    Prefetch pf;
+    if (nxt < nblk) {
+      const int bz = nxt % nbz, by = (nxt / nbz) % nby, bx = nxt / (nbz * nby);
+      const int ox = bx * kB, oy = by * kB, oz = bz * kB;
+      const bool inner = ox > 0 && ox + kB < n0 && oy > 0 && oy + kB < n1 && oz > 0 && oz + kB < n2;
+      if (inner) {
+        const float* base = rho + ((int64_t)(ox - 1) * n1 + (oy - 1)) * n2 + (oz - 1);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) pf[u] = base[loc[u]];
+      } else {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+          const int vox = min((int)threadIdx.x + u * kThreads, NS - 1);
+          const int z = vox % kH, y = (vox / kH) % kH, xx = vox / (kH * kH);
+          const int gx = wrap(ox + xx - 1, n0), gy = wrap(oy + y - 1, n1), gz = wrap(oz + z - 1, n2);
+          pf[u] = rho[((int64_t)gx * n1 + gy) * n2 + gz];
+        }
+      }
+    }
+    if (cur >= 0) {
+      const int bz = cur % nbz, by = (cur / nbz) % nby, bx = cur / (nbz * nby);
+      const int64_t e0 = (((int64_t)bx * kB * n1 + by * kB) * n2 + bz * kB);
+      const int64_t el = e0 + (int64_t)(2 * wave + m / 8) * n2 + m % 8;
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {  // output x plane q: staged planes q .. q+2
+        bf16x8 b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = xin[toff[j] + q * kH * kH];
+        const f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        const int64_t e = el + q * plane;
+        bf16x4 o;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i] + bsv[i]);
+        *reinterpret_cast<bf16x4*>(act + e * kCout + 4 * g) = o;
+      }
+    }
+    if (nxt >= nblk) break;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = threadIdx.x + u * kThreads;
+      if (i < NS) xin[i] = (__bf16)(((pf[u] - 1.f) * c) * inv);
+    }
+    __syncthreads();
+    cur = nxt;
+    nxt += G;
   }
 }
 
@@ -298,6 +616,20 @@ unsigned nblocks(const int32_t* shape) {
   return (unsigned)((int64_t)(shape[0] / conv::kB) * (shape[1] / conv::kB) * (shape[2] / conv::kB));
 }
 
+// Workgroups of a persistent conv grid: what fits on the device at once (the kernel's
+// occupancy times the CU count), at most the number of blocks, rounded up to a multiple of 8
+// (one share per XCD).
+int persistent_grid(const void* kern, int64_t nblk, unsigned* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  PM_HIP_TRY(hipGetDevice(&dev));
+  PM_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  PM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, conv::kThreads, 0));
+  int64_t g = (int64_t)std::max(cus, 1) * std::max(per_cu, 1);
+  g = std::min<int64_t>(g, nblk);
+  *grid = (unsigned)(ceil_div(g, 8) * 8);
+  return PM_SUCCESS;
+}
+
 constexpr int kWGrid = 1024;  // persistent workgroups of the weight-gradient pass
 
 }  // namespace
@@ -317,20 +649,52 @@ int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, 
   const __bf16* wb = static_cast<const __bf16*>(w);
   const __bf16* rb = static_cast<const __bf16*>(res);
   __bf16* ab = static_cast<__bf16*>(act);
-  const dim3 grid(nblocks(shape)), block(conv::kThreads);
-  if (cin == 8 && gelu)
-    hipLaunchKernelGGL((conv::k_conv_fwd<8, true>), grid, block, 0, s, xb, wb, bias, rb, pre, ab, shape[0], shape[1],
-                       shape[2], (int)pre_channels);
-  else if (cin == 8)
-    hipLaunchKernelGGL((conv::k_conv_fwd<8, false>), grid, block, 0, s, xb, wb, bias, rb, pre, ab, shape[0], shape[1],
-                       shape[2], (int)pre_channels);
-  else if (gelu)
-    hipLaunchKernelGGL((conv::k_conv_fwd<16, true>), grid, block, 0, s, xb, wb, bias, rb, pre, ab, shape[0],
-                       shape[1], shape[2], (int)pre_channels);
-  else
-    hipLaunchKernelGGL((conv::k_conv_fwd<16, false>), grid, block, 0, s, xb, wb, bias, rb, pre, ab, shape[0],
-                       shape[1], shape[2], (int)pre_channels);
+  if (res) PM_TRY(check_ptr16(res, "res"));
+  PM_RETURN_IF(pre && reinterpret_cast<uintptr_t>(pre) % 16 != 0, PM_ERROR_INVALID_ARGUMENT,
+               "pre must be 16-byte aligned");
+  PM_RETURN_IF(act && reinterpret_cast<uintptr_t>(act) % 16 != 0, PM_ERROR_INVALID_ARGUMENT,
+               "act must be 16-byte aligned");
+  PM_RETURN_IF((int64_t)conv::kH * shape[1] * shape[2] * cin >= (int64_t(1) << 31) ||
+                   (int64_t)shape[0] * shape[1] * shape[2] / (conv::kB * conv::kB * conv::kB) >= (int64_t(1) << 31),
+               PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
+  const int64_t nblk = nblocks(shape);
+  const int key = (cin == 16 ? 4 : 0) + (gelu ? 2 : 0) + (res ? 1 : 0);
+  void (*kern)(const __bf16*, const __bf16*, const float*, const __bf16*, float*, __bf16*, int, int, int, int) =
+      nullptr;
+  switch (key) {
+    case 0: kern = conv::k_conv_fwd<8, false, false>; break;
+    case 1: kern = conv::k_conv_fwd<8, false, true>; break;
+    case 2: kern = conv::k_conv_fwd<8, true, false>; break;
+    case 3: kern = conv::k_conv_fwd<8, true, true>; break;
+    case 4: kern = conv::k_conv_fwd<16, false, false>; break;
+    case 5: kern = conv::k_conv_fwd<16, false, true>; break;
+    case 6: kern = conv::k_conv_fwd<16, true, false>; break;
+    default: kern = conv::k_conv_fwd<16, true, true>; break;
+  }
+  unsigned grid = 0;
+  PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblk, &grid));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, xb, wb, bias, rb, pre, ab, shape[0], shape[1],
+                     shape[2], (int)pre_channels);
   PM_LAUNCHED("k_conv_fwd", s);
+  return PM_SUCCESS;
+}
+
+int pm_conv3d_stem(const float* rho, const int32_t shape[3], float omega_m, float a, float sigma8,
+                   const float* stats, const void* w, const float* bias, void* act, void* stream) {
+  PM_TRY(check_shape(shape));
+  PM_RETURN_IF(rho == nullptr || stats == nullptr || w == nullptr || bias == nullptr, PM_ERROR_INVALID_ARGUMENT,
+               "NULL argument");
+  PM_TRY(check_ptr16(act, "act"));
+  PM_RETURN_IF((int64_t)shape[0] * shape[1] * shape[2] / (conv::kB * conv::kB * conv::kB) >= (int64_t(1) << 31) ||
+                   (int64_t)conv::kH * shape[1] * shape[2] >= (int64_t(1) << 31),
+               PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
+  hipStream_t s = as_stream(stream);
+  unsigned grid = 0;
+  PM_TRY(persistent_grid(reinterpret_cast<const void*>(conv::k_conv_stem), nblocks(shape), &grid));
+  hipLaunchKernelGGL(conv::k_conv_stem, dim3(grid), dim3(conv::kThreads), 0, s, rho, 1.5f * omega_m, stats, a,
+                     omega_m, sigma8, static_cast<const __bf16*>(w), bias, static_cast<__bf16*>(act), shape[0],
+                     shape[1], shape[2]);
+  PM_LAUNCHED("k_conv_stem", s);
   return PM_SUCCESS;
 }
 

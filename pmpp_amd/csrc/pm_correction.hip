@@ -332,8 +332,8 @@ int pm_correction_features(const float* rho, int64_t n, float omega_m, float a, 
                            const float* gain_params, void* x0, float* stats, void* workspace,
                            size_t workspace_bytes, void* stream) {
   PM_TRY(check_mesh(n));
-  PM_RETURN_IF(rho == nullptr || gain_params == nullptr || x0 == nullptr || stats == nullptr,
-               PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF(rho == nullptr || gain_params == nullptr || stats == nullptr, PM_ERROR_INVALID_ARGUMENT,
+               "NULL argument");
   PM_RETURN_IF(reinterpret_cast<uintptr_t>(x0) % 16 != 0, PM_ERROR_INVALID_ARGUMENT, "x0 must be 16-byte aligned");
   PM_RETURN_IF(workspace == nullptr || workspace_bytes < kPartialBytes, PM_ERROR_WORKSPACE,
                "correction workspace too small");
@@ -346,6 +346,7 @@ int pm_correction_features(const float* rho, int64_t n, float omega_m, float a, 
   hipLaunchKernelGGL(corr::k_source_scale, dim3(1), dim3(64), 0, s, part, (int)nb, n, max_residual, gain_params, a,
                      omega_m, sigma8, stats);
   PM_LAUNCHED("k_source_scale", s);
+  if (x0 == nullptr) return PM_SUCCESS;  // stats only (the fused inference stem reads rho)
   hipLaunchKernelGGL(corr::k_features, dim3(corr::grid_for(n, 8)), dim3(corr::kBlock), 0, s, rho, n, c, stats, a,
                      omega_m, sigma8, static_cast<uint4*>(x0));
   PM_LAUNCHED("k_features", s);

@@ -48,6 +48,25 @@ def test_forward_matches_torch(device, cin, shape, gelu):
     np.testing.assert_array_equal(pre1[..., 0].cpu().numpy(), pre[..., 0].cpu().numpy())
 
 
+@pytest.mark.parametrize("cin,res", [(16, True), (8, False)])
+def test_forward_persistent_grid_walks_every_block(device, cin, res):
+    """A mesh of 1024 blocks of 8^3: more blocks than workgroups resident on the device, so
+    each workgroup of the persistent grid takes several blocks (and the grid a ragged tail)."""
+    shape = (128, 64, 64)
+    g = torch.Generator().manual_seed(11 + cin)
+    x = _bf(torch.randn(shape + (cin,), generator=g))
+    w = _bf(torch.randn(27, 16, cin, generator=g) * 0.2)
+    b = torch.randn(16, generator=g)
+    r = _bf(torch.randn(shape + (16,), generator=g)) if res else None
+    pre, act = conv3d(x.to(device, torch.bfloat16), w.to(device, torch.bfloat16), b.to(device),
+                      res=r.to(device, torch.bfloat16) if res else None, gelu=True)
+    ref = _ref_conv(x, w, b) + (r if res else 0)
+    scale = ref.abs().max().item()
+    np.testing.assert_allclose(pre.cpu().numpy(), ref.numpy(), atol=1e-4 * scale)
+    np.testing.assert_allclose(act.float().cpu().numpy(), _bf(Fn.gelu(ref, approximate="tanh")).numpy(),
+                               atol=1e-2 * scale)
+
+
 def test_backward_data_and_weight_match_autograd(device):
     shape = (16, 16, 8)
     g = torch.Generator().manual_seed(3)

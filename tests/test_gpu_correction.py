@@ -161,6 +161,36 @@ def test_network_and_backward_match_float32_torch(device, depth, shape):
     assert torch.all(grads[~live] == 0)
 
 
+@pytest.mark.parametrize("shape", [(16, 8, 24), (32, 32, 48), (64, 64, 64)])
+def test_inference_stem_from_density_matches_the_feature_path(device, shape):
+    """Inference reads the density straight into the stem, with the constant input channels
+    folded into its bias; training writes the 8-channel input x0 and runs the generic conv.
+    Both compute the same float32 pre-activations up to summation order, so the stem's bf16
+    activations agree within one bf16 ulp (at most 2^-7 relative, plus an absolute floor for
+    values near zero), on a few per cent of the values at most, and the residual within the
+    network's bf16 bar.  The meshes hold wrapped
+    and interior blocks and, at 64^3, more blocks than the persistent grid's workgroups."""
+    model = _randomised(init_mesh_cnn_potential_correction(seed=7, depth=2, device=device), 7)
+    rng = np.random.default_rng(8)
+    rho = torch.from_numpy((1 + 0.5 * rng.normal(size=shape)).astype(np.float32)).to(device)
+    a, om = 0.45, 0.29
+    fused, _ = evaluate_residual(model, rho, a, om)
+    ref, tape = evaluate_residual(model, rho, a, om, keep=True)
+    # stem activations of both paths (the tape keeps the feature path's)
+    from pmpp_amd import _lib
+    from pmpp_amd._torch import ptr, stream_handle
+    act = torch.empty(shape + (16,), dtype=torch.bfloat16, device=device)
+    _lib.call("pm_conv3d_stem", ptr(rho), _lib.int3(shape), om, a, float(model.sigma8_value), ptr(tape.stats),
+              ptr(model.weight(model.layers[0])), ptr(model.bias(model.layers[0])), ptr(act),
+              stream_handle(device))
+    got, exp = act.float().cpu(), tape.acts[0].float().cpu()
+    tol = 2.0 ** -7 * exp.abs() + 1e-3 * exp.abs().max()
+    assert torch.all((got - exp).abs() <= tol), float(((got - exp).abs() - tol).max())
+    assert float((got != exp).float().mean()) < 0.02
+    err = torch.linalg.norm((fused - ref).cpu()) / torch.linalg.norm(ref.cpu())
+    assert err < 2e-3, float(err)
+
+
 def test_zero_initialised_correction_is_exactly_the_plain_force(device):
     conf = Configuration(2.0, (16, 16, 16), mesh_shape=1)
     ptcl = _ptcl(conf, 0.7, 5, device)
