@@ -179,11 +179,13 @@ def cpu_baseline(dev, omega_m):
 def nbody_run(args):
     """The full single-GPU run of the 512^3 / 1024^3 / 1000 Mpc/h workload: 2LPT at a = 1/64,
     20 KDK steps to a = 1 (plain), the reverse sweep of that run, the same run with the learned
-    correction (BF16 network, forward), the memory high-water mark, and one plain force step on
-    a 2048^3 mesh for comparison.  Prints one JSON line (not the headline metric)."""
+    correction (BF16 network, forward: the three-layer network stem -> hidden -> output, and the
+    four-layer one with a residual block), the memory high-water mark, and the same plain run on
+    a 2048^3 mesh (4x the particle grid) for comparison.  Prints one JSON line (not the
+    headline metric)."""
     import torch
 
-    from pmpp_amd import Configuration, Particles, gravity
+    from pmpp_amd import Configuration
     from pmpp_amd.corrections import init_mesh_cnn_potential_correction
     from pmpp_amd.initial_conditions import LCDM, linear_modes, lpt, white_noise
     from pmpp_amd.nbody.solver import nbody, nbody_adj
@@ -212,25 +214,30 @@ def nbody_run(args):
                       acc=torch.zeros_like(out.acc))
     _, t_adj = timed(lambda: nbody_adj(out, cot, cosmo, conf))
     log(f"[run] reverse sweep in {t_adj:.3f} s")
-    del cot
-    model = init_mesh_cnn_potential_correction(seed=0, depth=2, device=dev)
-    corr, t_corr = timed(lambda: nbody(ptcl, cosmo, conf, correction=model, observer=lambda a, p: None))
-    log(f"[run] corrected: {nsteps} steps in {t_corr:.3f} s")
+    del cot, out
+    corrected = {}
+    for name, depth, block in (("three_layer", 2, "plain"), ("residual_block", 2, "residual")):
+        model = init_mesh_cnn_potential_correction(seed=0, depth=depth, block=block, device=dev)
+        nbody(ptcl, cosmo, conf, correction=model, observer=lambda a, p: None)  # warm (allocator)
+        _, t_corr = timed(lambda: nbody(ptcl, cosmo, conf, correction=model, observer=lambda a, p: None))
+        corrected[name] = {"conv_layers": len(model.layers), "steps_per_s": nsteps / t_corr, "s": t_corr}
+        log(f"[run] corrected ({name}, {len(model.layers)} conv layers): {nsteps} steps in {t_corr:.3f} s")
+        del model
     peak = torch.cuda.max_memory_allocated() / 2 ** 30
-    fin = out.pos(dtype=torch.float64)
-    del out, corr, ptcl, model
     torch.cuda.empty_cache()
-    conf2 = conf.replace(mesh_shape=4)
-    p2 = Particles.from_pos(conf2, fin)
-    del fin
-    gravity(1.0, p2, cosmo.Omega_m, conf2)  # warm (workspace allocation)
-    reps = 3
-    _, t2 = timed(lambda: [gravity(1.0, p2, cosmo.Omega_m, conf2) for _ in range(reps)])
+    conf4 = conf.replace(mesh_shape=4)
+    nbody(ptcl, cosmo, conf4, observer=lambda a, p: None)  # warm (allocator), as for the corrected runs
+    out4, t4 = timed(lambda: nbody(ptcl, cosmo, conf4, observer=lambda a, p: None))
+    log(f"[run] plain on the 2048^3 mesh: {nsteps} steps in {t4:.3f} s")
+    del out4
     res = {
         "metric": "N-body run steps/s (512^3 particles, 1024^3 mesh, 1000 Mpc/h, 20 KDK steps)",
         "steps_per_s_plain": nsteps / t_plain, "steps_per_s_reverse": nsteps / t_adj,
-        "steps_per_s_corrected": nsteps / t_corr, "memory_high_water_GiB": round(peak, 2),
-        "force_step_ms_2048_mesh": t2 / reps * 1e3, "n_gpus": 1, "dtype": "f32 (network bf16)",
+        "steps_per_s_corrected": corrected["three_layer"]["steps_per_s"], "corrected": corrected,
+        "memory_high_water_GiB": round(peak, 2),
+        "steps_per_s_plain_2048_mesh": nsteps / t4,
+        "corrected_vs_plain_2048_wall": corrected["three_layer"]["s"] / t4,
+        "n_gpus": 1, "dtype": "f32 (network bf16)",
         "data": "synthetic (2LPT LCDM field from seeded white noise)",
         "config": {"workload": "N-body run: 512^3 particles, 1024^3 mesh, 1000 Mpc/h, a = 1/64 -> 1",
                    "steps": nsteps, "comparison_mesh": 2048},

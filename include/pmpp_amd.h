@@ -177,13 +177,22 @@ int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid,
  * an extra irfftn + rfftn per force.  rho_add = NULL is pm_gravity.  The VJP also writes
  * rho_cot = dL/d rho_add (the real mesh, overwritten) when rho_cot is not NULL; workspaces
  * are those of pm_gravity_workspace / pm_gravity_vjp_workspace.  pm_gravity_density writes
- * the step's own CIC density (the mesh the correction network reads) and stops there. */
+ * the step's own CIC density (the mesh the correction network reads) and stops there;
+ * pm_gravity_corrected_resume then finishes that step with rho_add (the spectral solve and
+ * the gather, without bucketing and scattering the particles again): its workspace must
+ * still hold what the preceding pm_gravity_density call on the same particles, desc and
+ * stream left in it (the network may run in between on other buffers).  Together they are
+ * the reference's corrected force (nbody/gravity.py:626-640: density once, then the
+ * corrected potential and the force). */
 int pm_gravity_density(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                        const pm_gravity_desc* desc, float* rho, void* workspace, size_t workspace_bytes,
                        void* stream);
 int pm_gravity_corrected(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                          const pm_gravity_desc* desc, const float* rho_add, float* acc, void* workspace,
                          size_t workspace_bytes, void* stream);
+int pm_gravity_corrected_resume(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                                const pm_gravity_desc* desc, const float* rho_add, float* acc, void* workspace,
+                                size_t workspace_bytes, void* stream);
 int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                              const pm_gravity_desc* desc, const float* rho_add, const float* acc_cot,
                              float* disp_cot, float* omega_cot, float* rho_cot, void* workspace,
@@ -191,8 +200,9 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
 
 /* ------------------------------------------------------------ N-body updates ----
  * Elementwise stages of the symplectic integrator on (n,3) float32 arrays (16-byte aligned):
- * pm_kick   vel  += factor * acc    replaces kick              (nbody/integrator.py:512-538)
- * pm_drift  disp += factor * vel    replaces drift             (nbody/integrator.py:203-237)
+ * pm_kick   vel_out  = vel  + factor * acc   replaces kick      (nbody/integrator.py:512-538)
+ * pm_drift  disp_out = disp + factor * vel   replaces drift     (nbody/integrator.py:203-237)
+ *           (the output may be the input: an in-place update)
  * Their reverse passes reconstruct the input state from the output and pull cotangents:
  * pm_kick_adj   vel -= factor*acc;  acc_cot += factor*vel_cot;  *dot = sum(vel_cot*acc)
  *               replaces kick_adj                               (nbody/integrator.py:541-582)
@@ -201,8 +211,8 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
  * *dot (a device double) is the factor's cotangent, which the caller chains into the
  * cosmology cotangent; it is reduced deterministically in a workspace of
  * pm_nbody_adj_workspace bytes.  acc_cot may be NULL (not accumulated). */
-int pm_kick(float* vel, const float* acc, int64_t n, float factor, void* stream);
-int pm_drift(float* disp, const float* vel, int64_t n, float factor, void* stream);
+int pm_kick(const float* vel, const float* acc, float* vel_out, int64_t n, float factor, void* stream);
+int pm_drift(const float* disp, const float* vel, float* disp_out, int64_t n, float factor, void* stream);
 int pm_nbody_adj_workspace(size_t* bytes);
 int pm_kick_adj(float* vel, const float* acc, const float* vel_cot, float* acc_cot, int64_t n, float factor,
                 double* dot, void* workspace, size_t workspace_bytes, void* stream);
@@ -243,7 +253,10 @@ int pm_power_spectrum_bwd(const float* spec, const int32_t shape[3], int32_t mas
  *                      pm_correction_features would write from the density rho (float32
  *                      mesh) and its stats, without writing x0: the constant input channels
  *                      (a, Omega_m, sigma8) are folded into the bias (inference; training
- *                      keeps x0 for the weight gradient). */
+ *                      keeps x0 for the weight gradient).
+ * pm_conv3d_out_residual  the output layer fused with pm_correction_residual (inference):
+ *                      rho_add = g (rho - 1) + max_residual tanh(conv(x; w[27][16][16]) + bias,
+ *                      channel 0) sigma / c, without writing the output pre-activation. */
 int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, const void* res, int32_t gelu,
                   float* pre, int32_t pre_channels, void* act, const int32_t shape[3], void* stream);
 int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const float* gact, float* dx,
@@ -253,6 +266,9 @@ int pm_conv3d_bwd_weight(const float* dz, const void* x, int32_t cin, const int3
                          void* workspace, size_t workspace_bytes, void* stream);
 int pm_conv3d_stem(const float* rho, const int32_t shape[3], float omega_m, float a, float sigma8,
                    const float* stats, const void* w, const float* bias, void* act, void* stream);
+int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, const float* rho,
+                           const int32_t shape[3], float omega_m, float max_residual, const float* stats,
+                           float* rho_add, void* stream);
 
 /* ------------------------------------------------------------ learned correction ----
  * The elementwise stages around the conv stack of the mesh CNN correction

@@ -3,8 +3,10 @@
 The network is the reference's periodic residual CNN (``mesh_cnn.py:54-105``): a 3^3 stem
 convolution with GELU, ``depth - 1`` residual blocks (conv, GELU, conv, + input, GELU), a
 zero-initialised 3^3 output convolution bounded by ``max_residual * tanh``, and a
-conditioning-only "direct gain" ``max_residual * tanh(Linear([a, Omega_m, sigma8]))``.  The
-MI355X build differs in three deliberate ways:
+conditioning-only "direct gain" ``max_residual * tanh(Linear([a, Omega_m, sigma8]))``.  With
+``block="plain"`` each block is one GELU convolution instead (no residual), so ``depth=2``
+is the three-layer network stem -> hidden -> output.  The MI355X build differs in three
+deliberate ways:
 
 * hidden width is 16 channels (one MFMA tile of output channels) and activations are bf16
   with float32 accumulation (``pm_conv3d_*``);
@@ -59,10 +61,15 @@ class Layer:
         return self.nweights + CHANNELS
 
 
-def param_layout(depth: int):
+BLOCKS = ("residual", "plain")
+
+
+def param_layout(depth: int, block: str = "residual"):
     """(conv layers, gain offset, total size) of the flat parameter vector."""
     if depth < 1:
         raise ValueError("depth must be >= 1")
+    if block not in BLOCKS:
+        raise ValueError(f"block must be one of {BLOCKS}")
     layers: List[Layer] = []
     off = 0
 
@@ -74,7 +81,8 @@ def param_layout(depth: int):
     add("stem", IN_CHANNELS)
     for b in range(depth - 1):
         add(f"block_{b}_conv1", CHANNELS)
-        add(f"block_{b}_conv2", CHANNELS)
+        if block == "residual":
+            add(f"block_{b}_conv2", CHANNELS)
     add("out", CHANNELS)
     gain = off
     return layers, gain, _align(gain + 4)
@@ -90,10 +98,11 @@ class MeshCNNPotentialCorrection:
     output_init_scale: float = 1e-2
     sigma8_value: float = 0.8
     params_bf16: Optional[torch.Tensor] = None
+    block: str = "residual"
 
     def __post_init__(self):
         require_device(self.params, "params")
-        layers, gain, total = param_layout(self.depth)
+        layers, gain, total = param_layout(self.depth, self.block)
         if self.params.dtype != torch.float32 or self.params.numel() != total or not self.params.is_contiguous():
             raise ValueError(f"params must be a contiguous float32 vector of {total} elements")
         self.layers = layers
@@ -129,9 +138,9 @@ def _truncated_lecun(gen, fan_in, n):
     return t
 
 
-def initial_params(seed=0, depth=2, output_init_scale=1e-2) -> torch.Tensor:
+def initial_params(seed=0, depth=2, output_init_scale=1e-2, block="residual") -> torch.Tensor:
     """The flat float32 initial parameters on the host (see ``init_mesh_cnn_potential_correction``)."""
-    layers, gain, total = param_layout(depth)
+    layers, gain, total = param_layout(depth, block)
     gen = torch.Generator().manual_seed(int(seed))
     host = torch.zeros(total, dtype=torch.float32)
     for layer in layers:
@@ -146,13 +155,13 @@ def initial_params(seed=0, depth=2, output_init_scale=1e-2) -> torch.Tensor:
 
 
 def init_mesh_cnn_potential_correction(seed=0, depth=2, max_residual=0.1, output_init_scale=1e-2, sigma8_value=0.8,
-                                       device="cuda"):
+                                       device="cuda", block="residual"):
     """A freshly initialised correction (``mesh_cnn.py:163-214``): LeCun truncated-normal
     convolutions with zero biases, a zero output layer and a small random gain layer, so the
     initial correction is exactly zero."""
-    host = initial_params(seed, depth, output_init_scale)
+    host = initial_params(seed, depth, output_init_scale, block)
     return MeshCNNPotentialCorrection(params=host.to(device), depth=depth, max_residual=max_residual,
-                                      output_init_scale=output_init_scale, sigma8_value=sigma8_value)
+                                      output_init_scale=output_init_scale, sigma8_value=sigma8_value, block=block)
 
 
 @dataclasses.dataclass
@@ -211,18 +220,28 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     pres.append(pre)
     acts.append(x)
     for i in range(model.depth - 1):
+        if model.block == "plain":
+            l1 = layers[1 + i]
+            p1, x = conv3d(x, model.weight(l1), model.bias(l1), gelu=True, want_pre=keep)
+            pres.append(p1)
+            acts.append(x)
+            continue
         l1, l2 = layers[1 + 2 * i], layers[2 + 2 * i]
         p1, h = conv3d(x, model.weight(l1), model.bias(l1), gelu=True, want_pre=keep)
         p2, x = conv3d(h, model.weight(l2), model.bias(l2), res=x, gelu=True, want_pre=keep)
         pres += [p1, p2]
         acts += [h, x]
+    rho_add = torch.empty_like(rho)
+    if not keep:
+        # inference: the output layer writes rho_add directly (its pre-activation is not kept)
+        _lib.call("pm_conv3d_out_residual", ptr(x), ptr(model.weight(layers[-1])), ptr(model.bias(layers[-1])),
+                  ptr(rho), _lib.int3(shape), om, float(model.max_residual), ptr(stats), ptr(rho_add), s)
+        return rho_add, None
     out, _ = conv3d(x, model.weight(layers[-1]), model.bias(layers[-1]), gelu=False, want_pre=True, want_act=False,
                     pre_channels=1)
-    rho_add = torch.empty_like(rho)
     _lib.call("pm_correction_residual", ptr(out), ptr(rho), n, om, float(model.max_residual), ptr(stats),
               ptr(rho_add), s)
-    tape = Tape(rho, stats, x0, pres, acts, out, a, om) if keep else None
-    return rho_add, tape
+    return rho_add, Tape(rho, stats, x0, pres, acts, out, a, om)
 
 
 def _stem_weight_16(model):
@@ -262,6 +281,12 @@ def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, gr
     wgrad(layers[-1], dz, tape.acts[-1])
     dz = conv3d_bwd_data(dz, model.weight(layers[-1]), gact=tape.pres[-1])
     for i in reversed(range(model.depth - 1)):
+        if model.block == "plain":
+            # dz is the gradient of block i's pre-activation; its input is acts[i] = gelu(pres[i])
+            l1 = layers[1 + i]
+            wgrad(l1, dz, tape.acts[i])
+            dz = conv3d_bwd_data(dz, model.weight(l1), gact=tape.pres[i])
+            continue
         l1, l2 = layers[1 + 2 * i], layers[2 + 2 * i]
         x_in, pre_in = tape.acts[2 * i], tape.pres[2 * i]
         h, pre1 = tape.acts[2 * i + 1], tape.pres[2 * i + 1]

@@ -6,11 +6,14 @@ correction by P(k) and cross-correlation against a held-out fine run.
 
 Defaults: 128^3 particles in a 256 Mpc/h box, a coarse mesh of 256^3 (``--coarse 2``) and fine
 runs on a mesh 4x finer per axis (1024^3), 2LPT at a = 1/64 then ``--steps`` KDK steps to a = 1,
-training pairs from every other step of ``--seeds`` fine runs, and seed 1000 held out.  Prints
-one JSON line: the relative P(k) error against the fine run over the band 0.3 k_Nyq < k <
-k_Nyq (k_Nyq the particle Nyquist) and the mean cross-correlation there, for the plain and the
-corrected coarse runs, the training loss before and after, and the BF16 convolution throughput
-(TFLOP/s of ``k_conv_fwd`` from HIP events, against the 2.5 PFLOP/s dense BF16 peak).
+training pairs from every other step of ``--seeds`` fine runs, and seed 1000 held out; the
+network is the three-layer one (``--depth 2 --block plain``: stem, one hidden GELU conv,
+output).  Prints one JSON line: the relative P(k) error against the fine run over the band
+0.3 k_Nyq < k < k_Nyq (k_Nyq the particle Nyquist) and up to k = 1 h/Mpc (mean and maximum),
+the mean cross-correlation in the band, for the plain and the corrected coarse runs, the
+training loss before and after, and the inference network's time per evaluation with its
+throughput from HIP events (useful and MFMA-issued TFLOP/s, against the 2.5 PFLOP/s dense BF16
+peak).
 """
 
 from __future__ import annotations
@@ -57,12 +60,19 @@ def conv_throughput(model, rho, a, omega_m, reps=5):
     cnt = ctypes.c_int32(0)
     _lib.call("pm_timing_read", ms, names, cap, ctypes.byref(cnt))
     _lib.call("pm_timing_enable", 0)
-    t = sum(ms[i] for i in range(cnt.value) if names[i] == b"k_conv_fwd") / reps
+    conv = (b"k_conv_fwd", b"k_conv_stem", b"k_conv_out_residual")
+    t = sum(ms[i] for i in range(cnt.value) if names[i] in conv) / reps
     n = rho.numel()
-    flops = 0.0
-    for layer in model.layers:
-        flops += 2.0 * TAPS * CHANNELS * layer.cin * n
-    return flops / (t * 1e-3) / 1e12, t
+    # useful: the stem's one live spatial channel (its constant channels fold into the bias),
+    # 16 -> 16 hidden layers, the output layer's one channel; issued: what the MFMAs compute
+    # (the stem's 27 taps in one K = 32 step, 15 steps of 16 x 16 x 32 per hidden / output tile)
+    useful = 2.0 * TAPS * CHANNELS * n  # stem
+    issued = 2.0 * 32 * CHANNELS * n
+    for layer in model.layers[1:]:
+        cout = 1 if layer.name == "out" else CHANNELS
+        useful += 2.0 * TAPS * CHANNELS * cout * n
+        issued += 2.0 * 15 * 32 * CHANNELS * n
+    return useful / (t * 1e-3) / 1e12, issued / (t * 1e-3) / 1e12, t
 
 
 def main(argv=None):
@@ -77,6 +87,7 @@ def main(argv=None):
     ap.add_argument("--lr", default="3e-4", help="learning rate, or a comma-separated list to compare")
     ap.add_argument("--max-residual", type=float, default=0.1)
     ap.add_argument("--depth", type=int, default=2)
+    ap.add_argument("--block", default="plain", choices=["plain", "residual"])
     args = ap.parse_args(argv)
     dev = torch.device("cuda", 0)
     cosmo = LCDM()
@@ -101,7 +112,7 @@ def main(argv=None):
     model = None
     for lr in [float(v) for v in str(args.lr).split(",")]:
         model = init_mesh_cnn_potential_correction(seed=0, depth=args.depth, max_residual=args.max_residual,
-                                                   device=dev)
+                                                   device=dev, block=args.block)
         opt = build_correction_optimizer(lr)
         st = opt.init(model.params)
         rng = np.random.default_rng(0)
@@ -121,23 +132,30 @@ def main(argv=None):
         kn = np.pi / conf.ptcl_spacing
         band = (k > 0.3 * kn) & (k < kn)
         fine = res["pk_fine"].cpu().numpy()[band]
+        k1 = (k > 0) & (k <= 1.0)
+        e1p = np.abs(res["pk_plain"].cpu().numpy()[k1] / res["pk_fine"].cpu().numpy()[k1] - 1)
+        e1c = np.abs(res["pk_corrected"].cpu().numpy()[k1] / res["pk_fine"].cpu().numpy()[k1] - 1)
         results.append({
             "lr": lr, "train_s": round(t_train, 2), "loss_curve": curve,
             "force_loss_plain": loss_plain, "force_loss_corrected": pair_loss(model, sample),
             "pk_err_plain": float(np.abs(res["pk_plain"].cpu().numpy()[band] / fine - 1).mean()),
             "pk_err_corrected": float(np.abs(res["pk_corrected"].cpu().numpy()[band] / fine - 1).mean()),
+            "pk_err_k1_plain": {"mean": float(e1p.mean()), "max": float(e1p.max())},
+            "pk_err_k1_corrected": {"mean": float(e1c.mean()), "max": float(e1c.max())},
             "r_plain": float(res["r_plain"].cpu().numpy()[band].mean()),
             "r_corrected": float(res["r_corrected"].cpu().numpy()[band].mean()),
         })
         _log(f"[study] lr {lr:g}: {json.dumps(results[-1])}")
     out = {"particles": args.particles, "coarse_mesh": conf.mesh_shape[0], "fine_mesh": conf_fine.mesh_shape[0],
            "box": args.box, "seeds": args.seeds, "pairs": len(pairs), "iters": args.iters,
-           "max_residual": args.max_residual, "runs": results}
+           "max_residual": args.max_residual, "depth": args.depth, "block": args.block,
+           "conv_layers": len(model.layers) if model is not None else None, "runs": results}
     rho = density(pairs[-1][1], cosmo.Omega_m, conf)
-    tflops, ms = conv_throughput(model, rho, 1.0, cosmo.Omega_m)  # the last model's layers
+    useful, issued, ms = conv_throughput(model, rho, 1.0, cosmo.Omega_m)  # the last model's layers
     out["conv_fwd_ms"] = round(ms, 4)
-    out["conv_fwd_tflops"] = round(tflops, 1)
-    out["conv_mfma_frac"] = round(tflops / BF16_PEAK_TFLOPS, 4)
+    out["conv_fwd_tflops_useful"] = round(useful, 1)
+    out["conv_fwd_tflops_issued"] = round(issued, 1)
+    out["conv_mfma_frac"] = round(issued / BF16_PEAK_TFLOPS, 4)
     print(json.dumps(out))
 
 

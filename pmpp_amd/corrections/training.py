@@ -140,10 +140,30 @@ def gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot):
 
 
 def corrected_gravity(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrection):
-    """Coarse-mesh accelerations with the learned correction (no gradient)."""
-    rho = density(ptcl, cosmo, conf)
+    """Coarse-mesh accelerations with the learned correction (no gradient).
+
+    One force step: the density (``pm_gravity_density``), the network on it, then the same
+    step resumed with the residual (``pm_gravity_corrected_resume``), which reuses the
+    bucketed particles and the density the first call left in the workspace (held here
+    across the network's own allocations)."""
+    dev = ptcl.disp.device
+    ensure_init(dev)
+    d = gravity_desc(conf, _omega(cosmo))
+    pm = as_contig(ptcl.pmid, torch.int16, "pmid")
+    dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
+    n = _check_particles(pm, dp)
+    valid = _valid_u8(ptcl)
+    nbytes = _lib.size_query("pm_gravity_workspace", d, n)
+    ws = workspace(nbytes, dev)
+    s = stream_handle(dev)
+    rho = torch.empty(tuple(int(v) for v in conf.mesh_shape), dtype=torch.float32, device=dev)
+    _lib.call("pm_gravity_density", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(rho), ptr(ws), nbytes, s)
     rho_add, _ = evaluate_residual(correction, rho, a, _omega(cosmo))
-    return gravity_with_residual(ptcl, cosmo, conf, rho_add)
+    del rho
+    acc = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    _lib.call("pm_gravity_corrected_resume", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(rho_add), ptr(acc), ptr(ws),
+              nbytes, s)
+    return acc
 
 
 def corrected_gravity_vjp(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrection, acc_cot):

@@ -105,7 +105,12 @@ struct PlaneTaps {
   }
 };
 
-template <int CIN, bool GELU, bool RES>
+// Epilogue modes of the forward: kPlain writes pre and/or act; kOutResidual is the output
+// layer fused with the correction's residual stage (inference): rho_add = gain (rho - 1) +
+// amp tanh(pre channel 0), the same arithmetic as k_residual of pm_correction.hip.
+enum Epilogue { kPlain = 0, kOutResidual = 1 };
+
+template <int CIN, bool GELU, bool RES, int EPI>
 struct PlaneWalk {
   using T = PlaneTaps<CIN>;
   static constexpr int KS = T::KS;
@@ -125,6 +130,28 @@ struct PlaneWalk {
   float* pre;
   __bf16* act;
   int pch;
+  // per-tile values loaded at the start of the block (before the next block's prefetch, so
+  // that waiting for them does not wait for the prefetch): the residual's 4 channels (RES)
+  // or the density (kOutResidual)
+  typedef uint32_t Early __attribute__((ext_vector_type(2 * kB)));
+  Early early;
+  const float* rho;
+  float* rho_add;
+  float gain, amp;
+
+  __device__ __forceinline__ void load_early() {
+    if constexpr (RES) {
+#pragma unroll
+      for (int t = 0; t < kB; ++t) {
+        const uint2 v = *reinterpret_cast<const uint2*>(res + (el + t * plane) * kCout + 4 * g);
+        early[2 * t] = v.x;
+        early[2 * t + 1] = v.y;
+      }
+    } else if constexpr (EPI == kOutResidual) {
+#pragma unroll
+      for (int t = 0; t < kB; ++t) early[t] = __float_as_uint(rho[el + t * plane]);
+    }
+  }
 
   __device__ __forceinline__ void read_plane(int buf, int p) {
 #pragma unroll
@@ -138,11 +165,15 @@ struct PlaneWalk {
     const int64_t e = el + t * plane;
     float zv[4];
     bf16x4 r;
-    if constexpr (RES) r = *reinterpret_cast<const bf16x4*>(res + e * kCout + 4 * g);
+    if constexpr (RES) r = __builtin_bit_cast(bf16x4, make_uint2(early[2 * t], early[2 * t + 1]));
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       zv[i] = c[i] + bsv[i];
       if constexpr (RES) zv[i] += (float)r[i];
+    }
+    if constexpr (EPI == kOutResidual) {
+      if (g == 0) rho_add[e] = gain * (__uint_as_float(early[t]) - 1.f) + amp * tanhf(zv[0]);
+      return;
     }
     if (pre) {
       if (pch == kCout)
@@ -196,12 +227,15 @@ struct PlaneWalk {
 // 8-channel half ([half][voxel][8]), so the 16-byte reads of an 8-lane group cover 128
 // contiguous bytes (no bank conflicts).  The global addresses of a block are the thread's
 // fixed offsets plus the block origin, except for blocks whose halo wraps around the mesh.
-template <int CIN, bool GELU, bool RES>
+template <int CIN, bool GELU, bool RES, int EPI = kPlain>
 __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
                                                           const float* __restrict__ bias,
                                                           const __bf16* __restrict__ res, float* __restrict__ pre,
-                                                          __bf16* __restrict__ act, int n0, int n1, int n2, int pch) {
-  using Walk = PlaneWalk<CIN, GELU, RES>;
+                                                          __bf16* __restrict__ act, int n0, int n1, int n2, int pch,
+                                                          const float* __restrict__ rho = nullptr,
+                                                          const float* __restrict__ stats = nullptr, float c = 0.f,
+                                                          float r = 0.f, float* __restrict__ rho_add = nullptr) {
+  using Walk = PlaneWalk<CIN, GELU, RES, EPI>;
   using Taps = PlaneTaps<CIN>;
   constexpr int TPS = Taps::TPS, KS = Taps::KS;
   constexpr int V = CIN / 8;        // 8-channel halves of a voxel (LDS planes)
@@ -250,6 +284,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
   wk.pre = pre;
   wk.act = act;
   wk.pch = pch;
+  wk.rho = rho;
+  wk.rho_add = rho_add;
+  if constexpr (EPI == kOutResidual) {
+    wk.gain = stats[1];
+    wk.amp = r * stats[0] / c;
+  }
   // this thread's staged vectors: element offset from the block's (-1, -1, -1) corner when
   // the halo does not wrap (fits int32: the caller checks kH * n1 * n2 * CIN < 2^31)
   typedef int Offsets __attribute__((ext_vector_type(PER)));
@@ -269,6 +309,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
   typedef uint32_t Prefetch __attribute__((ext_vector_type(4 * PER)));  // a value, never a private array
   int cur = -1, nxt = first;
   for (;;) {
+    if (cur >= 0) {
+      const int bz = cur % nbz, by = (cur / nbz) % nby, bx = cur / (nbz * nby);
+      const int64_t e0 = (((int64_t)bx * kB * n1 + by * kB) * n2 + bz * kB);  // block origin voxel
+      wk.el = e0 + (int64_t)(2 * wave + m / 8) * n2 + m % 8;
+      wk.load_early();
+    }
     Prefetch pf;
     if (nxt < nblk) {
       const int bz = nxt % nbz, by = (nxt / nbz) % nby, bx = nxt / (nbz * nby);
@@ -300,9 +346,6 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
       }
     }
     if (cur >= 0) {
-      const int bz = cur % nbz, by = (cur / nbz) % nby, bx = cur / (nbz * nby);
-      const int64_t e0 = (((int64_t)bx * kB * n1 + by * kB) * n2 + bz * kB);  // block origin voxel
-      wk.el = e0 + (int64_t)(2 * wave + m / 8) * n2 + m % 8;
       wk.read_plane(0, 0);
       wk.template walk<0>();
     }
@@ -659,8 +702,8 @@ int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, 
                PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
   const int64_t nblk = nblocks(shape);
   const int key = (cin == 16 ? 4 : 0) + (gelu ? 2 : 0) + (res ? 1 : 0);
-  void (*kern)(const __bf16*, const __bf16*, const float*, const __bf16*, float*, __bf16*, int, int, int, int) =
-      nullptr;
+  void (*kern)(const __bf16*, const __bf16*, const float*, const __bf16*, float*, __bf16*, int, int, int, int,
+               const float*, const float*, float, float, float*) = nullptr;
   switch (key) {
     case 0: kern = conv::k_conv_fwd<8, false, false>; break;
     case 1: kern = conv::k_conv_fwd<8, false, true>; break;
@@ -674,7 +717,7 @@ int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, 
   unsigned grid = 0;
   PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblk, &grid));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, xb, wb, bias, rb, pre, ab, shape[0], shape[1],
-                     shape[2], (int)pre_channels);
+                     shape[2], (int)pre_channels, nullptr, nullptr, 0.f, 0.f, nullptr);
   PM_LAUNCHED("k_conv_fwd", s);
   return PM_SUCCESS;
 }
@@ -695,6 +738,27 @@ int pm_conv3d_stem(const float* rho, const int32_t shape[3], float omega_m, floa
                      omega_m, sigma8, static_cast<const __bf16*>(w), bias, static_cast<__bf16*>(act), shape[0],
                      shape[1], shape[2]);
   PM_LAUNCHED("k_conv_stem", s);
+  return PM_SUCCESS;
+}
+
+int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, const float* rho,
+                           const int32_t shape[3], float omega_m, float max_residual, const float* stats,
+                           float* rho_add, void* stream) {
+  PM_TRY(check_shape(shape));
+  PM_TRY(check_ptr16(x, "x"));
+  PM_RETURN_IF(w == nullptr || bias == nullptr || rho == nullptr || stats == nullptr || rho_add == nullptr,
+               PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF((int64_t)conv::kH * shape[1] * shape[2] * 16 >= (int64_t(1) << 31) ||
+                   (int64_t)shape[0] * shape[1] * shape[2] / (conv::kB * conv::kB * conv::kB) >= (int64_t(1) << 31),
+               PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
+  hipStream_t s = as_stream(stream);
+  auto kern = conv::k_conv_fwd<16, false, false, conv::kOutResidual>;
+  unsigned grid = 0;
+  PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, static_cast<const __bf16*>(x),
+                     static_cast<const __bf16*>(w), bias, nullptr, nullptr, nullptr, shape[0], shape[1], shape[2], 1,
+                     rho, stats, 1.5f * omega_m, max_residual, rho_add);
+  PM_LAUNCHED("k_conv_out_residual", s);
   return PM_SUCCESS;
 }
 

@@ -1120,6 +1120,25 @@ int pm_gravity_corrected(const int16_t* pmid, const float* disp, const uint8_t* 
   return grav::gather_stage(b.pmid_s, b.disp_s, b.valid_s, valid, n, desc, b, b.r0, b.r1, b.r2, acc, s);
 }
 
+int pm_gravity_corrected_resume(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                                const pm_gravity_desc* desc, const float* rho_add, float* acc, void* workspace,
+                                size_t workspace_bytes, void* stream) {
+  hipStream_t s = as_stream(stream);
+  grav::Layout L;
+  grav::Buffers b{};
+  Arena ws(workspace, workspace_bytes);
+  fft::SpecTables tb;
+  // the same carve as the pm_gravity_density call: the bucketed particles and the density
+  // mesh it left are at the same places
+  int st = grav::prepare(desc, n, workspace, workspace_bytes, false, L, b, ws, tb, s);
+  if (st) return st;
+  PM_RETURN_IF(n > 0 && (pmid == nullptr || disp == nullptr || acc == nullptr), PM_ERROR_INVALID_ARGUMENT,
+               "pmid/disp/acc NULL");
+  st = grav::force_mesh_stage(desc, L, b, tb, s, false, rho_add);
+  if (st) return st;
+  return grav::gather_stage(b.pmid_s, b.disp_s, b.valid_s, valid, n, desc, b, b.r0, b.r1, b.r2, acc, s);
+}
+
 int pm_gravity_density(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                        const pm_gravity_desc* desc, float* rho, void* workspace, size_t workspace_bytes,
                        void* stream) {

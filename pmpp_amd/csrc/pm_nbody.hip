@@ -20,18 +20,20 @@ inline unsigned blocks_for(int64_t n4) {
 }
 
 // y += f * x over m floats (y, x 16-byte aligned)
-__global__ void __launch_bounds__(kBlock) k_axpy(float* __restrict__ y, const float* __restrict__ x, int64_t m,
+// out = y + f x (out may be y: in place; one pass either way)
+__global__ void __launch_bounds__(kBlock) k_axpy(const float* y, const float* __restrict__ x, float* out, int64_t m,
                                                  float f) {
   const int64_t m4 = m / 4;
-  float4* y4 = reinterpret_cast<float4*>(y);
+  const float4* y4 = reinterpret_cast<const float4*>(y);
   const float4* x4 = reinterpret_cast<const float4*>(x);
+  float4* o4 = reinterpret_cast<float4*>(out);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m4; i += (int64_t)gridDim.x * blockDim.x) {
     float4 a = y4[i], b = x4[i];
-    y4[i] = make_float4(a.x + b.x * f, a.y + b.y * f, a.z + b.z * f, a.w + b.w * f);
+    o4[i] = make_float4(a.x + b.x * f, a.y + b.y * f, a.z + b.z * f, a.w + b.w * f);
   }
   for (int64_t i = 4 * m4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * blockDim.x)
-    y[i] = y[i] + x[i] * f;
+    out[i] = y[i] + x[i] * f;
 }
 
 __device__ __forceinline__ double block_sum(double v) {
@@ -92,14 +94,15 @@ int check_vec(const void* p, const char* what) {
   return PM_SUCCESS;
 }
 
-int update(float* y, const float* x, int64_t n, float f, void* stream, const char* name) {
+int update(const float* y, const float* x, float* out, int64_t n, float f, void* stream, const char* name) {
   PM_RETURN_IF(n < 0, PM_ERROR_INVALID_ARGUMENT, "n must be >= 0");
   if (n == 0) return PM_SUCCESS;
-  PM_TRY(check_vec(y, "output"));
+  PM_TRY(check_vec(y, "state"));
   PM_TRY(check_vec(x, "input"));
+  PM_TRY(check_vec(out, "output"));
   hipStream_t s = as_stream(stream);
   const int64_t m = 3 * n;
-  hipLaunchKernelGGL(k_axpy, dim3(blocks_for(m / 4 + 1)), dim3(kBlock), 0, s, y, x, m, f);
+  hipLaunchKernelGGL(k_axpy, dim3(blocks_for(m / 4 + 1)), dim3(kBlock), 0, s, y, x, out, m, f);
   PM_LAUNCHED(name, s);
   return PM_SUCCESS;
 }
@@ -133,12 +136,12 @@ using namespace pm;
 
 extern "C" {
 
-int pm_kick(float* vel, const float* acc, int64_t n, float factor, void* stream) {
-  return nbody::update(vel, acc, n, factor, stream, "k_kick");
+int pm_kick(const float* vel, const float* acc, float* vel_out, int64_t n, float factor, void* stream) {
+  return nbody::update(vel, acc, vel_out, n, factor, stream, "k_kick");
 }
 
-int pm_drift(float* disp, const float* vel, int64_t n, float factor, void* stream) {
-  return nbody::update(disp, vel, n, factor, stream, "k_drift");
+int pm_drift(const float* disp, const float* vel, float* disp_out, int64_t n, float factor, void* stream) {
+  return nbody::update(disp, vel, disp_out, n, factor, stream, "k_drift");
 }
 
 int pm_nbody_adj_workspace(size_t* bytes) {

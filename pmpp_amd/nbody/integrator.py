@@ -87,17 +87,19 @@ def _n(ptcl) -> int:
 
 
 def drift(a_vel, a_prev, a_next, ptcl, cosmo, conf):
-    """disp += D vel (integrator.py:203-237)."""
-    disp = ptcl.disp.clone()
-    _lib.call("pm_drift", ptr(disp), ptr(as_contig(ptcl.vel, torch.float32, "vel")), _n(ptcl),
+    """disp += D vel (integrator.py:203-237), into a new array (one pass)."""
+    src = as_contig(ptcl.disp, torch.float32, "disp")
+    disp = torch.empty_like(src)
+    _lib.call("pm_drift", ptr(src), ptr(as_contig(ptcl.vel, torch.float32, "vel")), ptr(disp), _n(ptcl),
               _f32(drift_factor(a_vel, a_prev, a_next, cosmo, conf)), stream_handle(disp.device))
     return ptcl.replace(disp=disp)
 
 
 def kick(a_acc, a_prev, a_next, ptcl, cosmo, conf):
-    """vel += K acc (integrator.py:512-538)."""
-    vel = ptcl.vel.clone()
-    _lib.call("pm_kick", ptr(vel), ptr(as_contig(ptcl.acc, torch.float32, "acc")), _n(ptcl),
+    """vel += K acc (integrator.py:512-538), into a new array (one pass)."""
+    src = as_contig(ptcl.vel, torch.float32, "vel")
+    vel = torch.empty_like(src)
+    _lib.call("pm_kick", ptr(src), ptr(as_contig(ptcl.acc, torch.float32, "acc")), ptr(vel), _n(ptcl),
               _f32(kick_factor(a_acc, a_prev, a_next, cosmo, conf)), stream_handle(vel.device))
     return ptcl.replace(vel=vel)
 

@@ -121,6 +121,10 @@ def _torch_residual(model, rho, a, om, params):
     w, b = layer(0)
     x = _bf(gelu(_torch_conv(x0, w, b, 8)))
     for i in range(model.depth - 1):
+        if model.block == "plain":
+            w1, b1 = layer(1 + i)
+            x = _bf(gelu(_torch_conv(x, w1, b1, 16)))
+            continue
         w1, b1 = layer(1 + 2 * i)
         w2, b2 = layer(2 + 2 * i)
         h = _bf(gelu(_torch_conv(x, w1, b1, 16)))
@@ -132,9 +136,12 @@ def _torch_residual(model, rho, a, om, params):
     return g * (rho - 1.0) + r * torch.tanh(out) * sigma / c
 
 
-@pytest.mark.parametrize("depth,shape", [(1, (8, 8, 8)), (2, (16, 8, 24)), (3, (16, 16, 16))])
-def test_network_and_backward_match_float32_torch(device, depth, shape):
-    model = _randomised(init_mesh_cnn_potential_correction(seed=depth, depth=depth, device=device), depth)
+@pytest.mark.parametrize("depth,shape,block", [(1, (8, 8, 8), "residual"), (2, (16, 8, 24), "residual"),
+                                               (3, (16, 16, 16), "residual"), (2, (16, 8, 24), "plain"),
+                                               (3, (8, 16, 16), "plain")])
+def test_network_and_backward_match_float32_torch(device, depth, shape, block):
+    model = _randomised(init_mesh_cnn_potential_correction(seed=depth, depth=depth, device=device, block=block),
+                        depth)
     rng = np.random.default_rng(depth)
     rho = torch.from_numpy((1 + 0.5 * rng.normal(size=shape)).astype(np.float32)).to(device)
     a, om = 0.7, 0.31
@@ -189,6 +196,31 @@ def test_inference_stem_from_density_matches_the_feature_path(device, shape):
     assert float((got != exp).float().mean()) < 0.02
     err = torch.linalg.norm((fused - ref).cpu()) / torch.linalg.norm(ref.cpu())
     assert err < 2e-3, float(err)
+    # the fused output layer + residual stage against the output conv and the residual kernel
+    # on the same activations: the same MFMA sums and the same elementwise arithmetic
+    x = tape.acts[-1]
+    L = model.layers[-1]
+    rho_add = torch.empty_like(rho)
+    _lib.call("pm_conv3d_out_residual", ptr(x), ptr(model.weight(L)), ptr(model.bias(L)), ptr(rho),
+              _lib.int3(shape), om, float(model.max_residual), ptr(tape.stats), ptr(rho_add), stream_handle(device))
+    np.testing.assert_allclose(rho_add.cpu().numpy(), ref.cpu().numpy(), rtol=1e-6,
+                               atol=1e-6 * float(ref.abs().max()))
+
+
+@pytest.mark.parametrize("ng,ratio", [(16, 2), (32, 2), (16, 1)])
+def test_corrected_force_resumes_the_density_step(device, ng, ratio):
+    """corrected_gravity buckets and scatters once (pm_gravity_density), runs the network and
+    resumes the same step (pm_gravity_corrected_resume); that equals the force step run from
+    scratch with the same residual (pm_gravity_corrected), bit for bit."""
+    from pmpp_amd.corrections.training import corrected_gravity
+    conf = Configuration(2.0, (ng,) * 3, mesh_shape=ratio)
+    ptcl = _ptcl(conf, 0.7, ng + 1, device)
+    model = _randomised(init_mesh_cnn_potential_correction(seed=3, depth=2, device=device), 3)
+    a = 0.6
+    got = corrected_gravity(a, ptcl, 0.3, conf, model)
+    rho_add, _ = evaluate_residual(model, density(ptcl, 0.3, conf), a, 0.3)
+    ref = gravity_with_residual(ptcl, 0.3, conf, rho_add)
+    np.testing.assert_array_equal(got.cpu().numpy(), ref.cpu().numpy())
 
 
 def test_zero_initialised_correction_is_exactly_the_plain_force(device):

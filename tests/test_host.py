@@ -71,10 +71,11 @@ def test_correction_parameter_layout_and_initialisation():
 
     from pmpp_amd.corrections.mesh_cnn import initial_params, param_layout
 
-    for depth in (1, 2, 4):
-        layers, gain, total = param_layout(depth)
+    for depth, block, nconv in [(1, "residual", 2), (2, "residual", 4), (4, "residual", 8), (2, "plain", 3),
+                                (4, "plain", 5)]:
+        layers, gain, total = param_layout(depth, block)
         assert [L.name for L in layers][0] == "stem" and layers[-1].name == "out"
-        assert len(layers) == 2 * depth
+        assert len(layers) == nconv
         ends = [L.offset + L.size for L in layers]
         # 64-element alignment (16-byte aligned bf16 weights and float32 biases), no overlap
         assert all(L.offset % 64 == 0 for L in layers) and gain % 64 == 0
