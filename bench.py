@@ -245,6 +245,59 @@ def nbody_run(args):
     print(json.dumps(res), flush=True)
 
 
+def network_run(args):
+    """The learned correction's inference network alone on the full-scale workload's coarse
+    mesh: the CIC density of the 512^3-particle 2LPT state (a = 1) on the 1024^3 mesh, then
+    ``--steps`` evaluations of the three-layer BF16 network (statistics, stem from the density,
+    hidden GELU conv, output layer fused with the residual).  Prints one JSON line with the
+    per-kernel times from the library's HIP events (not the headline metric)."""
+    import torch
+
+    from pmpp_amd import Configuration, _lib
+    from pmpp_amd._torch import stream_handle
+    from pmpp_amd.corrections import init_mesh_cnn_potential_correction
+    from pmpp_amd.corrections.mesh_cnn import evaluate_residual
+    from pmpp_amd.corrections.training import density
+    from pmpp_amd.initial_conditions import LCDM, particles_2lpt
+
+    dev = torch.device("cuda", 0)
+    ng, ratio, box = CONFIGS[args.config]
+    cosmo = LCDM()
+    conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio)
+    ptcl = particles_2lpt(conf, cosmo, seed=0, jitter_seed=1, device=dev, a=1.0)
+    rho = density(ptcl, cosmo.Omega_m, conf)
+    del ptcl
+    model = init_mesh_cnn_potential_correction(seed=0, depth=2, block="plain", device=dev)
+    for _ in range(args.warmup):
+        evaluate_residual(model, rho, 1.0, cosmo.Omega_m)
+    torch.cuda.synchronize()
+    stream = stream_handle(dev)
+    cap = 16 * args.steps + 16
+    _lib.call("pm_timing_enable", cap)
+    _lib.call("pm_timing_mark", b"begin", stream)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        evaluate_residual(model, rho, 1.0, cosmo.Omega_m)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (ctypes.c_float * cap)()
+    names = (ctypes.c_char_p * cap)()
+    cnt = ctypes.c_int32(0)
+    _lib.call("pm_timing_read", ms, names, cap, ctypes.byref(cnt))
+    _lib.call("pm_timing_enable", 0)
+    per = {}
+    for i in range(cnt.value):
+        k = names[i].decode()
+        per[k] = per.get(k, 0.0) + ms[i] / args.steps
+    nvox = conf.mesh_size
+    print(json.dumps({
+        "metric": "inference network time per force step (three-layer BF16 mesh CNN)",
+        "ms_per_eval": (t1 - t0) * 1e3 / args.steps, "voxels": nvox, "steps": args.steps,
+        "kernels_ms": {k: round(v, 4) for k, v in sorted(per.items(), key=lambda kv: -kv[1])},
+        "config": {"workload": f"network on the {ng * ratio}^3 density of {ng}^3 particles, {box:g} Mpc/h"},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +309,8 @@ def main():
                     help="time the full 20-step run (plain, reverse sweep, with the learned correction) and a "
                          "plain 2048^3-mesh force step instead of the force-step benchmark")
     ap.add_argument("--run-steps", type=int, default=20)
+    ap.add_argument("--network", action="store_true",
+                    help="time the learned correction's inference network on the workload's coarse mesh instead")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (the known-size kernel that "
                          "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -270,6 +325,10 @@ def main():
 
     if args.nbody_run:
         nbody_run(args)
+        return
+
+    if args.network:
+        network_run(args)
         return
 
     if args.copy_calibration:

@@ -2,7 +2,7 @@
 correction by P(k) and cross-correlation against a held-out fine run.
 
     python -m pmpp_amd.corrections.study [--particles 128] [--box 256] [--coarse 2] [--fine 4]
-                                         [--seeds 32] [--iters 800] [--steps 10] [--lr 3e-4]
+                                         [--seeds 32] [--iters 800] [--steps 10] [--lr 1e-4]
 
 Defaults: 128^3 particles in a 256 Mpc/h box, a coarse mesh of 256^3 (``--coarse 2``) and fine
 runs on a mesh 4x finer per axis (1024^3), 2LPT at a = 1/64 then ``--steps`` KDK steps to a = 1,
@@ -84,7 +84,7 @@ def main(argv=None):
     ap.add_argument("--seeds", type=int, default=32)
     ap.add_argument("--iters", type=int, default=800)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--lr", default="3e-4", help="learning rate, or a comma-separated list to compare")
+    ap.add_argument("--lr", default="1e-4", help="learning rate, or a comma-separated list to compare")
     ap.add_argument("--max-residual", type=float, default=0.1)
     ap.add_argument("--depth", type=int, default=2)
     ap.add_argument("--block", default="plain", choices=["plain", "residual"])
