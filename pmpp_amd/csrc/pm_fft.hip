@@ -168,24 +168,30 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
 }
 
 // --------------------------------------------------------------- z passes ----
-template <int M>
+template <int M, bool C2R>
 struct ZCfg {
   static constexpr int P = Plan<M>::P;
-  // long lines fill LDS (one slot per line): smaller workgroups let four of them share a CU
+  // long lines fill LDS (one slot per line): smaller workgroups let several of them share a CU
   // and interleave their load / transform / store phases
   static constexpr int THREADS = M >= 256 ? 128 : 256;
   static constexpr int LINES = THREADS / P;
-  // floats per line slot: holds M+1 float2, and LSTR = P (mod 64) so the 64/P lines of a wave
-  // start on disjoint groups of the 64 LDS banks (the exchange rows of one line are P apart)
-  static constexpr int LSTR_MIN = 2 * (M + 2);
+  // floats per line slot, with LSTR = P (mod 64) so the 64/P lines of a wave start on
+  // disjoint groups of the 64 LDS banks (the exchange rows of one line are P apart).  R2C
+  // pairs bins k and M - k of the complex transform through a natural-order copy of the line
+  // (M+1 float2); C2R pairs them through a real-only copy in two rounds, so its slot only holds
+  // the larger of that copy (M + 1 floats) and the transform's real/imaginary exchange
+  // (M + M/N2 floats), and twice as many lines fit on a CU.
+  static constexpr int LSTR_MIN =
+      C2R ? (M + M / Plan<M>::N2 > M + 1 ? M + M / Plan<M>::N2 : M + 1) : 2 * (M + 2);
   static constexpr int LSTR = ((LSTR_MIN + 63) / 64) * 64 + (P % 64);
   static constexpr int LDS_FLOATS = LINES * LSTR;
 };
 
 template <int M, bool C2R>
-__global__ void __launch_bounds__(ZCfg<M>::THREADS) k_zpass(ZPass zp, SpecTables tb, const float2* __restrict__ tw) {
+__global__ void __launch_bounds__((ZCfg<M, C2R>::THREADS)) k_zpass(ZPass zp, SpecTables tb,
+                                                                 const float2* __restrict__ tw) {
   using PL = Plan<M>;
-  using ZC = ZCfg<M>;
+  using ZC = ZCfg<M, C2R>;
   constexpr int P = PL::P, E = PL::E, N2 = PL::N2;
   __shared__ float lds[ZC::LDS_FLOATS];
   const int p = threadIdx.x % P;
@@ -234,6 +240,7 @@ __global__ void __launch_bounds__(ZCfg<M>::THREADS) k_zpass(ZPass zp, SpecTables
       zp.cnyq[lg * zp.nyq_stride] = make_float2(z0.x - z0.y, 0.f);
     }
   } else {
+    (void)nat;
     const float2* srcm = zp.cmain + lgl * zp.pitch_c;
     float2 xn = zp.cnyq[lgl * zp.nyq_stride];  // cnyq is a valid address even when nyq == 0
     const int klim = zp.kz_lim < M ? zp.kz_lim : M;
@@ -249,21 +256,34 @@ __global__ void __launch_bounds__(ZCfg<M>::THREADS) k_zpass(ZPass zp, SpecTables
       for (int j = 0; j < E; ++j) v[j] = c_imul(tb.negk[2][p + P * j], v[j]);
       xn = c_imul(tb.negk[2][M], xn);
     }
+    // pair bins k (the X-layout slot) and M - k through a natural-order real copy of the line,
+    // real parts first, then imaginary parts (the slot holds M + 1 floats, not M + 1 float2):
+    // e = (X[k] + conj X[M-k]) / 2, d = X[k] - conj X[M-k]
+    float* rl = lds + line * ZC::LSTR;
+    float ex[E], dx[E];
 #pragma unroll
-    for (int j = 0; j < E; ++j) nat[p + P * j] = v[j];
-    if (p == 0) nat[M] = xn;
+    for (int j = 0; j < E; ++j) rl[p + P * j] = v[j].x;
+    if (p == 0) rl[M] = xn.x;
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      int k = xrow<M>(p, s);
-      float2 xk = nat[k];
-      float2 xm = nat[M - k];
-      if (k == 0) {  // irfft uses only the real parts of the DC and Nyquist bins
-        xk.y = 0.f;
-        xm.y = 0.f;
-      }
-      float2 e = c_scale(c_add(xk, c_conj(xm)), 0.5f);
-      float2 o = c_scale(c_mulconj(c_sub(xk, c_conj(xm)), table_tw<2 * M, false>(tw, k)), 0.5f);
+      const int k = xrow<M>(p, s);
+      const float a = rl[k], b = rl[M - k];
+      ex[s] = a + b;
+      dx[s] = a - b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) rl[p + P * j] = v[j].y;
+    if (p == 0) rl[M] = xn.y;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int k = xrow<M>(p, s);
+      // irfft uses only the real parts of the DC and Nyquist bins
+      const float a = k == 0 ? 0.f : rl[k], b = k == 0 ? 0.f : rl[M - k];
+      float2 e = c_scale(make_float2(ex[s], a - b), 0.5f);
+      float2 o = c_scale(c_mulconj(make_float2(dx[s], a + b), table_tw<2 * M, false>(tw, k)), 0.5f);
       v[s] = make_float2(e.x - o.y, e.y + o.x);  // e + i o
     }
     __syncthreads();
@@ -357,7 +377,7 @@ int launch_cols_n(int mode, const ColPass& cp, int batches, const SpecTables& tb
 
 template <int M>
 int launch_z_n(bool c2r, const ZPass& zp, const SpecTables& tb, hipStream_t s) {
-  using ZC = ZCfg<M>;
+  using ZC = ZCfg<M, true>;  // THREADS and LINES are the same for both directions
   const float2* tw = twiddle_table();
   PM_RETURN_IF(tw == nullptr, PM_ERROR_HIP, "twiddle table not initialised");
   if (zp.nlines == 0) return PM_SUCCESS;

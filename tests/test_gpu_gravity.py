@@ -77,12 +77,19 @@ def test_banded_spectrum_on_non_cubic_meshes(device, grid, ratio):
     assert np.linalg.norm(acc - ref) / np.linalg.norm(ref) < 1e-5
 
 
-def test_clustered_particles_exceed_one_gather_chunk(device):
-    # every particle in one brick: several gather chunks and heavy LDS-atomic contention
-    conf = Configuration(1.0, (32, 32, 32), mesh_shape=1, brick=(8, 8, 8))
+@pytest.mark.parametrize("brick", [(8, 8, 8), (16, 16, 32)])
+def test_clustered_particles_exceed_one_gather_chunk(device, brick):
+    # 90 % of the particles in one brick: several gather chunks of 2048 particles (the last
+    # one partial) and heavy LDS-atomic contention; the rest leave most bricks and some brick
+    # columns empty.  (8, 8, 8) runs the runtime-shape tile kernels, (16, 16, 32) the
+    # compile-time ones (the pipelined gather).
+    conf = Configuration(1.0, (32, 32, 32), mesh_shape=1, brick=brick)
     rng = np.random.default_rng(9)
     n = 32 ** 3
-    pmid = torch.as_tensor(rng.integers(0, 8, (n, 3)).astype(np.int16), device=device)
+    pm = rng.integers(0, 8, (n, 3))
+    far = rng.random(n) < 0.1
+    pm[far] = rng.integers(0, 32, (int(far.sum()), 3))
+    pmid = torch.as_tensor(pm.astype(np.int16), device=device)
     disp = torch.as_tensor(rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32), device=device)
     ptcl = Particles(conf, pmid, disp)
     acc = gravity(1.0, ptcl, 0.3, conf).cpu().numpy()
