@@ -26,7 +26,7 @@ namespace corr {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxBlocks = 512;  // partial-sum slots per reduction
+constexpr int kMaxBlocks = 2048;  // partial-sum slots per reduction
 
 unsigned grid_for(int64_t n, int per_thread = 4) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, (int64_t)kBlock * per_thread), kMaxBlocks));
@@ -50,15 +50,27 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ p, int
 }
 
 // ----------------------------------------------------------- features ----
+// float4 loads with four independent float64 chains per thread (rho is 16-byte aligned); the
+// partition of the mesh over threads is fixed by the grid, so the sum is deterministic
 __global__ void __launch_bounds__(kBlock) k_source_sumsq(const float* __restrict__ rho, int64_t n, float c,
                                                          double* __restrict__ partials) {
   __shared__ double lds[kBlock / 64];
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const float s = (rho[i] - 1.f) * c;
-    acc += (double)(s * s);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  const int64_t n4 = n / 4;
+  const float4* r4 = reinterpret_cast<const float4*>(rho);
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += (int64_t)gridDim.x * kBlock) {
+    const float4 r = r4[i];
+    const float s0 = (r.x - 1.f) * c, s1 = (r.y - 1.f) * c, s2 = (r.z - 1.f) * c, s3 = (r.w - 1.f) * c;
+    a0 += (double)(s0 * s0);
+    a1 += (double)(s1 * s1);
+    a2 += (double)(s2 * s2);
+    a3 += (double)(s3 * s3);
   }
-  const double t = block_sum(acc, lds);
+  for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const float s = (rho[i] - 1.f) * c;
+    a0 += (double)(s * s);
+  }
+  const double t = block_sum((a0 + a1) + (a2 + a3), lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
@@ -340,7 +352,8 @@ int pm_correction_features(const float* rho, int64_t n, float omega_m, float a, 
   hipStream_t s = as_stream(stream);
   const float c = 1.5f * omega_m;
   double* part = static_cast<double*>(workspace);
-  const unsigned nb = corr::grid_for(n);
+  PM_RETURN_IF(reinterpret_cast<uintptr_t>(rho) % 16 != 0, PM_ERROR_INVALID_ARGUMENT, "rho must be 16-byte aligned");
+  const unsigned nb = corr::grid_for(n, 16);
   hipLaunchKernelGGL(corr::k_source_sumsq, dim3(nb), dim3(corr::kBlock), 0, s, rho, n, c, part);
   PM_LAUNCHED("k_source_sumsq", s);
   hipLaunchKernelGGL(corr::k_source_scale, dim3(1), dim3(64), 0, s, part, (int)nb, n, max_residual, gain_params, a,
