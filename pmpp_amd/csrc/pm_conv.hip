@@ -68,20 +68,40 @@ __device__ __forceinline__ float gelu_fast(float x) {
 // ----------------------------------------------------------------------------------------
 // Forward convolution, walking x planes.
 //
+// Blocks are BX x 8 x BZ = 512 output voxels, staged with a one-voxel halo (SX x SY x SZ).
+// With BZ = 8 an M tile is 2 y-rows x 8 z of one x plane; with BZ = 16 (BX = 4) it is one
+// y-row of 16 consecutive z, so the 16 lanes of a fragment read 256 contiguous bytes of a
+// channel half (all 64 LDS banks once; the 2 x 8 tile's rows sit SZ voxels apart and collide)
+// and an epilogue store covers 512 contiguous bytes.
+// Epilogue modes of the forward: kPlain writes pre and/or act; kOutResidual is the output
+// layer fused with the correction's residual stage (inference): rho_add = gain (rho - 1) +
+// amp tanh(pre channel 0), the same arithmetic as k_residual of pm_correction.hip.
+enum Epilogue { kPlain = 0, kOutResidual = 1 };
+
+template <int BZ_>
+struct FwdGeom {
+  static constexpr int BZ = BZ_;
+  static constexpr int BY = 8;
+  static constexpr int BX = kVox / (BY * BZ);  // 8 or 4
+  static constexpr int SX = BX + 2, SY = BY + 2, SZ = BZ + 2;
+  static constexpr int NS = SX * SY * SZ;  // staged voxels
+  static constexpr int ROWS = BZ / 8;      // y-rows a wave walks one after the other
+  static_assert(BZ == 8 || BZ == 16, "block z extent");
+};
+
 // The 27 taps are grouped by dx: within one dx the 9 (dy, dz) taps j = 3 dy' + dz' (dy', dz'
 // in 0..2) are spread over KS K steps of TPS taps (TPS = 32 / CIN; padded slots past j = 8 have
-// zero weights).  A wave owns the 2 y-rows x 8 z (16 voxels = one M tile) of every x plane
-// of the block.  Staged input plane p (p = 0..9) feeds output planes p (dx = -1), p - 1
-// (dx = 0) and p - 2 (dx = +1): the wave reads plane p's KS fragments from LDS once and
-// issues 3 KS MFMAs into three rotating accumulators, and reads plane p + 1 while those run.
-// That is KS LDS reads per M tile instead of one per K step, with the read latency covered
-// by the previous plane's matrix work.
-template <int CIN>
+// zero weights).  A wave walks the SX staged x planes of its tile rows: staged plane p feeds
+// output planes p (dx = -1), p - 1 (dx = 0) and p - 2 (dx = +1), so the wave reads plane p's KS
+// fragments from LDS once and issues 3 KS MFMAs into three rotating accumulators, and reads
+// plane p + 1 while those run: KS LDS reads per M tile and plane instead of one per K step,
+// with the read latency covered by the previous plane's matrix work.
+template <int CIN, int SZ>
 struct PlaneTaps {
   static constexpr int TPS = 32 / CIN;
   static constexpr int KS = (9 + TPS - 1) / TPS;
   // staged (y, z) offset of in-plane tap j
-  static constexpr int off2(int j) { return (j / 3) * kH + j % 3; }
+  static constexpr int off2(int j) { return (j / 3) * SZ + j % 3; }
   // offset of slot t of step k relative to the step's slot 0 (padded slots: 0)
   static constexpr int diff(int k, int t) { return k * TPS + t < 9 ? off2(k * TPS + t) - off2(k * TPS) : 0; }
   // steps whose slots have the same relative offsets share a per-lane LDS base (a class)
@@ -105,15 +125,12 @@ struct PlaneTaps {
   }
 };
 
-// Epilogue modes of the forward: kPlain writes pre and/or act; kOutResidual is the output
-// layer fused with the correction's residual stage (inference): rho_add = gain (rho - 1) +
-// amp tanh(pre channel 0), the same arithmetic as k_residual of pm_correction.hip.
-enum Epilogue { kPlain = 0, kOutResidual = 1 };
-
-template <int CIN, bool GELU, bool RES, int EPI>
+template <int CIN, bool GELU, bool RES, int EPI, int BZ>
 struct PlaneWalk {
-  using T = PlaneTaps<CIN>;
+  using Geo = FwdGeom<BZ>;
+  using T = PlaneTaps<CIN, Geo::SZ>;
   static constexpr int KS = T::KS;
+  static constexpr int BX = Geo::BX;
   static constexpr T taps{};
   // weights of (dx' = 0..2, step k): A[cout = m][k = 8 g + j]
   bf16x8 wa[3 * KS];
@@ -123,8 +140,9 @@ struct PlaneWalk {
   f32x4 acc[3];
   const __bf16* xin;
   // epilogue
-  int64_t el;     // lane's output voxel in plane 0 of the block
+  int64_t el;     // lane's output voxel in plane 0 (row 0) of the block
   int64_t plane;  // voxels per x plane of the mesh
+  int64_t rowz;   // voxels per y row of the mesh (n2)
   int g;
   const __bf16* res;
   float* pre;
@@ -139,40 +157,46 @@ struct PlaneWalk {
   float* rho_add;
   float gain, amp;
 
+  __device__ __forceinline__ int64_t voxel(int r, int t) const { return el + r * rowz + t * plane; }
+
   __device__ __forceinline__ void load_early() {
-    if constexpr (RES) {
 #pragma unroll
-      for (int t = 0; t < kB; ++t) {
-        const uint2 v = *reinterpret_cast<const uint2*>(res + (el + t * plane) * kCout + 4 * g);
-        early[2 * t] = v.x;
-        early[2 * t + 1] = v.y;
+    for (int r = 0; r < Geo::ROWS; ++r) {
+#pragma unroll
+      for (int t = 0; t < BX; ++t) {
+        const int q = r * BX + t;
+        if constexpr (RES) {
+          const uint2 v = *reinterpret_cast<const uint2*>(res + voxel(r, t) * kCout + 4 * g);
+          early[2 * q] = v.x;
+          early[2 * q + 1] = v.y;
+        } else if constexpr (EPI == kOutResidual) {
+          early[q] = __float_as_uint(rho[voxel(r, t)]);
+        }
       }
-    } else if constexpr (EPI == kOutResidual) {
-#pragma unroll
-      for (int t = 0; t < kB; ++t) early[t] = __float_as_uint(rho[el + t * plane]);
     }
   }
 
-  __device__ __forceinline__ void read_plane(int buf, int p) {
+  __device__ __forceinline__ void read_plane(int buf, int r, int p) {
 #pragma unroll
     for (int k = 0; k < KS; ++k)
       frag[buf][k] = *reinterpret_cast<const bf16x8*>(
-          xin + cbase[taps.cls[k]] + (p * kH * kH + T::off2(k * T::TPS)) * 8);
+          xin + cbase[taps.cls[k]] + (r * Geo::SZ + p * Geo::SY * Geo::SZ + T::off2(k * T::TPS)) * 8);
   }
 
-  __device__ __forceinline__ void finish(int t, const f32x4& c) {
-    // lane holds output channels 4g .. 4g+3 of its voxel in output plane t
-    const int64_t e = el + t * plane;
+  __device__ __forceinline__ void finish(int r, int t, const f32x4& c) {
+    // lane holds output channels 4g .. 4g+3 of its voxel in output plane t of row r
+    const int64_t e = voxel(r, t);
+    const int q = r * BX + t;
     float zv[4];
-    bf16x4 r;
-    if constexpr (RES) r = __builtin_bit_cast(bf16x4, make_uint2(early[2 * t], early[2 * t + 1]));
+    bf16x4 rv;
+    if constexpr (RES) rv = __builtin_bit_cast(bf16x4, make_uint2(early[2 * q], early[2 * q + 1]));
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       zv[i] = c[i] + bsv[i];
-      if constexpr (RES) zv[i] += (float)r[i];
+      if constexpr (RES) zv[i] += (float)rv[i];
     }
     if constexpr (EPI == kOutResidual) {
-      if (g == 0) rho_add[e] = gain * (__uint_as_float(early[t]) - 1.f) + amp * tanhf(zv[0]);
+      if (g == 0) rho_add[e] = gain * (__uint_as_float(early[q]) - 1.f) + amp * tanhf(zv[0]);
       return;
     }
     if (pre) {
@@ -189,26 +213,31 @@ struct PlaneWalk {
     }
   }
 
-  // plane P: its fragments are in frag[P % 2]; plane P + 1 is read into the other buffer
-  template <int P>
+  // staged plane P of row R: its fragments are in frag[P % 2]; plane P + 1 is read into the
+  // other buffer (and after the last plane of a row, plane 0 of the next row)
+  template <int R, int P>
   __device__ __forceinline__ void walk() {
-    if constexpr (P + 1 < kH) read_plane((P + 1) % 2, P + 1);
-    if constexpr (P < kB) acc[P % 3] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (P + 1 < Geo::SX)
+      read_plane((P + 1) % 2, R, P + 1);
+    else if constexpr (R + 1 < Geo::ROWS)
+      read_plane((P + 1) % 2, R + 1, 0);
+    if constexpr (P < BX) acc[P % 3] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       // output plane t = P - dx' takes dx' = P - t
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        constexpr int dummy = 0;
-        (void)dummy;
         const int t = P - d;
-        if (t >= 0 && t < kB)
+        if (t >= 0 && t < BX)
           acc[(P - d + 3) % 3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[d * KS + k], frag[P % 2][k],
                                                                        acc[(P - d + 3) % 3], 0, 0, 0);
       }
     }
-    if constexpr (P >= 2) finish(P - 2, acc[(P - 2) % 3]);
-    if constexpr (P + 1 < kH) walk<P + 1>();
+    if constexpr (P >= 2) finish(R, P - 2, acc[(P - 2) % 3]);
+    if constexpr (P + 1 < Geo::SX)
+      walk<R, P + 1>();
+    else if constexpr (R + 1 < Geo::ROWS)
+      walk<R + 1, 0>();  // the plane count per row is even, so frag parity carries over
   }
 };
 
@@ -216,18 +245,17 @@ struct PlaneWalk {
 // float32 pre-activations (when pre != null; the first pch of the 16 channels per voxel) and
 // bf16 activations (when act != null).
 //
-// A persistent grid walks the 8^3 output blocks; gridDim.x is a multiple of 8 and workgroup
-// w (on XCD w % 8) takes block it * G + (w % 8) * (G / 8) + w / 8 of iteration it, so the
-// blocks of one iteration on one XCD are consecutive (z neighbours share their halo planes in
-// that XCD's L2).  The weights stay in registers as MFMA A fragments for the whole kernel
+// A persistent grid walks the output blocks; gridDim.x is a multiple of 8 and workgroup w (on
+// XCD w % 8) takes block it * G + (w % 8) * (G / 8) + w / 8 of iteration it, so the blocks of
+// one iteration on one XCD are consecutive (z neighbours share their halo planes in that XCD's
+// L2).  The weights stay in registers as MFMA A fragments for the whole kernel
 // (D[cout][voxel] = W[cout][K] X[K][voxel]); the next block's input tile is loaded into
 // registers while the current block is computed from LDS, so the loads of one block are in
-// flight together and overlap the matrix work of the previous one.  The residual is read in
-// the epilogue (by then the prefetch has landed).  The staged tile is stored as one plane per
-// 8-channel half ([half][voxel][8]), so the 16-byte reads of an 8-lane group cover 128
-// contiguous bytes (no bank conflicts).  The global addresses of a block are the thread's
-// fixed offsets plus the block origin, except for blocks whose halo wraps around the mesh.
-template <int CIN, bool GELU, bool RES, int EPI = kPlain>
+// flight together and overlap the matrix work of the previous one.  The staged tile is stored
+// as one plane per 8-channel half ([half][voxel][8]).  The global addresses of a block are the
+// thread's fixed offsets plus the block origin, except for blocks whose halo wraps around the
+// mesh.
+template <int CIN, bool GELU, bool RES, int EPI = kPlain, int BZ = 8>
 __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
                                                           const float* __restrict__ bias,
                                                           const __bf16* __restrict__ res, float* __restrict__ pre,
@@ -235,15 +263,18 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
                                                           const float* __restrict__ rho = nullptr,
                                                           const float* __restrict__ stats = nullptr, float c = 0.f,
                                                           float r = 0.f, float* __restrict__ rho_add = nullptr) {
-  using Walk = PlaneWalk<CIN, GELU, RES, EPI>;
-  using Taps = PlaneTaps<CIN>;
+  using Geo = FwdGeom<BZ>;
+  using Walk = PlaneWalk<CIN, GELU, RES, EPI, BZ>;
+  using Taps = PlaneTaps<CIN, Geo::SZ>;
   constexpr int TPS = Taps::TPS, KS = Taps::KS;
-  constexpr int V = CIN / 8;        // 8-channel halves of a voxel (LDS planes)
-  constexpr int NS = kH * kH * kH;  // staged voxels
-  constexpr int ITEMS = NS * V;     // 16-byte vectors staged per block
+  constexpr int V = CIN / 8;  // 8-channel halves of a voxel (LDS planes)
+  constexpr int NS = Geo::NS, SZ = Geo::SZ, SY = Geo::SY;
+  constexpr int BX = Geo::BX, BY = Geo::BY;
+  constexpr int ITEMS = NS * V;  // 16-byte vectors staged per block
   constexpr int PER = (ITEMS + kThreads - 1) / kThreads;
   constexpr Taps taps{};
   static_assert(taps.ncls <= 4, "step classes");
+  static_assert(Geo::SX % 2 == 0, "even plane count (fragment buffer parity across rows)");
   __shared__ __align__(16) __bf16 xin[V * NS * 8];
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int m = lane % 16, g = lane / 16;
@@ -266,19 +297,23 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
   // weights and bias are in registers before the block loop starts (otherwise the wait for
   // them at the loop head also waits for the prefetch in flight)
   __builtin_amdgcn_s_waitcnt(0);
+  // this lane's voxel in the block's plane 0 (row 0): BZ = 8: y = 2 wave + m / 8, z = m % 8;
+  // BZ = 16: y = 2 wave (+ the row walked), z = m
+  const int ly = BZ == 8 ? 2 * wave + m / 8 : 2 * wave;
+  const int lz = BZ == 8 ? m % 8 : m;
   {
-    // this lane's output voxel: y = 2 wave + m / 8, z = m % 8 (staged corner of its taps)
-    const int corner = (2 * wave + m / 8) * kH + m % 8;
+    const int corner = ly * SZ + lz;  // staged corner of its taps
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int cc = 0; cc < 4; ++cc) {
       int d = 0;
 #pragma unroll
-      for (int t = 1; t < TPS; ++t) d = gt == t ? Taps::diff(taps.rep[c < taps.ncls ? c : 0], t) : d;
-      wk.cbase[c] = (gh * NS + corner + d) * 8;
+      for (int t = 1; t < TPS; ++t) d = gt == t ? Taps::diff(taps.rep[cc < taps.ncls ? cc : 0], t) : d;
+      wk.cbase[cc] = (gh * NS + corner + d) * 8;
     }
   }
   wk.xin = xin;
   wk.plane = (int64_t)n1 * n2;
+  wk.rowz = n2;
   wk.g = g;
   wk.res = res;
   wk.pre = pre;
@@ -291,18 +326,18 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
     wk.amp = r * stats[0] / c;
   }
   // this thread's staged vectors: element offset from the block's (-1, -1, -1) corner when
-  // the halo does not wrap (fits int32: the caller checks kH * n1 * n2 * CIN < 2^31)
+  // the halo does not wrap (fits int32: the caller checks SX * n1 * n2 * CIN < 2^31)
   typedef int Offsets __attribute__((ext_vector_type(PER)));
   Offsets loc;
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     const int it = min((int)threadIdx.x + u * kThreads, ITEMS - 1);
     const int v = it % V, vox = it / V;
-    const int z = vox % kH, y = (vox / kH) % kH, xx = vox / (kH * kH);
+    const int z = vox % SZ, y = (vox / SZ) % SY, xx = vox / (SZ * SY);
     loc[u] = ((xx * n1 + y) * n2 + z) * CIN + 8 * v;
   }
-  const int nbz = n2 / kB, nby = n1 / kB;
-  const int nblk = (n0 / kB) * nby * nbz;  // < 2^31: the caller checks the mesh size
+  const int nbz = n2 / BZ, nby = n1 / BY;
+  const int nblk = (n0 / BX) * nby * nbz;  // < 2^31: the caller checks the mesh size
   const int G = gridDim.x, per = G / 8, first = (blockIdx.x % 8) * per + blockIdx.x / 8;
   // iteration i loads block nxt into registers, computes block cur (staged in LDS by the
   // previous iteration) while those loads are in flight, then stages nxt
@@ -311,15 +346,15 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
   for (;;) {
     if (cur >= 0) {
       const int bz = cur % nbz, by = (cur / nbz) % nby, bx = cur / (nbz * nby);
-      const int64_t e0 = (((int64_t)bx * kB * n1 + by * kB) * n2 + bz * kB);  // block origin voxel
-      wk.el = e0 + (int64_t)(2 * wave + m / 8) * n2 + m % 8;
+      const int64_t e0 = (((int64_t)bx * BX * n1 + by * BY) * n2 + bz * BZ);  // block origin voxel
+      wk.el = e0 + (int64_t)ly * n2 + lz;
       wk.load_early();
     }
     Prefetch pf;
     if (nxt < nblk) {
       const int bz = nxt % nbz, by = (nxt / nbz) % nby, bx = nxt / (nbz * nby);
-      const int ox = bx * kB, oy = by * kB, oz = bz * kB;
-      const bool inner = ox > 0 && ox + kB < n0 && oy > 0 && oy + kB < n1 && oz > 0 && oz + kB < n2;
+      const int ox = bx * BX, oy = by * BY, oz = bz * BZ;
+      const bool inner = ox > 0 && ox + BX < n0 && oy > 0 && oy + BY < n1 && oz > 0 && oz + BZ < n2;
       if (inner) {
         const __bf16* base = x + (((int64_t)(ox - 1) * n1 + (oy - 1)) * n2 + (oz - 1)) * CIN;
 #pragma unroll
@@ -335,7 +370,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
         for (int u = 0; u < PER; ++u) {
           const int it = min((int)threadIdx.x + u * kThreads, ITEMS - 1);
           const int v = it % V, vox = it / V;
-          const int z = vox % kH, y = (vox / kH) % kH, xx = vox / (kH * kH);
+          const int z = vox % SZ, y = (vox / SZ) % SY, xx = vox / (SZ * SY);
           const int gx = wrap(ox + xx - 1, n0), gy = wrap(oy + y - 1, n1), gz = wrap(oz + z - 1, n2);
           const uint4 t = *reinterpret_cast<const uint4*>(x + (((int64_t)gx * n1 + gy) * n2 + gz) * CIN + 8 * v);
           pf[4 * u] = t.x;
@@ -346,8 +381,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
       }
     }
     if (cur >= 0) {
-      wk.read_plane(0, 0);
-      wk.template walk<0>();
+      wk.read_plane(0, 0, 0);
+      wk.template walk<0, 0>();
     }
     if (nxt >= nblk) break;
     __syncthreads();  // the MFMAs of block cur are done with the tile
@@ -655,6 +690,11 @@ int check_ptr16(const void* p, const char* what) {
   return PM_SUCCESS;
 }
 
+// the forward conv's 4 x 8 x 16 blocks (conflict-free fragment reads) when the mesh divides
+// into them; 8^3 blocks otherwise
+bool wide_blocks(const int32_t* shape) { return shape[0] % 4 == 0 && shape[2] % 16 == 0; }
+
+// blocks of 512 output voxels (8^3, or 4 x 8 x 16)
 unsigned nblocks(const int32_t* shape) {
   return (unsigned)((int64_t)(shape[0] / conv::kB) * (shape[1] / conv::kB) * (shape[2] / conv::kB));
 }
@@ -701,18 +741,29 @@ int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, 
                    (int64_t)shape[0] * shape[1] * shape[2] / (conv::kB * conv::kB * conv::kB) >= (int64_t(1) << 31),
                PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
   const int64_t nblk = nblocks(shape);
-  const int key = (cin == 16 ? 4 : 0) + (gelu ? 2 : 0) + (res ? 1 : 0);
+  const int key = (cin == 16 ? 4 : 0) + (gelu ? 2 : 0) + (res ? 1 : 0) + (wide_blocks(shape) ? 8 : 0);
   void (*kern)(const __bf16*, const __bf16*, const float*, const __bf16*, float*, __bf16*, int, int, int, int,
                const float*, const float*, float, float, float*) = nullptr;
   switch (key) {
-    case 0: kern = conv::k_conv_fwd<8, false, false>; break;
-    case 1: kern = conv::k_conv_fwd<8, false, true>; break;
-    case 2: kern = conv::k_conv_fwd<8, true, false>; break;
-    case 3: kern = conv::k_conv_fwd<8, true, true>; break;
-    case 4: kern = conv::k_conv_fwd<16, false, false>; break;
-    case 5: kern = conv::k_conv_fwd<16, false, true>; break;
-    case 6: kern = conv::k_conv_fwd<16, true, false>; break;
-    default: kern = conv::k_conv_fwd<16, true, true>; break;
+#define PM_CONV_CASE(K, C, G, R, Z) \
+  case K: kern = conv::k_conv_fwd<C, G, R, conv::kPlain, Z>; break;
+    PM_CONV_CASE(0, 8, false, false, 8)
+    PM_CONV_CASE(1, 8, false, true, 8)
+    PM_CONV_CASE(2, 8, true, false, 8)
+    PM_CONV_CASE(3, 8, true, true, 8)
+    PM_CONV_CASE(4, 16, false, false, 8)
+    PM_CONV_CASE(5, 16, false, true, 8)
+    PM_CONV_CASE(6, 16, true, false, 8)
+    PM_CONV_CASE(7, 16, true, true, 8)
+    PM_CONV_CASE(8, 8, false, false, 16)
+    PM_CONV_CASE(9, 8, false, true, 16)
+    PM_CONV_CASE(10, 8, true, false, 16)
+    PM_CONV_CASE(11, 8, true, true, 16)
+    PM_CONV_CASE(12, 16, false, false, 16)
+    PM_CONV_CASE(13, 16, false, true, 16)
+    PM_CONV_CASE(14, 16, true, false, 16)
+    default: kern = conv::k_conv_fwd<16, true, true, conv::kPlain, 16>; break;
+#undef PM_CONV_CASE
   }
   unsigned grid = 0;
   PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblk, &grid));
@@ -752,7 +803,8 @@ int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, cons
                    (int64_t)shape[0] * shape[1] * shape[2] / (conv::kB * conv::kB * conv::kB) >= (int64_t(1) << 31),
                PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
   hipStream_t s = as_stream(stream);
-  auto kern = conv::k_conv_fwd<16, false, false, conv::kOutResidual>;
+  auto kern = wide_blocks(shape) ? conv::k_conv_fwd<16, false, false, conv::kOutResidual, 16>
+                                 : conv::k_conv_fwd<16, false, false, conv::kOutResidual, 8>;
   unsigned grid = 0;
   PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, static_cast<const __bf16*>(x),

@@ -48,11 +48,12 @@ def test_forward_matches_torch(device, cin, shape, gelu):
     np.testing.assert_array_equal(pre1[..., 0].cpu().numpy(), pre[..., 0].cpu().numpy())
 
 
+@pytest.mark.parametrize("shape", [(128, 64, 64), (160, 64, 40)])
 @pytest.mark.parametrize("cin,res", [(16, True), (8, False)])
-def test_forward_persistent_grid_walks_every_block(device, cin, res):
-    """A mesh of 1024 blocks of 8^3: more blocks than workgroups resident on the device, so
-    each workgroup of the persistent grid takes several blocks (and the grid a ragged tail)."""
-    shape = (128, 64, 64)
+def test_forward_persistent_grid_walks_every_block(device, cin, res, shape):
+    """Meshes of 1024 blocks of 4 x 8 x 16 (z a multiple of 16) and of 800 blocks of 8^3 (z
+    not): more blocks than workgroups resident on the device, so each workgroup of the
+    persistent grid takes several blocks (and the grid a ragged tail), in both block shapes."""
     g = torch.Generator().manual_seed(11 + cin)
     x = _bf(torch.randn(shape + (cin,), generator=g))
     w = _bf(torch.randn(27, 16, cin, generator=g) * 0.2)
