@@ -298,6 +298,55 @@ def network_run(args):
     }), flush=True)
 
 
+def vjp_run(args):
+    """The reverse of the force step (``pm_gravity_vjp``: forward meshes recomputed, the gather
+    transpose, the spectral adjoint, the scatter adjoint) on the headline workload with a
+    random acceleration cotangent, ``--steps`` times after ``--warmup``.  Prints one JSON line
+    with the per-kernel times from the library's HIP events (not the headline metric)."""
+    import torch
+
+    from pmpp_amd import Configuration, _lib
+    from pmpp_amd._torch import ensure_init, stream_handle
+    from pmpp_amd.initial_conditions import LCDM, particles_2lpt
+    from pmpp_amd.nbody.gravity import gravity_vjp
+
+    dev = torch.device("cuda", 0)
+    ng, ratio, box = CONFIGS[args.config]
+    cosmo = LCDM()
+    conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio)
+    ptcl = particles_2lpt(conf, cosmo, seed=0, jitter_seed=1, device=dev, a=1.0)
+    ensure_init(dev)
+    g = torch.Generator(device=dev).manual_seed(5)
+    cot = torch.randn((len(ptcl), 3), device=dev, generator=g)
+    for _ in range(args.warmup):
+        gravity_vjp(ptcl, cosmo, conf, cot)
+    torch.cuda.synchronize()
+    stream = stream_handle(dev)
+    cap = 64 * args.steps + 16
+    _lib.call("pm_timing_enable", cap)
+    _lib.call("pm_timing_mark", b"begin", stream)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gravity_vjp(ptcl, cosmo, conf, cot)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (ctypes.c_float * cap)()
+    names = (ctypes.c_char_p * cap)()
+    cnt = ctypes.c_int32(0)
+    _lib.call("pm_timing_read", ms, names, cap, ctypes.byref(cnt))
+    _lib.call("pm_timing_enable", 0)
+    per = {}
+    for i in range(cnt.value):
+        k = names[i].decode()
+        per[k] = per.get(k, 0.0) + ms[i] / args.steps
+    print(json.dumps({
+        "metric": "force-step VJP time (acc cotangent -> disp and Omega_m cotangents)",
+        "ms_per_vjp": (t1 - t0) * 1e3 / args.steps, "particles": len(ptcl), "steps": args.steps,
+        "kernels_ms": {k: round(v, 4) for k, v in sorted(per.items(), key=lambda kv: -kv[1])},
+        "config": {"workload": f"PM force step VJP: {ng}^3 particles, {ng * ratio}^3 mesh, {box:g} Mpc/h"},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,6 +360,8 @@ def main():
     ap.add_argument("--run-steps", type=int, default=20)
     ap.add_argument("--network", action="store_true",
                     help="time the learned correction's inference network on the workload's coarse mesh instead")
+    ap.add_argument("--vjp", action="store_true",
+                    help="time the force step's VJP on the workload instead")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (the known-size kernel that "
                          "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -329,6 +380,10 @@ def main():
 
     if args.network:
         network_run(args)
+        return
+
+    if args.vjp:
+        vjp_run(args)
         return
 
     if args.copy_calibration:

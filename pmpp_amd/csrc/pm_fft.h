@@ -304,9 +304,14 @@ struct ColPass {
   // expand the band to full rows, the rows outside it are zero), 0 = output (forward passes).
   int keep = -1, compact_in = 0;
   int ky_split = 0x7fffffff, ky_shift = 0;
+  // kColAdj: the y and z components' spectra (in is the x component's)
+  const float2* in1 = nullptr;
+  const float2* in2 = nullptr;
 };
 
-enum ColMode { kColFwd = 0, kColInv = 1, kColInv2 = 2, kColGreen = 3 };
+// kColAdj: the x pass of the force step's spectral adjoint: forward transforms of the three
+// cotangent spectra, out = mask * (-1/k^2) * sum_i conj(D_i) in_i, inverse transform
+enum ColMode { kColFwd = 0, kColInv = 1, kColInv2 = 2, kColGreen = 3, kColAdj = 4 };
 
 // Line pass along the contiguous last axis (real <-> half spectrum, split main + Nyquist).
 struct ZPass {

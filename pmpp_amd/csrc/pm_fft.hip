@@ -102,7 +102,47 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
   auto idx = [&](int row) { return row * T + col + (row / N2) * CF::PADK; };
 
   float2 v[E];
-  if constexpr (MODE == kColFwd || MODE == kColGreen) {
+  if constexpr (MODE == kColAdj) {
+    // Spectral adjoint of the force step, x pass (the columns are kz, the batches the packed ky
+    // rows, as for kColGreen): sum_i conj(D_i) hat g_i with conj(i negk) = -i negk, then
+    // mask * (-1/k^2), then the inverse x transform.  The y and z symbols are constant along
+    // an x column, so by linearity those two components are combined before their transform:
+    // U = X(negk_y g_y + negk_z g_z), then s = -i (negk_x X(g_x) + U); two lines live per
+    // thread instead of three.
+    int ky = (int)(b * io.ky_b) + cgl * io.ky_c;
+    ky += ky > io.ky_split ? io.ky_shift : 0;
+    int kz = (int)(b * io.kz_b) + cgl * io.kz_c + io.kz0;
+    const float kyv = tb.k[1][ky], kzv = tb.k[2][kz];
+    const float nky = tb.negk[1][ky], nkz = tb.negk[2][kz];
+    const bool keep_yz = cok && tb.mask[1][ky] != 0.f && tb.mask[2][kz] != 0.f;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int64_t o = lbase + (int64_t)nrow<N>(p, s) * io.s_row;
+      const float2 gy = io.in1[o], gz = io.in2[o];
+      v[s] = make_float2(nky * gy.x + nkz * gz.x, nky * gy.y + nkz * gz.y);
+    }
+    forward_core<N>(v, my, p, idx, tw);
+    asm volatile("" ::: "memory");  // the x component's loads after U's transform
+    float2 w[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) w[s] = io.in[lbase + (int64_t)nrow<N>(p, s) * io.s_row];
+    forward_core<N>(w, my, p, idx, tw);
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const float kxv = spec_k(xrow<N>(p, s), N, false, tb.period);
+      const float nkx = spec_negk(kxv, tb);
+      const float tx = nkx * w[s].x + v[s].x, ty = nkx * w[s].y + v[s].y;  // s = -i t
+      const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+      const bool keep = keep_yz && spec_keep(kxv, tb) && k2 != 0.f;
+      v[s] = keep ? make_float2(-ty / k2, tx / k2) : make_float2(0.f, 0.f);  // -s / k^2
+    }
+    inverse_core<N>(v, my, p, idx, tw);
+    if (cok) {
+#pragma unroll
+      for (int s = 0; s < E; ++s) io.out0[obase + (int64_t)nrow<N>(p, s) * o_row] = v[s];
+    }
+    return;
+  } else if constexpr (MODE == kColFwd || MODE == kColGreen) {
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = io.in[lbase + (int64_t)nrow<N>(p, s) * io.s_row];
     forward_core<N>(v, my, p, idx, tw);
@@ -371,6 +411,7 @@ int launch_cols_n(int mode, const ColPass& cp, int batches, const SpecTables& tb
     case kColInv: return launch_cols_cfg<N, T1, kColInv, 1>(cp, batches, tb, dax, s, "fft_cols_inv");
     case kColInv2: return launch_cols_cfg<N, T2, kColInv2, 2>(cp, batches, tb, dax, s, "fft_cols_inv2");
     case kColGreen: return launch_cols_cfg<N, T2, kColGreen, 2>(cp, batches, tb, dax, s, "fft_cols_green");
+    case kColAdj: return launch_cols_cfg<N, T1, kColAdj, 1>(cp, batches, tb, dax, s, "fft_cols_adjoint");
     default: return fail(PM_ERROR_INVALID_ARGUMENT, "bad column mode");
   }
 }

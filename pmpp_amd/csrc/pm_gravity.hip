@@ -680,28 +680,6 @@ __global__ void __launch_bounds__(kBlock) k_scatter_bwd_sorted(
   for (int a = 0; a < 3; ++a) disp_cot[i * 3 + a] = disp_cot_s[j * 3 + a] + dc[a] / g.a1;
 }
 
-// src_hat <- G(k) * sum_i conj(D_i(k)) * hat_i on a pitched half spectrum.
-__global__ void k_adjoint_combine(float2* __restrict__ a, const float2* __restrict__ b,
-                                  const float2* __restrict__ c, int nx, int ny, int nzh, int64_t pitch,
-                                  fft::SpecTables tb) {
-  int64_t total = (int64_t)nx * ny * nzh;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int kz = (int)(t % nzh);
-    int ky = (int)((t / nzh) % ny);
-    int kx = (int)(t / ((int64_t)nzh * ny));
-    int64_t i = ((int64_t)kx * ny + ky) * pitch + kz;
-    float kxv = tb.k[0][kx], kyv = tb.k[1][ky], kzv = tb.k[2][kz];
-    float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-    bool keep = tb.mask[0][kx] != 0.f && tb.mask[1][ky] != 0.f && tb.mask[2][kz] != 0.f && k2 != 0.f;
-    // conj(i*negk) = -i*negk
-    float2 s = fft::c_imul(-tb.negk[0][kx], a[i]);
-    s = fft::c_add(s, fft::c_imul(-tb.negk[1][ky], b[i]));
-    s = fft::c_add(s, fft::c_imul(-tb.negk[2][kz], c[i]));
-    a[i] = keep ? make_float2(-s.x / k2, -s.y / k2) : make_float2(0.f, 0.f);
-  }
-}
-
 // omega_cot += 1.5 * sum_x src_cot(x) * (rho(x) [+ rho_add(x)] - 1); dens_cot = src_cot * scale,
 // in place, and copied to rho_cot when given (the cotangent of an added density)
 __global__ void __launch_bounds__(kBlock) k_source_bwd(float* __restrict__ src_cot,
@@ -849,11 +827,11 @@ bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const in
   b.r0 = ws.take<float>(L.nreal);
   b.r1 = ws.take<float>(L.nreal);
   b.r2 = ws.take<float>(L.nreal);
-  // banded force pipeline (see force_mesh_stage): s1-sized ca, cc, ce and s2-sized cb, cd;
-  // the reverse pass also runs full-spectrum FFTs in ca, cb and cc
-  b.ca = ws.take<float2>(reverse ? L.ncplx : L.s1);
-  b.cb = ws.take<float2>(reverse ? L.ncplx : L.s2);
-  b.cc = ws.take<float2>(reverse ? L.ncplx : L.s1);
+  // banded force pipeline (see force_mesh_stage and adjoint_spectral_stage): s1-sized ca,
+  // cc, ce and s2-sized cb, cd
+  b.ca = ws.take<float2>(L.s1);
+  b.cb = ws.take<float2>(L.s2);
+  b.cc = ws.take<float2>(L.s1);
   b.cd = ws.take<float2>(L.s2);
   b.ce = ws.take<float2>(L.s1);
   if (reverse) {
@@ -1055,6 +1033,77 @@ int force_mesh_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b, cons
   return PM_SUCCESS;
 }
 
+// Spectral adjoint of the force meshes: src_cot (b.g1, real) = irfftn(G * sum_i conj(D_i) *
+// rfftn(g_i)) for the three cotangent meshes g_i (b.g1, b.g2, b.g3; the symbol is Hermitian, so
+// the transpose of irfftn(H rfftn(.)) is irfftn(conj(H) rfftn(.))).  G * D_i vanishes outside
+// the particle-Nyquist band, so the passes are force_mesh_stage's in reverse on the same band:
+// three z r2c passes storing the kz band, three y passes packing the |ky| band, one x pass
+// that transforms the three spectra, combines them mode by mode and inverts (kColAdj), the y
+// inverse expanding the band and the z c2r.  Buffers: ca, cc, ce (s1) and cb, cd (s2).
+int adjoint_spectral_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b, const fft::SpecTables& tb,
+                           hipStream_t s) {
+  using namespace fft;
+  const bool zband = L.nkz <= L.m;
+  const bool ypack = L.ky_keep >= 0;
+  const int64_t p1 = L.pitch1;
+  const int ncols = L.nkz;
+  float* reals[3] = {b.g1, b.g2, b.g3};
+  float2* s1buf[3] = {b.ca, b.cc, b.ce};
+  ZPass zr{};
+  zr.pitch_r = L.nz;
+  zr.pitch_c = p1;
+  zr.nyq_stride = p1;
+  zr.nlines = (int64_t)L.nx * L.ny;
+  zr.src_shift = 0.f;
+  zr.src_scale = 1.f;
+  zr.kz_lim = L.nkz;
+  zr.nyq = zband ? 0 : 1;
+  for (int c = 0; c < 3; ++c) {
+    zr.rin = reals[c];
+    zr.cmain = s1buf[c];
+    zr.cnyq = zband ? s1buf[c] : s1buf[c] + L.m;
+    PM_TRY(launch_z(L.nz, false, zr, tb, s));
+  }
+  // y forward, packed: ca -> cb, cc -> cd, then ce -> ca (ca is free again; s2 <= s1)
+  float2* s2buf[3] = {b.cb, b.cd, b.ca};
+  for (int c = 0; c < 3; ++c) {
+    ColPass yf{s1buf[c], s2buf[c], nullptr, p1, 1, (int64_t)L.ny * p1, ncols, 0, 0, 0, 0, 0};
+    yf.o_row = p1;
+    yf.o_batch = (int64_t)L.nky * p1;
+    yf.keep = L.ky_keep;
+    PM_TRY(launch_cols(L.ny, kColFwd, yf, L.nx, tb, 1, s));
+  }
+  // x: three forward transforms, the combined symbol, one inverse -> cc (packed band)
+  ColPass xa{s2buf[0], b.cc, nullptr, (int64_t)L.nky * p1, 1, p1, ncols, 1, 0, 0, 1, 0};
+  xa.in1 = s2buf[1];
+  xa.in2 = s2buf[2];
+  if (ypack) {
+    xa.ky_split = L.ky_keep;
+    xa.ky_shift = L.ny - L.nky;
+  }
+  PM_TRY(launch_cols(L.nx, kColAdj, xa, L.nky, tb, 0, s));
+  // y inverse expanding the band: cc -> ce
+  ColPass yi{b.cc, b.ce, nullptr, p1, 1, (int64_t)L.nky * p1, ncols, 0, 0, 0, 0, 0};
+  yi.o_row = p1;
+  yi.o_batch = (int64_t)L.ny * p1;
+  yi.keep = L.ky_keep;
+  yi.compact_in = 1;
+  PM_TRY(launch_cols(L.ny, kColInv, yi, L.nx, tb, 1, s));
+  ZPass zc{};
+  zc.pitch_r = L.nz;
+  zc.pitch_c = p1;
+  zc.nyq_stride = p1;
+  zc.nlines = (int64_t)L.nx * L.ny;
+  zc.out_scale = (float)(2.0 / ((double)L.nx * L.ny * L.nz));
+  zc.kz_lim = L.nkz;
+  zc.nyq = zband ? 0 : 1;
+  zc.cmain = b.ce;
+  zc.cnyq = zband ? b.ce : b.ce + L.m;
+  zc.rout = b.g1;
+  zc.deriv = 0;
+  return launch_z(L.nz, true, zc, tb, s);
+}
+
 int prepare(const pm_gravity_desc* d, int64_t n, void* workspace, size_t workspace_bytes_, bool reverse,
             Layout& L, Buffers& b, Arena& ws, fft::SpecTables& tb, hipStream_t s) {
   PM_TRY(check_desc(d, n));
@@ -1219,23 +1268,8 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
       PM_LAUNCHED("k_gather3_bwd_sorted", s);
     }
   }
-  // spectral adjoint: src_cot = irfftn(G * sum_i conj(D_i) rfftn(g_i))  (the symbol is
-  // Hermitian, so the transpose of irfftn(H rfftn(.)) is irfftn(conj(H) rfftn(.)))
-  fft::SpecTables none{};
-  st = fft::rfft3d_pitched(b.g1, b.ca, desc->mesh_shape, L.pitch, 0.f, 1.f, none, s);
-  if (st) return st;
-  st = fft::rfft3d_pitched(b.g2, b.cb, desc->mesh_shape, L.pitch, 0.f, 1.f, none, s);
-  if (st) return st;
-  st = fft::rfft3d_pitched(b.g3, b.cc, desc->mesh_shape, L.pitch, 0.f, 1.f, none, s);
-  if (st) return st;
-  {
-    int64_t total = (int64_t)L.nx * L.ny * (L.m + 1);
-    unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(total, 256), 65536);
-    hipLaunchKernelGGL(grav::k_adjoint_combine, dim3(blocks), dim3(256), 0, s, b.ca, b.cb, b.cc, L.nx, L.ny,
-                       L.m + 1, L.pitch, tb);
-    PM_LAUNCHED("k_adjoint_combine", s);
-  }
-  st = fft::irfft3d_pitched(b.ca, b.g1, desc->mesh_shape, L.pitch, none, s);
+  // spectral adjoint: src_cot (g1) = irfftn(G * sum_i conj(D_i) rfftn(g_i)) on the band
+  st = grav::adjoint_spectral_stage(desc, L, b, tb, s);
   if (st) return st;
   {
     const float coef = 1.5f * desc->omega_m;
