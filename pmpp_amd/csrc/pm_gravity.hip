@@ -569,6 +569,110 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_columns(
   }
 }
 
+// The displacement cotangent of the force step, tiled like k_gather_columns: for sorted
+// particle j, dc = sum_m coef_m(j) sum_c M_m(c) grad W_c(j) with the meshes M = (F_x, F_y, F_z,
+// dens_cot) and coef = (acc_cot of j, x, y, z, from the sorted component-major copies; the
+// particle value), written as dc / a1 through perm: the reverse of the three gathers
+// (pallas.py:321-355) and the scatter adjoint's displacement term (pallas.py:358-409) in one
+// pass.  grad W_c = (s_x w_y w_z, w_x s_y w_z, w_x w_y s_z) with s = sign(-(f - b)), sign(0) =
+// 0 (the corner_grad convention).  Each mesh's tile is staged in turn; a particle's partial sums
+// wait in its thread's own LDS slots.  With a zero particle value the density term is skipped.
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kTileThreads) k_disp_cot_columns(
+    const int16_t* __restrict__ pmid, const float* __restrict__ disp, const int32_t* __restrict__ bin_off,
+    const int32_t* __restrict__ perm, Geom g, Bricks k, const float* __restrict__ f0, const float* __restrict__ f1,
+    const float* __restrict__ f2, const float* __restrict__ dcot, const float* __restrict__ cot_s, int64_t n,
+    float val, float* __restrict__ disp_cot) {
+  constexpr int kLds = BX ? (BX + 1) * (BY + 1) * (BZ + 2) : kTileMaxFloats;
+  constexpr int kPl = BX ? (BX + 1) * (BY + 1) : kPlaneMax;
+  __shared__ __align__(16) float tile[kLds];
+  __shared__ float res[3][kGatherCap];
+  __shared__ float plane[2][4][kPl];  // [parity of bz][mesh]: lower z plane of brick bz
+  const BrickShape<BX, BY, BZ> B(k);
+  const int col = column_of(blockIdx.x, gridDim.x);
+  const int cx = col / k.nb[1], cy = col - cx * k.nb[1];
+  const int rows = B.rows(), P = B.pitch(), nb2 = k.nb[2];
+  const int nm = val != 0.f ? 4 : 3;
+  // the brick above the top one is brick 0 (periodic)
+  for (int m = 0; m < nm; ++m)
+    load_plane(plane[nb2 & 1][m], m == 0 ? f0 : (m == 1 ? f1 : (m == 2 ? f2 : dcot)), B, k, cx, cy, 0);
+  for (int bz = nb2 - 1; bz >= 0; --bz) {
+    const int o[3] = {cx * B.b0, cy * B.b1, bz * B.b2};
+    const int64_t bid = (int64_t)col * nb2 + bz;
+    const int64_t j0 = bin_off[bid], j1 = bin_off[bid + 1];
+    const int up = (bz + 1) & 1, me = bz & 1;
+    if (j0 == j1) {  // no particles: only this brick's lower planes are needed (by the brick below)
+      __syncthreads();
+      for (int m = 0; m < nm; ++m)
+        load_plane(plane[me][m], m == 0 ? f0 : (m == 1 ? f1 : (m == 2 ? f2 : dcot)), B, k, cx, cy, bz);
+      continue;
+    }
+    for (int64_t s0 = j0; s0 < j1; s0 += kGatherCap) {
+      const int64_t s1 = s0 + kGatherCap < j1 ? s0 + kGatherCap : j1;
+#pragma unroll 1
+      for (int m = 0; m < nm; ++m) {
+        const float* __restrict__ F = m == 0 ? f0 : (m == 1 ? f1 : (m == 2 ? f2 : dcot));
+        __syncthreads();  // the previous mesh's pass is done with the tile
+        stage_rows<BX, BY, BZ>(tile, F, B, k, cx, cy, bz);
+        for (int r = threadIdx.x; r < rows; r += kTileThreads) tile[r * P + B.b2] = plane[up][m][r];
+        __syncthreads();
+        for (int r = threadIdx.x; r < rows; r += kTileThreads) plane[me][m][r] = tile[r * P];
+#pragma unroll
+        for (int h = 0; h < kGatherPPT; h += kGatherBatch) {
+          int bs[kGatherBatch][3];
+          float fr[kGatherBatch][3];
+          float cf[kGatherBatch];
+#pragma unroll
+          for (int u = 0; u < kGatherBatch; ++u) {  // the batch's particle loads first (clamped)
+            int64_t j = s0 + threadIdx.x + (int64_t)(h + u) * kTileThreads;
+            j = j < s1 ? j : s1 - 1;
+            base_frac(pmid, disp, j, g.a1, k.n, bs[u], fr[u]);
+            cf[u] = m < 3 ? cot_s[(int64_t)m * n + j] : val;
+          }
+#pragma unroll
+          for (int u = 0; u < kGatherBatch; ++u) {
+            const int jl = threadIdx.x + (h + u) * kTileThreads;
+            if (s0 + jl >= s1) continue;
+            const int c0i = local_cell(B, bs[u], o);
+            float w[3][2], sg[3][2];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+#pragma unroll
+              for (int bb = 0; bb < 2; ++bb) {
+                const float delta = fr[u][a] - (float)bb;
+                w[a][bb] = 1.f - fabsf(delta);
+                sg[a][bb] = cic_sign(delta);
+              }
+            }
+            float gs[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+              const int bx = c & 1, by = (c >> 1) & 1, bzz = (c >> 2) & 1;
+              const float v = tile[c0i + B.corner_off(c)];
+              gs[0] = gs[0] + v * (sg[0][bx] * w[1][by] * w[2][bzz]);
+              gs[1] = gs[1] + v * (w[0][bx] * sg[1][by] * w[2][bzz]);
+              gs[2] = gs[2] + v * (w[0][bx] * w[1][by] * sg[2][bzz]);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) res[a][jl] = (m == 0 ? 0.f : res[a][jl]) + cf[u] * gs[a];
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kGatherPPT; ++q) {
+        const int jl = threadIdx.x + q * kTileThreads;
+        if (s0 + jl < s1) {
+          const int64_t i = perm[s0 + jl];
+          disp_cot[i * 3 + 0] = res[0][jl] / g.a1;
+          disp_cot[i * 3 + 1] = res[1][jl] / g.a1;
+          disp_cot[i * 3 + 2] = res[2][jl] / g.a1;
+        }
+      }
+    }
+  }
+}
+
 // invalid particles (sorted last, past the last brick) get a zero acceleration
 __global__ void k_zero_invalid(const int32_t* __restrict__ perm, const int32_t* __restrict__ bin_off,
                                int64_t nbins, int64_t n, float* __restrict__ acc) {
@@ -839,7 +943,7 @@ bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const in
     b.g1 = ws.take<float>(L.nreal);
     b.g2 = ws.take<float>(L.nreal);
     b.g3 = ws.take<float>(L.nreal);
-    b.disp_cot_s = ws.take<float>(3 * n);
+    b.disp_cot_s = b.tiled ? nullptr : ws.take<float>(3 * n);
     b.cot_s = b.tiled ? ws.take<float>(3 * n) : nullptr;
     b.cot_pmax = b.tiled ? ws.take<float>(kCotBlocks + 1) : nullptr;
   }
@@ -948,6 +1052,31 @@ int gather_stage(const int16_t* pmid_s, const float* disp_s, const uint8_t* vali
   hipLaunchKernelGGL(k_gather3_sorted, dim3(grid_for(n)), dim3(kBlock), 0, s, pmid_s, disp_s, valid_s, b.perm, n, g,
                      fx, fy, fz, acc);
   PM_LAUNCHED("k_gather3_sorted", s);
+  return PM_SUCCESS;
+}
+
+// Tiled displacement cotangent of the reverse pass: F = (b.g0, b.r1, b.r2), the density
+// cotangent in b.g1, the sorted acc_cot copies in b.cot_s; particles outside every brick (the
+// invalid ones) get zero.
+int disp_cot_tiled(const pm_gravity_desc* d, const Buffers& b, int64_t n, const uint8_t* valid, float* disp_cot,
+                   hipStream_t s) {
+  const Geom g = step_geom(d);
+  Bricks k = make_bricks(d->mesh_shape, d->brick);
+  const int64_t nbins = (int64_t)k.nb[0] * k.nb[1] * k.nb[2];
+  const unsigned ncols = (unsigned)((int64_t)k.nb[0] * k.nb[1]);
+  const float val = d->particle_value;
+  if (is_brick(k, 16, 16, 32))
+    hipLaunchKernelGGL((k_disp_cot_columns<16, 16, 32>), dim3(ncols), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
+                       b.bins, b.perm, g, k, b.g0, b.r1, b.r2, b.g1, b.cot_s, n, val, disp_cot);
+  else
+    hipLaunchKernelGGL((k_disp_cot_columns<0, 0, 0>), dim3(ncols), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
+                       b.bins, b.perm, g, k, b.g0, b.r1, b.r2, b.g1, b.cot_s, n, val, disp_cot);
+  PM_LAUNCHED("k_disp_cot_columns", s);
+  if (valid) {
+    unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), 4096);
+    hipLaunchKernelGGL(k_zero_invalid, dim3(blocks), dim3(kBlock), 0, s, b.perm, b.bins, nbins, n, disp_cot);
+    PM_LAUNCHED("k_zero_invalid", s);
+  }
   return PM_SUCCESS;
 }
 
@@ -1236,13 +1365,9 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
   // gather^T: acc_cot scattered into three cotangent meshes, displacement cotangent from F.
   // With brick tiling each component goes through the tiled scatter (sorted, scaled copies of
   // acc_cot); otherwise through float atomics beside the displacement cotangent.
+  // The tiled path takes the displacement cotangent after the spectral adjoint, in one pass
+  // over F and the density cotangent.
   if (b.tiled) {
-    if (n > 0) {
-      hipLaunchKernelGGL(grav::k_gather3_bwd_sorted<false>, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s,
-                         b.pmid_s, b.disp_s, b.valid_s, b.perm, n, g, b.g0, b.r1, b.r2, acc_cot, nullptr, nullptr,
-                         nullptr, b.disp_cot_s);
-      PM_LAUNCHED("k_gather3_bwd_sorted", s);
-    }
     const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, grav::kBlock), grav::kCotBlocks));
     hipLaunchKernelGGL(grav::k_cot_sorted, dim3(nb), dim3(grav::kBlock), 0, s, b.perm, acc_cot, n, b.cot_s,
                        b.cot_pmax);
@@ -1278,6 +1403,7 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
                        omega_cot);
     PM_LAUNCHED("k_source_bwd", s);
   }
+  if (n > 0 && b.tiled) return grav::disp_cot_tiled(desc, b, n, valid, disp_cot, s);
   if (n > 0) {
     hipLaunchKernelGGL(grav::k_scatter_bwd_sorted, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s, b.pmid_s,
                        b.disp_s, b.valid_s, b.perm, n, g, desc->particle_value, b.g1, b.disp_cot_s, disp_cot);

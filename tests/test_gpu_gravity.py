@@ -153,6 +153,44 @@ def test_force_vjp_matches_oracle(device, ng, ratio):
     assert float(omega.grad) == pytest.approx(oref, rel=1e-3)
 
 
+@pytest.mark.parametrize("brick,clustered,valid_frac", [
+    ((8, 8, 8), False, 0.8),    # runtime-shape tiled reverse, invalid particles zeroed
+    ((16, 16, 32), False, None),  # compile-time tiled reverse
+    ((16, 16, 32), True, 0.9),  # one brick holds several 2048-particle chunks
+    ((3, 3, 3), False, 0.8),    # brick does not divide the mesh: untiled reverse
+])
+def test_force_vjp_across_brick_tilings(device, brick, clustered, valid_frac):
+    conf = Configuration(2.0, (16, 16, 16), mesh_shape=2, brick=brick)
+    rng = np.random.default_rng(23)
+    n = 16 ** 3
+    if clustered:
+        pm = rng.integers(0, 12, (n, 3))
+        far = rng.random(n) < 0.1
+        pm[far] = rng.integers(0, 32, (int(far.sum()), 3))
+        base = Particles(conf, torch.as_tensor(pm.astype(np.int16), device=device),
+                         torch.as_tensor(rng.uniform(-1.5, 1.5, (n, 3)).astype(np.float32), device=device))
+    else:
+        base = _ptcl(conf, 0.6, 29, device)
+    unused = None if valid_frac is None else torch.as_tensor(rng.random(n) > valid_frac, device=device)
+    ptcl = base.replace(unused_index=unused)
+    disp = ptcl.disp.clone().requires_grad_(True)
+    omega = torch.tensor(0.3, device=device, requires_grad=True)
+    acc = gravity(1.0, ptcl.replace(disp=disp), type("C", (), {"Omega_m": omega})(), conf)
+    ac = rng.normal(size=acc.shape).astype(np.float32)
+    (acc * torch.as_tensor(ac, device=device)).sum().backward()
+    masks = spectral.particle_nyquist_masks(conf.mesh_shape, conf.ptcl_grid_shape, conf.ptcl_spacing,
+                                            conf.cell_size)
+    valid = None if unused is None else ~unused.cpu().numpy()
+    dref, oref = spectral.gravity_vjp(ptcl.pmid.cpu().numpy().astype(np.int64),
+                                      ptcl.disp.cpu().numpy().astype(np.float64), 0.3, ac, conf.mesh_shape,
+                                      conf.cell_size, conf.ptcl_num, masks, valid=valid)
+    got = disp.grad.cpu().numpy()
+    assert np.linalg.norm(got - dref) / np.linalg.norm(dref) < 1e-3
+    assert float(omega.grad) == pytest.approx(oref, rel=1e-3)
+    if valid is not None:
+        assert np.all(got[~valid] == 0)
+
+
 def test_particle_subsets_empty_sets_and_all_invalid(device):
     # a particle count that is not a whole lattice (odd, bucketed in plain index chunks), the
     # empty set, and a set whose particles are all invalid
