@@ -525,6 +525,17 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_columns(
     }
     for (int64_t s0 = j0; s0 < j1; s0 += kGatherCap) {
       const int64_t s1 = s0 + kGatherCap < j1 ? s0 + kGatherCap : j1;
+      // each particle's tile cell and fractions, once for the three components (clamped
+      // loads; their latency overlaps the first tile's staging)
+      int c0[kGatherPPT];
+      float fr[kGatherPPT][3];
+#pragma unroll
+      for (int q = 0; q < kGatherPPT; ++q) {
+        int64_t j = s0 + threadIdx.x + (int64_t)q * kTileThreads;
+        int bs[3];
+        base_frac(pmid, disp, j < s1 ? j : s1 - 1, g.a1, k.n, bs, fr[q]);
+        c0[q] = local_cell(B, bs, o);
+      }
 #pragma unroll 1
       for (int comp = 0; comp < 3; ++comp) {
         const float* __restrict__ F = comp == 0 ? f0 : (comp == 1 ? f1 : f2);
@@ -534,24 +545,13 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_columns(
         __syncthreads();
         for (int r = threadIdx.x; r < rows; r += kTileThreads) plane[me][comp][r] = tile[r * P];
 #pragma unroll
-        for (int h = 0; h < kGatherPPT; h += kGatherBatch) {
-          int bs[kGatherBatch][3];
-          float fr[kGatherBatch][3];
+        for (int q = 0; q < kGatherPPT; ++q) {
+          const int jl = threadIdx.x + q * kTileThreads;
+          if (s0 + jl >= s1) continue;
+          float sum = 0.f;
 #pragma unroll
-          for (int u = 0; u < kGatherBatch; ++u) {  // the batch's particle loads first (clamped)
-            int64_t j = s0 + threadIdx.x + (int64_t)(h + u) * kTileThreads;
-            base_frac(pmid, disp, j < s1 ? j : s1 - 1, g.a1, k.n, bs[u], fr[u]);
-          }
-#pragma unroll
-          for (int u = 0; u < kGatherBatch; ++u) {
-            const int jl = threadIdx.x + (h + u) * kTileThreads;
-            if (s0 + jl >= s1) continue;
-            const int c0i = local_cell(B, bs[u], o);
-            float sum = 0.f;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) sum = sum + tile[c0i + B.corner_off(c)] * frac_weight(fr[u], c);
-            res[comp][jl] = sum;
-          }
+          for (int c = 0; c < 8; ++c) sum = sum + tile[c0[q] + B.corner_off(c)] * frac_weight(fr[q], c);
+          res[comp][jl] = sum;
         }
       }
       __syncthreads();
@@ -609,6 +609,16 @@ __global__ void __launch_bounds__(kTileThreads) k_disp_cot_columns(
     }
     for (int64_t s0 = j0; s0 < j1; s0 += kGatherCap) {
       const int64_t s1 = s0 + kGatherCap < j1 ? s0 + kGatherCap : j1;
+      // each particle's tile cell and fractions, once for every mesh
+      int c0[kGatherPPT];
+      float fr[kGatherPPT][3];
+#pragma unroll
+      for (int q = 0; q < kGatherPPT; ++q) {
+        int64_t j = s0 + threadIdx.x + (int64_t)q * kTileThreads;
+        int bs[3];
+        base_frac(pmid, disp, j < s1 ? j : s1 - 1, g.a1, k.n, bs, fr[q]);
+        c0[q] = local_cell(B, bs, o);
+      }
 #pragma unroll 1
       for (int m = 0; m < nm; ++m) {
         const float* __restrict__ F = m == 0 ? f0 : (m == 1 ? f1 : (m == 2 ? f2 : dcot));
@@ -617,45 +627,39 @@ __global__ void __launch_bounds__(kTileThreads) k_disp_cot_columns(
         for (int r = threadIdx.x; r < rows; r += kTileThreads) tile[r * P + B.b2] = plane[up][m][r];
         __syncthreads();
         for (int r = threadIdx.x; r < rows; r += kTileThreads) plane[me][m][r] = tile[r * P];
+        float cf[kGatherPPT];
 #pragma unroll
-        for (int h = 0; h < kGatherPPT; h += kGatherBatch) {
-          int bs[kGatherBatch][3];
-          float fr[kGatherBatch][3];
-          float cf[kGatherBatch];
+        for (int q = 0; q < kGatherPPT; ++q) {  // this mesh's coefficients (clamped loads)
+          const int64_t j = s0 + threadIdx.x + (int64_t)q * kTileThreads;
+          cf[q] = m < 3 ? cot_s[(int64_t)m * n + (j < s1 ? j : s1 - 1)] : val;
+        }
 #pragma unroll
-          for (int u = 0; u < kGatherBatch; ++u) {  // the batch's particle loads first (clamped)
-            int64_t j = s0 + threadIdx.x + (int64_t)(h + u) * kTileThreads;
-            j = j < s1 ? j : s1 - 1;
-            base_frac(pmid, disp, j, g.a1, k.n, bs[u], fr[u]);
-            cf[u] = m < 3 ? cot_s[(int64_t)m * n + j] : val;
+        for (int q = 0; q < kGatherPPT; ++q) {
+          const int jl = threadIdx.x + q * kTileThreads;
+          if (s0 + jl >= s1) continue;
+          float w[3][2], sg[3][2];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            float f = fr[q][a];
+            asm volatile("" : "+v"(f));  // weights per mesh pass: kept out of registers between passes
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) {
+              const float delta = f - (float)bb;
+              w[a][bb] = 1.f - fabsf(delta);
+              sg[a][bb] = cic_sign(delta);
+            }
+          }
+          float gs[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            const int bx = c & 1, by = (c >> 1) & 1, bzz = (c >> 2) & 1;
+            const float v = tile[c0[q] + B.corner_off(c)];
+            gs[0] = gs[0] + v * (sg[0][bx] * w[1][by] * w[2][bzz]);
+            gs[1] = gs[1] + v * (w[0][bx] * sg[1][by] * w[2][bzz]);
+            gs[2] = gs[2] + v * (w[0][bx] * w[1][by] * sg[2][bzz]);
           }
 #pragma unroll
-          for (int u = 0; u < kGatherBatch; ++u) {
-            const int jl = threadIdx.x + (h + u) * kTileThreads;
-            if (s0 + jl >= s1) continue;
-            const int c0i = local_cell(B, bs[u], o);
-            float w[3][2], sg[3][2];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-#pragma unroll
-              for (int bb = 0; bb < 2; ++bb) {
-                const float delta = fr[u][a] - (float)bb;
-                w[a][bb] = 1.f - fabsf(delta);
-                sg[a][bb] = cic_sign(delta);
-              }
-            }
-            float gs[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-              const int bx = c & 1, by = (c >> 1) & 1, bzz = (c >> 2) & 1;
-              const float v = tile[c0i + B.corner_off(c)];
-              gs[0] = gs[0] + v * (sg[0][bx] * w[1][by] * w[2][bzz]);
-              gs[1] = gs[1] + v * (w[0][bx] * sg[1][by] * w[2][bzz]);
-              gs[2] = gs[2] + v * (w[0][bx] * w[1][by] * sg[2][bzz]);
-            }
-#pragma unroll
-            for (int a = 0; a < 3; ++a) res[a][jl] = (m == 0 ? 0.f : res[a][jl]) + cf[u] * gs[a];
-          }
+          for (int a = 0; a < 3; ++a) res[a][jl] = (m == 0 ? 0.f : res[a][jl]) + cf[q] * gs[a];
         }
       }
       __syncthreads();

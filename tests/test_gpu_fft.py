@@ -12,7 +12,8 @@ from oracle import spectral
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(2, 2, 2), (4, 4, 4), (8, 16, 4), (2, 64, 8), (32, 32, 32), (64, 128, 32), (16, 8, 256),
-          (4, 2048, 2), (2048, 2, 4), (4, 4, 2048), (128, 64, 512), (1024, 4, 16)]
+          (4, 2048, 2), (2048, 2, 4), (4, 4, 2048), (128, 64, 512), (1024, 4, 16), (2, 2, 1024),
+          (8, 4, 1024), (16, 2, 1024)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)

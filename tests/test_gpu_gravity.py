@@ -191,6 +191,27 @@ def test_force_vjp_across_brick_tilings(device, brick, clustered, valid_frac):
         assert np.all(got[~valid] == 0)
 
 
+def test_force_and_vjp_on_1024_point_z_lines(device):
+    # an 8 x 8 x 1024 mesh: the banded z passes of 1024-point lines (r2c, the derivative c2r,
+    # and the adjoint's plain c2r) against the oracle
+    conf = Configuration(1.0, (4, 4, 512), mesh_shape=2)
+    ptcl = _ptcl(conf, 0.5, 31, device)
+    ref, masks = _oracle_acc(conf, ptcl, 0.3)
+    disp = ptcl.disp.clone().requires_grad_(True)
+    omega = torch.tensor(0.3, device=device, requires_grad=True)
+    acc = gravity(1.0, ptcl.replace(disp=disp), type("C", (), {"Omega_m": omega})(), conf)
+    got = acc.detach().cpu().numpy()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+    ac = np.random.default_rng(5).normal(size=acc.shape).astype(np.float32)
+    (acc * torch.as_tensor(ac, device=device)).sum().backward()
+    dref, oref = spectral.gravity_vjp(ptcl.pmid.cpu().numpy().astype(np.int64),
+                                      ptcl.disp.cpu().numpy().astype(np.float64), 0.3, ac, conf.mesh_shape,
+                                      conf.cell_size, conf.ptcl_num, masks)
+    g = disp.grad.cpu().numpy()
+    assert np.linalg.norm(g - dref) / np.linalg.norm(dref) < 1e-3
+    assert float(omega.grad) == pytest.approx(oref, rel=1e-3)
+
+
 def test_particle_subsets_empty_sets_and_all_invalid(device):
     # a particle count that is not a whole lattice (odd, bucketed in plain index chunks), the
     # empty set, and a set whose particles are all invalid
