@@ -794,26 +794,56 @@ __global__ void __launch_bounds__(kBlock) k_source_bwd(float* __restrict__ src_c
                                                        const float* __restrict__ rho,
                                                        const float* __restrict__ rho_add,
                                                        float* __restrict__ rho_cot, int64_t n,
-                                                       float scale, float* __restrict__ omega_cot) {
-  float acc = 0.f;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    float sc = src_cot[i];
+                                                       float scale, double* __restrict__ part) {
+  // sum_x src_cot (rho - 1): float products, float64 sums (per thread, then per block in a
+  // fixed order), one partial per block; float4 accesses when every array is 16-byte aligned
+  const bool vec = ((reinterpret_cast<uintptr_t>(src_cot) | reinterpret_cast<uintptr_t>(rho) |
+                     reinterpret_cast<uintptr_t>(rho_add) | reinterpret_cast<uintptr_t>(rho_cot)) & 15) == 0;
+  const int64_t n4 = vec ? n / 4 : 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t g0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  for (int64_t q = g0; q < n4; q += stride) {
+    const float4 sc = reinterpret_cast<const float4*>(src_cot)[q];
+    float4 r = reinterpret_cast<const float4*>(rho)[q];
+    if (rho_add) {
+      const float4 ra = reinterpret_cast<const float4*>(rho_add)[q];
+      r = make_float4(r.x + ra.x, r.y + ra.y, r.z + ra.z, r.w + ra.w);
+    }
+    acc += (double)(sc.x * (r.x - 1.f)) + (double)(sc.y * (r.y - 1.f)) + (double)(sc.z * (r.z - 1.f)) +
+           (double)(sc.w * (r.w - 1.f));
+    const float4 o = make_float4(sc.x * scale, sc.y * scale, sc.z * scale, sc.w * scale);
+    reinterpret_cast<float4*>(src_cot)[q] = o;
+    if (rho_cot) reinterpret_cast<float4*>(rho_cot)[q] = o;
+  }
+  for (int64_t i = 4 * n4 + g0; i < n; i += stride) {
+    const float sc = src_cot[i];
     float r = rho[i];
     if (rho_add) r += rho_add[i];
-    acc += sc * (r - 1.f);
+    acc += (double)(sc * (r - 1.f));
     src_cot[i] = sc * scale;
     if (rho_cot) rho_cot[i] = sc * scale;
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-  __shared__ float part[kBlock / 64];
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = acc;
+  __shared__ double wsum[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x / 64] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float t = 0.f;
-    for (int w = 0; w < kBlock / 64; ++w) t += part[w];
-    atomic_add_f32(omega_cot, 1.5f * t);
+    double t = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) t += wsum[w];
+    part[blockIdx.x] = t;
   }
 }
+
+// omega_cot[0] += 1.5 * (the block partials of k_source_bwd, summed in block order)
+__global__ void k_source_bwd_sum(const double* __restrict__ part, int nb, float* __restrict__ omega_cot) {
+  if (threadIdx.x != 0) return;
+  double t = 0.0;
+  for (int b = 0; b < nb; ++b) t += part[b];
+  omega_cot[0] += 1.5f * (float)t;
+}
+
+constexpr int kSrcBlocks = 2048;  // blocks of k_source_bwd (and its partials)
 
 struct Layout {
   int nx, ny, nz, m;
@@ -910,6 +940,7 @@ struct Buffers {
   float* disp_cot_s;
   float* cot_s;      // sorted acc_cot, component-major (tiled reverse)
   float* cot_pmax;   // per-block maxima, then the fixed-point scale
+  double* src_part;  // per-block partials of the Omega_m cotangent
 };
 
 bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const int32_t* brick, bool reverse,
@@ -950,6 +981,7 @@ bool carve(Arena& ws, const Layout& L, int64_t n, const int32_t* shape, const in
     b.disp_cot_s = b.tiled ? nullptr : ws.take<float>(3 * n);
     b.cot_s = b.tiled ? ws.take<float>(3 * n) : nullptr;
     b.cot_pmax = b.tiled ? ws.take<float>(kCotBlocks + 1) : nullptr;
+    b.src_part = ws.take<double>(kSrcBlocks);
   }
   return ws.ok();
 }
@@ -1402,10 +1434,13 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
   if (st) return st;
   {
     const float coef = 1.5f * desc->omega_m;
-    unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(L.nreal, 256 * 4), 4096);
-    hipLaunchKernelGGL(grav::k_source_bwd, dim3(blocks), dim3(256), 0, s, b.g1, b.r0, rho_add, rho_cot, L.nreal, coef,
-                       omega_cot);
+    const unsigned blocks =
+        (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(L.nreal, grav::kBlock * 16), grav::kSrcBlocks));
+    hipLaunchKernelGGL(grav::k_source_bwd, dim3(blocks), dim3(grav::kBlock), 0, s, b.g1, b.r0, rho_add, rho_cot,
+                       L.nreal, coef, b.src_part);
     PM_LAUNCHED("k_source_bwd", s);
+    hipLaunchKernelGGL(grav::k_source_bwd_sum, dim3(1), dim3(64), 0, s, b.src_part, (int)blocks, omega_cot);
+    PM_LAUNCHED("k_source_bwd_sum", s);
   }
   if (n > 0 && b.tiled) return grav::disp_cot_tiled(desc, b, n, valid, disp_cot, s);
   if (n > 0) {
