@@ -169,6 +169,21 @@ int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid,
                    const pm_gravity_desc* desc, const float* acc_cot, float* disp_cot,
                    float* omega_cot, void* workspace, size_t workspace_bytes, void* stream);
 
+/* The reverse sweep's force pair (nbody/integrator.py:781-837: after drift_adj the force is
+ * recomputed at the reconstructed positions, and the next stage's force_adj pulls its
+ * cotangent back at those same positions).  pm_gravity_keep is pm_gravity on a workspace of
+ * pm_gravity_vjp_workspace bytes that keeps the bucketed particles, the density and the force
+ * meshes in it; pm_gravity_vjp_resume is pm_gravity_vjp that starts from them instead of
+ * recomputing them.  Its workspace must still hold what the preceding pm_gravity_keep call on
+ * the same particles (pmid, disp, valid bit for bit), desc and stream left in it; calls that
+ * change the positions or reuse the workspace in between invalidate it.  Results equal
+ * pm_gravity / pm_gravity_vjp. */
+int pm_gravity_keep(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                    const pm_gravity_desc* desc, float* acc, void* workspace, size_t workspace_bytes, void* stream);
+int pm_gravity_vjp_resume(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                          const pm_gravity_desc* desc, const float* acc_cot, float* disp_cot, float* omega_cot,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
 /* The force step with a learned correction (nbody/gravity.py:419-430, 635-640 with
  * apply_potential_correction, corrections/core.py:286-334): rho_add (the real mesh, Nx*Ny*Nz
  * float32, C order) is added to the CIC density before S = 1.5*Omega_m*(rho + rho_add - 1),

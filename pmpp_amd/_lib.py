@@ -32,6 +32,7 @@ EXPORTED_SYMBOLS = (
     "pm_spectral_workspace", "pm_spectral_force_multiply",
     "pm_gravity_workspace", "pm_gravity_vjp_workspace", "pm_gravity", "pm_gravity_vjp",
     "pm_gravity_density", "pm_gravity_corrected", "pm_gravity_corrected_resume", "pm_gravity_corrected_vjp",
+    "pm_gravity_keep", "pm_gravity_vjp_resume",
     "pm_kick", "pm_drift", "pm_nbody_adj_workspace", "pm_kick_adj", "pm_drift_adj",
     "pm_power_spectrum", "pm_power_spectrum_bwd",
     "pm_conv3d_fwd", "pm_conv3d_bwd_data", "pm_conv3d_bwd_weight_workspace", "pm_conv3d_bwd_weight",
@@ -106,6 +107,10 @@ _SIGNATURES = {
                        ctypes.c_int),
     "pm_gravity_density": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, ctypes.c_size_t, _vp],
                            ctypes.c_int),
+    "pm_gravity_keep": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, ctypes.c_size_t, _vp],
+                        ctypes.c_int),
+    "pm_gravity_vjp_resume": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, _vp,
+                               ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_gravity_corrected": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, ctypes.c_size_t,
                               _vp], ctypes.c_int),
     "pm_gravity_corrected_resume": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, ctypes.c_size_t,

@@ -1378,10 +1378,12 @@ int pm_gravity_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid,
                                   workspace, workspace_bytes, stream);
 }
 
-int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
-                             const pm_gravity_desc* desc, const float* rho_add, const float* acc_cot,
-                             float* disp_cot, float* omega_cot, float* rho_cot, void* workspace,
-                             size_t workspace_bytes, void* stream) {
+// The force VJP; with resume, the forward meshes (and the bucketed particles) are those a
+// pm_gravity_keep call left in the same workspace for the same particles.
+static int vjp_impl(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                    const pm_gravity_desc* desc, const float* rho_add, const float* acc_cot, float* disp_cot,
+                    float* omega_cot, float* rho_cot, void* workspace, size_t workspace_bytes, void* stream,
+                    bool resume) {
   hipStream_t s = as_stream(stream);
   grav::Layout L;
   grav::Buffers b{};
@@ -1394,10 +1396,12 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
   PM_RETURN_IF(omega_cot == nullptr, PM_ERROR_INVALID_ARGUMENT, "omega_cot NULL");
   const Geom g = grav::step_geom(desc);
   // recompute the forward meshes: rho stays in r0, F_x goes to g0, F_y r1, F_z r2
-  st = grav::density_stage(pmid, disp, valid, n, desc, L, b, ws, s);
-  if (st) return st;
-  st = grav::force_mesh_stage(desc, L, b, tb, s, true, rho_add);
-  if (st) return st;
+  if (!resume) {
+    st = grav::density_stage(pmid, disp, valid, n, desc, L, b, ws, s);
+    if (st) return st;
+    st = grav::force_mesh_stage(desc, L, b, tb, s, true, rho_add);
+    if (st) return st;
+  }
   // gather^T: acc_cot scattered into three cotangent meshes, displacement cotangent from F.
   // With brick tiling each component goes through the tiled scatter (sorted, scaled copies of
   // acc_cot); otherwise through float atomics beside the displacement cotangent.
@@ -1449,6 +1453,40 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
     PM_LAUNCHED("k_scatter_bwd_sorted", s);
   }
   return PM_SUCCESS;
+}
+
+int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                             const pm_gravity_desc* desc, const float* rho_add, const float* acc_cot,
+                             float* disp_cot, float* omega_cot, float* rho_cot, void* workspace,
+                             size_t workspace_bytes, void* stream) {
+  return vjp_impl(pmid, disp, valid, n, desc, rho_add, acc_cot, disp_cot, omega_cot, rho_cot, workspace,
+                  workspace_bytes, stream, false);
+}
+
+int pm_gravity_keep(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                    const pm_gravity_desc* desc, float* acc, void* workspace, size_t workspace_bytes, void* stream) {
+  hipStream_t s = as_stream(stream);
+  grav::Layout L;
+  grav::Buffers b{};
+  Arena ws(workspace, workspace_bytes);
+  fft::SpecTables tb;
+  // the VJP's carve: the meshes stay where pm_gravity_vjp_resume finds them
+  int st = grav::prepare(desc, n, workspace, workspace_bytes, true, L, b, ws, tb, s);
+  if (st) return st;
+  PM_RETURN_IF(n > 0 && (pmid == nullptr || disp == nullptr || acc == nullptr), PM_ERROR_INVALID_ARGUMENT,
+               "pmid/disp/acc NULL");
+  st = grav::density_stage(pmid, disp, valid, n, desc, L, b, ws, s);
+  if (st) return st;
+  st = grav::force_mesh_stage(desc, L, b, tb, s, true, nullptr);
+  if (st) return st;
+  return grav::gather_stage(b.pmid_s, b.disp_s, b.valid_s, valid, n, desc, b, b.g0, b.r1, b.r2, acc, s);
+}
+
+int pm_gravity_vjp_resume(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                          const pm_gravity_desc* desc, const float* acc_cot, float* disp_cot, float* omega_cot,
+                          void* workspace, size_t workspace_bytes, void* stream) {
+  return vjp_impl(pmid, disp, valid, n, desc, nullptr, acc_cot, disp_cot, omega_cot, nullptr, workspace,
+                  workspace_bytes, stream, true);
 }
 
 int pm_spectral_workspace(const pm_gravity_desc* desc, size_t* bytes) {

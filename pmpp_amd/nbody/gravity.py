@@ -118,12 +118,26 @@ def gravity(a, ptcl, cosmo, conf, correction=None):
     return _GravityFn.apply(ptcl.disp, omega, ptcl.pmid.detach(), _valid_u8(ptcl), conf)
 
 
-def gravity_vjp(ptcl, cosmo, conf, acc_cot):
-    """(disp cotangent (N,3) tensor, Omega_m cotangent float) of ``gravity`` for ``acc_cot``.
+class KeptForce:
+    """A workspace holding the forward meshes of one force evaluation (``pm_gravity_keep``)
+    for the VJP at the same particles (``pm_gravity_vjp_resume``): the reverse sweep recomputes
+    the force at the reconstructed positions and pulls the next cotangent back there
+    (integrator.py:781-837).  ``key`` names the particle arrays (and their versions) the
+    meshes belong to; a different key means the workspace holds nothing usable."""
 
-    The reverse of the force step without autograd bookkeeping (``pm_gravity_vjp``), as the
-    N-body reverse sweep calls it once per drift stage (integrator.py:620-686).
-    """
+    def __init__(self):
+        self.ws = None
+        self.nbytes = 0
+        self.key = None
+
+    @staticmethod
+    def key_of(ptcl, omega):
+        return (ptcl.pmid.data_ptr(), ptcl.pmid._version, ptcl.disp.data_ptr(), ptcl.disp._version,
+                None if ptcl.unused_index is None else (ptcl.unused_index.data_ptr(), ptcl.unused_index._version),
+                len(ptcl), float(omega))
+
+
+def _vjp_call(ptcl, cosmo, conf, acc_cot, kept=None):
     omega = cosmo.Omega_m if hasattr(cosmo, "Omega_m") else cosmo
     dev = ptcl.disp.device
     ensure_init(dev)
@@ -132,12 +146,55 @@ def gravity_vjp(ptcl, cosmo, conf, acc_cot):
     dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
     n = _check_particles(pm, dp)
     nbytes = _lib.size_query("pm_gravity_vjp_workspace", d, n)
-    ws = workspace(nbytes, dev)
+    resume = (kept is not None and kept.ws is not None and kept.nbytes == nbytes
+              and kept.key == KeptForce.key_of(ptcl, omega))
+    if kept is not None:  # the sweep's VJPs all run in its one kept workspace
+        if kept.ws is None or kept.ws.numel() < nbytes or kept.ws.device != dev:
+            kept.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        kept.nbytes = nbytes
+        ws = kept.ws
+    else:
+        ws = workspace(nbytes, dev)
     ac = as_contig(acc_cot.detach(), torch.float32, "acc_cot")
     disp_cot = torch.empty((n, 3), dtype=torch.float32, device=dev)
     omega_cot = torch.zeros((), dtype=torch.float32, device=dev)
     valid = _valid_u8(ptcl)
-    _lib.call("pm_gravity_vjp", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(ac), ptr(disp_cot), ptr(omega_cot),
-              ptr(ws), nbytes, stream_handle(dev))
+    _lib.call("pm_gravity_vjp_resume" if resume else "pm_gravity_vjp", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(ac),
+              ptr(disp_cot), ptr(omega_cot), ptr(ws), nbytes, stream_handle(dev))
+    if kept is not None:
+        kept.key = None  # the VJP's cotangent meshes have overwritten part of the workspace
     return disp_cot, float(omega_cot.item())
+
+
+def gravity_vjp(ptcl, cosmo, conf, acc_cot, kept=None):
+    """(disp cotangent (N,3) tensor, Omega_m cotangent float) of ``gravity`` for ``acc_cot``.
+
+    The reverse of the force step without autograd bookkeeping (``pm_gravity_vjp``), as the
+    N-body reverse sweep calls it once per drift stage (integrator.py:620-686).  With a
+    ``KeptForce`` that ``gravity_keep`` filled for these same particles, the forward meshes
+    are taken from it (``pm_gravity_vjp_resume``) instead of being recomputed.
+    """
+    return _vjp_call(ptcl, cosmo, conf, acc_cot, kept)
+
+
+def gravity_keep(ptcl, cosmo, conf, kept):
+    """``gravity`` (accelerations (N,3), no autograd) that leaves its forward meshes in
+    ``kept`` for a following ``gravity_vjp`` at the same particles (``pm_gravity_keep``)."""
+    omega = cosmo.Omega_m if hasattr(cosmo, "Omega_m") else cosmo
+    dev = ptcl.disp.device
+    ensure_init(dev)
+    d = gravity_desc(conf, float(omega))
+    pm = as_contig(ptcl.pmid, torch.int16, "pmid")
+    dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
+    n = _check_particles(pm, dp)
+    nbytes = _lib.size_query("pm_gravity_vjp_workspace", d, n)
+    if kept.ws is None or kept.ws.numel() < nbytes or kept.ws.device != dev:
+        kept.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    kept.nbytes = nbytes
+    acc = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    valid = _valid_u8(ptcl)
+    _lib.call("pm_gravity_keep", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(acc), ptr(kept.ws), nbytes,
+              stream_handle(dev))
+    kept.key = KeptForce.key_of(ptcl, omega)
+    return acc
 

@@ -165,6 +165,9 @@ class AdjointState:
         self.ws = workspace(nbytes, dev)
         self.ws_bytes = nbytes
         self.stream = stream_handle(dev)
+        from .gravity import KeptForce
+
+        self.kept = KeptForce()  # forward meshes of the last force recompute, for the next VJP
 
     def particles(self):
         return self.ptcl.replace(disp=self.disp, vel=self.vel, acc=self.acc)
@@ -184,6 +187,7 @@ def kick_adj(a_acc, a_prev, a_next, st: AdjointState, cosmo, conf):
 def drift_adj_from_output(a_vel, a_prev, a_next, st: AdjointState, cosmo, conf):
     """Reverse a drift from its output: disp -= D vel, vel_cot += D disp_cot (:429-474)."""
     D = drift_factor(a_vel, a_prev, a_next, cosmo, conf)
+    st.kept.key = None  # the positions change under any kept force meshes
     _lib.call("pm_drift_adj", ptr(st.disp), ptr(st.vel), ptr(st.disp_cot), ptr(st.vel_cot), _n(st.ptcl), _f32(D),
               ptr(st.dot), ptr(st.ws), st.ws_bytes, st.stream)
     st.omega_cot += _factor_dOm("drift_factor", a_vel, a_prev, a_next, cosmo, conf) * st.take_dot()
@@ -195,7 +199,7 @@ def force_adj(a, st: AdjointState, cosmo, conf, correction=None):
     if correction is None:
         from .gravity import gravity_vjp
 
-        d_disp, d_om = gravity_vjp(st.particles(), cosmo, conf, st.acc_cot)
+        d_disp, d_om = gravity_vjp(st.particles(), cosmo, conf, st.acc_cot, kept=st.kept)
     else:
         from ..corrections.training import corrected_gravity_vjp
 
@@ -215,5 +219,11 @@ def integrate_adj(a_prev, a_next, st: AdjointState, cosmo, conf, correction=None
         _, a_vel, a_disp, a_disp_next, a_acc_in = stage
         force_adj(a_disp_next, st, cosmo, conf, correction=correction)
         drift_adj_from_output(a_vel, a_disp, a_disp_next, st, cosmo, conf)
-        # restore the acceleration the earlier kick consumed
-        st.acc = force(a_acc_in, st.particles(), cosmo, conf, correction=correction).acc.contiguous()
+        # restore the acceleration the earlier kick consumed; uncorrected, the force keeps its
+        # meshes for the VJP at these same positions in the next stage
+        if correction is None:
+            from .gravity import gravity_keep
+
+            st.acc = gravity_keep(st.particles(), cosmo, conf, st.kept).contiguous()
+        else:
+            st.acc = force(a_acc_in, st.particles(), cosmo, conf, correction=correction).acc.contiguous()

@@ -212,6 +212,34 @@ def test_force_and_vjp_on_1024_point_z_lines(device):
     assert float(omega.grad) == pytest.approx(oref, rel=1e-3)
 
 
+@pytest.mark.parametrize("brick,valid_frac", [((16, 16, 32), 0.85), ((3, 3, 3), None)])
+def test_kept_force_and_resumed_vjp_equal_the_plain_calls(device, brick, valid_frac):
+    # pm_gravity_keep / pm_gravity_vjp_resume (the reverse sweep's pair at one set of positions)
+    # against pm_gravity / pm_gravity_vjp: bit for bit with brick tiling (fixed-point scatters,
+    # per-particle gathers); the untiled path scatters with float atomics, whose order (and so
+    # the last bits) differs between any two calls
+    from pmpp_amd.nbody.gravity import KeptForce, gravity_keep, gravity_vjp
+
+    conf = Configuration(2.0, (16, 16, 16), mesh_shape=2, brick=brick)
+    ptcl = _ptcl(conf, 0.7, 37, device, valid_frac)
+    cosmo = type("C", (), {"Omega_m": 0.3})()
+    ac = torch.as_tensor(np.random.default_rng(41).normal(size=(len(ptcl), 3)).astype(np.float32), device=device)
+    kept = KeptForce()
+    acc_keep = gravity_keep(ptcl, cosmo, conf, kept)
+    acc = gravity(1.0, ptcl, 0.3, conf)
+    d_res, o_res = gravity_vjp(ptcl, cosmo, conf, ac, kept=kept)
+    assert kept.key is None  # used once
+    d_ref, o_ref = gravity_vjp(ptcl, cosmo, conf, ac)
+    if brick == (16, 16, 32):
+        assert torch.equal(acc_keep, acc)
+        assert torch.equal(d_res, d_ref)
+        assert o_res == o_ref
+    else:
+        assert float((acc_keep - acc).norm() / acc.norm()) < 1e-6
+        assert float((d_res - d_ref).norm() / d_ref.norm()) < 1e-6
+        assert o_res == pytest.approx(o_ref, rel=1e-6)
+
+
 def test_particle_subsets_empty_sets_and_all_invalid(device):
     # a particle count that is not a whole lattice (odd, bucketed in plain index chunks), the
     # empty set, and a set whose particles are all invalid
