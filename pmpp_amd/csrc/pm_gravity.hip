@@ -418,7 +418,6 @@ __global__ void __launch_bounds__(kBlock) k_fold_xy(Bricks k, const float* __res
 // more than kGatherCap particles (rare: ~2x the mean occupancy at 1/8 particle per cell).
 constexpr int kGatherCap = 2048;
 constexpr int kGatherPPT = kGatherCap / kTileThreads;
-constexpr int kGatherBatch = 2;  // particles whose loads are in flight together
 
 // Stage rows z in [0, b2) of one component's tile for brick (cx, cy, bz) (rows wrap in x, y).
 template <int BX, int BY, int BZ>
@@ -689,9 +688,9 @@ __global__ void k_zero_invalid(const int32_t* __restrict__ perm, const int32_t* 
   }
 }
 
-// Reverse of the three gathers (gather^T = scatter of acc_cot, plus the displacement
-// cotangent from the mesh values), pallas.py:321-355 per component.
-template <bool SCATTER>
+// Reverse of the three gathers without brick tiling (gather^T = float-atomic scatter of
+// acc_cot, plus the displacement cotangent from the mesh values), pallas.py:321-355 per
+// component.
 __global__ void __launch_bounds__(kBlock) k_gather3_bwd_sorted(
     const int16_t* __restrict__ pmid, const float* __restrict__ disp, const uint8_t* __restrict__ valid,
     const int32_t* __restrict__ perm, int64_t n, Geom g, const float* __restrict__ fx,
@@ -712,11 +711,9 @@ __global__ void __launch_bounds__(kBlock) k_gather3_bwd_sorted(
       float w = corner_weight(ax, c);
       float gr[3];
       corner_grad(ax, c, gr);
-      if (SCATTER) {
-        atomic_add_f32(gx + flat, ac[0] * w);
-        atomic_add_f32(gy + flat, ac[1] * w);
-        atomic_add_f32(gz + flat, ac[2] * w);
-      }
+      atomic_add_f32(gx + flat, ac[0] * w);
+      atomic_add_f32(gy + flat, ac[1] * w);
+      atomic_add_f32(gz + flat, ac[2] * w);
       float inner = ac[0] * fx[flat] + ac[1] * fy[flat] + ac[2] * fz[flat];
 #pragma unroll
       for (int a = 0; a < 3; ++a) dc[a] = dc[a] + inner * gr[a];
@@ -1427,7 +1424,7 @@ static int vjp_impl(const int16_t* pmid, const float* disp, const uint8_t* valid
     st = grav::fill(b.g3, L.nreal, s);
     if (st) return st;
     if (n > 0) {
-      hipLaunchKernelGGL(grav::k_gather3_bwd_sorted<true>, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s,
+      hipLaunchKernelGGL(grav::k_gather3_bwd_sorted, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s,
                          b.pmid_s, b.disp_s, b.valid_s, b.perm, n, g, b.g0, b.r1, b.r2, acc_cot, b.g1, b.g2, b.g3,
                          b.disp_cot_s);
       PM_LAUNCHED("k_gather3_bwd_sorted", s);

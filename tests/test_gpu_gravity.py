@@ -82,7 +82,7 @@ def test_clustered_particles_exceed_one_gather_chunk(device, brick):
     # 90 % of the particles in one brick: several gather chunks of 2048 particles (the last
     # one partial) and heavy LDS-atomic contention; the rest leave most bricks and some brick
     # columns empty.  (8, 8, 8) runs the runtime-shape tile kernels, (16, 16, 32) the
-    # compile-time ones (the pipelined gather).
+    # compile-time ones.
     conf = Configuration(1.0, (32, 32, 32), mesh_shape=1, brick=brick)
     rng = np.random.default_rng(9)
     n = 32 ** 3
