@@ -113,9 +113,28 @@ def gravity(a, ptcl, cosmo, conf, correction=None):
         return _CorrectedGravityFn.apply(ptcl.disp, omega, ptcl, conf, correction, float(a))
     del a
     omega = cosmo.Omega_m if hasattr(cosmo, "Omega_m") else cosmo
+    if not isinstance(omega, torch.Tensor) and not (torch.is_grad_enabled() and ptcl.disp.requires_grad):
+        # nothing to differentiate: a plain call, with Omega_m passed by value (a device tensor
+        # made from it would be read back to the host, a synchronisation per force)
+        return _gravity_plain(ptcl, float(omega), conf)
     if not isinstance(omega, torch.Tensor):
         omega = torch.tensor(float(omega), dtype=torch.float32, device=ptcl.disp.device)
     return _GravityFn.apply(ptcl.disp, omega, ptcl.pmid.detach(), _valid_u8(ptcl), conf)
+
+
+def _gravity_plain(ptcl, omega, conf):
+    dev = ptcl.disp.device
+    ensure_init(dev)
+    d = gravity_desc(conf, omega)
+    pm = as_contig(ptcl.pmid.detach(), torch.int16, "pmid")
+    dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
+    n = _check_particles(pm, dp)
+    nbytes = _lib.size_query("pm_gravity_workspace", d, n)
+    ws = workspace(nbytes, dev)
+    acc = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    valid = _valid_u8(ptcl)
+    _lib.call("pm_gravity", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(acc), ptr(ws), nbytes, stream_handle(dev))
+    return acc
 
 
 class KeptForce:
