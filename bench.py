@@ -134,20 +134,26 @@ def measured_traffic(kernel, config):
 
 
 def cpu_baseline(dev, omega_m):
-    """Float32 NumPy/SciPy port of the force step, timed on host cores on a 128^3/256^3 sample."""
+    """The reference's operator sequence restated in float32 on the host cores: the 256^3-particle /
+    512^3-mesh force step (lattice + jitter + 2LPT at a = 1 in a 256 Mpc/h box, the same input the
+    GPU takes for that workload), 1 warm-up then the median of 5; and the whole 64^3-particle /
+    128^3-mesh / 128 Mpc/h run (white noise -> 2LPT at a = 1/64 -> 10 KDK steps), 1 warm-up then
+    one timed run.  CIC in C/OpenMP, FFTs in scipy with one worker per core."""
     import numpy as np
 
-    from oracle.cpu_baseline import force_step
+    from oracle.cpu_baseline import force_step, nbody_run
     from pmpp_amd import Configuration
     from pmpp_amd.initial_conditions import LCDM, particles_2lpt
 
-    ng, ratio, box = 128, 2, 250.0  # same construction and density as the 512^3/1000 Mpc/h workload
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    ng, ratio, box = CONFIGS["c2"]
     conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio)
     ptcl = particles_2lpt(conf, LCDM(), seed=0, jitter_seed=1, device=dev, a=1.0)
     pm = ptcl.pmid.cpu().numpy()
     dp = ptcl.disp.cpu().numpy()
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    del ptcl
     args = (pm, dp, omega_m, conf.mesh_shape, conf.cell_size, conf.ptcl_grid_shape, conf.ptcl_spacing, cores)
     force_step(*args)  # warm-up
     times = []
@@ -156,6 +162,10 @@ def cpu_baseline(dev, omega_m):
         force_step(*args)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
+    nbody_run(64, 2, 128.0, 10, 0, LCDM(), workers=cores)  # warm-up
+    t0 = time.perf_counter()
+    nbody_run(64, 2, 128.0, 10, 0, LCDM(), workers=cores)
+    t_c1 = time.perf_counter() - t0
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -170,9 +180,15 @@ def cpu_baseline(dev, omega_m):
         "unit": "particles/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"median of 5 float32 NumPy/SciPy force steps at {ng}^3 particles on a {ng * ratio}^3 mesh "
-                  f"({box:g} Mpc/h, same construction and particle density as the GPU workload; "
-                  f"{med:.2f} s/step, {1.0 / med:.3f} steps/s); CPU: {model or platform.processor()}",
+        "sample": f"median of 5 float32 force steps at {ng}^3 particles on a {ng * ratio}^3 mesh ({box:g} Mpc/h, "
+                  f"lattice + jitter + 2LPT at a = 1: {med:.2f} s/step, {1.0 / med:.3f} steps/s; CIC in C/OpenMP, "
+                  f"scipy FFTs, {cores} threads); CPU: {model or platform.processor()}",
+        "c2_force_step": {"s_per_step": round(med, 4), "steps_per_s": round(1.0 / med, 4),
+                          "particles_per_s": conf.ptcl_num / med, "particles": conf.ptcl_num,
+                          "mesh": list(conf.mesh_shape)},
+        "c1_end_to_end": {"s": round(t_c1, 3), "kdk_steps": 10, "steps_per_s": round(10 / t_c1, 3),
+                          "workload": "64^3 particles, 128^3 mesh, 128 Mpc/h: white noise (seed 0) -> 2LPT at "
+                                      "a = 1/64 -> 10 KDK steps to a = 1 (float64 ICs, float32 force)"},
     }
 
 

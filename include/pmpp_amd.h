@@ -3,8 +3,9 @@
  *
  * Every entry point is extern "C", takes caller-owned device pointers plus plain
  * sizes, enqueues work on the caller's HIP stream (passed as void*, a hipStream_t),
- * never synchronises, never allocates device memory and returns a status code; the
- * text of the last error on the calling host thread is available from pm_last_error().
+ * never synchronises (pm_init excepted), never allocates device memory and returns a
+ * status code; the text of the last error on the calling host thread is available from
+ * pm_last_error().
  * Temporaries come only from a caller-provided workspace whose size is reported by the
  * matching *_workspace() query.  This is the ownership/threading contract of the
  * reference's native FFI handlers (cuda/route_kernels.cu:1-7, 738-800; XLA owns every
@@ -27,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 1
+#define PM_ABI_VERSION 2
 
 enum pm_status {
   PM_SUCCESS = 0,
@@ -73,8 +74,11 @@ typedef struct pm_gravity_desc {
 
 const char* pm_last_error(void);
 int pm_abi_version(void);
-/* Prepares per-device constant tables on the current device; called implicitly by
- * every entry point, exposed so a caller can do it before HIP graph capture. */
+/* Prepares the per-device constant tables (the FFT twiddle table) on the current device and
+ * synchronises it: the only entry point that does either.  It must run once per device
+ * before any entry point that transforms (pm_rfft3d, pm_irfft3d, the pm_gravity family);
+ * those return PM_ERROR_INVALID_ARGUMENT until it has.  Thread-safe; later calls return at
+ * once.  (The reference's FFI library has no such step: XLA's FFT needs none.) */
 int pm_init(void);
 
 /* Per-kernel timing for measurement harnesses (no counterpart in the reference, whose
@@ -119,13 +123,29 @@ int pm_cic_scatter_bwd(const int16_t* pmid, const float* disp, const uint8_t* va
  * key/order contract: pmid_to_idx (src/pmpp/core/utils.py:292-326), the sentinel
  * (src/pmpp/distributed/routing.py:112-116) and the stable block-scan compaction
  * (cuda/route_kernels.cu:194-225).  perm[j] = input index of the j-th sorted particle;
- * bin_offsets (optional, num_bricks+2 int32) delimits each brick. */
+ * bin_offsets (optional, num_bricks+2 int32) delimits each brick; pmid_sorted (n,3) and
+ * disp_sorted (n,3) (each optional) receive the payload in sorted order. */
 int pm_sort_cells_workspace(int64_t n, const int32_t mesh_shape[3], const int32_t brick[3],
                             size_t* bytes);
 int pm_sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                   const int32_t mesh_shape[3], float cell_size, const int32_t brick[3],
-                  int32_t* perm, int32_t* bin_offsets, void* workspace, size_t workspace_bytes,
-                  void* stream);
+                  int32_t* perm, int32_t* bin_offsets, int16_t* pmid_sorted, float* disp_sorted,
+                  void* workspace, size_t workspace_bytes, void* stream);
+/* The force step's own binning (pm_gravity runs it internally): the same keys, bins and
+ * bin_offsets (num_bricks+2 int32, required) as pm_sort_cells, but the order inside a bin is
+ * unspecified (atomic arrival order), which the step's order-independent tile kernels allow.
+ * perm, pmid_sorted (n,3), disp_sorted (n,3) and valid_sorted (n) are all written.
+ * ptcl_grid (optional) names the Lagrangian grid the particles were generated on in C order;
+ * when its product is n and it is a multiple of (4, 8, 64), particles are read in blocks of
+ * that grid (speed only; any value gives a valid binning).  No counterpart in the reference,
+ * whose single-device path does not order particles (nbody/gravity.py:813). */
+int pm_bucket_cells_workspace(int64_t n, const int32_t mesh_shape[3], const int32_t brick[3],
+                              size_t* bytes);
+int pm_bucket_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                    const int32_t mesh_shape[3], float cell_size, const int32_t brick[3],
+                    const int32_t ptcl_grid[3], int32_t* perm, int32_t* bin_offsets, int16_t* pmid_sorted,
+                    float* disp_sorted, uint8_t* valid_sorted, void* workspace, size_t workspace_bytes,
+                    void* stream);
 
 /* --------------------------------------------------------------- FFT -----
  * pm_rfft3d  = jnp.fft.rfftn  (unnormalised forward)     used at nbody/gravity.py:436

@@ -102,6 +102,21 @@ def _flat(idx, spatial_shape):
     return (safe[..., 0] * s[1] + safe[..., 1]) * s[2] + safe[..., 2]
 
 
+def _accumulate(out, flat, contrib):
+    """out[flat[i]] += contrib[i] for every i (repeated indices accumulate), per channel.
+
+    ``numpy.bincount`` sums its weights in input order, as ``numpy.add.at`` does, at a
+    fraction of the cost."""
+    n = out.shape[0]
+    if out.ndim == 1:
+        out += np.bincount(flat, weights=contrib, minlength=n)
+        return
+    rest = out.reshape(n, -1)
+    cc = contrib.reshape(contrib.shape[0], -1)
+    for c in range(rest.shape[1]):
+        rest[:, c] += np.bincount(flat, weights=cc[:, c], minlength=n)
+
+
 def _broadcast_val(val, n, valid):
     v = np.asarray(val, dtype=np.float64)
     scalar = v.ndim == 0
@@ -133,7 +148,7 @@ def scatter(pmid, disp, val, mesh, cell_size, mesh_shape, offset=0.0, mesh_cell_
     w = np.where(ok, frac, 0.0)
     out = mesh.reshape((-1,) + chan)
     contrib = v[:, None, ...] * w.reshape(w.shape + (1,) * len(chan))
-    np.add.at(out, flat.reshape(-1), contrib.reshape((-1,) + chan))
+    _accumulate(out, flat.reshape(-1), contrib.reshape((-1,) + chan))
     return out.reshape(mesh.shape)
 
 
@@ -151,6 +166,20 @@ def gather(pmid, disp, mesh, cell_size, mesh_shape, offset=0.0, mesh_cell_size=N
     if valid is not None:
         m = np.asarray(valid, dtype=bool).reshape((-1,) + (1,) * len(chan))
         out = np.where(m, out, 0.0)
+    return out
+
+
+def gather_many(pmid, disp, meshes, cell_size, mesh_shape, valid=None):
+    """``gather`` of several scalar meshes at the same particles, sharing one ``enmesh``;
+    returns (N, len(meshes)).  Equal to stacking ``gather`` of each mesh (test-pinned)."""
+    spatial = np.asarray(meshes[0]).shape[:3]
+    idx, frac = enmesh(pmid, disp, cell_size, mesh_shape, 0.0, None, spatial)
+    ok = _in_bounds(idx, spatial)
+    flat = _flat(idx, spatial)
+    w = np.where(ok, frac, 0.0)
+    out = np.stack([(np.asarray(m, dtype=np.float64).reshape(-1)[flat] * w).sum(axis=1) for m in meshes], axis=-1)
+    if valid is not None:
+        out = np.where(np.asarray(valid, dtype=bool)[:, None], out, 0.0)
     return out
 
 
@@ -217,5 +246,5 @@ def gather_adj(pmid, disp, mesh, val_cot, cell_size, mesh_shape, offset=0.0, mes
     w = np.where(ok, frac, 0.0)
     mesh_cot = np.zeros(mesh.shape, dtype=np.float64).reshape((-1,) + chan)
     contrib = vc[:, None, ...] * w.reshape(w.shape + (1,) * len(chan))
-    np.add.at(mesh_cot, flat.reshape(-1), contrib.reshape((-1,) + chan))
+    _accumulate(mesh_cot, flat.reshape(-1), contrib.reshape((-1,) + chan))
     return disp_cot, mesh_cot.reshape(mesh.shape)

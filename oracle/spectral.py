@@ -11,8 +11,11 @@ Follows (read as text, never imported):
   streamed force: scatter, delta*1.5*Omega_m, rfftn, mask, Poisson, -ik_i, irfftn, gather.
 * ``src/pmpp/cic/scatter.py:258-265``        -- default particle value N_m/N_p on valid particles.
 
-The FFTs are ``numpy.fft.rfftn``/``irfftn`` (pocketfft, a library routine), pinned in
-``tests/test_oracle_spectral.py`` against a brute-force O(N^2) DFT on small meshes.  The
+The FFTs are pocketfft's ``rfftn``/``irfftn`` (a library routine; ``scipy.fft`` with one worker
+thread per allowed core, the same transform ``numpy.fft`` runs on one thread), pinned in
+``tests/test_oracle_spectral.py`` against a brute-force O(N^2) DFT on small meshes.  The CIC
+stages run over particle chunks of ``CHUNK`` so meshes of 512^3 cells and 16.8M particles fit
+in host memory; chunking changes only the float64 summation order.  The
 force is pinned by the closed form of a single plane wave (a = -A k0 sin(k0.x)/k0^2 of the
 source A cos(k0.x)), by zero force on a uniform lattice, by momentum conservation, and by the
 linearity in Omega_m (gravity.py:630).  ``gravity_vjp`` is pinned by central finite differences.
@@ -20,11 +23,51 @@ linearity in Omega_m (gravity.py:630).  ``gravity_vjp`` is pinned by central fin
 
 from __future__ import annotations
 
-import numpy as np
+import os
 
-from .cic import scatter, gather, scatter_adj, gather_adj
+import numpy as np
+import scipy.fft
+
+from .cic import scatter, gather, gather_many, scatter_adj, gather_adj
 
 F32_EPS = float(np.finfo(np.float32).eps)
+CHUNK = 1 << 21  # particles per CIC chunk
+_WORKERS = max(1, len(os.sched_getaffinity(0)))
+
+
+def rfftn(x):
+    """Unnormalised forward real 3-D DFT (numpy.fft.rfftn convention)."""
+    return scipy.fft.rfftn(np.asarray(x, dtype=np.float64), workers=_WORKERS)
+
+
+def irfftn(x, s):
+    """1/N normalised inverse of ``rfftn`` onto a real mesh of shape ``s``."""
+    return scipy.fft.irfftn(x, s=tuple(s), axes=(0, 1, 2), workers=_WORKERS)
+
+
+def _chunks(n):
+    return [(i, min(i + CHUNK, n)) for i in range(0, max(n, 1), CHUNK)] if n else []
+
+
+def _sel(a, lo, hi):
+    return None if a is None else np.asarray(a)[lo:hi]
+
+
+def scatter_chunked(pmid, disp, val, mesh_shape, cell_size, valid=None):
+    """CIC density of per-particle values ``val`` (N,), particle chunk by chunk."""
+    dens = np.zeros(tuple(mesh_shape))
+    for lo, hi in _chunks(np.asarray(pmid).shape[0]):
+        dens = scatter(pmid[lo:hi], disp[lo:hi], val[lo:hi], dens, cell_size, mesh_shape, valid=_sel(valid, lo, hi))
+    return dens
+
+
+def gather_chunked(pmid, disp, meshes, cell_size, mesh_shape, valid=None):
+    """CIC gather of each mesh of ``meshes`` at the particles, chunk by chunk: (N, len(meshes))."""
+    n = np.asarray(pmid).shape[0]
+    out = np.zeros((n, len(meshes)))
+    for lo, hi in _chunks(n):
+        out[lo:hi] = gather_many(pmid[lo:hi], disp[lo:hi], meshes, cell_size, mesh_shape, valid=_sel(valid, lo, hi))
+    return out
 
 
 def fftfreq_angular(n, spacing, last=False):
@@ -87,12 +130,13 @@ def green_multiplier(mesh_shape, cell_size, masks=()):
 def force_meshes(density, omega_m, mesh_shape, cell_size, masks=()):
     """Real-space force components (F_x, F_y, F_z) from a density mesh (gravity.py:626-633, 506-543)."""
     src = (np.asarray(density, dtype=np.float64) - 1.0) * (1.5 * float(omega_m))
-    src_hat = np.fft.rfftn(src)
+    src_hat = rfftn(src)
+    del src
     pot = src_hat * green_multiplier(mesh_shape, cell_size, masks)
+    del src_hat
     nk = neg_k(mesh_shape, cell_size)
     shapes = [(-1, 1, 1), (1, -1, 1), (1, 1, -1)]
-    return tuple(np.fft.irfftn(1j * nk[i].reshape(shapes[i]) * pot, s=tuple(mesh_shape), axes=(0, 1, 2))
-                 for i in range(3))
+    return tuple(irfftn(1j * nk[i].reshape(shapes[i]) * pot, tuple(mesh_shape)) for i in range(3))
 
 
 def default_value(n, mesh_shape, ptcl_num, valid=None):
@@ -109,14 +153,16 @@ def gravity(pmid, disp, omega_m, mesh_shape, cell_size, ptcl_num, masks=(), vali
     ``rho_add`` (a real mesh) is added to the CIC density before the source is formed: the
     learned correction applied as a source residual (the potential-residual hook of
     corrections/core.py:286-334 moved in front of the Poisson solve)."""
-    n = np.asarray(pmid).shape[0]
+    pmid = np.asarray(pmid)
+    disp = np.asarray(disp, dtype=np.float64)
+    n = pmid.shape[0]
     val = default_value(n, mesh_shape, ptcl_num, valid)
-    dens = scatter(pmid, disp, val, np.zeros(tuple(mesh_shape)), cell_size, mesh_shape, valid=valid)
+    dens = scatter_chunked(pmid, disp, val, mesh_shape, cell_size, valid=valid)
     if rho_add is not None:
         dens = dens + np.asarray(rho_add, dtype=np.float64)
     comps = force_meshes(dens, omega_m, mesh_shape, cell_size, masks)
-    acc = np.stack([gather(pmid, disp, c, cell_size, mesh_shape, valid=valid) for c in comps], axis=-1)
-    return acc
+    del dens
+    return gather_chunked(pmid, disp, comps, cell_size, mesh_shape, valid=valid)
 
 
 def gravity_vjp(pmid, disp, omega_m, acc_cot, mesh_shape, cell_size, ptcl_num, masks=(), valid=None,
@@ -131,9 +177,11 @@ def gravity_vjp(pmid, disp, omega_m, acc_cot, mesh_shape, cell_size, ptcl_num, m
     is also that of ``rho_add``) as a third item when ``rho_add`` is given.
     """
     mesh_shape = tuple(mesh_shape)
-    n = np.asarray(pmid).shape[0]
+    pmid = np.asarray(pmid)
+    disp = np.asarray(disp, dtype=np.float64)
+    n = pmid.shape[0]
     val = default_value(n, mesh_shape, ptcl_num, valid)
-    dens = scatter(pmid, disp, val, np.zeros(mesh_shape), cell_size, mesh_shape, valid=valid)
+    dens = scatter_chunked(pmid, disp, val, mesh_shape, cell_size, valid=valid)
     if rho_add is not None:
         dens = dens + np.asarray(rho_add, dtype=np.float64)
     comps = force_meshes(dens, omega_m, mesh_shape, cell_size, masks)
@@ -141,9 +189,14 @@ def gravity_vjp(pmid, disp, omega_m, acc_cot, mesh_shape, cell_size, ptcl_num, m
     disp_cot = np.zeros((n, 3))
     mesh_cots = []
     for i in range(3):
-        dc, mc = gather_adj(pmid, disp, comps[i], acc_cot[:, i], cell_size, mesh_shape, valid=valid)
-        disp_cot += dc
+        mc = np.zeros(mesh_shape)
+        for lo, hi in _chunks(n):
+            dc, m = gather_adj(pmid[lo:hi], disp[lo:hi], comps[i], acc_cot[lo:hi, i], cell_size, mesh_shape,
+                               valid=_sel(valid, lo, hi))
+            disp_cot[lo:hi] += dc
+            mc += m
         mesh_cots.append(mc)
+    del comps
     # F_i = irfftn(D_i * G * rfftn(S)), D_i = i neg_k_i.  For a real linear map
     # F = A S its transpose A^T is obtained from the adjoint-of-irfftn / rfftn pair:
     # A^T g = irfftn(conj(D_i) G rfftn(g)) exactly, because the composite is a real
@@ -154,12 +207,14 @@ def gravity_vjp(pmid, disp, omega_m, acc_cot, mesh_shape, cell_size, ptcl_num, m
     src_cot = np.zeros(mesh_shape)
     for i in range(3):
         d = 1j * nk[i].reshape(shapes[i])
-        src_cot += np.fft.irfftn(np.conj(d) * g * np.fft.rfftn(mesh_cots[i]), s=mesh_shape, axes=(0, 1, 2))
+        src_cot += irfftn(np.conj(d) * g * rfftn(mesh_cots[i]), mesh_shape)
     src = dens - 1.0
     omega_cot = 1.5 * float(np.sum(src_cot * src))
     dens_cot = src_cot * (1.5 * float(omega_m))
-    dc, _ = scatter_adj(pmid, disp, val, dens_cot, cell_size, mesh_shape, valid=valid)
-    disp_cot += dc
+    for lo, hi in _chunks(n):
+        dc, _ = scatter_adj(pmid[lo:hi], disp[lo:hi], val[lo:hi], dens_cot, cell_size, mesh_shape,
+                            valid=_sel(valid, lo, hi))
+        disp_cot[lo:hi] += dc
     if rho_add is not None:
         return disp_cot, omega_cot, dens_cot
     return disp_cot, omega_cot

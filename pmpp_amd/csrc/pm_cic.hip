@@ -197,7 +197,7 @@ extern "C" {
 
 const char* pm_last_error(void) { return g_last_error.c_str(); }
 int pm_abi_version(void) { return PM_ABI_VERSION; }
-int pm_init(void) { return ensure_device_tables(); }
+int pm_init(void) { return init_device_tables(); }
 
 int pm_timing_enable(int32_t max_events) {
   Timing& t = g_timing;

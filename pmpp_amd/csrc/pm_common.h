@@ -40,7 +40,8 @@ int fail(int code, const std::string& msg);
     if (_st) return _st; \
   } while (0)
 
-int ensure_device_tables();  // twiddle tables for the current device
+int init_device_tables();     // pm_init: upload the twiddle table of the current device (synchronises)
+int require_device_tables();  // error unless pm_init has run on the current device
 
 // Optional per-kernel timing (pm_timing_* in the ABI): when enabled on the calling host
 // thread, mark() records a HIP event on the stream after a kernel launch.

@@ -1,5 +1,6 @@
 // FFT passes (real<->half-spectrum along z, complex along y/x) and the fused spectral
 // force passes of the PM step.  See pm_fft.h for the decomposition.
+#include <atomic>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -15,23 +16,25 @@ static __device__ float2 g_twiddle[kTwN];
 
 namespace {
 std::mutex g_tw_mutex;
-const float2* g_tw_ptr[64] = {nullptr};
+std::atomic<const float2*> g_tw_ptr[64];
 }  // namespace
 
 const float2* twiddle_table() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-  return g_tw_ptr[dev];
+  return g_tw_ptr[dev].load(std::memory_order_acquire);
 }
 
 }  // namespace fft
 
-int ensure_device_tables() {
+// The one place that uploads the table and synchronises (pm_init): every other entry point
+// only enqueues work on its stream, so it can run inside HIP graph capture.
+int init_device_tables() {
   int dev = 0;
   PM_HIP_TRY(hipGetDevice(&dev));
   PM_RETURN_IF(dev < 0 || dev >= 64, PM_ERROR_UNSUPPORTED, "device ordinal >= 64");
   std::lock_guard<std::mutex> lock(fft::g_tw_mutex);
-  if (fft::g_tw_ptr[dev]) return PM_SUCCESS;
+  if (fft::g_tw_ptr[dev].load(std::memory_order_acquire)) return PM_SUCCESS;
   std::vector<float2> host(fft::kTwN);
   for (int i = 0; i < fft::kTwN; ++i) {
     // exp(-2 pi i j / 4096) in double, rounded once to float
@@ -46,7 +49,13 @@ int ensure_device_tables() {
   void* ptr = nullptr;
   PM_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(fft::g_twiddle)));
   PM_HIP_TRY(hipDeviceSynchronize());
-  fft::g_tw_ptr[dev] = static_cast<const float2*>(ptr);
+  fft::g_tw_ptr[dev].store(static_cast<const float2*>(ptr), std::memory_order_release);
+  return PM_SUCCESS;
+}
+
+int require_device_tables() {
+  PM_RETURN_IF(fft::twiddle_table() == nullptr, PM_ERROR_INVALID_ARGUMENT,
+               "pm_init has not run on the current device (it prepares the FFT twiddle table)");
   return PM_SUCCESS;
 }
 
@@ -584,7 +593,7 @@ int pm_rfft3d(const float* in, float* out_complex, const int32_t shape[3], void*
                "NULL argument");
   PM_RETURN_IF(!fft::supported_shape(shape), PM_ERROR_UNSUPPORTED,
                "FFT axes must be powers of two in [2, 2048]");
-  int st = ensure_device_tables();
+  int st = require_device_tables();
   if (st) return st;
   PM_RETURN_IF((const void*)in == (const void*)out_complex, PM_ERROR_INVALID_ARGUMENT,
                "pm_rfft3d needs distinct input and output buffers");
@@ -601,7 +610,7 @@ int pm_irfft3d(float* in_complex, float* out, const int32_t shape[3], void* work
                "NULL argument");
   PM_RETURN_IF(!fft::supported_shape(shape), PM_ERROR_UNSUPPORTED,
                "FFT axes must be powers of two in [2, 2048]");
-  int st = ensure_device_tables();
+  int st = require_device_tables();
   if (st) return st;
   PM_RETURN_IF((const void*)in_complex == (const void*)out, PM_ERROR_INVALID_ARGUMENT,
                "pm_irfft3d needs distinct input and output buffers");

@@ -143,6 +143,26 @@ __device__ __forceinline__ int column_of(int L, int G) {
   return (L % 8) * (G / 8) + L / 8;
 }
 
+// The column walks that read a tile plus its one-cell x/y halo (gather, displacement cotangent):
+// XCD x (workgroups L with L % 8 == x) takes the columns cx in [x R, (x + 1) R), R = nb0 / 8, and
+// visits them in patches of PX x 8 columns, patch after patch, so the workgroups an XCD runs at
+// the same time (about 64) form a compact patch of the (cx, cy) plane and read the halo rows
+// they share into one L2 at nearly the same time, instead of a 64-column strip whose x-halo
+// rows are read again, long after, by the strip beside it.  Other shapes keep column_of.
+__device__ __forceinline__ int patch_column_of(int L, int G, int nb0, int nb1) {
+  const int R = nb0 / 8;
+  const int PX = R < 8 ? R : 8;
+  if (G % 8 != 0 || G != nb0 * nb1 || nb0 % 8 != 0 || nb1 % 8 != 0 || PX == 0 || R % PX != 0) return column_of(L, G);
+  const int x = L % 8, j = L / 8;
+  const int per = PX * 8;
+  const int p = j / per, w = j - p * per;
+  const int npy = nb1 / 8;
+  const int pxi = p / npy, pyi = p - pxi * npy;
+  const int cx = x * R + pxi * PX + w / 8;
+  const int cy = pyi * 8 + (w & 7);
+  return cx * nb1 + cy;
+}
+
 constexpr int kTileThreads = 512;
 constexpr int kBatch = 8;  // independent loads kept in flight per thread in the runtime-shape tile loops
 
@@ -503,7 +523,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_columns(
   __shared__ float res[3][kGatherCap];
   __shared__ float plane[2][3][kPl];  // [parity of bz][component]: lower z plane of brick bz
   const BrickShape<BX, BY, BZ> B(k);
-  const int col = column_of(blockIdx.x, gridDim.x);
+  const int col = patch_column_of(blockIdx.x, gridDim.x, k.nb[0], k.nb[1]);
   const int cx = col / k.nb[1], cy = col - cx * k.nb[1];
   const int rows = B.rows(), P = B.pitch(), nb2 = k.nb[2];
   // the brick above the top one is brick 0 (periodic)
@@ -588,7 +608,7 @@ __global__ void __launch_bounds__(kTileThreads) k_disp_cot_columns(
   __shared__ float res[3][kGatherCap];
   __shared__ float plane[2][4][kPl];  // [parity of bz][mesh]: lower z plane of brick bz
   const BrickShape<BX, BY, BZ> B(k);
-  const int col = column_of(blockIdx.x, gridDim.x);
+  const int col = patch_column_of(blockIdx.x, gridDim.x, k.nb[0], k.nb[1]);
   const int cx = col / k.nb[1], cy = col - cx * k.nb[1];
   const int rows = B.rows(), P = B.pitch(), nb2 = k.nb[2];
   const int nm = val != 0.f ? 4 : 3;
@@ -728,14 +748,14 @@ __global__ void __launch_bounds__(kBlock) k_gather3_bwd_sorted(
 __global__ void __launch_bounds__(kBlock) k_cot_sorted(const int32_t* __restrict__ perm,
                                                        const float* __restrict__ acc_cot, int64_t n,
                                                        float* __restrict__ cot_s, float* __restrict__ pmax) {
-  float mx = 0.f;
+  float mx = 0.f;  // +inf once a non-finite value is seen (fmaxf alone would skip a NaN)
   for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
     const int64_t i = perm[j];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float v = acc_cot[i * 3 + c];
       cot_s[c * n + j] = v;
-      mx = fmaxf(mx, fabsf(v));
+      mx = isfinite(v) ? fmaxf(mx, fabsf(v)) : INFINITY;
     }
   }
   for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
@@ -749,12 +769,16 @@ __global__ void __launch_bounds__(kBlock) k_cot_sorted(const int32_t* __restrict
   }
 }
 
-// scale[0] = 1 / max |acc_cot| (1 when all are zero): the values enter the fixed point <= 1
+// scale[0] = 1 / max |acc_cot| (1 when all are zero; capped at 2^127 when the maximum is so
+// small that its reciprocal overflows): the values enter the fixed point at magnitude <= 1.
+// A non-finite cotangent makes the scale NaN, and the tiled scatter's output scale (the
+// reciprocal) with it, so every mesh cotangent and with them every particle's displacement
+// cotangent and the Omega_m cotangent come out NaN, as the unfused float chain would give.
 __global__ void k_cot_scale(const float* __restrict__ pmax, int nb, float* __restrict__ scale) {
   if (threadIdx.x != 0) return;
   float m = 0.f;
   for (int i = 0; i < nb; ++i) m = fmaxf(m, pmax[i]);
-  scale[0] = m > 0.f ? 1.f / m : 1.f;
+  scale[0] = !(m <= 3.402823466e38f) ? __int_as_float(0x7fc00000) : (m > 0.f ? fminf(1.f / m, 1.7014118e38f) : 1.f);
 }
 
 // Scatter adjoint for the density: disp_cot += v * sum_c dens_cot[c] grad_c / cell, then the
@@ -1269,7 +1293,7 @@ int adjoint_spectral_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b
 int prepare(const pm_gravity_desc* d, int64_t n, void* workspace, size_t workspace_bytes_, bool reverse,
             Layout& L, Buffers& b, Arena& ws, fft::SpecTables& tb, hipStream_t s) {
   PM_TRY(check_desc(d, n));
-  PM_TRY(ensure_device_tables());
+  PM_TRY(require_device_tables());
   size_t need = workspace_bytes(d, n, reverse);
   PM_RETURN_IF(workspace == nullptr || workspace_bytes_ < need, PM_ERROR_WORKSPACE,
                "gravity workspace too small: need " + std::to_string(need) + " bytes");

@@ -10,7 +10,7 @@ int check_sort_args(int64_t n, const int32_t* mesh_shape, const int32_t* brick);
 size_t workspace_bytes(int64_t n, const int32_t* mesh_shape, const int32_t* brick);
 int sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                const int32_t* mesh_shape, float cell_size, const int32_t* brick, int32_t* perm,
-               int32_t* bin_offsets, Arena& ws, hipStream_t s);
+               int32_t* bin_offsets, int16_t* pmid_s, float* disp_s, Arena& ws, hipStream_t s);
 
 // Unstable bucketing by the same key (gravity step): bin_offsets (num_bricks + 2 entries, the
 // last bucket holding invalid particles), perm and the payload gathered into bucket order.

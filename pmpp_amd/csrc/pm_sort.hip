@@ -217,6 +217,20 @@ __global__ void k_bin_bounds(const int32_t* __restrict__ keys, int64_t n, int nu
   for (int b = lo; b <= hi; ++b) bounds[b] = (int32_t)j;
 }
 
+// sorted payload: pmid_s[j] = pmid[perm[j]], disp_s[j] = disp[perm[j]] (either may be null)
+__global__ void k_gather_payload(const int32_t* __restrict__ perm, const int16_t* __restrict__ pmid,
+                                 const float* __restrict__ disp, int64_t n, int16_t* __restrict__ pmid_s,
+                                 float* __restrict__ disp_s) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = perm[j];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (pmid_s) pmid_s[j * 3 + a] = pmid[i * 3 + a];
+      if (disp_s) disp_s[j * 3 + a] = disp[i * 3 + a];
+    }
+  }
+}
+
 int scan_exclusive(const int32_t* in, int32_t* out, int64_t len, int32_t* partial, hipStream_t s) {
   if (len == 0) return PM_SUCCESS;
   int64_t blocks = ceil_div(len, kScanChunk);
@@ -250,10 +264,8 @@ KeyGeom make_key_geom(const int32_t* mesh_shape, float cell, const int32_t* bric
 }
 
 // ---- unstable bucketing by key (the gravity step's binning) ----
-// Each wave groups its lanes by key (leader loop over ballots) and issues one global atomic
-// per distinct key; nearly sorted inputs (particles in Lagrangian order) have few keys per
-// wave.  The order inside a bucket depends on atomic arrival order, which the tile kernels
-// do not see: the scatter sums in exact integer arithmetic and the gather is per particle.
+// The order inside a bucket depends on atomic arrival order, which the tile kernels do not
+// see: the scatter sums in exact integer arithmetic and the gather is per particle.
 __device__ __forceinline__ int raw_cell_key(const KeyGeom& g, const int p[3], const float d[3], bool ok) {
   if (!ok) return g.sentinel;
   int c[3];
@@ -491,7 +503,7 @@ int digit_bits_for(int bits, int& passes) {
 
 int sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                const int32_t* mesh_shape, float cell_size, const int32_t* brick, int32_t* perm,
-               int32_t* bin_offsets, Arena& ws, hipStream_t s) {
+               int32_t* bin_offsets, int16_t* pmid_s, float* disp_s, Arena& ws, hipStream_t s) {
   KeyGeom g = make_key_geom(mesh_shape, cell_size, brick);
   if (n == 0) {
     if (bin_offsets) {
@@ -548,6 +560,11 @@ int sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int
       PM_LAUNCHED("k_bin_bounds", s);
     }
   }
+  if (pmid_s || disp_s) {
+    const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, 256), 8192);
+    hipLaunchKernelGGL(k_gather_payload, dim3(blocks), dim3(256), 0, s, perm, pmid, disp, n, pmid_s, disp_s);
+    PM_LAUNCHED("k_gather_payload", s);
+  }
   return PM_SUCCESS;
 }
 
@@ -569,8 +586,8 @@ int pm_sort_cells_workspace(int64_t n, const int32_t mesh_shape[3], const int32_
 
 int pm_sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                   const int32_t mesh_shape[3], float cell_size, const int32_t brick[3],
-                  int32_t* perm, int32_t* bin_offsets, void* workspace, size_t workspace_bytes,
-                  void* stream) {
+                  int32_t* perm, int32_t* bin_offsets, int16_t* pmid_sorted, float* disp_sorted,
+                  void* workspace, size_t workspace_bytes, void* stream) {
   int st = sort::check_sort_args(n, mesh_shape, brick);
   if (st) return st;
   PM_RETURN_IF(!(cell_size > 0.f), PM_ERROR_INVALID_ARGUMENT, "cell_size must be positive");
@@ -580,8 +597,38 @@ int pm_sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, 
   PM_RETURN_IF(workspace == nullptr || workspace_bytes < need, PM_ERROR_WORKSPACE,
                "sort workspace too small: need " + std::to_string(need) + " bytes");
   Arena ws(workspace, workspace_bytes);
-  return sort::sort_cells(pmid, disp, valid, n, mesh_shape, cell_size, brick, perm, bin_offsets, ws,
-                          as_stream(stream));
+  return sort::sort_cells(pmid, disp, valid, n, mesh_shape, cell_size, brick, perm, bin_offsets, pmid_sorted,
+                          disp_sorted, ws, as_stream(stream));
+}
+
+int pm_bucket_cells_workspace(int64_t n, const int32_t mesh_shape[3], const int32_t brick[3], size_t* bytes) {
+  int st = sort::check_sort_args(n, mesh_shape, brick);
+  if (st) return st;
+  PM_RETURN_IF(bytes == nullptr, PM_ERROR_INVALID_ARGUMENT, "bytes is NULL");
+  *bytes = sort::bucket_workspace_bytes(mesh_shape, brick);
+  return PM_SUCCESS;
+}
+
+int pm_bucket_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                    const int32_t mesh_shape[3], float cell_size, const int32_t brick[3], const int32_t ptcl_grid[3],
+                    int32_t* perm, int32_t* bin_offsets, int16_t* pmid_sorted, float* disp_sorted,
+                    uint8_t* valid_sorted, void* workspace, size_t workspace_bytes, void* stream) {
+  int st = sort::check_sort_args(n, mesh_shape, brick);
+  if (st) return st;
+  PM_RETURN_IF(!(cell_size > 0.f), PM_ERROR_INVALID_ARGUMENT, "cell_size must be positive");
+  PM_RETURN_IF(bin_offsets == nullptr, PM_ERROR_INVALID_ARGUMENT, "bin_offsets NULL");
+  PM_RETURN_IF(n > 0 && (pmid == nullptr || disp == nullptr || perm == nullptr || pmid_sorted == nullptr ||
+                         disp_sorted == nullptr || valid_sorted == nullptr),
+               PM_ERROR_INVALID_ARGUMENT, "pmid/disp/perm/sorted payload NULL");
+  if (ptcl_grid)
+    for (int a = 0; a < 3; ++a)
+      PM_RETURN_IF(ptcl_grid[a] < 0, PM_ERROR_INVALID_ARGUMENT, "ptcl_grid must be non-negative");
+  const size_t need = sort::bucket_workspace_bytes(mesh_shape, brick);
+  PM_RETURN_IF(workspace == nullptr || workspace_bytes < need, PM_ERROR_WORKSPACE,
+               "bucket workspace too small: need " + std::to_string(need) + " bytes");
+  Arena ws(workspace, workspace_bytes);
+  return sort::bucket_cells(pmid, disp, valid, n, mesh_shape, cell_size, brick, ptcl_grid, perm, bin_offsets,
+                            pmid_sorted, disp_sorted, valid_sorted, ws, as_stream(stream));
 }
 
 }  // extern "C"

@@ -168,6 +168,7 @@ def _vjp_call(ptcl, cosmo, conf, acc_cot, kept=None):
     resume = (kept is not None and kept.ws is not None and kept.nbytes == nbytes
               and kept.key == KeptForce.key_of(ptcl, omega))
     if kept is not None:  # the sweep's VJPs all run in its one kept workspace
+        kept.key = None  # the VJP's cotangent meshes overwrite part of the workspace
         if kept.ws is None or kept.ws.numel() < nbytes or kept.ws.device != dev:
             kept.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         kept.nbytes = nbytes
@@ -180,20 +181,20 @@ def _vjp_call(ptcl, cosmo, conf, acc_cot, kept=None):
     valid = _valid_u8(ptcl)
     _lib.call("pm_gravity_vjp_resume" if resume else "pm_gravity_vjp", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(ac),
               ptr(disp_cot), ptr(omega_cot), ptr(ws), nbytes, stream_handle(dev))
-    if kept is not None:
-        kept.key = None  # the VJP's cotangent meshes have overwritten part of the workspace
-    return disp_cot, float(omega_cot.item())
+    return disp_cot, omega_cot
 
 
-def gravity_vjp(ptcl, cosmo, conf, acc_cot, kept=None):
-    """(disp cotangent (N,3) tensor, Omega_m cotangent float) of ``gravity`` for ``acc_cot``.
+def gravity_vjp(ptcl, cosmo, conf, acc_cot, kept=None, device_omega=False):
+    """(disp cotangent (N,3) tensor, Omega_m cotangent) of ``gravity`` for ``acc_cot``.
 
     The reverse of the force step without autograd bookkeeping (``pm_gravity_vjp``), as the
     N-body reverse sweep calls it once per drift stage (integrator.py:620-686).  With a
     ``KeptForce`` that ``gravity_keep`` filled for these same particles, the forward meshes
-    are taken from it (``pm_gravity_vjp_resume``) instead of being recomputed.
+    are taken from it (``pm_gravity_vjp_resume``) instead of being recomputed.  The Omega_m
+    cotangent is a float, or with ``device_omega`` a 0-d device tensor (no host round trip).
     """
-    return _vjp_call(ptcl, cosmo, conf, acc_cot, kept)
+    disp_cot, omega_cot = _vjp_call(ptcl, cosmo, conf, acc_cot, kept)
+    return disp_cot, (omega_cot if device_omega else float(omega_cot.item()))
 
 
 def gravity_keep(ptcl, cosmo, conf, kept):
@@ -207,6 +208,7 @@ def gravity_keep(ptcl, cosmo, conf, kept):
     dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
     n = _check_particles(pm, dp)
     nbytes = _lib.size_query("pm_gravity_vjp_workspace", d, n)
+    kept.key = None  # the call overwrites the workspace: it names nothing until it has succeeded
     if kept.ws is None or kept.ws.numel() < nbytes or kept.ws.device != dev:
         kept.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     kept.nbytes = nbytes

@@ -159,7 +159,9 @@ class AdjointState:
         self.disp_cot = disp_cot.clone().contiguous()
         self.vel_cot = vel_cot.clone().contiguous()
         self.acc_cot = acc_cot.clone().contiguous()
-        self.omega_cot = 0.0
+        # the Omega_m cotangent accumulates on the device (float64): reading it back after every
+        # stage would stall the stream between the per-step kernels
+        self.omega_cot_dev = torch.zeros((), dtype=torch.float64, device=dev)
         self.dot = torch.zeros(1, dtype=torch.float64, device=dev)
         nbytes = _lib.size_query("pm_nbody_adj_workspace")
         self.ws = workspace(nbytes, dev)
@@ -172,8 +174,14 @@ class AdjointState:
     def particles(self):
         return self.ptcl.replace(disp=self.disp, vel=self.vel, acc=self.acc)
 
-    def take_dot(self) -> float:
-        return float(self.dot.item())
+    @property
+    def omega_cot(self) -> float:
+        """The accumulated Omega_m cotangent (one host read)."""
+        return float(self.omega_cot_dev.item())
+
+    def add_dot(self, coef: float) -> None:
+        """omega_cot += coef * (the factor cotangent the last adjoint update wrote), on the device."""
+        self.omega_cot_dev.add_(self.dot[0], alpha=float(coef))
 
 
 def kick_adj(a_acc, a_prev, a_next, st: AdjointState, cosmo, conf):
@@ -181,7 +189,7 @@ def kick_adj(a_acc, a_prev, a_next, st: AdjointState, cosmo, conf):
     K = kick_factor(a_acc, a_prev, a_next, cosmo, conf)
     _lib.call("pm_kick_adj", ptr(st.vel), ptr(st.acc), ptr(st.vel_cot), ptr(st.acc_cot), _n(st.ptcl), _f32(K),
               ptr(st.dot), ptr(st.ws), st.ws_bytes, st.stream)
-    st.omega_cot += _factor_dOm("kick_factor", a_acc, a_prev, a_next, cosmo, conf) * st.take_dot()
+    st.add_dot(_factor_dOm("kick_factor", a_acc, a_prev, a_next, cosmo, conf))
 
 
 def drift_adj_from_output(a_vel, a_prev, a_next, st: AdjointState, cosmo, conf):
@@ -190,7 +198,7 @@ def drift_adj_from_output(a_vel, a_prev, a_next, st: AdjointState, cosmo, conf):
     st.kept.key = None  # the positions change under any kept force meshes
     _lib.call("pm_drift_adj", ptr(st.disp), ptr(st.vel), ptr(st.disp_cot), ptr(st.vel_cot), _n(st.ptcl), _f32(D),
               ptr(st.dot), ptr(st.ws), st.ws_bytes, st.stream)
-    st.omega_cot += _factor_dOm("drift_factor", a_vel, a_prev, a_next, cosmo, conf) * st.take_dot()
+    st.add_dot(_factor_dOm("drift_factor", a_vel, a_prev, a_next, cosmo, conf))
 
 
 def force_adj(a, st: AdjointState, cosmo, conf, correction=None):
@@ -199,13 +207,13 @@ def force_adj(a, st: AdjointState, cosmo, conf, correction=None):
     if correction is None:
         from .gravity import gravity_vjp
 
-        d_disp, d_om = gravity_vjp(st.particles(), cosmo, conf, st.acc_cot, kept=st.kept)
+        d_disp, d_om = gravity_vjp(st.particles(), cosmo, conf, st.acc_cot, kept=st.kept, device_omega=True)
     else:
         from ..corrections.training import corrected_gravity_vjp
 
         d_disp, d_om = corrected_gravity_vjp(a, st.particles(), cosmo, conf, correction, st.acc_cot)
     st.disp_cot += d_disp
-    st.omega_cot += d_om
+    st.omega_cot_dev += d_om
     st.acc_cot.zero_()
 
 

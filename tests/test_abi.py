@@ -59,3 +59,17 @@ def test_null_arguments_are_rejected_before_any_launch():
     lib = _lib.load_library()
     assert lib.pm_gravity(None, None, None, 10, None, None, None, 0, None) == _lib.PM_ERROR_INVALID_ARGUMENT
     assert lib.pm_cic_gather(None, None, None, 5, 1, None, None, None, None) == _lib.PM_ERROR_INVALID_ARGUMENT
+
+
+def test_transforms_require_pm_init():
+    # pm_init is the one entry point that uploads tables and synchronises; before it has run on
+    # a device, the transforming entry points refuse with an error instead of doing it implicitly
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("checked where no device has been initialised")
+    lib = _lib.load_library()
+    dummy = ctypes.c_void_p(256)
+    st = lib.pm_rfft3d(dummy, ctypes.c_void_p(512), _lib.int3((8, 8, 8)), None, 0, None)
+    assert st == _lib.PM_ERROR_INVALID_ARGUMENT
+    assert b"pm_init" in lib.pm_last_error()

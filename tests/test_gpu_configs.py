@@ -1,0 +1,261 @@
+"""The HIP path at the sizes of the benchmark workloads, against the float64 oracle where the
+oracle finishes in seconds and through size-independent properties where it does not.
+
+* 64^3 particles / 128^3 mesh / 128 Mpc/h: white noise (seed 0) -> linear modes -> 2LPT at
+  a = 1/64 -> 10 KDK steps to a = 1, against the oracle's run from the same initial state:
+  trajectory within 1e-4 (relative L2), CIC-density P(k) within 1e-4 per shell.  This grid
+  takes the tiled particle-grid chunks of the bucketing and the (16, 16, 32) brick kernels.
+* 1024-point x and y axes (particle grids (512, 8, 64) and (8, 512, 64) on a 2x mesh): the
+  banded column passes at 1024 points (forward, Green + two inverses, two-output y inverse,
+  adjoint x pass) against the oracle: force 1e-5, VJP 1e-3.
+* 256^3 particles / 512^3 mesh / 256 Mpc/h (lattice + jitter + 2LPT at a = 1): the force
+  against the float64 oracle at 1e-5 (relative L2).
+* 512^3 particles / 1024^3 mesh / 1000 Mpc/h, the benchmark's own input: momentum
+  conservation, linearity in Omega_m, the fused step against the unfused chain of single
+  operators (scatter, rfftn, spectral multiply, irfftn, gather), the fused VJP against the
+  unfused adjoint chain, and bitwise independence of the particle order.
+* 128^3 particles / 256^3 mesh, 16 steps: the gradient of a log-P(k) loss with respect to
+  the white noise (hand-written adjoints of P(k), CIC, the N-body sweep and LPT) against
+  float32 central differences (eps = 1e-3) along 8 random white-noise Fourier modes.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cic as ocic
+from oracle import nbody as onb
+from oracle import power_spectrum as ops
+from oracle import spectral
+from pmpp_amd import Configuration, Particles, gravity
+from pmpp_amd.analysis import density_to_pk, density_to_pk_vjp
+from pmpp_amd.cic import scatter
+from pmpp_amd.cic.kernels import hip_gather, hip_gather_bwd, hip_scatter, hip_scatter_bwd
+from pmpp_amd.initial_conditions import LCDM, linear_modes, lpt, lpt_vjp, particles_2lpt, white_noise
+from pmpp_amd.nbody.gravity import KeptForce, gravity_desc, gravity_keep, gravity_vjp
+from pmpp_amd.nbody.solver import nbody
+from pmpp_amd.numerics import irfftn, rfftn, spectral_force_multiply
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _masks(conf):
+    return spectral.particle_nyquist_masks(conf.mesh_shape, conf.ptcl_grid_shape, conf.ptcl_spacing, conf.cell_size)
+
+
+def _np(t, dtype=np.float64):
+    return t.detach().cpu().numpy().astype(dtype)
+
+
+def test_c1_plain_run_matches_float64_oracle(device):
+    a0 = 1 / 64
+    conf = Configuration(2.0, (64,) * 3, mesh_shape=2, a_start=a0, a_nbody_maxstep=(1 - a0) / 10)
+    assert len(conf.a_nbody) == 11
+    cos = LCDM()
+    p0 = lpt(linear_modes(white_noise(0, conf, device=device), cos, conf), cos, conf)
+    out = nbody(p0, cos, conf)
+    print(f"[c1] GPU run done: {conf.ptcl_num} particles, mesh {conf.mesh_shape}", flush=True)
+    S = onb.Setup(conf.mesh_shape, conf.cell_size, conf.ptcl_num, _masks(conf), cos.Omega_m, conf.growth_a,
+                  conf.a_lpt_step)
+    pm = _np(p0.pmid, np.int64)
+    d, v, a = onb.nbody(S, pm, _np(p0.disp), _np(p0.vel), conf.a_nbody)
+    errs = (_rel(_np(out.disp), d), _rel(_np(out.vel), v), _rel(_np(out.acc), a))
+    print(f"[c1] trajectory rel err disp {errs[0]:.2e} vel {errs[1]:.2e} acc {errs[2]:.2e}", flush=True)
+    assert max(errs) < 1e-4
+    # the CIC density power spectrum of the final state
+    k, pk, nm = density_to_pk(scatter(out, conf), conf)
+    val = spectral.default_value(conf.ptcl_num, conf.mesh_shape, conf.ptcl_num)
+    rho = ocic.scatter(pm, d, val, np.zeros(conf.mesh_shape), conf.cell_size, conf.mesh_shape)
+    k0, pk0, nm0 = ops.power_spectrum(rho, conf.cell_size, "CIC")
+    np.testing.assert_array_equal(_np(nm), nm0)
+    perr = np.abs(_np(pk) / pk0 - 1)
+    print(f"[c1] P(k) max rel err {perr.max():.2e} over {len(pk0)} shells", flush=True)
+    assert perr.max() < 1e-4
+
+
+def _jittered(conf, scale, seed, device):
+    base = Particles.gen_grid(conf, device=device)
+    rng = np.random.default_rng(seed)
+    d = base.disp.cpu().numpy() + rng.normal(size=base.disp.shape) * scale * conf.ptcl_spacing
+    return base.replace(disp=torch.as_tensor(d.astype(np.float32), device=device))
+
+
+@pytest.mark.parametrize("grid", [(512, 8, 64), (8, 512, 64)])
+def test_1024_point_column_passes_and_tiled_bucketing(device, grid):
+    conf = Configuration(1.0, grid, mesh_shape=2)
+    assert max(conf.mesh_shape[:2]) == 1024 and conf.brick == (16, 16, 32)
+    ptcl = _jittered(conf, 0.6, sum(grid), device)
+    pm, dp = _np(ptcl.pmid, np.int64), _np(ptcl.disp)
+    acc = _np(gravity(1.0, ptcl, 0.3, conf))
+    ref = spectral.gravity(pm, dp, 0.3, conf.mesh_shape, conf.cell_size, conf.ptcl_num, _masks(conf))
+    err = _rel(acc, ref)
+    print(f"[1024] grid {grid}: force rel err {err:.2e}", flush=True)
+    assert err < 1e-5
+    cot = np.random.default_rng(2).normal(size=acc.shape).astype(np.float32)
+    cot_t = torch.as_tensor(cot, device=device)
+    cosmo = type("C", (), {"Omega_m": 0.3})()
+    d_gpu, o_gpu = gravity_vjp(ptcl, cosmo, conf, cot_t)
+    dref, oref = spectral.gravity_vjp(pm, dp, 0.3, cot, conf.mesh_shape, conf.cell_size, conf.ptcl_num, _masks(conf))
+    gerr = _rel(_np(d_gpu), dref)
+    print(f"[1024] grid {grid}: vjp rel err {gerr:.2e}, Omega_m {o_gpu:.6e} vs {oref:.6e}", flush=True)
+    assert gerr < 1e-3
+    assert o_gpu == pytest.approx(oref, rel=1e-3)
+    # the reverse sweep's kept force and resumed VJP equal the plain calls bit for bit
+    kept = KeptForce()
+    acc_keep = gravity_keep(ptcl, cosmo, conf, kept)
+    d_res, o_res = gravity_vjp(ptcl, cosmo, conf, cot_t, kept=kept)
+    assert np.array_equal(_np(acc_keep, np.float32), acc.astype(np.float32))
+    assert torch.equal(d_res, d_gpu) and o_res == o_gpu
+
+
+def test_c2_force_matches_float64_oracle(device):
+    conf = Configuration(1.0, (256,) * 3, mesh_shape=2)
+    ptcl = particles_2lpt(conf, LCDM(), seed=0, jitter_seed=1, device=device, a=1.0)
+    acc = _np(gravity(1.0, ptcl, 0.3, conf))
+    print(f"[c2] GPU force done: {conf.ptcl_num} particles, mesh {conf.mesh_shape}", flush=True)
+    ref = spectral.gravity(_np(ptcl.pmid, np.int64), _np(ptcl.disp), 0.3, conf.mesh_shape, conf.cell_size,
+                           conf.ptcl_num, _masks(conf))
+    err = _rel(acc, ref)
+    print(f"[c2] force rel err {err:.2e}", flush=True)
+    assert err < 1e-5
+    np.testing.assert_allclose(acc, ref, atol=3e-5 * np.abs(ref).max())
+
+
+def _unfused_force(ptcl, conf, omega, desc):
+    """The force as a chain of single operators: CIC scatter, S = 1.5 Omega_m (rho - 1), rfftn,
+    mask x Green x (-i k_i), irfftn and CIC gather per component (float32 throughout)."""
+    kw = dict(offset=0.0, particle_cell_size=conf.cell_size, global_shape=tuple(conf.mesh_shape))
+    val = float(conf.mesh_size / conf.ptcl_num)
+    rho = hip_scatter(ptcl.pmid, ptcl.disp, val, torch.zeros(conf.mesh_shape, device=ptcl.disp.device), **kw)
+    src = (rho - 1.0) * np.float32(1.5 * omega)
+    shat = rfftn(src)
+    del src
+    comps, acc = [], []
+    for i in range(3):
+        f = irfftn(spectral_force_multiply(shat, desc, i), s=conf.mesh_shape)
+        acc.append(hip_gather(ptcl.pmid, ptcl.disp, f, **kw))
+        comps.append(f)
+    return torch.stack(acc, dim=1), rho, comps
+
+
+def _unfused_vjp(ptcl, conf, omega, desc, rho, comps, cot):
+    """The force VJP as the chain of single adjoints: gather^T per component, the spectral
+    adjoint (conjugate symbol), the source scale and the scatter^T."""
+    kw = dict(offset=0.0, particle_cell_size=conf.cell_size, global_shape=tuple(conf.mesh_shape))
+    disp_cot = torch.zeros_like(ptcl.disp)
+    src_cot = torch.zeros(conf.mesh_shape, device=ptcl.disp.device)
+    for i in range(3):
+        dc, mc = hip_gather_bwd(ptcl.pmid, ptcl.disp, comps[i], cot[:, i].contiguous(), **kw)
+        disp_cot += dc
+        src_cot += irfftn(spectral_force_multiply(rfftn(mc), desc, i, adjoint=True), s=conf.mesh_shape)
+        del mc
+    omega_cot = 1.5 * float((src_cot.double() * (rho.double() - 1.0)).sum())
+    dens_cot = src_cot * np.float32(1.5 * omega)
+    dc, _ = hip_scatter_bwd(ptcl.pmid, ptcl.disp, float(conf.mesh_size / conf.ptcl_num), dens_cot, **kw)
+    return disp_cot + dc, omega_cot
+
+
+def test_c5_full_size_properties(device):
+    conf = Configuration(1000.0 / 512, (512,) * 3, mesh_shape=2)
+    cos = LCDM()
+    ptcl = particles_2lpt(conf, cos, seed=0, jitter_seed=1, device=device, a=1.0)
+    omega = cos.Omega_m
+    acc = gravity(1.0, ptcl, omega, conf)
+    print(f"[c5] force done: {conf.ptcl_num} particles, mesh {conf.mesh_shape}", flush=True)
+    # momentum: sum_p acc_p = sum_g F_g rho_g / v vanishes (the force operator is antisymmetric)
+    tot = acc.double().sum(0).abs()
+    scale = acc.double().abs().sum(0)
+    print(f"[c5] |sum acc| / sum |acc| = {(tot / scale).tolist()}", flush=True)
+    assert float((tot / scale).max()) < 1e-6
+    # linear in Omega_m (the fused 1.5 Omega_m source)
+    acc2 = gravity(1.0, ptcl, 2 * omega, conf)
+    lin = float((acc2 - 2 * acc).norm() / (2 * acc).norm())
+    print(f"[c5] linearity in Omega_m: {lin:.2e}", flush=True)
+    assert lin < 1e-6
+    del acc2
+    # the fused step against the chain of single operators
+    desc = gravity_desc(conf, omega)
+    acc_u, rho, comps = _unfused_force(ptcl, conf, omega, desc)
+    fe = float((acc - acc_u).norm() / acc_u.norm())
+    print(f"[c5] fused vs unfused force: {fe:.2e}", flush=True)
+    assert fe < 1e-5
+    # the fused VJP against the chain of single adjoints
+    g = torch.Generator(device=device).manual_seed(3)
+    cot = torch.randn(acc.shape, device=device, generator=g)
+    d_f, o_f = gravity_vjp(ptcl, cos, conf, cot)
+    d_u, o_u = _unfused_vjp(ptcl, conf, omega, desc, rho, comps, cot)
+    del rho, comps
+    ve = float((d_f - d_u).norm() / d_u.norm())
+    print(f"[c5] fused vs unfused VJP: {ve:.2e}; Omega_m cotangent {o_f:.6e} vs {o_u:.6e}", flush=True)
+    assert ve < 1e-5
+    assert o_f == pytest.approx(o_u, rel=1e-5)
+    # the unstable bucketing and the fixed-point scatter make the force independent of the
+    # particle order, bit for bit
+    perm = torch.randperm(len(ptcl), generator=torch.Generator(device=device).manual_seed(4), device=device)
+    shuffled = ptcl.replace(pmid=ptcl.pmid[perm].contiguous(), disp=ptcl.disp[perm].contiguous(),
+                            unused_index=None)
+    assert torch.equal(gravity(1.0, shuffled, omega, conf), acc[perm])
+
+
+def test_c3_gradient_matches_central_differences(device):
+    a0 = 1 / 64
+    conf = Configuration(2.0, (128,) * 3, mesh_shape=2, a_start=a0, a_nbody_maxstep=(1 - a0) / 16)
+    assert len(conf.a_nbody) == 17
+    cos = LCDM()
+    w = white_noise(2, conf, real=True, device=device)
+    nb = 32
+
+    def forward(noise):
+        p0 = lpt(linear_modes(noise, cos, conf), cos, conf)
+        return p0
+
+    def loss_of(dens, target):
+        _, pk, _ = density_to_pk(dens, conf)
+        p = pk.double()[:nb]
+        return float(((p.log() - target.log()) ** 2).mean()), pk
+
+    with torch.no_grad():
+        fin = nbody(forward(w), cos, conf)
+        _, pk = loss_of(scatter(fin, conf), torch.ones(nb, dtype=torch.float64, device=device))
+    target = (1.2 * pk.double()[:nb] * torch.linspace(0.9, 1.1, nb, dtype=torch.float64, device=device)).detach()
+
+    # hand-written reverse chain: P(k) adjoint, scatter adjoint, N-body sweep, LPT adjoint
+    p0 = forward(w)
+    disp0 = p0.disp.clone().requires_grad_(True)
+    vel0 = p0.vel.clone().requires_grad_(True)
+    fin = nbody(p0.replace(disp=disp0, vel=vel0), cos, conf)
+    dens = scatter(fin, conf)
+    _, pk = loss_of(dens.detach(), target)
+    p = pk.double().cpu().numpy()
+    t = target.cpu().numpy()
+    pk_cot = np.zeros(len(p))
+    pk_cot[:nb] = 2 * (np.log(p[:nb]) - np.log(t)) / (nb * p[:nb])
+    dens.backward(density_to_pk_vjp(dens.detach(), conf, pk_cot))
+    g = lpt_vjp(w, cos, conf, disp0.grad, vel0.grad).double()
+    print("[c3] adjoint done", flush=True)
+
+    # central differences along single white-noise Fourier modes (real-space plane waves)
+    shape = conf.ptcl_grid_shape
+    rng = np.random.default_rng(8)
+    eps = 1e-3
+    got, fds = [], []
+    for _ in range(8):
+        e = torch.zeros((shape[0], shape[1], shape[2] // 2 + 1), dtype=torch.complex64)
+        m = rng.integers(1, 6, size=3) * rng.choice([-1, 1], size=3)
+        ix, iy, iz = int(m[0]) % shape[0], int(m[1]) % shape[1], abs(int(m[2]))
+        e[ix, iy, iz] = 1.0 if rng.random() < 0.5 else 1j
+        v = irfftn(e.to(device), s=shape) * math.sqrt(math.prod(shape))  # orthonormal inverse
+        with torch.no_grad():
+            lp, _ = loss_of(scatter(nbody(forward(w + eps * v), cos, conf), conf), target)
+            lm, _ = loss_of(scatter(nbody(forward(w - eps * v), cos, conf), conf), target)
+        fds.append((lp - lm) / (2 * eps))
+        got.append(float((g * v.double()).sum()))
+    got, fds = np.array(got), np.array(fds)
+    print("[c3] adjoint vs central differences:\n", np.c_[got, fds], flush=True)
+    np.testing.assert_allclose(got, fds, rtol=2e-2, atol=2e-2 * np.abs(fds).max())

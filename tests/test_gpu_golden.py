@@ -63,7 +63,7 @@ def test_sort_matches_golden(device, with_valid):
     perm = torch.empty(n, dtype=torch.int32, device=device)
     offs = torch.empty(len(g["bins"]), dtype=torch.int32, device=device)
     _lib.call("pm_sort_cells", ptr(P), ptr(D), ptr(V), n, _lib.int3(ms), ctypes.c_float(float(g["cell"])),
-              _lib.int3(brick), ptr(perm), ptr(offs), ptr(ws), nbytes, stream_handle(device))
+              _lib.int3(brick), ptr(perm), ptr(offs), None, None, ptr(ws), nbytes, stream_handle(device))
     torch.cuda.synchronize()
     np.testing.assert_array_equal(perm.cpu().numpy(), g["perm"] if with_valid else g["perm_all"])
     if with_valid:

@@ -74,3 +74,30 @@ def test_single_mode_power_is_exact(device):
     s = 3 - 1
     assert pk[s] == pytest.approx(2 * A * A * 16 ** 3 / nm[s], rel=1e-5)
     assert np.abs(np.delete(pk, s)).max() < 1e-9
+
+
+def test_cic_window_of_a_randomly_placed_particle_on_the_gpu(device):
+    # the device shell sums against the closed form the oracle is pinned by: averaged over its
+    # offset in a cell, a CIC particle's per-mode power is prod_i [1 - 2/3 sin^2(pi m_i/N)],
+    # and the CIC deconvolution divides it by prod_i sinc^4(m_i/N)
+    from pmpp_amd.cic import scatter
+    from pmpp_amd import Particles
+    from tests.test_oracle_power_spectrum import _expected_shell_mean
+
+    n, cell = 16, 1.5
+    conf = Configuration(cell, (n,) * 3, mesh_shape=1)
+    g = np.array([0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0)])
+    offs = np.array([[a, b, c] for a in g for b in g for c in g]) * cell
+    pks = []
+    for o in offs:
+        p = Particles(conf, torch.tensor([[3, 5, 7]], dtype=torch.int16, device=device),
+                      torch.tensor([o], dtype=torch.float32, device=device))
+        mesh = scatter(p, conf, val=torch.ones(1, device=device))
+        pks.append(delta_to_pk(mesh, conf, mas="CIC")[1].cpu().numpy())
+
+    def per_mode(a, b, c):
+        return np.prod([(1 - 2.0 / 3.0 * np.sin(np.pi * x / n) ** 2) / np.sinc(x / n) ** 4 for x in (a, b, c)])
+
+    pk = np.mean(pks, axis=0)
+    exp = _expected_shell_mean(n, per_mode)[:len(pk)] * cell ** 3 / n ** 3
+    np.testing.assert_allclose(pk, exp, rtol=1e-5)

@@ -17,12 +17,30 @@ def _dft3_bruteforce(x):
     return out
 
 
-def test_numpy_rfftn_matches_bruteforce_dft():
+def test_oracle_rfftn_matches_bruteforce_dft():
     rng = np.random.default_rng(0)
     x = rng.normal(size=(4, 6, 8))
     full = _dft3_bruteforce(x)
+    np.testing.assert_allclose(spectral.rfftn(x), full[..., :5], atol=1e-11)
     np.testing.assert_allclose(np.fft.rfftn(x), full[..., :5], atol=1e-11)
-    np.testing.assert_allclose(np.fft.irfftn(np.fft.rfftn(x), s=x.shape), x, atol=1e-13)
+    np.testing.assert_allclose(spectral.irfftn(spectral.rfftn(x), x.shape), x, atol=1e-13)
+
+
+def test_chunked_cic_equals_single_pass(monkeypatch):
+    # the particle chunking of the large-mesh oracle changes only the summation order
+    rng = np.random.default_rng(4)
+    ms, n = (8, 6, 10), 1000
+    pm = rng.integers(0, 6, (n, 3))
+    dp = rng.normal(size=(n, 3)) * 2.0
+    val = rng.random(n)
+    valid = rng.random(n) > 0.2
+    ref = cic.scatter(pm, dp, val, np.zeros(ms), 0.7, ms, valid=valid)
+    meshes = [rng.normal(size=ms) for _ in range(3)]
+    gref = np.stack([cic.gather(pm, dp, m, 0.7, ms, valid=valid) for m in meshes], axis=-1)
+    monkeypatch.setattr(spectral, "CHUNK", 97)
+    np.testing.assert_allclose(spectral.scatter_chunked(pm, dp, val, ms, 0.7, valid=valid), ref, rtol=1e-13, atol=1e-13)
+    np.testing.assert_array_equal(spectral.gather_chunked(pm, dp, meshes, 0.7, ms, valid=valid), gref)
+    np.testing.assert_array_equal(cic.gather_many(pm, dp, meshes, 0.7, ms, valid=valid), gref)
 
 
 def test_poisson_symbol_and_nyquist_support_closed_forms():
