@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 2
+#define PM_ABI_VERSION 3
 
 enum pm_status {
   PM_SUCCESS = 0,
@@ -75,7 +75,8 @@ typedef struct pm_gravity_desc {
 const char* pm_last_error(void);
 int pm_abi_version(void);
 /* Prepares the per-device constant tables (the FFT twiddle table) on the current device and
- * synchronises it: the only entry point that does either.  It must run once per device
+ * synchronises it: the only entry point that does either.  It also resolves the current
+ * device's address of the static scratch that pm_bucket_cells_noalloc uses.  It must run once per device
  * before any entry point that transforms (pm_rfft3d, pm_irfft3d, the pm_gravity family);
  * those return PM_ERROR_INVALID_ARGUMENT until it has.  Thread-safe; later calls return at
  * once.  (The reference's FFI library has no such step: XLA's FFT needs none.) */
@@ -146,6 +147,19 @@ int pm_bucket_cells(const int16_t* pmid, const float* disp, const uint8_t* valid
                     const int32_t ptcl_grid[3], int32_t* perm, int32_t* bin_offsets, int16_t* pmid_sorted,
                     float* disp_sorted, uint8_t* valid_sorted, void* workspace, size_t workspace_bytes,
                     void* stream);
+/* pm_bucket_cells without a workspace: no counterpart in the reference.  It is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc): the bin counters and scan partials live in a static 8 MiB device
+ * array of the library (one per device, held until the process ends), so the call allocates
+ * nothing.  Same arguments, outputs and errors as pm_bucket_cells; meshes with more than
+ * 2,096,446 bricks return PM_ERROR_UNSUPPORTED, and PM_ERROR_INVALID_ARGUMENT until pm_init
+ * has run on the current device.  Every call on a device shares that device's scratch (the
+ * counters are cleared by a kernel at the start of each call): calls that overlap on one
+ * device, on different streams, race. */
+int pm_bucket_cells_noalloc(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                            const int32_t mesh_shape[3], float cell_size, const int32_t brick[3],
+                            const int32_t ptcl_grid[3], int32_t* perm, int32_t* bin_offsets, int16_t* pmid_sorted,
+                            float* disp_sorted, uint8_t* valid_sorted, void* stream);
 
 /* --------------------------------------------------------------- FFT -----
  * pm_rfft3d  = jnp.fft.rfftn  (unnormalised forward)     used at nbody/gravity.py:436

@@ -14,7 +14,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 PM_SUCCESS = 0
 PM_ERROR_INVALID_ARGUMENT = 1
@@ -28,6 +28,7 @@ EXPORTED_SYMBOLS = (
     "pm_timing_enable", "pm_timing_mark", "pm_timing_read",
     "pm_cic_scatter", "pm_cic_gather", "pm_cic_gather_bwd", "pm_cic_scatter_bwd",
     "pm_sort_cells_workspace", "pm_sort_cells", "pm_bucket_cells_workspace", "pm_bucket_cells",
+    "pm_bucket_cells_noalloc",
     "pm_fft_workspace", "pm_rfft3d", "pm_irfft3d",
     "pm_spectral_workspace", "pm_spectral_force_multiply",
     "pm_gravity_workspace", "pm_gravity_vjp_workspace", "pm_gravity", "pm_gravity_vjp",
@@ -96,6 +97,8 @@ _SIGNATURES = {
     "pm_bucket_cells_workspace": ([_i64, _i32x3, _i32x3, _sz_p], ctypes.c_int),
     "pm_bucket_cells": ([_vp, _vp, _vp, _i64, _i32x3, _f32, _i32x3, _i32x3, _vp, _vp, _vp, _vp, _vp, _vp,
                          ctypes.c_size_t, _vp], ctypes.c_int),
+    "pm_bucket_cells_noalloc": ([_vp, _vp, _vp, _i64, _i32x3, _f32, _i32x3, _i32x3, _vp, _vp, _vp, _vp, _vp, _vp],
+                                ctypes.c_int),
     "pm_fft_workspace": ([_i32x3, _sz_p], ctypes.c_int),
     "pm_rfft3d": ([_vp, _vp, _i32x3, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_irfft3d": ([_vp, _vp, _i32x3, _vp, ctypes.c_size_t, _vp], ctypes.c_int),

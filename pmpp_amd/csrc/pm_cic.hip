@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "pm_common.h"
+#include "pm_sort.h"
 
 namespace pm {
 
@@ -197,7 +198,11 @@ extern "C" {
 
 const char* pm_last_error(void) { return g_last_error.c_str(); }
 int pm_abi_version(void) { return PM_ABI_VERSION; }
-int pm_init(void) { return init_device_tables(); }
+int pm_init(void) {
+  int st = init_device_tables();
+  if (st) return st;
+  return sort::init_noalloc_scratch();
+}
 
 int pm_timing_enable(int32_t max_events) {
   Timing& t = g_timing;

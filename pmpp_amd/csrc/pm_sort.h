@@ -21,5 +21,8 @@ int bucket_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, i
                  float cell_size, const int32_t* brick, const int32_t* ptcl_grid, int32_t* perm, int32_t* bin_offsets,
                  int16_t* pmid_s, float* disp_s, uint8_t* valid_s, Arena& ws, hipStream_t s);
 
+// pm_init: resolve the current device's address of the allocation-free binning scratch.
+int init_noalloc_scratch();
+
 }  // namespace sort
 }  // namespace pm

@@ -10,6 +10,7 @@
 // The key of the first pass is computed on the fly from (pmid, disp): the raveled brick
 // index of the containing cell, with invalid particles mapped to the sentinel.
 #include <algorithm>
+#include <atomic>
 
 #include "pm_common.h"
 #include "pm_sort.h"
@@ -568,6 +569,46 @@ int sort_cells(const int16_t* pmid, const float* disp, const uint8_t* valid, int
   return PM_SUCCESS;
 }
 
+// ------------------------------------------------ allocation-free binning ----
+// pm_bucket_cells_noalloc takes its bin counters and scan partials from this file-scope
+// device array instead of a caller workspace.  On the host the array's name is only a
+// stand-in, so host code never passes it to a kernel or a copy: pm_init resolves the
+// array's address on each device once (hipGetSymbolAddress, with that device current) and
+// the entry point uses the address of the current device.  Every call on a device shares
+// that device's copy, so calls that overlap on one device (two streams) race.
+constexpr size_t kNoAllocScratchBytes = size_t(8) << 20;  // 8 MiB: up to 2,096,446 bricks
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::atomic<void*> g_scratch_ptr[kMaxDevices];
+
+int init_noalloc_scratch() {
+  int dev = 0;
+  PM_HIP_TRY(hipGetDevice(&dev));
+  PM_RETURN_IF(dev < 0 || dev >= kMaxDevices, PM_ERROR_UNSUPPORTED, "device ordinal >= 64");
+  if (g_scratch_ptr[dev].load(std::memory_order_acquire)) return PM_SUCCESS;
+  void* p = nullptr;
+  PM_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  PM_RETURN_IF(p == nullptr, PM_ERROR_HIP, "hipGetSymbolAddress returned NULL for the binning scratch");
+  g_scratch_ptr[dev].store(p, std::memory_order_release);
+  return PM_SUCCESS;
+}
+
+int check_bucket_args(const int16_t* pmid, const float* disp, int64_t n, const int32_t* mesh_shape, float cell_size,
+                      const int32_t* brick, const int32_t* ptcl_grid, const int32_t* perm, const int32_t* bin_offsets,
+                      const int16_t* pmid_s, const float* disp_s, const uint8_t* valid_s) {
+  int st = check_sort_args(n, mesh_shape, brick);
+  if (st) return st;
+  PM_RETURN_IF(!(cell_size > 0.f), PM_ERROR_INVALID_ARGUMENT, "cell_size must be positive");
+  PM_RETURN_IF(bin_offsets == nullptr, PM_ERROR_INVALID_ARGUMENT, "bin_offsets NULL");
+  PM_RETURN_IF(n > 0 && (pmid == nullptr || disp == nullptr || perm == nullptr || pmid_s == nullptr ||
+                         disp_s == nullptr || valid_s == nullptr),
+               PM_ERROR_INVALID_ARGUMENT, "pmid/disp/perm/sorted payload NULL");
+  if (ptcl_grid)
+    for (int a = 0; a < 3; ++a)
+      PM_RETURN_IF(ptcl_grid[a] < 0, PM_ERROR_INVALID_ARGUMENT, "ptcl_grid must be non-negative");
+  return PM_SUCCESS;
+}
+
 }  // namespace sort
 }  // namespace pm
 
@@ -613,20 +654,36 @@ int pm_bucket_cells(const int16_t* pmid, const float* disp, const uint8_t* valid
                     const int32_t mesh_shape[3], float cell_size, const int32_t brick[3], const int32_t ptcl_grid[3],
                     int32_t* perm, int32_t* bin_offsets, int16_t* pmid_sorted, float* disp_sorted,
                     uint8_t* valid_sorted, void* workspace, size_t workspace_bytes, void* stream) {
-  int st = sort::check_sort_args(n, mesh_shape, brick);
+  int st = sort::check_bucket_args(pmid, disp, n, mesh_shape, cell_size, brick, ptcl_grid, perm, bin_offsets,
+                                   pmid_sorted, disp_sorted, valid_sorted);
   if (st) return st;
-  PM_RETURN_IF(!(cell_size > 0.f), PM_ERROR_INVALID_ARGUMENT, "cell_size must be positive");
-  PM_RETURN_IF(bin_offsets == nullptr, PM_ERROR_INVALID_ARGUMENT, "bin_offsets NULL");
-  PM_RETURN_IF(n > 0 && (pmid == nullptr || disp == nullptr || perm == nullptr || pmid_sorted == nullptr ||
-                         disp_sorted == nullptr || valid_sorted == nullptr),
-               PM_ERROR_INVALID_ARGUMENT, "pmid/disp/perm/sorted payload NULL");
-  if (ptcl_grid)
-    for (int a = 0; a < 3; ++a)
-      PM_RETURN_IF(ptcl_grid[a] < 0, PM_ERROR_INVALID_ARGUMENT, "ptcl_grid must be non-negative");
   const size_t need = sort::bucket_workspace_bytes(mesh_shape, brick);
   PM_RETURN_IF(workspace == nullptr || workspace_bytes < need, PM_ERROR_WORKSPACE,
                "bucket workspace too small: need " + std::to_string(need) + " bytes");
   Arena ws(workspace, workspace_bytes);
+  return sort::bucket_cells(pmid, disp, valid, n, mesh_shape, cell_size, brick, ptcl_grid, perm, bin_offsets,
+                            pmid_sorted, disp_sorted, valid_sorted, ws, as_stream(stream));
+}
+
+int pm_bucket_cells_noalloc(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                            const int32_t mesh_shape[3], float cell_size, const int32_t brick[3],
+                            const int32_t ptcl_grid[3], int32_t* perm, int32_t* bin_offsets, int16_t* pmid_sorted,
+                            float* disp_sorted, uint8_t* valid_sorted, void* stream) {
+  int st = sort::check_bucket_args(pmid, disp, n, mesh_shape, cell_size, brick, ptcl_grid, perm, bin_offsets,
+                                   pmid_sorted, disp_sorted, valid_sorted);
+  if (st) return st;
+  const size_t need = sort::bucket_workspace_bytes(mesh_shape, brick);
+  PM_RETURN_IF(need > sort::kNoAllocScratchBytes, PM_ERROR_UNSUPPORTED,
+               "pm_bucket_cells_noalloc: " + std::to_string(need) + " bytes of bin counters exceed its " +
+                   std::to_string(sort::kNoAllocScratchBytes) +
+                   "-byte static scratch (at most 2,096,446 bricks): use pm_bucket_cells with a workspace");
+  int dev = 0;
+  PM_HIP_TRY(hipGetDevice(&dev));
+  void* p = (dev >= 0 && dev < sort::kMaxDevices) ? sort::g_scratch_ptr[dev].load(std::memory_order_acquire)
+                                                  : nullptr;
+  PM_RETURN_IF(p == nullptr, PM_ERROR_INVALID_ARGUMENT,
+               "pm_init has not run on the current device (it resolves the binning scratch)");
+  Arena ws(p, sort::kNoAllocScratchBytes);
   return sort::bucket_cells(pmid, disp, valid, n, mesh_shape, cell_size, brick, ptcl_grid, perm, bin_offsets,
                             pmid_sorted, disp_sorted, valid_sorted, ws, as_stream(stream));
 }
