@@ -194,15 +194,15 @@ def cpu_baseline(dev, omega_m):
 
 def nbody_run(args):
     """The full single-GPU run of the 512^3 / 1024^3 / 1000 Mpc/h workload: 2LPT at a = 1/64,
-    20 KDK steps to a = 1 (plain), the reverse sweep of that run, the same run with the learned
-    correction (BF16 network, forward: the three-layer network stem -> hidden -> output, and the
-    four-layer one with a residual block), the memory high-water mark, and the same plain run on
-    a 2048^3 mesh (4x the particle grid) for comparison; every run is timed after one warm run.
+    20 KDK steps to a = 1 (plain), the reverse sweep of that run, the same run with the shipped
+    trained correction (the three-layer BF16 network) and its reverse sweep (the network's
+    backward over x-slabs), the memory high-water marks, and the same plain run on a 2048^3 mesh
+    (4x the particle grid) for comparison; every forward run is timed after one warm run.
     Prints one JSON line (not the headline metric)."""
     import torch
 
     from pmpp_amd import Configuration
-    from pmpp_amd.corrections import init_mesh_cnn_potential_correction
+    from pmpp_amd.corrections import load_correction
     from pmpp_amd.initial_conditions import LCDM, linear_modes, lpt, white_noise
     from pmpp_amd.nbody.solver import nbody, nbody_adj
 
@@ -234,14 +234,24 @@ def nbody_run(args):
     log(f"[run] reverse sweep in {t_adj:.3f} s")
     del cot, out
     corrected = {}
-    for name, depth, block in (("three_layer", 2, "plain"), ("residual_block", 2, "residual")):
-        model = init_mesh_cnn_potential_correction(seed=0, depth=depth, block=block, device=dev)
-        nbody(ptcl, cosmo, conf, correction=model, observer=lambda a, p: None)  # warm (allocator)
-        _, t_corr = timed(lambda: nbody(ptcl, cosmo, conf, correction=model, observer=lambda a, p: None))
-        corrected[name] = {"conv_layers": len(model.layers), "steps_per_s": nsteps / t_corr, "s": t_corr}
-        log(f"[run] corrected ({name}, {len(model.layers)} conv layers): {nsteps} steps in {t_corr:.3f} s")
-        del model
+    model = load_correction(device=dev)
+    nbody(ptcl, cosmo, conf, correction=model, observer=lambda a, p: None)  # warm (allocator)
+    out, t_corr = timed(lambda: nbody(ptcl, cosmo, conf, correction=model, observer=lambda a, p: None))
+    corrected = {"conv_layers": len(model.layers), "steps_per_s": nsteps / t_corr, "s": t_corr,
+                 "weights": "shipped trained correction"}
+    log(f"[run] corrected ({len(model.layers)} conv layers, trained): {nsteps} steps in {t_corr:.3f} s")
     peak = torch.cuda.max_memory_allocated() / 2 ** 30
+    cot = out.replace(disp=torch.randn(out.disp.shape, generator=g).to(dev), vel=torch.zeros_like(out.vel),
+                      acc=torch.zeros_like(out.acc))
+    torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats()
+    _, t_cadj = timed(lambda: nbody_adj(out, cot, cosmo, conf, correction=model))
+    peak_cadj = torch.cuda.max_memory_allocated() / 2 ** 30
+    log(f"[run] reverse sweep with the correction in {t_cadj:.3f} s, memory high-water mark {peak_cadj:.1f} GiB")
+    corrected["reverse_steps_per_s"] = nsteps / t_cadj
+    corrected["reverse_s"] = t_cadj
+    corrected["reverse_memory_high_water_GiB"] = round(peak_cadj, 2)
+    del cot, out, model
     torch.cuda.empty_cache()
     conf4 = conf.replace(mesh_shape=4)
     nbody(ptcl, cosmo, conf4, observer=lambda a, p: None)  # warm (allocator), as for the corrected runs
@@ -251,16 +261,207 @@ def nbody_run(args):
     res = {
         "metric": "N-body run steps/s (512^3 particles, 1024^3 mesh, 1000 Mpc/h, 20 KDK steps)",
         "steps_per_s_plain": nsteps / t_plain, "steps_per_s_reverse": nsteps / t_adj,
-        "steps_per_s_corrected": corrected["three_layer"]["steps_per_s"], "corrected": corrected,
+        "steps_per_s_corrected": corrected["steps_per_s"], "corrected": corrected,
         "memory_high_water_GiB": round(peak, 2),
         "steps_per_s_plain_2048_mesh": nsteps / t4,
-        "corrected_vs_plain_2048_wall": corrected["three_layer"]["s"] / t4,
+        "corrected_vs_plain_2048_wall": corrected["s"] / t4,
         "n_gpus": 1, "dtype": "f32 (network bf16)",
         "data": "synthetic (2LPT LCDM field from seeded white noise)",
         "config": {"workload": "N-body run: 512^3 particles, 1024^3 mesh, 1000 Mpc/h, a = 1/64 -> 1",
                    "steps": nsteps, "comparison_mesh": 2048},
     }
     print(json.dumps(res), flush=True)
+
+
+def _parse_floats(text):
+    return [float(v) for v in str(text).split(",") if v]
+
+
+def train_correction_run(args):
+    """The learned-correction workload: 128^3 particles in a 250 Mpc/h box (the 1.95 Mpc/h
+    particle spacing of the full-scale workload), coarse mesh 256^3 (2x the particle grid), fine
+    mesh 512^3 (4x), 2LPT at a = 1/64 then ``--run-steps`` KDK steps to a = 1.
+
+    Stage 1 (force pairs): every other step of the fine runs of seeds 100 .. 100 +
+    ``--train-seeds`` - 1, re-anchored on the coarse mesh with the fine force as the target;
+    each (kernel symmetry, max_residual, learning rate) of the grid trains the three-layer
+    network for ``--train-iters`` Adamax steps on the relative force error.  Stage 2 (whole
+    runs): the best stage-1 network is fine-tuned for ``--e2e-iters`` steps on whole corrected
+    runs of seeds 100 .. 100 + ``--e2e-seeds`` - 1 (loss: mean squared log P(k) error plus the
+    mean 1 - r against the fine run over 0 < k <= 1 h/Mpc, through the N-body reverse sweep),
+    keeping the checkpoint with the best validation score.  Every choice uses the validation
+    seed 999 alone (score: largest |P/P_fine - 1| up to k = 1 h/Mpc plus 30 x any shortfall of
+    the cross-correlation below the plain run's); only the final network is judged, on the
+    held-out seeds 1000 and 1001.  Densities are measured on the coarse mesh.  ``--save PATH``
+    writes the weights there (the package ships them as
+    ``pmpp_amd/corrections/weights/mesh_cnn_2x_from_4x.npz``).  Prints one JSON line."""
+    import torch
+
+    from pmpp_amd import Configuration
+    from pmpp_amd.corrections import (compare_runs, fine_run_pairs, mean_force_loss, pk_errors, save_correction,
+                                      train_correction)
+    from pmpp_amd.corrections.training import RunTarget, train_correction_runs
+    from pmpp_amd.initial_conditions import LCDM
+
+    dev = torch.device("cuda", 0)
+    cosmo = LCDM()
+    ng, box, a0 = 128, 250.0, 1 / 64
+    conf = Configuration(box / ng, (ng,) * 3, mesh_shape=2, a_start=a0, a_nbody_maxstep=(1 - a0) / args.run_steps)
+    conf_fine = conf.replace(mesh_shape=4)
+
+    def judge(model, seed):
+        return pk_errors(compare_runs(model, cosmo, conf, conf_fine, seed=seed, device=dev, measure_conf=conf))
+
+    def score(v):
+        c, p = v["corrected"], v["plain"]
+        return c["pk_err_max"] + 30.0 * max(0.0, p["r_mean"] - c["r_mean"])
+
+    t0 = time.perf_counter()
+    pairs = fine_run_pairs(cosmo, conf, conf_fine, seeds=range(100, 100 + args.train_seeds), every=2, device=dev)
+    val_pairs = fine_run_pairs(cosmo, conf, conf_fine, seeds=[999], every=2, device=dev)
+    torch.cuda.synchronize()
+    log(f"[train] {len(pairs)} training pairs from {args.train_seeds} fine runs in {time.perf_counter() - t0:.1f} s")
+    val_plain = mean_force_loss(None, val_pairs, cosmo, conf)
+    cands = []
+    grid = [(sym, mr, lr) for sym in args.symmetry.split(",") for mr in _parse_floats(args.max_residual)
+            for lr in _parse_floats(args.lr)]
+    for sym, mr, lr in grid:
+        tag = f"symmetry {sym} max_residual {mr:g} lr {lr:g}"
+        t0 = time.perf_counter()
+        model, losses = train_correction(
+            pairs, cosmo, conf, iters=args.train_iters, learning_rate=lr, max_residual=mr, symmetry=sym, device=dev,
+            progress=lambda it, m, tag=tag: log(f"[train] {tag} step {it}: mean training loss {m:.6f}"))
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        val_loss = mean_force_loss(model, val_pairs, cosmo, conf)
+        val = judge(model, 999)
+        q = max(1, len(losses) // 10)
+        cands.append({"symmetry": sym, "max_residual": mr, "lr": lr, "train_s": round(t_train, 2),
+                      "train_loss_first_tenth": float(losses[:q].mean()),
+                      "train_loss_last_tenth": float(losses[-q:].mean()),
+                      "val_force_loss": val_loss, "val_force_loss_plain": val_plain, "val": val, "score": score(val),
+                      "model": model})
+        log(f"[train] {tag}: val force loss {val_loss:.6f} (plain {val_plain:.6f}), "
+            f"val P(k) err to k=1: max {val['corrected']['pk_err_max']:.4f} "
+            f"(plain {val['plain']['pk_err_max']:.4f}), r {val['corrected']['r_mean']:.6f} "
+            f"(plain {val['plain']['r_mean']:.6f})")
+    best = min(cands, key=lambda c: c["score"])
+    model = best["model"]
+    n_pairs = len(pairs)
+    del pairs
+    stage2 = None
+    if args.e2e_iters > 0:
+        t0 = time.perf_counter()
+        targets = [RunTarget(sd, cosmo, conf, conf_fine, device=dev) for sd in range(100, 100 + args.e2e_seeds)]
+        vals = []
+
+        def validate(m):
+            v = judge(m, 999)
+            vals.append(v)
+            log(f"[e2e] validation: P(k) err to k=1 max {v['corrected']['pk_err_max']:.4f} "
+                f"mean {v['corrected']['pk_err_mean']:.4f}, r {v['corrected']['r_mean']:.6f} "
+                f"(plain {v['plain']['r_mean']:.6f})")
+            return score(v)
+
+        e2e_losses, scores = train_correction_runs(
+            model, targets, cosmo, conf, iters=args.e2e_iters, learning_rate=args.e2e_lr, r_weight=args.r_weight,
+            validate=validate, validate_every=args.e2e_validate_every,
+            progress=lambda it, l: log(f"[e2e] step {it}: run loss {l:.6f}") if it % 10 == 0 else None)
+        torch.cuda.synchronize()
+        stage2 = {"seeds": f"100-{99 + args.e2e_seeds}", "iters": args.e2e_iters, "lr": args.e2e_lr,
+                  "r_weight": args.r_weight, "s": round(time.perf_counter() - t0, 2),
+                  "loss_first_tenth": float(e2e_losses[:max(1, len(e2e_losses) // 10)].mean()),
+                  "loss_last_tenth": float(e2e_losses[-max(1, len(e2e_losses) // 10):].mean()),
+                  "validation_scores": [(int(i), round(float(v), 5)) for i, v in scores],
+                  "kept_step": int(min(scores, key=lambda t: t[1])[0])}
+        del targets
+    judged = {}
+    for sd in (1000, 1001):
+        judge_pairs = fine_run_pairs(cosmo, conf, conf_fine, seeds=[sd], every=2, device=dev)
+        judged[sd] = {**judge(model, sd), "force_loss_corrected": mean_force_loss(model, judge_pairs, cosmo, conf),
+                      "force_loss_plain": mean_force_loss(None, judge_pairs, cosmo, conf)}
+        log(f"[train] judge seed {sd}: {json.dumps(judged[sd])}")
+    recipe = {"particles": ng, "box": box, "coarse_mesh": 2 * ng, "fine_mesh": 4 * ng, "kdk_steps": args.run_steps,
+              "a_start": a0, "train_seeds": f"100-{99 + args.train_seeds}", "pairs": n_pairs,
+              "iters": args.train_iters, "validation_seed": 999, "judge_seed": 1000,
+              "max_residual": best["max_residual"], "lr": best["lr"], "symmetry": best["symmetry"],
+              "e2e_iters": args.e2e_iters, "e2e_seeds": f"100-{99 + args.e2e_seeds}", "e2e_lr": args.e2e_lr,
+              "r_weight": args.r_weight}
+    if args.save:
+        save_correction(model, args.save, **recipe)
+        log(f"[train] wrote {args.save}")
+    out = {
+        "metric": "learned correction: P(k) error of the corrected 2x-mesh run vs the 4x-mesh run (held-out seeds)",
+        "recipe": recipe,
+        "judge": judged,
+        "stage1_candidates": [{k: v for k, v in c.items() if k != "model"} for c in cands],
+        "stage2": stage2,
+        "config": {"workload": f"learned correction: {ng}^3 particles, {box:g} Mpc/h, coarse {2 * ng}^3 vs fine "
+                               f"{4 * ng}^3, {args.run_steps} KDK steps"},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def pk_error_run(dev, nsteps=20, seed=0):
+    """The second half of the headline metric at full scale: from one seed's white noise, the
+    512^3-particle / 1000 Mpc/h run (2LPT at a = 1/64, ``nsteps`` KDK steps to a = 1) on the
+    1024^3 mesh with the shipped trained correction, and the plain run on the 2048^3 mesh (4x
+    the particle grid); both final densities are measured on the 1024^3 mesh (CIC, deconvolved),
+    and each run is timed after one warm run.  Returns a dict (P(k) error and cross-correlation
+    over 0 < k <= 1 h/Mpc for the corrected and the plain 2x run, and the wall times)."""
+    import torch
+
+    from pmpp_amd import Configuration
+    from pmpp_amd.analysis import density_to_cross_correlation, density_to_pk
+    from pmpp_amd.cic import scatter
+    from pmpp_amd.corrections import correction_meta, load_correction
+    from pmpp_amd.initial_conditions import LCDM, linear_modes, lpt, white_noise
+    from pmpp_amd.nbody.particles import Particles
+    from pmpp_amd.nbody.solver import nbody
+
+    ng, ratio, box = CONFIGS["c5"]
+    a0 = 1 / 64
+    cosmo = LCDM()
+    conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio, a_start=a0, a_nbody_maxstep=(1 - a0) / nsteps)
+    conf4 = conf.replace(mesh_shape=4)
+    model = load_correction(device=dev)
+    modes = linear_modes(white_noise(seed, conf, device=dev), cosmo, conf)
+
+    def timed(c, corr):
+        p0 = lpt(modes, cosmo, c)
+        nbody(p0, cosmo, c, correction=corr, observer=lambda a, p: None)  # warm (allocator, workspaces)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = nbody(p0, cosmo, c, correction=corr, observer=lambda a, p: None)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        del p0
+        rho = scatter(Particles.from_pos(conf, out.pos(dtype=torch.float64)), conf)
+        del out
+        torch.cuda.empty_cache()
+        return rho, t
+
+    log("[pk] corrected run on the 1024^3 mesh")
+    d_corr, t_corr = timed(conf, model)
+    log(f"[pk] corrected: {nsteps} steps in {t_corr:.3f} s; plain run on the 1024^3 mesh")
+    d_plain, t_plain = timed(conf, None)
+    log(f"[pk] plain 2x: {t_plain:.3f} s; plain run on the 2048^3 mesh")
+    d_fine, t_fine = timed(conf4, None)
+    log(f"[pk] plain 4x: {t_fine:.3f} s")
+    k, pk_fine, _ = density_to_pk(d_fine, conf)
+    _, r_c, _, pk_c, _, _ = density_to_cross_correlation(d_corr, d_fine, conf)
+    _, r_p, _, pk_p, _, _ = density_to_cross_correlation(d_plain, d_fine, conf)
+    k = k.cpu().numpy()
+    sel = (k > 0) & (k <= 1.0)
+    res = {"kmax_h_per_Mpc": 1.0, "shells": int(sel.sum()), "seed": seed, "kdk_steps": nsteps,
+           "measured_on": "1024^3 mesh, CIC deconvolved", "weights": correction_meta()}
+    for name, pk, r in (("corrected_2x", pk_c, r_c), ("plain_2x", pk_p, r_p)):
+        e = abs(pk.cpu().numpy()[sel] / pk_fine.cpu().numpy()[sel] - 1)
+        res[name] = {"pk_err_mean": round(float(e.mean()), 5), "pk_err_max": round(float(e.max()), 5),
+                     "r_mean": round(float(r.cpu().numpy()[sel].mean()), 6)}
+    res["wall_s"] = {"corrected_2x": round(t_corr, 3), "plain_2x": round(t_plain, 3), "plain_4x": round(t_fine, 3)}
+    res["corrected_vs_plain_4x_wall"] = round(t_corr / t_fine, 4)
+    return res
 
 
 def network_run(args):
@@ -380,6 +581,24 @@ def main():
                     help="time the learned correction's inference network on the workload's coarse mesh instead")
     ap.add_argument("--vjp", action="store_true",
                     help="time the force step's VJP on the workload instead")
+    ap.add_argument("--train-correction", action="store_true",
+                    help="train the learned correction on the 128^3 / 250 Mpc/h coarse (2x) / fine (4x) workload, "
+                         "choose it on a validation seed and judge it on a held-out one, instead")
+    ap.add_argument("--train-seeds", type=int, default=32)
+    ap.add_argument("--train-iters", type=int, default=2000)
+    ap.add_argument("--max-residual", default="0.5")
+    ap.add_argument("--lr", default="1e-3")
+    ap.add_argument("--symmetry", default="none", help="kernel symmetry of the trained network: none, cubic, or both")
+    ap.add_argument("--e2e-iters", type=int, default=300)
+    ap.add_argument("--e2e-seeds", type=int, default=16)
+    ap.add_argument("--e2e-lr", type=float, default=1e-4)
+    ap.add_argument("--e2e-validate-every", type=int, default=50)
+    ap.add_argument("--r-weight", type=float, default=1.0)
+    ap.add_argument("--save", default=None,
+                    help="with --train-correction: write the chosen weights to this path (the package ships "
+                         "pmpp_amd/corrections/weights/mesh_cnn_2x_from_4x.npz)")
+    ap.add_argument("--no-pk-error", action="store_true",
+                    help="skip the corrected 1024^3 vs plain 2048^3 P(k) comparison of the default run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (the known-size kernel that "
                          "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -402,6 +621,11 @@ def main():
 
     if args.vjp:
         vjp_run(args)
+        return
+
+    if args.train_correction:
+        args.run_steps = args.run_steps or 20
+        train_correction_run(args)
         return
 
     if args.copy_calibration:
@@ -492,6 +716,17 @@ def main():
         "strict": {"step_bytes": strict_bytes,
                    "frac": round(strict_bytes / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)},
     }
+    del ws, acc, pm, dp, ptcl
+    torch.cuda.empty_cache()
+    pk_err = None
+    from pmpp_amd.corrections.mesh_cnn import TRAINED_DEFAULT, TRAINED_DIR
+    if not args.no_pk_error and args.config == "c5":
+        if (TRAINED_DIR / TRAINED_DEFAULT).exists():
+            log("[bench] P(k) of the corrected 1024^3 run against the plain 2048^3 run")
+            pk_err = pk_error_run(dev)
+        else:
+            log("[bench] no trained correction weights shipped: P(k) comparison skipped")
+        torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (float32 NumPy/SciPy port)")
@@ -514,6 +749,7 @@ def main():
         "config": {"workload": f"PM force step: {ng}^3 particles, {ng * ratio}^3 mesh, {box:g} Mpc/h",
                    "particles": n, "mesh": list(conf.mesh_shape), "box": box, "brick": list(conf.brick)},
         "roofline": roofline,
+        "pk_error_vs_4x_mesh": pk_err,
         "cpu_baseline": cpu,
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},

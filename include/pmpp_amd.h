@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 3
+#define PM_ABI_VERSION 4
 
 enum pm_status {
   PM_SUCCESS = 0,
@@ -327,6 +327,8 @@ int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, cons
  *                         g = max_residual tanh(gain_params . [a, Omega_m, sigma8, 1]) into
  *                         stats[0..2] (device floats); x0 = bf16 [S/sigma, a, Omega_m, sigma8,
  *                         0, 0, 0, 0] per cell (n x 8 bf16, 16-byte aligned; NULL: stats only).
+ *                         With gain_params NULL, stats are read instead (those of the whole
+ *                         mesh when rho is a slab of it) and only x0 is written.
  * pm_correction_residual  rho_add = g (rho - 1) + max_residual tanh(out) sigma / c, out the
  *                         output layer's float32 pre-activation of channel 0 (n floats).
  * pm_correction_residual_bwd  the output layer's pre-activation gradient dz_out (n x 16,
@@ -334,8 +336,8 @@ int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, cons
  *                         rho_cot.
  * pm_correction_input_bwd the cotangents through the network's inputs: rho_bar (n floats,
  *                         overwritten) = d<rho_cot, rho_add>/d rho and omega_cot[0] += its
- *                         Omega_m derivative, from dx0 (n x 16 float32, the gradient of the
- *                         network input; channels 0 and 2 are read), the forward's out and
+ *                         Omega_m derivative, from dx0 (n x 4 float32, channels 0..3 of the
+ *                         gradient of the network input; 0 and 2 are read), the forward's out and
  *                         stats.  With the scatter adjoint of rho_bar this is reverse mode of
  *                         the corrected force through the density. 
  * pm_force_loss           loss[0] = sum |acc - target|^2 / sum |target|^2 (device doubles,

@@ -14,7 +14,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 PM_SUCCESS = 0
 PM_ERROR_INVALID_ARGUMENT = 1

@@ -157,6 +157,40 @@ def _pk_vjp(mesh, conf, pk_cot, mas, density):
     return irfftn(grad, s=shape)
 
 
+def _cross_vjp(mesh, other, conf, pab_cot, mas, density):
+    """d/d(mesh) of <pab_cot, P_ab(k)> with ``other`` (b) fixed: per mode the shell coefficient
+    times the window times b's spectrum (half the auto-spectrum's 2 a)."""
+    from ..numerics import irfftn
+
+    dev = mesh.device
+    shape = tuple(mesh.shape)
+    ns = num_shells(shape)
+    _, nmodes = shell_statistics(shape, float(conf.cell_size))
+    cot = np.asarray(pab_cot, dtype=np.float64)
+    n = min(len(nmodes), ns - 1)
+    coef = np.zeros(ns, dtype=np.float64)
+    coef[1:1 + n] = float(conf.cell_size) ** 3 * cot / nmodes[:n]
+    coef_d = torch.from_numpy(coef.astype(np.float32)).to(dev)
+    spec = rfftn(other.contiguous())
+    grad = torch.empty_like(spec)
+    _lib.call("pm_power_spectrum_bwd", ptr(torch.view_as_real(spec)), _lib.int3(shape), _mas(mas), ns, int(density),
+              ptr(coef_d), ptr(torch.view_as_real(grad)), stream_handle(dev))
+    return irfftn(grad, s=shape)
+
+
+def density_to_cross_correlation_vjp(density_a, density_b, conf, r_cot, mas="CIC", eps=1e-30):
+    """d/d(density_a) of <r_cot, r(k)> for ``density_to_cross_correlation`` with ``density_b``
+    fixed: r = P_ab / sqrt(P_aa P_bb), so dr = dP_ab / sqrt(P_aa P_bb) - r dP_aa / (2 P_aa)
+    (up to a constant field, as ``density_to_pk_vjp``)."""
+    _, r, pab, pa, pb, _ = density_to_cross_correlation(density_a, density_b, conf, mas, eps)
+    r, pa, pb = (x.double().cpu().numpy() for x in (r, pa, pb))
+    rc = np.asarray(r_cot.detach().cpu() if isinstance(r_cot, torch.Tensor) else r_cot, dtype=np.float64)
+    c_ab = rc / np.sqrt(np.maximum(pa * pb, eps))
+    c_aa = -rc * r / (2.0 * np.maximum(pa, eps))
+    return (_pk_vjp(density_a, conf, c_aa, mas, density=True) +
+            _cross_vjp(density_a, density_b, conf, c_ab, mas, density=True))
+
+
 def delta_to_pk_vjp(delta, conf, pk_cot, mas="CIC"):
     """d/d(delta) of <pk_cot, P(k)> for ``delta_to_pk`` (the reverse of the shell sums)."""
     return _pk_vjp(delta, conf, pk_cot, mas, density=False)

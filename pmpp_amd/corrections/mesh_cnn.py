@@ -62,6 +62,26 @@ class Layer:
 
 
 BLOCKS = ("residual", "plain")
+SYMMETRIES = ("none", "cubic")
+
+# taps of a 3^3 kernel grouped by their number of nonzero offsets (centre, faces, edges,
+# corners): the orbits of the cube's 48 rotations and reflections acting on the taps
+_TAP_ORBITS = [[t for t in range(27) if sum(((t // 9) % 3 != 1, (t // 3) % 3 != 1, t % 3 != 1)) == o]
+               for o in range(4)]
+
+
+def project_cubic(vec: torch.Tensor, layers) -> torch.Tensor:
+    """Projects the convolution kernels stored in the flat parameter-layout vector ``vec`` (in
+    place) onto the kernels that are invariant under the symmetries of the cube: each
+    (output, input) channel pair's 27 taps are replaced by their mean over each orbit.  A
+    network of such kernels with pointwise activations commutes with the cube's rotations and
+    reflections, as the PM force's window does; biases and the gain stay as they are."""
+    for layer in layers:
+        w = vec[layer.offset:layer.offset + layer.nweights].view(TAPS, -1)
+        for orbit in _TAP_ORBITS:
+            idx = torch.tensor(orbit, device=vec.device)
+            w[idx] = w[idx].mean(0, keepdim=True)
+    return vec
 
 
 def param_layout(depth: int, block: str = "residual"):
@@ -99,9 +119,12 @@ class MeshCNNPotentialCorrection:
     sigma8_value: float = 0.8
     params_bf16: Optional[torch.Tensor] = None
     block: str = "residual"
+    symmetry: str = "none"
 
     def __post_init__(self):
         require_device(self.params, "params")
+        if self.symmetry not in SYMMETRIES:
+            raise ValueError(f"symmetry must be one of {SYMMETRIES}")
         layers, gain, total = param_layout(self.depth, self.block)
         if self.params.dtype != torch.float32 or self.params.numel() != total or not self.params.is_contiguous():
             raise ValueError(f"params must be a contiguous float32 vector of {total} elements")
@@ -155,13 +178,25 @@ def initial_params(seed=0, depth=2, output_init_scale=1e-2, block="residual") ->
 
 
 def init_mesh_cnn_potential_correction(seed=0, depth=2, max_residual=0.1, output_init_scale=1e-2, sigma8_value=0.8,
-                                       device="cuda", block="residual"):
+                                       device="cuda", block="residual", symmetry="none"):
     """A freshly initialised correction (``mesh_cnn.py:163-214``): LeCun truncated-normal
     convolutions with zero biases, a zero output layer and a small random gain layer, so the
-    initial correction is exactly zero."""
+    initial correction is exactly zero.  ``symmetry="cubic"`` projects the kernels onto the
+    cube-symmetric ones (``project_cubic``), rescaled to the unprojected initial variance, and
+    training keeps them there."""
     host = initial_params(seed, depth, output_init_scale, block)
+    if symmetry == "cubic":
+        layers, _, _ = param_layout(depth, block)
+        for layer in layers:
+            w = host[layer.offset:layer.offset + layer.nweights].clone()
+            std0 = float(w.std())
+            project_cubic(host, [layer])
+            ws = host[layer.offset:layer.offset + layer.nweights]
+            if std0 > 0 and float(ws.std()) > 0:
+                ws.mul_(std0 / float(ws.std()))
     return MeshCNNPotentialCorrection(params=host.to(device), depth=depth, max_residual=max_residual,
-                                      output_init_scale=output_init_scale, sigma8_value=sigma8_value, block=block)
+                                      output_init_scale=output_init_scale, sigma8_value=sigma8_value, block=block,
+                                      symmetry=symmetry)
 
 
 @dataclasses.dataclass
@@ -182,11 +217,22 @@ def _workspace(dev):
     return workspace(_lib.size_query("pm_correction_workspace"), dev)
 
 
-def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=False):
+@dataclasses.dataclass
+class OutputTape:
+    """The forward's statistics and output-layer pre-activation, without the activations
+    (what ``network_input_vjp`` needs from the whole mesh)."""
+
+    stats: torch.Tensor
+    out: torch.Tensor
+
+
+def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=False, stats=None, keep_out=False):
     """Density residual ``rho_add`` of the correction for the density mesh ``rho`` (X, Y, Z).
 
     Returns ``(rho_add, tape)``; ``tape`` is None unless ``keep`` (then it holds the
-    activations for ``residual_backward``)."""
+    activations for ``residual_backward``) or ``keep_out`` (then an ``OutputTape``).  ``stats``
+    (the [sigma, gain, tanh] device floats of a forward on the whole mesh) are used instead of
+    this mesh's own: ``rho`` is then a slab of that mesh."""
     require_device(rho, "rho")
     if rho.dtype != torch.float32 or rho.ndim != 3:
         raise ValueError("rho must be a float32 (X, Y, Z) mesh")
@@ -197,23 +243,27 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     n = rho.numel()
     a, om = float(a), float(omega_m)
     ws = _workspace(dev)
-    stats = torch.empty(4, dtype=torch.float32, device=dev)
+    given = stats is not None
+    if not given:
+        stats = torch.empty(4, dtype=torch.float32, device=dev)
     s = stream_handle(dev)
     s8 = float(model.sigma8_value)
     layers = model.layers
     pres, acts = [], []
+    gain = None if given else ptr(model.gain_params)
     if keep:
         # the backward's stem weight gradient needs the network input x0
         x0 = torch.empty(shape + (IN_CHANNELS,), dtype=torch.bfloat16, device=dev)
-        _lib.call("pm_correction_features", ptr(rho), n, om, a, s8, float(model.max_residual),
-                  ptr(model.gain_params), ptr(x0), ptr(stats), ptr(ws), ws.numel(), s)
+        _lib.call("pm_correction_features", ptr(rho), n, om, a, s8, float(model.max_residual), gain, ptr(x0),
+                  ptr(stats), ptr(ws), ws.numel(), s)
         pre, x = conv3d(x0, model.weight(layers[0]), model.bias(layers[0]), gelu=True, want_pre=True)
     else:
         # inference: the stem reads the density directly (constant input channels folded
         # into its bias), so x0 is never written
         x0, pre = None, None
-        _lib.call("pm_correction_features", ptr(rho), n, om, a, s8, float(model.max_residual),
-                  ptr(model.gain_params), None, ptr(stats), ptr(ws), ws.numel(), s)
+        if not given:
+            _lib.call("pm_correction_features", ptr(rho), n, om, a, s8, float(model.max_residual), gain, None,
+                      ptr(stats), ptr(ws), ws.numel(), s)
         x = torch.empty(shape + (CHANNELS,), dtype=torch.bfloat16, device=dev)
         _lib.call("pm_conv3d_stem", ptr(rho), _lib.int3(shape), om, a, s8, ptr(stats),
                   ptr(model.weight(layers[0])), ptr(model.bias(layers[0])), ptr(x), s)
@@ -232,7 +282,7 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
         pres += [p1, p2]
         acts += [h, x]
     rho_add = torch.empty_like(rho)
-    if not keep:
+    if not keep and not keep_out:
         # inference: the output layer writes rho_add directly (its pre-activation is not kept)
         _lib.call("pm_conv3d_out_residual", ptr(x), ptr(model.weight(layers[-1])), ptr(model.bias(layers[-1])),
                   ptr(rho), _lib.int3(shape), om, float(model.max_residual), ptr(stats), ptr(rho_add), s)
@@ -241,6 +291,8 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
                     pre_channels=1)
     _lib.call("pm_correction_residual", ptr(out), ptr(rho), n, om, float(model.max_residual), ptr(stats),
               ptr(rho_add), s)
+    if not keep:
+        return rho_add, OutputTape(stats, out)
     return rho_add, Tape(rho, stats, x0, pres, acts, out, a, om)
 
 
@@ -253,29 +305,34 @@ def _stem_weight_16(model):
     return out
 
 
-def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, grads=None, inputs=False):
+def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, grads=None, inputs=False,
+                      weights=True, input_grad=False):
     """Gradient of ``<rho_cot, rho_add>`` with respect to the parameters (flat float32, the
     parameter layout); written into ``grads`` when given.  With ``inputs`` also the cotangents
     through the network's inputs: returns ``(grads, rho_bar, omega_bar)`` with ``rho_bar`` the
-    density cotangent (mesh) and ``omega_bar`` a device float (``pm_correction_input_bwd``)."""
+    density cotangent (mesh) and ``omega_bar`` a device float (``pm_correction_input_bwd``).
+    With ``input_grad`` it returns only the gradient of the network input x0 (16-channel
+    float32 mesh, channels 0..7 live); ``weights=False`` skips the weight gradients."""
     dev = tape.rho.device
     n = tape.rho.numel()
     shape = tuple(tape.rho.shape)
     if grads is None:
-        grads = torch.zeros_like(model.params)
+        grads = torch.zeros_like(model.params) if weights else torch.zeros(model.params.numel() - model.gain_offset,
+                                                                            dtype=torch.float32, device=dev)
     rc = rho_cot.contiguous()
     if rc.dtype != torch.float32 or tuple(rc.shape) != shape:
         raise ValueError("rho_cot must be a float32 mesh shaped like rho")
     ws = _workspace(dev)
     dz = torch.empty(shape + (CHANNELS,), dtype=torch.float32, device=dev)
-    g = model.gain_offset
+    g = model.gain_offset if weights else 0
     _lib.call("pm_correction_residual_bwd", ptr(rc), ptr(tape.out), ptr(tape.rho), n, tape.omega_m, tape.a,
               float(model.sigma8_value), float(model.max_residual), ptr(tape.stats), ptr(dz), ptr(grads[g:g + 4]),
               ptr(ws), ws.numel(), stream_handle(dev))
     layers = model.layers
 
     def wgrad(layer, dzl, x):
-        conv3d_bwd_weight(dzl, x, out=grads[layer.offset:layer.offset + layer.size])
+        if weights:
+            conv3d_bwd_weight(dzl, x, out=grads[layer.offset:layer.offset + layer.size])
 
     # output layer: input acts[-1] = gelu(pres[-1])
     wgrad(layers[-1], dz, tape.acts[-1])
@@ -295,12 +352,114 @@ def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, gr
         wgrad(l1, dz1, x_in)
         dz = conv3d_bwd_data(dz1, model.weight(l1), add=dz, gact=pre_in)
     wgrad(layers[0], dz, tape.x0)
-    if not inputs:
+    if not inputs and not input_grad:
         return grads
     dx0 = conv3d_bwd_data(dz, _stem_weight_16(model))
-    rho_bar = torch.empty_like(tape.rho)
-    omega_bar = torch.zeros((), dtype=torch.float32, device=dev)
-    _lib.call("pm_correction_input_bwd", ptr(rc), ptr(tape.out), ptr(tape.rho), ptr(dx0), n, tape.omega_m,
-              float(model.max_residual), ptr(tape.stats), ptr(model.gain_params), ptr(rho_bar), ptr(omega_bar),
-              ptr(ws), ws.numel(), stream_handle(dev))
+    if input_grad:
+        return dx0
+    rho_bar, omega_bar = _input_bwd(model, tape.rho, rc, tape.out, tape.stats, dx0[..., :4].contiguous(),
+                                    tape.omega_m)
     return grads, rho_bar, omega_bar
+
+
+def _input_bwd(model, rho, rho_cot, out, stats, dx0c, omega_m):
+    dev = rho.device
+    ws = _workspace(dev)
+    rho_bar = torch.empty_like(rho)
+    omega_bar = torch.zeros((), dtype=torch.float32, device=dev)
+    _lib.call("pm_correction_input_bwd", ptr(rho_cot), ptr(out), ptr(rho), ptr(dx0c), rho.numel(), float(omega_m),
+              float(model.max_residual), ptr(stats), ptr(model.gain_params), ptr(rho_bar), ptr(omega_bar),
+              ptr(ws), ws.numel(), stream_handle(dev))
+    return rho_bar, omega_bar
+
+
+# cells of the extended slab (slab + halo) that one network reverse pass may hold: its tape and
+# gradients take about 350 bytes per cell (44 GiB at this size)
+SLAB_CELLS = 1 << 27
+
+
+def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_m, tape: OutputTape, slab=None):
+    """``(rho_bar, omega_bar)``: the cotangents of ``<rho_cot, rho_add(rho, Omega_m)>`` through
+    the network's inputs (``pm_correction_input_bwd``), with the network's reverse pass run over
+    x-slabs so that its activations and gradients never exist for the whole mesh at once.
+
+    Each slab of ``slab`` x planes (a multiple of 8; default: the largest whose extended slab
+    holds at most ``SLAB_CELLS`` cells, or the whole mesh when that fits) is extended by a halo
+    of H planes on both sides (wrapped periodically), H = 2 x the number of conv layers rounded
+    up to a multiple of 8; the network's forward (with the whole mesh's statistics from
+    ``tape``) and its data backward run on the extended slab as if it were periodic.  Each
+    conv reaches one plane further, so after the 2 x layers convolutions of the forward and
+    the backward only the outer H planes hold wrapped values and the slab's own planes equal
+    the whole-mesh computation.  The input gradient of those planes is kept (4 of its 16
+    channels); the global terms (through sigma, the gain and Omega_m) follow on the whole mesh."""
+    require_device(rho, "rho")
+    dev = rho.device
+    shape = tuple(rho.shape)
+    n0, plane = shape[0], shape[1] * shape[2]
+    om = float(omega_m)
+    rc = rho_cot.contiguous()
+    halo = -(-2 * len(model.layers) // 8) * 8
+    if slab is None:
+        slab = n0 if n0 * plane <= SLAB_CELLS else max(8, (SLAB_CELLS // plane - 2 * halo) // 8 * 8)
+    if slab % 8 or slab < 8:
+        raise ValueError("slab must be a positive multiple of 8 planes")
+    dx0c = torch.empty(shape + (4,), dtype=torch.float32, device=dev)
+    if slab >= n0:
+        _, full = evaluate_residual(model, rho, a, om, keep=True, stats=tape.stats)
+        dx0c.copy_(residual_backward(model, full, rc, weights=False, input_grad=True)[..., :4])
+        del full
+    else:
+        for x0 in range(0, n0, slab):
+            s = min(slab, n0 - x0)
+            idx = torch.arange(x0 - halo, x0 + s + halo, device=dev).remainder_(n0)
+            rho_e = rho.index_select(0, idx)
+            _, ext = evaluate_residual(model, rho_e, a, om, keep=True, stats=tape.stats)
+            dx0_e = residual_backward(model, ext, rc.index_select(0, idx), weights=False, input_grad=True)
+            del ext
+            dx0c[x0:x0 + s] = dx0_e[halo:halo + s, ..., :4]
+            del dx0_e, rho_e
+    return _input_bwd(model, rho, rc, tape.out, tape.stats, dx0c, om)
+
+
+TRAINED_DIR = __import__("pathlib").Path(__file__).resolve().parent / "weights"
+TRAINED_DEFAULT = "mesh_cnn_2x_from_4x.npz"
+
+
+def save_correction(model: MeshCNNPotentialCorrection, path, **meta) -> None:
+    """Writes the float32 parameters and hyperparameters to an ``.npz`` file (plain arrays, no
+    pickled objects); ``meta`` adds numeric or string fields (the training recipe)."""
+    import numpy as np
+
+    fields = {"params": model.params.detach().cpu().numpy().astype(np.float32),
+              "depth": np.int64(model.depth), "max_residual": np.float64(model.max_residual),
+              "output_init_scale": np.float64(model.output_init_scale),
+              "sigma8_value": np.float64(model.sigma8_value), "block": np.str_(model.block),
+              "symmetry": np.str_(model.symmetry)}
+    for k, v in meta.items():
+        fields[f"meta_{k}"] = np.asarray(v)
+    with open(path, "wb") as f:
+        np.savez(f, **fields)
+
+
+def load_correction(path=None, device="cuda") -> MeshCNNPotentialCorrection:
+    """A correction saved by ``save_correction`` (default: the weights shipped with the package,
+    trained on coarse (2x) / fine (4x particle-grid) force pairs at 1.95 Mpc/h particle spacing).
+    Loaded with ``numpy.load(allow_pickle=False)``."""
+    import numpy as np
+
+    path = TRAINED_DIR / TRAINED_DEFAULT if path is None else path
+    with np.load(path, allow_pickle=False) as z:
+        params = torch.from_numpy(np.ascontiguousarray(z["params"], dtype=np.float32)).to(device)
+        return MeshCNNPotentialCorrection(params=params, depth=int(z["depth"]), max_residual=float(z["max_residual"]),
+                                          output_init_scale=float(z["output_init_scale"]),
+                                          sigma8_value=float(z["sigma8_value"]), block=str(z["block"]),
+                                          symmetry=str(z["symmetry"]) if "symmetry" in z.files else "none")
+
+
+def correction_meta(path=None) -> dict:
+    """The ``meta_*`` fields of a saved correction (its training recipe), as host values."""
+    import numpy as np
+
+    path = TRAINED_DIR / TRAINED_DEFAULT if path is None else path
+    with np.load(path, allow_pickle=False) as z:
+        return {k[5:]: z[k].item() if z[k].ndim == 0 else z[k].tolist() for k in z.files if k.startswith("meta_")}

@@ -25,7 +25,8 @@ import torch
 from .. import _lib
 from .._torch import as_contig, ensure_init, ptr, stream_handle, workspace
 from ..nbody.gravity import _check_particles, _valid_u8, gravity_desc
-from .mesh_cnn import MeshCNNPotentialCorrection, evaluate_residual, residual_backward
+from .mesh_cnn import (MeshCNNPotentialCorrection, evaluate_residual, network_input_vjp, project_cubic,
+                       residual_backward)
 
 
 @dataclasses.dataclass
@@ -107,6 +108,7 @@ def _residual_mesh(rho_add, conf):
 def gravity_with_residual(ptcl, cosmo, conf, rho_add):
     """Accelerations with the density residual ``rho_add`` (``pm_gravity_corrected``)."""
     dev = ptcl.disp.device
+    ensure_init(dev)
     d = gravity_desc(conf, _omega(cosmo))
     pm = as_contig(ptcl.pmid, torch.int16, "pmid")
     dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
@@ -123,6 +125,7 @@ def gravity_with_residual(ptcl, cosmo, conf, rho_add):
 def gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot):
     """(disp_cot, omega_cot (device float), rho_cot) of ``gravity_with_residual``."""
     dev = ptcl.disp.device
+    ensure_init(dev)
     d = gravity_desc(conf, _omega(cosmo))
     pm = as_contig(ptcl.pmid, torch.int16, "pmid")
     dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
@@ -166,18 +169,29 @@ def corrected_gravity(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrecti
     return acc
 
 
-def corrected_gravity_vjp(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrection, acc_cot):
+def corrected_gravity_vjp(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrection, acc_cot, slab=None,
+                          param_grads=None):
     """(disp_cot, omega_cot float) of ``corrected_gravity``: the force step's adjoint at the
     corrected source (``pm_gravity_corrected_vjp``) plus the path through the network's inputs
     (``pm_correction_input_bwd``), whose density cotangent reaches the particles through the
-    CIC scatter adjoint."""
+    CIC scatter adjoint.  The network's reverse runs over x-slabs (``network_input_vjp``);
+    with ``param_grads`` it runs on the whole mesh and also adds the parameters' gradient."""
     from ..cic.kernels import hip_scatter_bwd
 
     om = _omega(cosmo)
     rho = density(ptcl, cosmo, conf)
-    rho_add, tape = evaluate_residual(correction, rho, a, om, keep=True)
-    disp_cot, omega_cot, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
-    _, rho_bar, omega_bar = residual_backward(correction, tape, rho_cot, inputs=True)
+    if param_grads is not None:
+        rho_add, full = evaluate_residual(correction, rho, a, om, keep=True)
+        disp_cot, omega_cot, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
+        del rho_add
+        g, rho_bar, omega_bar = residual_backward(correction, full, rho_cot, inputs=True)
+        del full
+        param_grads += g
+    else:
+        rho_add, tape = evaluate_residual(correction, rho, a, om, keep_out=True)
+        disp_cot, omega_cot, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
+        del rho_add
+        rho_bar, omega_bar = network_input_vjp(correction, rho, rho_cot, a, om, tape, slab=slab)
     valid = _valid_u8(ptcl)
     dc, _ = hip_scatter_bwd(ptcl.pmid, ptcl.disp.detach(), conf.mesh_size / conf.ptcl_num, rho_bar, offset=0.0,
                             particle_cell_size=conf.cell_size, cell_size=None, global_shape=tuple(conf.mesh_shape),
@@ -201,13 +215,17 @@ def force_loss(acc, target, want_grad=True):
 
 
 def train_step(model, optimizer, state, ptcl, a, cosmo, conf, target, grads=None):
-    """One optimizer step on one snapshot; returns the loss before the step (device float64)."""
+    """One optimizer step on one snapshot; returns the loss before the step (device float64).
+    A model with ``symmetry="cubic"`` has its gradient projected onto the cube-symmetric
+    kernels (``project_cubic``) before the step, so its weights stay in that subspace."""
     rho = density(ptcl, cosmo, conf)
     rho_add, tape = evaluate_residual(model, rho, a, _omega(cosmo), keep=True)
     acc = gravity_with_residual(ptcl, cosmo, conf, rho_add)
     loss, acc_cot = force_loss(acc, target)
     _, _, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
     grads = residual_backward(model, tape, rho_cot, grads)
+    if model.symmetry == "cubic":
+        project_cubic(grads, model.layers)
     optimizer.step(model, grads, state)
     return loss[0]
 
@@ -239,11 +257,12 @@ def fine_run_pairs(cosmo, conf, conf_fine, seeds, every=1, device="cuda"):
     return pairs
 
 
-def compare_runs(model, cosmo, conf, conf_fine, seed, device="cuda"):
+def compare_runs(model, cosmo, conf, conf_fine, seed, device="cuda", measure_conf=None):
     """The judge of the correction: from one seed's initial conditions, the final density of
-    the plain coarse run, the corrected coarse run and the fine run, each measured on the fine
-    mesh.  Returns a dict with ``k``, ``pk_plain``, ``pk_corrected``, ``pk_fine`` and the
-    cross-correlations ``r_plain``, ``r_corrected`` against the fine run (torch, device)."""
+    the plain coarse run, the corrected coarse run and the fine run, each measured on the mesh
+    of ``measure_conf`` (default: the fine mesh).  Returns a dict with ``k``, ``pk_plain``,
+    ``pk_corrected``, ``pk_fine`` and the cross-correlations ``r_plain``, ``r_corrected``
+    against the fine run (torch, device)."""
     from ..analysis import density_to_cross_correlation, density_to_pk
     from ..cic import scatter
     from ..initial_conditions import linear_modes, lpt, white_noise
@@ -255,13 +274,168 @@ def compare_runs(model, cosmo, conf, conf_fine, seed, device="cuda"):
     corr = nbody(lpt(modes, cosmo, conf), cosmo, conf, correction=model)
     fine = nbody(lpt(modes, cosmo, conf_fine), cosmo, conf_fine, observer=lambda a, p: None)
 
+    mc = conf_fine if measure_conf is None else measure_conf
+
     def dens(p):
-        q = Particles.from_pos(conf_fine, p.pos(dtype=torch.float64))
-        return scatter(q, conf_fine)
+        q = Particles.from_pos(mc, p.pos(dtype=torch.float64))
+        return scatter(q, mc)
 
     d_plain, d_corr, d_fine = dens(plain), dens(corr), dens(fine)
-    k, pk_fine, _ = density_to_pk(d_fine, conf_fine)
-    _, r_plain, _, pk_plain, _, _ = density_to_cross_correlation(d_plain, d_fine, conf_fine)
-    _, r_corr, _, pk_corr, _, _ = density_to_cross_correlation(d_corr, d_fine, conf_fine)
+    del plain, corr, fine
+    k, pk_fine, _ = density_to_pk(d_fine, mc)
+    _, r_plain, _, pk_plain, _, _ = density_to_cross_correlation(d_plain, d_fine, mc)
+    _, r_corr, _, pk_corr, _, _ = density_to_cross_correlation(d_corr, d_fine, mc)
     return {"k": k, "pk_plain": pk_plain, "pk_corrected": pk_corr, "pk_fine": pk_fine, "r_plain": r_plain,
             "r_corrected": r_corr}
+
+
+def mean_force_loss(model, pairs, cosmo, conf):
+    """Mean relative squared force error |acc - t|^2 / |t|^2 over ``pairs`` of the coarse force
+    with the correction ``model`` (None: the plain force); a host float."""
+    from ..nbody.gravity import gravity
+
+    tot = None
+    for a, p, tgt in pairs:
+        acc = gravity(a, p, cosmo.Omega_m, conf, correction=model)
+        loss = force_loss(acc, tgt, want_grad=False)[0][0]
+        tot = loss if tot is None else tot + loss
+    return float(tot) / len(pairs)
+
+
+def train_correction(pairs, cosmo, conf, *, iters, learning_rate, max_residual, depth=2, block="plain", seed=0,
+                     gradient_clip_norm=100.0, symmetry="none", device="cuda", progress=None, progress_every=500):
+    """Trains a freshly initialised correction on ``pairs`` ([(a, coarse particles, target acc)],
+    ``fine_run_pairs``): ``iters`` Adamax steps (``build_correction_optimizer``), one pair per
+    step drawn by numpy's PCG64(``seed``).  ``progress(it, mean loss of the last steps)`` is
+    called every ``progress_every`` steps.  Returns ``(model, losses)``, the per-step training
+    losses as a host numpy array (read once at the end, so the loop never waits on the GPU)."""
+    import numpy as np
+
+    from .mesh_cnn import init_mesh_cnn_potential_correction
+
+    model = init_mesh_cnn_potential_correction(seed=seed, depth=depth, max_residual=max_residual, device=device,
+                                               block=block, symmetry=symmetry)
+    opt = build_correction_optimizer(learning_rate, gradient_clip_norm)
+    state = opt.init(model.params)
+    rng = np.random.default_rng(seed)
+    grads = torch.zeros_like(model.params)
+    losses = torch.empty(iters, dtype=torch.float64, device=model.params.device)
+    for it in range(iters):
+        a, p, tgt = pairs[int(rng.integers(len(pairs)))]
+        losses[it] = train_step(model, opt, state, p, a, cosmo, conf, tgt, grads=grads)
+        if progress is not None and (it + 1) % progress_every == 0:
+            progress(it + 1, float(losses[it + 1 - progress_every:it + 1].mean()))
+    return model, losses.cpu().numpy()
+
+
+def pk_errors(res, kmax=1.0):
+    """Summary of ``compare_runs``: over the shells 0 < k <= ``kmax`` (h/Mpc), the mean and the
+    maximum of |P / P_fine - 1| and the mean cross-correlation with the fine run, for the plain
+    and the corrected coarse runs (host floats)."""
+    k = res["k"].cpu().numpy()
+    sel = (k > 0) & (k <= kmax)
+    fine = res["pk_fine"].cpu().numpy()[sel]
+    out = {"kmax": kmax, "shells": int(sel.sum())}
+    for name in ("plain", "corrected"):
+        e = abs(res[f"pk_{name}"].cpu().numpy()[sel] / fine - 1)
+        out[name] = {"pk_err_mean": float(e.mean()), "pk_err_max": float(e.max()),
+                     "r_mean": float(res[f"r_{name}"].cpu().numpy()[sel].mean())}
+    return out
+
+
+class RunTarget:
+    """One seed's end-to-end training target: the coarse run's initial state and the final
+    density of the fine run from the same white noise, measured on the coarse mesh."""
+
+    def __init__(self, seed, cosmo, conf, conf_fine, device="cuda"):
+        from ..cic import scatter
+        from ..initial_conditions import linear_modes, lpt, white_noise
+        from ..nbody.particles import Particles
+        from ..nbody.solver import nbody
+
+        modes = linear_modes(white_noise(seed, conf, device=device), cosmo, conf)
+        self.seed = seed
+        self.p0 = lpt(modes, cosmo, conf)
+        fine = nbody(lpt(modes, cosmo, conf_fine), cosmo, conf_fine, observer=lambda a, p: None)
+        self.dens_fine = scatter(Particles.from_pos(conf, fine.pos(dtype=torch.float64)), conf)
+
+
+def run_loss(model, target: RunTarget, cosmo, conf, kmax=1.0, r_weight=1.0, grads=None):
+    """End-to-end loss of one corrected run and (with ``grads``) its parameter gradient: the
+    corrected coarse run from ``target.p0`` to the last scale factor, its final CIC density on
+    the coarse mesh, and over the shells 0 < k <= ``kmax``
+        L = mean (log P(k) - log P_fine(k))^2 + r_weight mean (1 - r(k))
+    with r the cross-correlation with the fine run.  The gradient runs through the P(k) and r
+    adjoints, the scatter adjoint and the N-body reverse sweep with the network's parameter
+    gradient (``nbody_adj(param_grads=...)``), added to ``grads``.  Returns the host float L."""
+    import numpy as np
+
+    from ..analysis import (density_to_cross_correlation, density_to_cross_correlation_vjp, density_to_pk,
+                            density_to_pk_vjp)
+    from ..cic import scatter
+    from ..nbody.solver import nbody, nbody_adj
+
+    with torch.no_grad():
+        out = nbody(target.p0, cosmo, conf, correction=model, observer=lambda a, p: None)
+    disp = out.disp.detach().clone().requires_grad_(True)
+    dens = scatter(out.replace(disp=disp), conf)
+    k, pk, _ = density_to_pk(dens.detach(), conf)
+    _, pkf, _ = density_to_pk(target.dens_fine, conf)
+    _, r, _, _, _, _ = density_to_cross_correlation(dens.detach(), target.dens_fine, conf)
+    k, pk, pkf, r = (x.double().cpu().numpy() for x in (k, pk, pkf, r))
+    sel = (k > 0) & (k <= kmax)
+    ns = int(sel.sum())
+    d = np.log(pk[sel]) - np.log(pkf[sel])
+    loss = float((d ** 2).mean() + r_weight * (1.0 - r[sel]).mean())
+    if grads is None:
+        return loss
+    pk_cot = np.zeros(len(pk))
+    pk_cot[sel] = 2.0 * d / (ns * pk[sel])
+    r_cot = np.zeros(len(pk))
+    r_cot[sel] = -r_weight / ns
+    dens_cot = density_to_pk_vjp(dens.detach(), conf, pk_cot)
+    if r_weight:
+        dens_cot += density_to_cross_correlation_vjp(dens.detach(), target.dens_fine, conf, r_cot)
+    dens.backward(dens_cot)
+    cot = out.replace(disp=disp.grad, vel=torch.zeros_like(out.vel), acc=torch.zeros_like(out.acc))
+    nbody_adj(out, cot, cosmo, conf, correction=model, param_grads=grads)
+    return loss
+
+
+def train_correction_runs(model, targets, cosmo, conf, *, iters, learning_rate, kmax=1.0, r_weight=1.0, seed=0,
+                          gradient_clip_norm=100.0, validate=None, validate_every=50, progress=None):
+    """Fine-tunes ``model`` end to end on whole corrected runs (``run_loss`` over ``targets``,
+    one drawn per step by numpy's PCG64(``seed``)) with Adamax.  Every ``validate_every`` steps
+    ``validate(model)`` returns a score (lower is better); the parameters of the best-scoring
+    checkpoint (the starting point included) are restored at the end.  Returns
+    ``(losses, [(step, score), ...])``."""
+    import numpy as np
+
+    opt = build_correction_optimizer(learning_rate, gradient_clip_norm)
+    state = opt.init(model.params)
+    rng = np.random.default_rng(seed)
+    grads = torch.zeros_like(model.params)
+    losses, scores = [], []
+    best = None
+    if validate is not None:
+        s0 = validate(model)
+        scores.append((0, s0))
+        best = (s0, model.params.detach().clone())
+    for it in range(iters):
+        t = targets[int(rng.integers(len(targets)))]
+        grads.zero_()
+        losses.append(run_loss(model, t, cosmo, conf, kmax=kmax, r_weight=r_weight, grads=grads))
+        if model.symmetry == "cubic":
+            project_cubic(grads, model.layers)
+        opt.step(model, grads, state)
+        if progress is not None:
+            progress(it + 1, losses[-1])
+        if validate is not None and (it + 1) % validate_every == 0:
+            sc = validate(model)
+            scores.append((it + 1, sc))
+            if sc < best[0]:
+                best = (sc, model.params.detach().clone())
+    if best is not None:
+        model.params.copy_(best[1])
+        model.sync_bf16()
+    return np.asarray(losses), scores

@@ -517,6 +517,133 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
   }
 }
 
+// Output layer: one output channel from 16 input channels.  The 16 -> 16 implicit GEMM would
+// spend 15 of its 16 matrix rows on channels that are thrown away, so this kernel puts the
+// taps on the rows instead: for every staged (x, y) column u of the block and each of the nine
+// in-plane offsets t = (dx, dy),
+//   D_t[u][z] = sum_{dz, ci} w[(dx, dy, dz)][0][ci] x[u][z + dz][ci]
+// is one MFMA row (K = 3 dz x 16 channels = 48, two K steps; rows 9..15 and the second step's
+// upper half carry zero weights), with the 16 z voxels of the column on the N axis.  The
+// output is then out[v] = sum_t D_t[v + (dx, dy)][z]: the D values go to LDS (over the staged
+// tile, which is no longer needed) and each output voxel sums its nine, in tap order.  Two
+// MFMAs per staged column replace 15 per 16 output voxels.
+//
+// Blocks are 4 x 8 x 16 output voxels (the mesh must divide into them) staged as 6 x 10 x 18
+// with a one-voxel periodic halo: 1080 voxels of 32 bytes, stored as one plane per 8-channel
+// half; the 60 staged columns x 9 offsets x 16 z of D take exactly the same 34560 bytes.
+// Epilogues: kPlain writes the pre-activation out + bias[0] (pre, one channel); kOutResidual
+// writes rho_add = gain (rho - 1) + amp tanh(out + bias[0]) (the residual stage).
+constexpr int kOBX = 4, kOBY = 8, kOBZ = 16;
+constexpr int kOSX = kOBX + 2, kOSY = kOBY + 2, kOSZ = kOBZ + 2;
+constexpr int kOCols = kOSX * kOSY;                        // 60 staged columns
+constexpr int kONS = kOCols * kOSZ;                        // 1080 staged voxels
+constexpr int kOItems = 2 * kONS;                          // 16-byte vectors staged
+constexpr int kOPer = (kOItems + kThreads - 1) / kThreads;  // 9 per thread
+constexpr int kOColsPerWave = kOCols / (kThreads / 64);     // 15
+static_assert(kOCols * 9 * kOBZ == kONS * 8, "D fills the staged tile exactly");
+static_assert(kOCols % (kThreads / 64) == 0, "columns per wave");
+
+template <int EPI>
+__global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
+                                                        const float* __restrict__ bias, float* __restrict__ pre,
+                                                        const float* __restrict__ rho, const float* __restrict__ stats,
+                                                        float c, float r, float* __restrict__ rho_add, int n0, int n1,
+                                                        int n2, int g8) {
+  __shared__ __align__(16) float lds[kONS * 8];
+  __bf16* xin = reinterpret_cast<__bf16*>(lds);
+  const int nbz = n2 / kOBZ, nby = n1 / kOBY;
+  const int nblk = (n0 / kOBX) * nby * nbz;
+  const int L = blockIdx.x;
+  const int q = (L % 8) * g8 + L / 8;  // XCD-aware: consecutive blocks on one XCD
+  if (q >= nblk) return;               // padding workgroup: uniform exit before any barrier
+  const int bz = q % nbz, by = (q / nbz) % nby, bx = q / (nbz * nby);
+  const int ox = bx * kOBX, oy = by * kOBY, oz = bz * kOBZ;
+  // stage the halo tile: item i is half (i % 2) of staged voxel i / 2 (global order, so
+  // consecutive threads read consecutive 16 bytes); LDS keeps the halves as two planes
+  uint4 pf[kOPer];
+  const bool inner = ox > 0 && ox + kOBX < n0 && oy > 0 && oy + kOBY < n1 && oz > 0 && oz + kOBZ < n2;
+#pragma unroll
+  for (int u = 0; u < kOPer; ++u) {
+    const int it = min((int)threadIdx.x + u * kThreads, kOItems - 1);
+    const int h = it % 2, vox = it / 2;
+    const int sz = vox % kOSZ, sy = (vox / kOSZ) % kOSY, sx = vox / (kOSZ * kOSY);
+    int gx = ox + sx - 1, gy = oy + sy - 1, gz = oz + sz - 1;
+    if (!inner) {
+      gx = wrap(gx, n0);
+      gy = wrap(gy, n1);
+      gz = wrap(gz, n2);
+    }
+    pf[u] = *reinterpret_cast<const uint4*>(x + (((int64_t)gx * n1 + gy) * n2 + gz) * kCout + 8 * h);
+  }
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int m = lane % 16, g = lane / 16;
+  // A fragments: row m = in-plane offset t = 3 (dx + 1) + (dy + 1), K = 8 g + j.
+  // step 0: g = 0, 1: dz = -1, channels 8 g + j; g = 2, 3: dz = 0, channels 8 (g - 2) + j
+  // step 1: g = 0, 1: dz = +1; g = 2, 3: zero
+  bf16x8 wa0, wa1;
+  {
+    const int ch = 8 * (g % 2);
+    const int dz0 = g < 2 ? 0 : 1;  // dz + 1 of step 0
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      wa0[j] = m < 9 ? w[((3 * m + dz0) * kCout) * kCout + ch + j] : (__bf16)0.f;
+      wa1[j] = (m < 9 && g < 2) ? w[((3 * m + 2) * kCout) * kCout + ch + j] : (__bf16)0.f;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kOPer; ++u) {
+    const int i = threadIdx.x + u * kThreads;
+    if (i < kOItems) *reinterpret_cast<uint4*>(xin + ((i % 2) * kONS + i / 2) * 8) = pf[u];
+  }
+  __syncthreads();
+  // B fragments of staged column col: lane (g, m) feeds voxel z = m (staged z = m + 1 + dz),
+  // channel half g % 2; step 0 reads dz = -1 (g < 2) or 0, step 1 reads dz = +1
+  const int h = g % 2;
+  const int sz0 = g < 2 ? m : m + 1, sz1 = m + 2;
+  f32x4 acc[kOColsPerWave];
+#pragma unroll
+  for (int k = 0; k < kOColsPerWave; ++k) {
+    const int col = wave + 4 * k;
+    const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(xin + (h * kONS + col * kOSZ + sz0) * 8);
+    const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(xin + (h * kONS + col * kOSZ + sz1) * 8);
+    f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa0, b0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, b1, d, 0, 0, 0);
+  }
+  __syncthreads();  // every wave is done reading the staged tile
+  // D over the tile: lds[(col * 9 + t) * 16 + z]; lane (g, m) holds t = 4 g + i of voxel z = m
+#pragma unroll
+  for (int k = 0; k < kOColsPerWave; ++k) {
+    const int col = wave + 4 * k;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (4 * g + i < 9) lds[(col * 9 + 4 * g + i) * kOBZ + m] = acc[k][i];
+  }
+  __syncthreads();
+  const float b0v = bias[0];
+  float gain = 0.f, amp = 0.f;
+  if constexpr (EPI == kOutResidual) {
+    gain = stats[1];
+    amp = r * stats[0] / c;
+  }
+#pragma unroll
+  for (int s = 0; s < (kOBX * kOBY * kOBZ) / kThreads; ++s) {
+    const int v = threadIdx.x + s * kThreads;
+    const int z = v % kOBZ, y = (v / kOBZ) % kOBY, xx = v / (kOBZ * kOBY);
+    float o = 0.f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int col = (xx + t / 3) * kOSY + (y + t % 3);  // staged column (x + 1 + dx, y + 1 + dy)
+      o += lds[(col * 9 + t) * kOBZ + z];
+    }
+    const float zv = o + b0v;
+    const int64_t e = ((int64_t)(ox + xx) * n1 + (oy + y)) * n2 + oz + z;
+    if constexpr (EPI == kOutResidual)
+      rho_add[e] = gain * (rho[e] - 1.f) + amp * tanhf(zv);
+    else
+      pre[e] = zv;
+  }
+}
+
 // Data gradient of a 16 -> 16 layer: dx[v][ci] = sum_tap sum_co dz[v - off(tap)][co] w[tap][co][ci]
 // (the same implicit GEMM with the taps mirrored and the weight matrix transposed); dz is the
 // float32 gradient of the pre-activations, rounded to bf16 for the matrix cores.  When gact
@@ -699,6 +826,25 @@ unsigned nblocks(const int32_t* shape) {
   return (unsigned)((int64_t)(shape[0] / conv::kB) * (shape[1] / conv::kB) * (shape[2] / conv::kB));
 }
 
+// The one-output-channel layer with the taps on the matrix rows (k_conv_out1): one workgroup
+// per 4 x 8 x 16 block, in the XCD-aware order (grid rounded up to a multiple of 8)
+int launch_out1(const __bf16* x, const __bf16* w, const float* bias, float* pre, const float* rho,
+                const float* stats, float c, float r, float* rho_add, const int32_t* shape, hipStream_t s) {
+  const int64_t nblk = (int64_t)shape[0] / conv::kOBX * (shape[1] / conv::kOBY) * (shape[2] / conv::kOBZ);
+  const int64_t g8 = ceil_div(nblk, 8);
+  PM_RETURN_IF(8 * g8 >= (int64_t(1) << 31), PM_ERROR_UNSUPPORTED, "conv3d: too many blocks");
+  if (rho_add)
+    hipLaunchKernelGGL(conv::k_conv_out1<conv::kOutResidual>, dim3((unsigned)(8 * g8)), dim3(conv::kThreads), 0, s, x,
+                       w, bias, pre, rho, stats, c, r, rho_add, shape[0], shape[1], shape[2], (int)g8);
+  else
+    hipLaunchKernelGGL(conv::k_conv_out1<conv::kPlain>, dim3((unsigned)(8 * g8)), dim3(conv::kThreads), 0, s, x, w,
+                       bias, pre, rho, stats, c, r, rho_add, shape[0], shape[1], shape[2], (int)g8);
+  PM_LAUNCHED("k_conv_out1", s);
+  return PM_SUCCESS;
+}
+
+bool out1_blocks(const int32_t* shape) { return shape[0] % conv::kOBX == 0 && shape[2] % conv::kOBZ == 0; }
+
 // Workgroups of a persistent conv grid: what fits on the device at once (the kernel's
 // occupancy times the CU count), at most the number of blocks, rounded up to a multiple of 8
 // (one share per XCD).
@@ -740,6 +886,8 @@ int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, 
   PM_RETURN_IF((int64_t)conv::kH * shape[1] * shape[2] * cin >= (int64_t(1) << 31) ||
                    (int64_t)shape[0] * shape[1] * shape[2] / (conv::kB * conv::kB * conv::kB) >= (int64_t(1) << 31),
                PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
+  if (cin == 16 && pre_channels == 1 && act == nullptr && !gelu && res == nullptr && pre && out1_blocks(shape))
+    return launch_out1(xb, wb, bias, pre, nullptr, nullptr, 0.f, 0.f, nullptr, shape, s);
   const int64_t nblk = nblocks(shape);
   const int key = (cin == 16 ? 4 : 0) + (gelu ? 2 : 0) + (res ? 1 : 0) + (wide_blocks(shape) ? 8 : 0);
   void (*kern)(const __bf16*, const __bf16*, const float*, const __bf16*, float*, __bf16*, int, int, int, int,
@@ -803,6 +951,9 @@ int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, cons
                    (int64_t)shape[0] * shape[1] * shape[2] / (conv::kB * conv::kB * conv::kB) >= (int64_t(1) << 31),
                PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
   hipStream_t s = as_stream(stream);
+  if (out1_blocks(shape))
+    return launch_out1(static_cast<const __bf16*>(x), static_cast<const __bf16*>(w), bias, nullptr, rho, stats,
+                       1.5f * omega_m, max_residual, rho_add, shape, s);
   auto kern = wide_blocks(shape) ? conv::k_conv_fwd<16, false, false, conv::kOutResidual, 16>
                                  : conv::k_conv_fwd<16, false, false, conv::kOutResidual, 8>;
   unsigned grid = 0;

@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(kBlock) k_input_sums(const float* __restrict__
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const float rc = rho_cot[i];
     const float d = rho[i] - 1.f;
-    const float4 g = dx0[i * 4];  // channels 0..3 of the input gradient
+    const float4 g = dx0[i];  // channels 0..3 of the input gradient
     s1 += (double)(rc * tanhf(out[i]));
     s2 += (double)(g.x * (d * c));
     s3 += (double)g.z;
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(kBlock) k_input_bwd(const float* __restrict__ 
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const float d = rho[i] - 1.f;
     const float S = d * c;
-    const float Sbar = dx0[i * 4].x * inv + sb * S;
+    const float Sbar = dx0[i].x * inv + sb * S;
     rho_bar[i] = g * rho_cot[i] + c * Sbar;
     acc += (double)(Sbar * d);
   }
@@ -344,13 +344,20 @@ int pm_correction_features(const float* rho, int64_t n, float omega_m, float a, 
                            const float* gain_params, void* x0, float* stats, void* workspace,
                            size_t workspace_bytes, void* stream) {
   PM_TRY(check_mesh(n));
-  PM_RETURN_IF(rho == nullptr || gain_params == nullptr || stats == nullptr, PM_ERROR_INVALID_ARGUMENT,
-               "NULL argument");
+  PM_RETURN_IF(rho == nullptr || stats == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF(gain_params == nullptr && x0 == nullptr, PM_ERROR_INVALID_ARGUMENT,
+               "gain_params NULL (stats given) needs x0");
   PM_RETURN_IF(reinterpret_cast<uintptr_t>(x0) % 16 != 0, PM_ERROR_INVALID_ARGUMENT, "x0 must be 16-byte aligned");
-  PM_RETURN_IF(workspace == nullptr || workspace_bytes < kPartialBytes, PM_ERROR_WORKSPACE,
-               "correction workspace too small");
   hipStream_t s = as_stream(stream);
   const float c = 1.5f * omega_m;
+  if (gain_params == nullptr) {  // stats given (a slab of a mesh whose statistics are known)
+    hipLaunchKernelGGL(corr::k_features, dim3(corr::grid_for(n, 8)), dim3(corr::kBlock), 0, s, rho, n, c, stats, a,
+                       omega_m, sigma8, static_cast<uint4*>(x0));
+    PM_LAUNCHED("k_features", s);
+    return PM_SUCCESS;
+  }
+  PM_RETURN_IF(workspace == nullptr || workspace_bytes < kPartialBytes, PM_ERROR_WORKSPACE,
+               "correction workspace too small");
   double* part = static_cast<double*>(workspace);
   PM_RETURN_IF(reinterpret_cast<uintptr_t>(rho) % 16 != 0, PM_ERROR_INVALID_ARGUMENT, "rho must be 16-byte aligned");
   const unsigned nb = corr::grid_for(n, 16);

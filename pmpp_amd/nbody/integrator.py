@@ -170,6 +170,9 @@ class AdjointState:
         from .gravity import KeptForce
 
         self.kept = KeptForce()  # forward meshes of the last force recompute, for the next VJP
+        # with a learned correction: accumulate the gradient of its parameters here (flat
+        # float32, the parameter layout) when set
+        self.param_grads = None
 
     def particles(self):
         return self.ptcl.replace(disp=self.disp, vel=self.vel, acc=self.acc)
@@ -211,7 +214,8 @@ def force_adj(a, st: AdjointState, cosmo, conf, correction=None):
     else:
         from ..corrections.training import corrected_gravity_vjp
 
-        d_disp, d_om = corrected_gravity_vjp(a, st.particles(), cosmo, conf, correction, st.acc_cot)
+        d_disp, d_om = corrected_gravity_vjp(a, st.particles(), cosmo, conf, correction, st.acc_cot,
+                                             param_grads=st.param_grads)
     st.disp_cot += d_disp
     st.omega_cot_dev += d_om
     st.acc_cot.zero_()

@@ -41,16 +41,19 @@ def _run(ptcl, cosmo, conf, reverse=False, correction=None, observer=None):
     return ptcl
 
 
-def nbody_adj(ptcl, ptcl_cot, cosmo, conf, reverse=False, correction=None):
+def nbody_adj(ptcl, ptcl_cot, cosmo, conf, reverse=False, correction=None, param_grads=None):
     """Reverse sweep from the final state (solver.py:385-444).
 
     ``ptcl`` is the final state (disp, vel, acc), ``ptcl_cot`` a Particles-like object whose
     ``disp``/``vel``/``acc`` hold the cotangents of that state.  Returns
     ``(ptcl_in, ptcl_cot_in, omega_m_cot)``: the reconstructed initial state, the cotangent
-    of the initial state and the Omega_m cotangent.
+    of the initial state and the Omega_m cotangent.  With a ``correction`` and
+    ``param_grads`` (a flat float32 tensor in its parameter layout), the gradient with respect
+    to the correction's parameters is added to ``param_grads`` (training through the run).
     """
     a = _schedule(conf, reverse)
     st = AdjointState(ptcl, ptcl_cot.disp, ptcl_cot.vel, ptcl_cot.acc)
+    st.param_grads = param_grads
     for a0, a1 in zip(a[:-1][::-1], a[1:][::-1]):
         integrate_adj(float(a0), float(a1), st, cosmo, conf, correction=correction)
     # the initial force (computed by nbody_init before the first step) is pulled through last

@@ -103,3 +103,20 @@ def test_weight_gradient_cin8(device):
     _ref_conv(x, wr, torch.zeros(16)).backward(dz)
     dw, db = conv3d_bwd_weight(dz.to(device), x.to(device, torch.bfloat16))
     np.testing.assert_allclose(dw.cpu().numpy(), wr.grad.numpy(), atol=1e-4 * wr.grad.abs().max().item())
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 16), (24, 16, 32), (32, 16, 48)])
+def test_one_channel_output_layer_matches_torch(device, shape):
+    """The one-output-channel layer (taps on the matrix rows, k_conv_out1: meshes that divide
+    into 4 x 8 x 16 blocks) against channel 0 of the float32 convolution, on meshes of one
+    block along y and z (every y and z halo wraps) and of several blocks along each axis."""
+    g = torch.Generator().manual_seed(sum(shape))
+    x = _bf(torch.randn(shape + (16,), generator=g))
+    w = _bf(torch.randn(27, 16, 16, generator=g) * 0.2)
+    b = torch.randn(16, generator=g)
+    pre1, act = conv3d(x.to(device, torch.bfloat16), w.to(device, torch.bfloat16), b.to(device), want_act=False,
+                       pre_channels=1)
+    assert act is None and pre1.shape == shape + (1,)
+    ref = _ref_conv(x, w, b)[..., 0]
+    scale = ref.abs().max().item()
+    np.testing.assert_allclose(pre1[..., 0].cpu().numpy(), ref.numpy(), atol=1e-4 * scale)

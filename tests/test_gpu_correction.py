@@ -433,3 +433,57 @@ def test_nbody_reverse_sweep_with_a_correction_matches_finite_differences(device
         fm = nbody(base.replace(disp=base.disp - eps * v), cos, conf, correction=model).disp
     fd = float(((fp - fm) * cot).sum()) / (2 * eps)
     assert float((disp.grad * v).sum()) == pytest.approx(fd, rel=5e-2)
+
+
+@pytest.mark.parametrize("slab", [8, 16])
+def test_network_reverse_by_slabs_equals_the_whole_mesh(device, slab):
+    """The corrected force's reverse with the network's backward run over x-slabs with a halo
+    (network_input_vjp) against the same backward on the whole mesh: the slab's own planes
+    see exactly the whole-mesh arithmetic, so the cotangents agree to summation order."""
+    from pmpp_amd.corrections.training import corrected_gravity_vjp
+
+    conf = Configuration(2.0, (32, 16, 16), mesh_shape=1)
+    ptcl = _ptcl(conf, 0.6, 41, device)
+    model = _randomised(init_mesh_cnn_potential_correction(seed=6, depth=2, device=device, block="plain"), 6,
+                        out_scale=0.2)
+    rng = np.random.default_rng(14)
+    ac = torch.from_numpy(rng.normal(size=(conf.ptcl_num, 3)).astype(np.float32)).to(device)
+    d_full, o_full = corrected_gravity_vjp(0.7, ptcl, 0.3, conf, model, ac)
+    d_slab, o_slab = corrected_gravity_vjp(0.7, ptcl, 0.3, conf, model, ac, slab=slab)
+    np.testing.assert_allclose(_np(d_slab), _np(d_full), rtol=1e-5, atol=1e-6 * np.abs(_np(d_full)).max())
+    assert o_slab == pytest.approx(o_full, rel=1e-5)
+
+
+def test_whole_run_loss_parameter_gradient_matches_central_differences(device):
+    """The end-to-end training gradient (log P(k) and cross-correlation loss of a corrected run,
+    through the P(k) and r adjoints, the scatter adjoint and the N-body reverse sweep with the
+    network's parameter gradient) against central differences along a random direction of the
+    float32 biases and gain parameters (the bf16 weight mirror does not see small steps)."""
+    from pmpp_amd.corrections.training import RunTarget, run_loss
+
+    cos = LCDM()
+    conf = Configuration(4.0, (16, 16, 16), mesh_shape=2, a_start=1 / 16, a_nbody_maxstep=0.25)
+    tgt = RunTarget(3, cos, conf, conf.replace(mesh_shape=4), device=device)
+    model = _randomised(init_mesh_cnn_potential_correction(seed=8, depth=2, block="plain", max_residual=1.0,
+                                                           device=device), 8, out_scale=0.2)
+    grads = torch.zeros_like(model.params)
+    run_loss(model, tgt, cos, conf, grads=grads)
+    g = torch.Generator().manual_seed(9)
+    v = torch.zeros(model.params.numel())
+    for layer in model.layers:
+        v[layer.offset + layer.nweights:layer.offset + layer.size] = torch.randn(16, generator=g)
+    v[model.gain_offset:model.gain_offset + 4] = torch.randn(4, generator=g)
+    v = v.to(device)
+    got = float((grads * v).sum())
+    base = model.params.detach().clone()
+    eps = 1e-2
+
+    def at(t):
+        model.params.copy_(base + t * v)
+        model.sync_bf16()
+        return run_loss(model, tgt, cos, conf)
+
+    fd = (at(eps) - at(-eps)) / (2 * eps)
+    at(0.0)
+    print(f"whole-run loss gradient . v: adjoint {got:.6e}, central differences {fd:.6e}")
+    assert got == pytest.approx(fd, rel=5e-2)
