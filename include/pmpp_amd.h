@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 4
+#define PM_ABI_VERSION 5
 
 enum pm_status {
   PM_SUCCESS = 0,
@@ -318,6 +318,15 @@ int pm_conv3d_stem(const float* rho, const int32_t shape[3], float omega_m, floa
 int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, const float* rho,
                            const int32_t shape[3], float omega_m, float max_residual, const float* stats,
                            float* rho_add, void* stream);
+/* pm_conv3d_stem_hidden  the inference stem (as pm_conv3d_stem) fused with a first hidden
+ *                        layer that is a plain GELU conv (16 -> 16, weights w_hidden [27][16][16],
+ *                        bias b_hidden): act = gelu(conv(gelu(stem(rho))) + b_hidden), bf16
+ *                        (n x 16).  The stem's activation stays in LDS.  Meshes whose x extent is
+ *                        a multiple of 4 and z extent a multiple of 16 (PM_ERROR_UNSUPPORTED
+ *                        otherwise: use the two kernels). */
+int pm_conv3d_stem_hidden(const float* rho, const int32_t shape[3], float omega_m, float a, float sigma8,
+                          const float* stats, const void* w_stem, const float* b_stem, const void* w_hidden,
+                          const float* b_hidden, void* act, void* stream);
 
 /* ------------------------------------------------------------ learned correction ----
  * The elementwise stages around the conv stack of the mesh CNN correction

@@ -251,6 +251,7 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     layers = model.layers
     pres, acts = [], []
     gain = None if given else ptr(model.gain_params)
+    fused = (not keep and model.block == "plain" and model.depth >= 2 and shape[0] % 4 == 0 and shape[2] % 16 == 0)
     if keep:
         # the backward's stem weight gradient needs the network input x0
         x0 = torch.empty(shape + (IN_CHANNELS,), dtype=torch.bfloat16, device=dev)
@@ -265,11 +266,17 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
             _lib.call("pm_correction_features", ptr(rho), n, om, a, s8, float(model.max_residual), gain, None,
                       ptr(stats), ptr(ws), ws.numel(), s)
         x = torch.empty(shape + (CHANNELS,), dtype=torch.bfloat16, device=dev)
-        _lib.call("pm_conv3d_stem", ptr(rho), _lib.int3(shape), om, a, s8, ptr(stats),
-                  ptr(model.weight(layers[0])), ptr(model.bias(layers[0])), ptr(x), s)
+        if fused:
+            # stem and first (plain) hidden layer in one kernel: the stem's activation stays in LDS
+            _lib.call("pm_conv3d_stem_hidden", ptr(rho), _lib.int3(shape), om, a, s8, ptr(stats),
+                      ptr(model.weight(layers[0])), ptr(model.bias(layers[0])), ptr(model.weight(layers[1])),
+                      ptr(model.bias(layers[1])), ptr(x), s)
+        else:
+            _lib.call("pm_conv3d_stem", ptr(rho), _lib.int3(shape), om, a, s8, ptr(stats),
+                      ptr(model.weight(layers[0])), ptr(model.bias(layers[0])), ptr(x), s)
     pres.append(pre)
     acts.append(x)
-    for i in range(model.depth - 1):
+    for i in range(1 if fused else 0, model.depth - 1):
         if model.block == "plain":
             l1 = layers[1 + i]
             p1, x = conv3d(x, model.weight(l1), model.bias(l1), gelu=True, want_pre=keep)

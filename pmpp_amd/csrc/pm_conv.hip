@@ -540,6 +540,182 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
   }
 }
 
+// Inference stem fused with a first hidden layer that is a plain GELU conv:
+// act = gelu(conv16(gelu(stem(rho))) + b1).  The hidden layer's staged input tile (6 x 10 x 18
+// voxels of 16 channels around a 4 x 8 x 16 block) is computed in LDS from the density instead
+// of being read from HBM: the block stages the bf16 source of its 8 x 12 x 20 density tile
+// (two-voxel halo), runs the stem's one-step MFMA over the 1080 staged voxels (68 tiles of 16
+// voxels; lane (g, m) feeds taps 8g .. 8g+7 of voxel m with the same arithmetic as
+// k_conv_stem, so the activations are k_conv_stem's bit for bit), writes their GELU as bf16
+// into the tile and walks the hidden conv over it exactly as k_conv_fwd does.  The stem
+// activation (32 bytes per voxel written, then read back 2.8x over by the hidden layer's halo
+// tiles) never reaches HBM; the price is the stem's MFMA on 2.1x the block's voxels, one MFMA
+// per 16 of them.  Persistent and prefetching (the next block's density tile in registers
+// while the current block is computed), with the hidden layer's block order.
+constexpr int kFRX = 8, kFRY = 12, kFRZ = 20;                // density tile: block + two-voxel halo
+constexpr int kFRN = kFRX * kFRY * kFRZ;                     // 1920
+constexpr int kFRPer = (kFRN + kThreads - 1) / kThreads;     // 8
+
+__global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* __restrict__ rho, float c,
+                                                                  const float* __restrict__ stats, float a, float om,
+                                                                  float s8, const __bf16* __restrict__ ws,
+                                                                  const float* __restrict__ bs,
+                                                                  const __bf16* __restrict__ wh,
+                                                                  const float* __restrict__ bh,
+                                                                  __bf16* __restrict__ act, int n0, int n1, int n2) {
+  using Geo = FwdGeom<16>;
+  using Walk = PlaneWalk<16, true, false, kPlain, 16>;
+  using Taps = PlaneTaps<16, Geo::SZ>;
+  constexpr int TPS = Taps::TPS, KS = Taps::KS;
+  constexpr int NS = Geo::NS, SZ = Geo::SZ, SY = Geo::SY;
+  constexpr int BX = Geo::BX, BY = Geo::BY, BZ = Geo::BZ;
+  constexpr int STEM_TILES = (NS + 15) / 16;  // 68
+  constexpr int CS = 8;                       // channels of the stem's weight tensor
+  constexpr Taps taps{};
+  static_assert(taps.ncls <= 4, "step classes");
+  __shared__ __align__(16) __bf16 xin[2 * NS * 8];
+  __shared__ __align__(16) __bf16 rt[kFRN];
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int m = lane % 16, g = lane / 16;
+  const int gt = g / 2, gh = g % 2;  // hidden K group: tap slot, channel half
+  Walk wk;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int j = k * TPS + gt;
+      const int tap = d * 9 + (j < 9 ? j : 0);
+      bf16x8 v = *reinterpret_cast<const bf16x8*>(wh + (tap * kCout + m) * kCout + 8 * gh);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) wk.wa[d * KS + k][i] = j < 9 ? v[i] : (__bf16)0.f;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) wk.bsv[i] = bh[4 * g + i];
+  // stem: A fragment (channel-0 weights of taps 8g .. 8g+7 for output channel m) and the bias of
+  // output channels 4g .. 4g+3 with the constant input channels folded in, as k_conv_stem
+  bf16x8 wsa;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int tap = 8 * g + j;
+    wsa[j] = tap < kTaps ? ws[(tap * kCout + m) * CS] : (__bf16)0.f;
+  }
+  const __bf16 ba = (__bf16)a, bo = (__bf16)om, bss = (__bf16)s8;
+  float sbv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int co = 4 * g + i;
+    float t = 0.f;
+    for (int tap = 0; tap < kTaps; ++tap) {
+      const __bf16* wt = ws + (tap * kCout + co) * CS;
+      t += (float)wt[1] * (float)ba + (float)wt[2] * (float)bo + (float)wt[3] * (float)bss;
+    }
+    sbv[i] = bs[co] + t;
+  }
+  const float inv = 1.f / stats[0];
+  __builtin_amdgcn_s_waitcnt(0);
+  // density-tile offsets of this lane's 8 stem taps relative to a staged voxel's own cell
+  // (padded taps read tap 26's cell; their weights are zero)
+  typedef int TapOffsets __attribute__((ext_vector_type(8)));
+  TapOffsets soff;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int t = min(8 * g + j, kTaps - 1);
+    soff[j] = ((t / 9 - 1) * kFRY + ((t / 3) % 3 - 1)) * kFRZ + (t % 3 - 1);
+  }
+  const int ly = 2 * wave, lz = m;
+  {
+    const int corner = ly * SZ + lz;
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+      int d = 0;
+#pragma unroll
+      for (int t = 1; t < TPS; ++t) d = gt == t ? Taps::diff(taps.rep[cc < taps.ncls ? cc : 0], t) : d;
+      wk.cbase[cc] = (gh * NS + corner + d) * 8;
+    }
+  }
+  wk.xin = xin;
+  wk.plane = (int64_t)n1 * n2;
+  wk.rowz = n2;
+  wk.g = g;
+  wk.res = nullptr;
+  wk.pre = nullptr;
+  wk.act = act;
+  wk.pch = kCout;
+  wk.rho = nullptr;
+  wk.rho_add = nullptr;
+  // this thread's density-tile cells: offset from the tile's (-2, -2, -2) corner
+  typedef int Offsets __attribute__((ext_vector_type(kFRPer)));
+  Offsets loc;
+#pragma unroll
+  for (int u = 0; u < kFRPer; ++u) {
+    const int it = min((int)threadIdx.x + u * kThreads, kFRN - 1);
+    const int z = it % kFRZ, y = (it / kFRZ) % kFRY, xx = it / (kFRZ * kFRY);
+    loc[u] = (xx * n1 + y) * n2 + z;
+  }
+  const int nbz = n2 / BZ, nby = n1 / BY;
+  const int nblk = (n0 / BX) * nby * nbz;
+  const int G = gridDim.x, per = G / 8, first = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  typedef float Prefetch __attribute__((ext_vector_type(kFRPer)));
+  int cur = -1, nxt = first;
+  for (;;) {
+    Prefetch pf;
+    if (nxt < nblk) {
+      int bx, by, bz;
+      block_of(nxt, n0 / BX, nby, nbz, bx, by, bz);
+      const int ox = bx * BX, oy = by * BY, oz = bz * BZ;
+      const bool inner = ox >= 2 && ox + BX + 2 <= n0 && oy >= 2 && oy + BY + 2 <= n1 && oz >= 2 && oz + BZ + 2 <= n2;
+      if (inner) {
+        const float* base = rho + ((int64_t)(ox - 2) * n1 + (oy - 2)) * n2 + (oz - 2);
+#pragma unroll
+        for (int u = 0; u < kFRPer; ++u) pf[u] = base[loc[u]];
+      } else {
+#pragma unroll
+        for (int u = 0; u < kFRPer; ++u) {
+          const int it = min((int)threadIdx.x + u * kThreads, kFRN - 1);
+          const int z = it % kFRZ, y = (it / kFRZ) % kFRY, xx = it / (kFRZ * kFRY);
+          const int gx = wrap(ox + xx - 2, n0), gy = wrap(oy + y - 2, n1), gz = wrap(oz + z - 2, n2);
+          pf[u] = rho[((int64_t)gx * n1 + gy) * n2 + gz];
+        }
+      }
+    }
+    if (cur >= 0) {
+      int bx, by, bz;
+      block_of(cur, n0 / BX, nby, nbz, bx, by, bz);
+      const int64_t e0 = (((int64_t)bx * BX * n1 + by * BY) * n2 + bz * BZ);
+      wk.el = e0 + (int64_t)ly * n2 + lz;
+      wk.read_plane(0, 0, 0);
+      wk.template walk<0, 0>();
+    }
+    if (nxt >= nblk) break;
+    __syncthreads();  // the hidden layer's MFMAs of block cur are done with the tile
+#pragma unroll
+    for (int u = 0; u < kFRPer; ++u) {
+      const int i = threadIdx.x + u * kThreads;
+      if (i < kFRN) rt[i] = (__bf16)(((pf[u] - 1.f) * c) * inv);
+    }
+    __syncthreads();
+    // the stem on the staged voxels of block nxt, into the hidden layer's tile
+    for (int t = wave; t < STEM_TILES; t += kThreads / 64) {
+      const int v = 16 * t + m;
+      const int vc = min(v, NS - 1);
+      const int sz = vc % SZ, sy = (vc / SZ) % SY, sx = vc / (SZ * SY);
+      const int own = ((sx + 1) * kFRY + (sy + 1)) * kFRZ + (sz + 1);
+      bf16x8 b;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) b[j] = rt[own + soff[j]];
+      const f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsa, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      bf16x4 o;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i] + sbv[i]);
+      if (v < NS) *reinterpret_cast<bf16x4*>(xin + ((g / 2) * NS + v) * 8 + 4 * (g % 2)) = o;
+    }
+    __syncthreads();
+    cur = nxt;
+    nxt += G;
+  }
+}
+
 // Output layer: one output channel from 16 input channels.  The 16 -> 16 implicit GEMM would
 // spend 15 of its 16 matrix rows on channels that are thrown away, so this kernel puts the
 // taps on the rows instead: for every staged (x, y) column u of the block and each of the nine
@@ -571,33 +747,11 @@ __global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict
                                                         const float* __restrict__ bias, float* __restrict__ pre,
                                                         const float* __restrict__ rho, const float* __restrict__ stats,
                                                         float c, float r, float* __restrict__ rho_add, int n0, int n1,
-                                                        int n2, int g8) {
+                                                        int n2) {
   __shared__ __align__(16) float lds[kONS * 8];
   __bf16* xin = reinterpret_cast<__bf16*>(lds);
   const int nbz = n2 / kOBZ, nby = n1 / kOBY;
   const int nblk = (n0 / kOBX) * nby * nbz;
-  const int L = blockIdx.x;
-  const int q = (L % 8) * g8 + L / 8;  // XCD-aware: consecutive blocks on one XCD
-  if (q >= nblk) return;               // padding workgroup: uniform exit before any barrier
-  const int bz = q % nbz, by = (q / nbz) % nby, bx = q / (nbz * nby);
-  const int ox = bx * kOBX, oy = by * kOBY, oz = bz * kOBZ;
-  // stage the halo tile: item i is half (i % 2) of staged voxel i / 2 (global order, so
-  // consecutive threads read consecutive 16 bytes); LDS keeps the halves as two planes
-  uint4 pf[kOPer];
-  const bool inner = ox > 0 && ox + kOBX < n0 && oy > 0 && oy + kOBY < n1 && oz > 0 && oz + kOBZ < n2;
-#pragma unroll
-  for (int u = 0; u < kOPer; ++u) {
-    const int it = min((int)threadIdx.x + u * kThreads, kOItems - 1);
-    const int h = it % 2, vox = it / 2;
-    const int sz = vox % kOSZ, sy = (vox / kOSZ) % kOSY, sx = vox / (kOSZ * kOSY);
-    int gx = ox + sx - 1, gy = oy + sy - 1, gz = oz + sz - 1;
-    if (!inner) {
-      gx = wrap(gx, n0);
-      gy = wrap(gy, n1);
-      gz = wrap(gz, n2);
-    }
-    pf[u] = *reinterpret_cast<const uint4*>(x + (((int64_t)gx * n1 + gy) * n2 + gz) * kCout + 8 * h);
-  }
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int m = lane % 16, g = lane / 16;
   // A fragments: row m = in-plane offset t = 3 (dx + 1) + (dy + 1), K = 8 g + j.
@@ -613,57 +767,130 @@ __global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict
       wa1[j] = (m < 9 && g < 2) ? w[((3 * m + 2) * kCout) * kCout + ch + j] : (__bf16)0.f;
     }
   }
-#pragma unroll
-  for (int u = 0; u < kOPer; ++u) {
-    const int i = threadIdx.x + u * kThreads;
-    if (i < kOItems) *reinterpret_cast<uint4*>(xin + ((i % 2) * kONS + i / 2) * 8) = pf[u];
-  }
-  __syncthreads();
-  // B fragments of staged column col: lane (g, m) feeds voxel z = m (staged z = m + 1 + dz),
-  // channel half g % 2; step 0 reads dz = -1 (g < 2) or 0, step 1 reads dz = +1
-  const int h = g % 2;
-  const int sz0 = g < 2 ? m : m + 1, sz1 = m + 2;
-  f32x4 acc[kOColsPerWave];
-#pragma unroll
-  for (int k = 0; k < kOColsPerWave; ++k) {
-    const int col = wave + 4 * k;
-    const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(xin + (h * kONS + col * kOSZ + sz0) * 8);
-    const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(xin + (h * kONS + col * kOSZ + sz1) * 8);
-    f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa0, b0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-    acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, b1, d, 0, 0, 0);
-  }
-  __syncthreads();  // every wave is done reading the staged tile
-  // D over the tile: lds[(col * 9 + t) * 16 + z]; lane (g, m) holds t = 4 g + i of voxel z = m
-#pragma unroll
-  for (int k = 0; k < kOColsPerWave; ++k) {
-    const int col = wave + 4 * k;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (4 * g + i < 9) lds[(col * 9 + 4 * g + i) * kOBZ + m] = acc[k][i];
-  }
-  __syncthreads();
   const float b0v = bias[0];
   float gain = 0.f, amp = 0.f;
   if constexpr (EPI == kOutResidual) {
     gain = stats[1];
     amp = r * stats[0] / c;
   }
+  __builtin_amdgcn_s_waitcnt(0);  // weights in registers before the prefetching loop
+  // staged items of this thread: item i is half (i % 2) of staged voxel i / 2 (global order, so
+  // consecutive threads read consecutive 16 bytes); element offset from the block's
+  // (-1, -1, -1) corner when the halo does not wrap (the caller checks it fits int32)
+  typedef int Offsets __attribute__((ext_vector_type(kOPer)));
+  Offsets loc;
 #pragma unroll
-  for (int s = 0; s < (kOBX * kOBY * kOBZ) / kThreads; ++s) {
-    const int v = threadIdx.x + s * kThreads;
-    const int z = v % kOBZ, y = (v / kOBZ) % kOBY, xx = v / (kOBZ * kOBY);
-    float o = 0.f;
+  for (int u = 0; u < kOPer; ++u) {
+    const int it = min((int)threadIdx.x + u * kThreads, kOItems - 1);
+    const int h = it % 2, vox = it / 2;
+    const int sz = vox % kOSZ, sy = (vox / kOSZ) % kOSY, sx = vox / (kOSZ * kOSY);
+    loc[u] = ((sx * n1 + sy) * n2 + sz) * kCout + 8 * h;
+  }
+  // B fragments of staged column col: lane (g, m) feeds voxel z = m (staged z = m + 1 + dz),
+  // channel half g % 2; step 0 reads dz = -1 (g < 2) or 0, step 1 reads dz = +1
+  const int h = g % 2;
+  const int sz0 = g < 2 ? m : m + 1, sz1 = m + 2;
+  constexpr int VPT = (kOBX * kOBY * kOBZ) / kThreads;  // output voxels per thread
+  const int G = gridDim.x, per = G / 8, first = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  typedef uint32_t Prefetch __attribute__((ext_vector_type(4 * kOPer)));  // a value, never a private array
+  int cur = -1, nxt = first;
+  int64_t ecur = 0;  // origin voxel of block cur
+  for (;;) {
+    // the current block's epilogue loads go first: the vector memory counter waits in order
+    float rv[VPT];
+    if (cur >= 0) {
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-      const int col = (xx + t / 3) * kOSY + (y + t % 3);  // staged column (x + 1 + dx, y + 1 + dy)
-      o += lds[(col * 9 + t) * kOBZ + z];
+      for (int s = 0; s < VPT; ++s) {
+        const int v = threadIdx.x + s * kThreads;
+        const int z = v % kOBZ, y = (v / kOBZ) % kOBY, xx = v / (kOBZ * kOBY);
+        const int64_t e = ecur + ((int64_t)xx * n1 + y) * n2 + z;
+        rv[s] = EPI == kOutResidual ? rho[e] : 0.f;
+      }
     }
-    const float zv = o + b0v;
-    const int64_t e = ((int64_t)(ox + xx) * n1 + (oy + y)) * n2 + oz + z;
-    if constexpr (EPI == kOutResidual)
-      rho_add[e] = gain * (rho[e] - 1.f) + amp * tanhf(zv);
-    else
-      pre[e] = zv;
+    Prefetch pf;
+    if (nxt < nblk) {
+      const int bz = nxt % nbz, by = (nxt / nbz) % nby, bx = nxt / (nbz * nby);
+      const int ox = bx * kOBX, oy = by * kOBY, oz = bz * kOBZ;
+      const bool inner = ox > 0 && ox + kOBX < n0 && oy > 0 && oy + kOBY < n1 && oz > 0 && oz + kOBZ < n2;
+      if (inner) {
+        const __bf16* base = x + (((int64_t)(ox - 1) * n1 + (oy - 1)) * n2 + (oz - 1)) * kCout;
+#pragma unroll
+        for (int u = 0; u < kOPer; ++u) {  // unconditional (clamped) loads, all in flight together
+          const uint4 t = *reinterpret_cast<const uint4*>(base + loc[u]);
+          pf[4 * u] = t.x;
+          pf[4 * u + 1] = t.y;
+          pf[4 * u + 2] = t.z;
+          pf[4 * u + 3] = t.w;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < kOPer; ++u) {
+          const int it = min((int)threadIdx.x + u * kThreads, kOItems - 1);
+          const int hh = it % 2, vox = it / 2;
+          const int sz = vox % kOSZ, sy = (vox / kOSZ) % kOSY, sx = vox / (kOSZ * kOSY);
+          const int gx = wrap(ox + sx - 1, n0), gy = wrap(oy + sy - 1, n1), gz = wrap(oz + sz - 1, n2);
+          const uint4 t = *reinterpret_cast<const uint4*>(x + (((int64_t)gx * n1 + gy) * n2 + gz) * kCout + 8 * hh);
+          pf[4 * u] = t.x;
+          pf[4 * u + 1] = t.y;
+          pf[4 * u + 2] = t.z;
+          pf[4 * u + 3] = t.w;
+        }
+      }
+    }
+    if (cur >= 0) {
+      f32x4 acc[kOColsPerWave];
+#pragma unroll
+      for (int k = 0; k < kOColsPerWave; ++k) {
+        const int col = wave + 4 * k;
+        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(xin + (h * kONS + col * kOSZ + sz0) * 8);
+        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(xin + (h * kONS + col * kOSZ + sz1) * 8);
+        f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa0, b0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, b1, d, 0, 0, 0);
+      }
+      __syncthreads();  // every wave is done reading the staged tile
+      // D over the tile: lds[(col * 9 + t) * 16 + z]; lane (g, m) holds t = 4 g + i of voxel z = m
+#pragma unroll
+      for (int k = 0; k < kOColsPerWave; ++k) {
+        const int col = wave + 4 * k;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (4 * g + i < 9) lds[(col * 9 + 4 * g + i) * kOBZ + m] = acc[k][i];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < VPT; ++s) {
+        const int v = threadIdx.x + s * kThreads;
+        const int z = v % kOBZ, y = (v / kOBZ) % kOBY, xx = v / (kOBZ * kOBY);
+        float o = 0.f;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+          const int col = (xx + t / 3) * kOSY + (y + t % 3);  // staged column (x + 1 + dx, y + 1 + dy)
+          o += lds[(col * 9 + t) * kOBZ + z];
+        }
+        const float zv = o + b0v;
+        const int64_t e = ecur + ((int64_t)xx * n1 + y) * n2 + z;
+        if constexpr (EPI == kOutResidual)
+          rho_add[e] = gain * (rv[s] - 1.f) + amp * tanhf(zv);
+        else
+          pre[e] = zv;
+      }
+    }
+    if (nxt >= nblk) break;
+    __syncthreads();  // the reduction is done with D
+#pragma unroll
+    for (int u = 0; u < kOPer; ++u) {
+      const int i = threadIdx.x + u * kThreads;
+      if (i < kOItems)
+        *reinterpret_cast<uint4*>(xin + ((i % 2) * kONS + i / 2) * 8) =
+            make_uint4(pf[4 * u], pf[4 * u + 1], pf[4 * u + 2], pf[4 * u + 3]);
+    }
+    __syncthreads();
+    cur = nxt;
+    {
+      const int bz = cur % nbz, by = (cur / nbz) % nby, bx = cur / (nbz * nby);
+      ecur = ((int64_t)bx * kOBX * n1 + by * kOBY) * n2 + bz * kOBZ;
+    }
+    nxt += G;
   }
 }
 
@@ -849,24 +1076,6 @@ unsigned nblocks(const int32_t* shape) {
   return (unsigned)((int64_t)(shape[0] / conv::kB) * (shape[1] / conv::kB) * (shape[2] / conv::kB));
 }
 
-// The one-output-channel layer with the taps on the matrix rows (k_conv_out1): one workgroup
-// per 4 x 8 x 16 block, in the XCD-aware order (grid rounded up to a multiple of 8)
-int launch_out1(const __bf16* x, const __bf16* w, const float* bias, float* pre, const float* rho,
-                const float* stats, float c, float r, float* rho_add, const int32_t* shape, hipStream_t s) {
-  const int64_t nblk = (int64_t)shape[0] / conv::kOBX * (shape[1] / conv::kOBY) * (shape[2] / conv::kOBZ);
-  const int64_t g8 = ceil_div(nblk, 8);
-  PM_RETURN_IF(8 * g8 >= (int64_t(1) << 31), PM_ERROR_UNSUPPORTED, "conv3d: too many blocks");
-  if (rho_add)
-    hipLaunchKernelGGL(conv::k_conv_out1<conv::kOutResidual>, dim3((unsigned)(8 * g8)), dim3(conv::kThreads), 0, s, x,
-                       w, bias, pre, rho, stats, c, r, rho_add, shape[0], shape[1], shape[2], (int)g8);
-  else
-    hipLaunchKernelGGL(conv::k_conv_out1<conv::kPlain>, dim3((unsigned)(8 * g8)), dim3(conv::kThreads), 0, s, x, w,
-                       bias, pre, rho, stats, c, r, rho_add, shape[0], shape[1], shape[2], (int)g8);
-  PM_LAUNCHED("k_conv_out1", s);
-  return PM_SUCCESS;
-}
-
-bool out1_blocks(const int32_t* shape) { return shape[0] % conv::kOBX == 0 && shape[2] % conv::kOBZ == 0; }
 
 // Workgroups of a persistent conv grid: what fits on the device at once (the kernel's
 // occupancy times the CU count), at most the number of blocks, rounded up to a multiple of 8
@@ -881,6 +1090,32 @@ int persistent_grid(const void* kern, int64_t nblk, unsigned* grid) {
   *grid = (unsigned)(ceil_div(g, 8) * 8);
   return PM_SUCCESS;
 }
+
+// The one-output-channel layer with the taps on the matrix rows (k_conv_out1): a persistent
+// grid over the 4 x 8 x 16 blocks (the next block's tile prefetched while one is computed)
+int launch_out1(const __bf16* x, const __bf16* w, const float* bias, float* pre, const float* rho,
+                const float* stats, float c, float r, float* rho_add, const int32_t* shape, hipStream_t s) {
+  const int64_t nblk = (int64_t)shape[0] / conv::kOBX * (shape[1] / conv::kOBY) * (shape[2] / conv::kOBZ);
+  PM_RETURN_IF(nblk >= (int64_t(1) << 31) ||
+                   (int64_t)conv::kOSX * shape[1] * shape[2] * conv::kCout >= (int64_t(1) << 31),
+               PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
+  unsigned grid = 0;
+  if (rho_add) {
+    auto kern = conv::k_conv_out1<conv::kOutResidual>;
+    PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblk, &grid));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, x, w, bias, pre, rho, stats, c, r, rho_add,
+                       shape[0], shape[1], shape[2]);
+  } else {
+    auto kern = conv::k_conv_out1<conv::kPlain>;
+    PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblk, &grid));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, x, w, bias, pre, rho, stats, c, r, rho_add,
+                       shape[0], shape[1], shape[2]);
+  }
+  PM_LAUNCHED("k_conv_out1", s);
+  return PM_SUCCESS;
+}
+
+bool out1_blocks(const int32_t* shape) { return shape[0] % conv::kOBX == 0 && shape[2] % conv::kOBZ == 0; }
 
 constexpr int kWGrid = 1024;  // persistent workgroups of the weight-gradient pass
 
@@ -960,6 +1195,29 @@ int pm_conv3d_stem(const float* rho, const int32_t shape[3], float omega_m, floa
                      omega_m, sigma8, static_cast<const __bf16*>(w), bias, static_cast<__bf16*>(act), shape[0],
                      shape[1], shape[2]);
   PM_LAUNCHED("k_conv_stem", s);
+  return PM_SUCCESS;
+}
+
+int pm_conv3d_stem_hidden(const float* rho, const int32_t shape[3], float omega_m, float a, float sigma8,
+                          const float* stats, const void* w_stem, const float* b_stem, const void* w_hidden,
+                          const float* b_hidden, void* act, void* stream) {
+  PM_TRY(check_shape(shape));
+  PM_RETURN_IF(!wide_blocks(shape), PM_ERROR_UNSUPPORTED,
+               "fused stem + hidden conv: the mesh's x extent must be a multiple of 4 and z of 16");
+  PM_RETURN_IF(rho == nullptr || stats == nullptr || w_stem == nullptr || b_stem == nullptr || b_hidden == nullptr,
+               PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_TRY(check_ptr16(w_hidden, "w_hidden"));
+  PM_TRY(check_ptr16(act, "act"));
+  PM_RETURN_IF((int64_t)shape[0] * shape[1] * shape[2] / (conv::kB * conv::kB * conv::kB) >= (int64_t(1) << 31) ||
+                   (int64_t)conv::kFRX * shape[1] * shape[2] >= (int64_t(1) << 31),
+               PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
+  hipStream_t s = as_stream(stream);
+  unsigned grid = 0;
+  PM_TRY(persistent_grid(reinterpret_cast<const void*>(conv::k_conv_stem_hidden), nblocks(shape), &grid));
+  hipLaunchKernelGGL(conv::k_conv_stem_hidden, dim3(grid), dim3(conv::kThreads), 0, s, rho, 1.5f * omega_m, stats, a,
+                     omega_m, sigma8, static_cast<const __bf16*>(w_stem), b_stem, static_cast<const __bf16*>(w_hidden),
+                     b_hidden, static_cast<__bf16*>(act), shape[0], shape[1], shape[2]);
+  PM_LAUNCHED("k_conv_stem_hidden", s);
   return PM_SUCCESS;
 }
 

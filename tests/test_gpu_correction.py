@@ -487,3 +487,30 @@ def test_whole_run_loss_parameter_gradient_matches_central_differences(device):
     at(0.0)
     print(f"whole-run loss gradient . v: adjoint {got:.6e}, central differences {fd:.6e}")
     assert got == pytest.approx(fd, rel=5e-2)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 16), (32, 32, 48), (64, 64, 64)])
+def test_fused_stem_and_hidden_layer_equal_the_two_kernels(device, shape):
+    """pm_conv3d_stem_hidden (the stem computed on the hidden layer's staged tile in LDS) against
+    pm_conv3d_stem followed by the hidden conv: the same arithmetic in the same order, so the
+    activations agree bit for bit; meshes of one block along y and z (every halo wraps) and of
+    several blocks along each axis."""
+    from pmpp_amd import _lib
+    from pmpp_amd._torch import ptr, stream_handle
+    from pmpp_amd.corrections import conv3d
+
+    model = _randomised(init_mesh_cnn_potential_correction(seed=9, depth=2, block="plain", device=device), 9)
+    rng = np.random.default_rng(10)
+    rho = torch.from_numpy((1 + 0.5 * rng.normal(size=shape)).astype(np.float32)).to(device)
+    a, om = 0.55, 0.31
+    _, tape = evaluate_residual(model, rho, a, om, keep_out=True)  # the statistics
+    L0, L1 = model.layers[0], model.layers[1]
+    s = stream_handle(device)
+    stem = torch.empty(shape + (16,), dtype=torch.bfloat16, device=device)
+    _lib.call("pm_conv3d_stem", ptr(rho), _lib.int3(shape), om, a, float(model.sigma8_value), ptr(tape.stats),
+              ptr(model.weight(L0)), ptr(model.bias(L0)), ptr(stem), s)
+    _, ref = conv3d(stem, model.weight(L1), model.bias(L1), gelu=True, want_pre=False)
+    got = torch.empty(shape + (16,), dtype=torch.bfloat16, device=device)
+    _lib.call("pm_conv3d_stem_hidden", ptr(rho), _lib.int3(shape), om, a, float(model.sigma8_value), ptr(tape.stats),
+              ptr(model.weight(L0)), ptr(model.bias(L0)), ptr(model.weight(L1)), ptr(model.bias(L1)), ptr(got), s)
+    np.testing.assert_array_equal(got.float().cpu().numpy(), ref.float().cpu().numpy())
