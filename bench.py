@@ -330,7 +330,7 @@ def train_correction_run(args):
         t0 = time.perf_counter()
         model, losses = train_correction(
             pairs, cosmo, conf, iters=args.train_iters, learning_rate=lr, max_residual=mr, symmetry=sym, device=dev,
-            progress=lambda it, m, tag=tag: log(f"[train] {tag} step {it}: mean training loss {m:.6f}"))
+            potential=args.potential, progress=lambda it, m, tag=tag: log(f"[train] {tag} step {it}: mean training loss {m:.6f}"))
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
         val_loss = mean_force_loss(model, val_pairs, cosmo, conf)
@@ -386,7 +386,7 @@ def train_correction_run(args):
               "iters": args.train_iters, "validation_seed": 999, "judge_seed": 1000,
               "max_residual": best["max_residual"], "lr": best["lr"], "symmetry": best["symmetry"],
               "e2e_iters": args.e2e_iters, "e2e_seeds": f"100-{99 + args.e2e_seeds}", "e2e_lr": args.e2e_lr,
-              "r_weight": args.r_weight}
+              "r_weight": args.r_weight, "potential": int(args.potential)}
     if args.save:
         save_correction(model, args.save, **recipe)
         log(f"[train] wrote {args.save}")
@@ -486,9 +486,9 @@ def network_run(args):
     ptcl = particles_2lpt(conf, cosmo, seed=0, jitter_seed=1, device=dev, a=1.0)
     rho = density(ptcl, cosmo.Omega_m, conf)
     del ptcl
-    model = init_mesh_cnn_potential_correction(seed=0, depth=2, block="plain", device=dev)
+    model = init_mesh_cnn_potential_correction(seed=0, depth=2, block="plain", device=dev, potential=args.potential)
     for _ in range(args.warmup):
-        evaluate_residual(model, rho, 1.0, cosmo.Omega_m)
+        evaluate_residual(model, rho, 1.0, cosmo.Omega_m, conf=conf)
     torch.cuda.synchronize()
     stream = stream_handle(dev)
     cap = 16 * args.steps + 16
@@ -496,7 +496,7 @@ def network_run(args):
     _lib.call("pm_timing_mark", b"begin", stream)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        evaluate_residual(model, rho, 1.0, cosmo.Omega_m)
+        evaluate_residual(model, rho, 1.0, cosmo.Omega_m, conf=conf)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (ctypes.c_float * cap)()
@@ -594,6 +594,8 @@ def main():
     ap.add_argument("--e2e-lr", type=float, default=1e-4)
     ap.add_argument("--e2e-validate-every", type=int, default=50)
     ap.add_argument("--r-weight", type=float, default=1.0)
+    ap.add_argument("--potential", action="store_true",
+                    help="with --train-correction or --network: the network also reads the potential channel")
     ap.add_argument("--save", default=None,
                     help="with --train-correction: write the chosen weights to this path (the package ships "
                          "pmpp_amd/corrections/weights/mesh_cnn_2x_from_4x.npz)")

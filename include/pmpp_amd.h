@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 5
+#define PM_ABI_VERSION 6
 
 enum pm_status {
   PM_SUCCESS = 0,
@@ -247,6 +247,19 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
                              float* disp_cot, float* omega_cot, float* rho_cot, void* workspace,
                              size_t workspace_bytes, void* stream);
 
+/* The particle-Nyquist-filtered potential of a real mesh, on the force step's spectral band:
+ *   out = add + irfftn(mask(k) * (-1/k^2) * rfftn(scale * (in + shift)))
+ * (nbody/gravity.py:174-196, 263-321, 626-633: the potential the reference's corrected force
+ * computes before apply_potential_correction, corrections/core.py:286-334, and feeds to the
+ * mesh CNN as its potential channel, corrections/mesh_cnn.py:20-27, 76-84).  shift = -1,
+ * scale = 1.5 Omega_m on a density mesh gives the potential of the force's source; the operator
+ * is symmetric, so shift = 0 applies its adjoint to a cotangent.  add (NULL: none) may alias
+ * out; in may alias out (it is consumed before out is written).  desc gives the mesh, cell size,
+ * particle spacing and mask; its omega_m is not used.  Workspace: pm_potential_workspace. */
+int pm_potential_workspace(const pm_gravity_desc* desc, size_t* bytes);
+int pm_gravity_potential(const float* in, float shift, float scale, const float* add, const pm_gravity_desc* desc,
+                         float* out, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------ N-body updates ----
  * Elementwise stages of the symplectic integrator on (n,3) float32 arrays (16-byte aligned):
  * pm_kick   vel_out  = vel  + factor * acc   replaces kick      (nbody/integrator.py:512-538)
@@ -300,9 +313,10 @@ int pm_power_spectrum_bwd(const float* spec, const int32_t shape[3], int32_t mas
  *                      gradient dz and layer input x; workspace from the query.
  * pm_conv3d_stem       act = gelu(conv(x0; w[27][16][8]) + bias) for the network input x0 that
  *                      pm_correction_features would write from the density rho (float32
- *                      mesh) and its stats, without writing x0: the constant input channels
- *                      (a, Omega_m, sigma8) are folded into the bias (inference; training
- *                      keeps x0 for the weight gradient).
+ *                      mesh), the potential phi (NULL: no potential channel) and their stats,
+ *                      without writing x0: the constant input channels (a, Omega_m, sigma8) are
+ *                      folded into the bias (inference; training keeps x0 for the weight
+ *                      gradient).
  * pm_conv3d_out_residual  the output layer fused with pm_correction_residual (inference):
  *                      rho_add = g (rho - 1) + max_residual tanh(conv(x; w[27][16][16]) + bias,
  *                      channel 0) sigma / c, without writing the output pre-activation. */
@@ -313,7 +327,7 @@ int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const f
 int pm_conv3d_bwd_weight_workspace(const int32_t shape[3], int32_t cin, size_t* bytes);
 int pm_conv3d_bwd_weight(const float* dz, const void* x, int32_t cin, const int32_t shape[3], float* dw_db,
                          void* workspace, size_t workspace_bytes, void* stream);
-int pm_conv3d_stem(const float* rho, const int32_t shape[3], float omega_m, float a, float sigma8,
+int pm_conv3d_stem(const float* rho, const float* phi, const int32_t shape[3], float omega_m, float a, float sigma8,
                    const float* stats, const void* w, const float* bias, void* act, void* stream);
 int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, const float* rho,
                            const int32_t shape[3], float omega_m, float max_residual, const float* stats,
@@ -324,9 +338,9 @@ int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, cons
  *                        (n x 16).  The stem's activation stays in LDS.  Meshes whose x extent is
  *                        a multiple of 4 and z extent a multiple of 16 (PM_ERROR_UNSUPPORTED
  *                        otherwise: use the two kernels). */
-int pm_conv3d_stem_hidden(const float* rho, const int32_t shape[3], float omega_m, float a, float sigma8,
-                          const float* stats, const void* w_stem, const float* b_stem, const void* w_hidden,
-                          const float* b_hidden, void* act, void* stream);
+int pm_conv3d_stem_hidden(const float* rho, const float* phi, const int32_t shape[3], float omega_m, float a,
+                          float sigma8, const float* stats, const void* w_stem, const float* b_stem,
+                          const void* w_hidden, const float* b_hidden, void* act, void* stream);
 
 /* ------------------------------------------------------------ learned correction ----
  * The elementwise stages around the conv stack of the mesh CNN correction
@@ -336,8 +350,12 @@ int pm_conv3d_stem_hidden(const float* rho, const int32_t shape[3], float omega_
  *                         g = max_residual tanh(gain_params . [a, Omega_m, sigma8, 1]) into
  *                         stats[0..2] (device floats); x0 = bf16 [S/sigma, a, Omega_m, sigma8,
  *                         0, 0, 0, 0] per cell (n x 8 bf16, 16-byte aligned; NULL: stats only).
- *                         With gain_params NULL, stats are read instead (those of the whole
- *                         mesh when rho is a slab of it) and only x0 is written.
+ *                         With the potential phi (float32 mesh, pm_gravity_potential of rho;
+ *                         NULL: no potential channel, mesh_cnn.py:20-27, 81-84) also
+ *                         stats[3] = sigma_p = sqrt(mean phi^2 + 1e-6) and x0 = bf16 [S/sigma,
+ *                         phi/sigma_p, a, Omega_m, sigma8, 0, 0, 0].  With gain_params NULL,
+ *                         stats are read instead (those of the whole mesh when rho and phi are
+ *                         slabs of it) and only x0 is written.
  * pm_correction_residual  rho_add = g (rho - 1) + max_residual tanh(out) sigma / c, out the
  *                         output layer's float32 pre-activation of channel 0 (n floats).
  * pm_correction_residual_bwd  the output layer's pre-activation gradient dz_out (n x 16,
@@ -346,9 +364,12 @@ int pm_conv3d_stem_hidden(const float* rho, const int32_t shape[3], float omega_
  * pm_correction_input_bwd the cotangents through the network's inputs: rho_bar (n floats,
  *                         overwritten) = d<rho_cot, rho_add>/d rho and omega_cot[0] += its
  *                         Omega_m derivative, from dx0 (n x 4 float32, channels 0..3 of the
- *                         gradient of the network input; 0 and 2 are read), the forward's out and
- *                         stats.  With the scatter adjoint of rho_bar this is reverse mode of
- *                         the corrected force through the density. 
+ *                         gradient of the network input), the forward's out and stats.  With the
+ *                         potential channel (phi not NULL) it also writes phi_bar (n floats), the
+ *                         cotangent of phi, whose path to the density the caller adds with
+ *                         pm_gravity_potential(phi_bar, 0, 1.5 Omega_m, rho_bar, ..., rho_bar);
+ *                         rho_bar holds the other paths.  With the scatter adjoint of rho_bar this
+ *                         is reverse mode of the corrected force through the density.
  * pm_force_loss           loss[0] = sum |acc - target|^2 / sum |target|^2 (device doubles,
  *                         loss[1] = 1 / sum |target|^2), acc_cot = d loss / d acc if not NULL.
  * pm_adamax_step          clip_by_global_norm(clip_norm) then adamax(learning_rate, b1, b2, eps)
@@ -358,17 +379,18 @@ int pm_conv3d_stem_hidden(const float* rho, const int32_t shape[3], float omega_
  * pm_to_bf16              dst = bf16(src).
  * Reductions are deterministic (fixed-order float64 partials); the workspace holds them. */
 int pm_correction_workspace(size_t* bytes);
-int pm_correction_features(const float* rho, int64_t n, float omega_m, float a, float sigma8, float max_residual,
-                           const float* gain_params, void* x0, float* stats, void* workspace,
+int pm_correction_features(const float* rho, const float* phi, int64_t n, float omega_m, float a, float sigma8,
+                           float max_residual, const float* gain_params, void* x0, float* stats, void* workspace,
                            size_t workspace_bytes, void* stream);
 int pm_correction_residual(const float* out, const float* rho, int64_t n, float omega_m, float max_residual,
                            const float* stats, float* rho_add, void* stream);
 int pm_correction_residual_bwd(const float* rho_cot, const float* out, const float* rho, int64_t n, float omega_m,
                                float a, float sigma8, float max_residual, const float* stats, float* dz_out,
                                float* gain_grad, void* workspace, size_t workspace_bytes, void* stream);
-int pm_correction_input_bwd(const float* rho_cot, const float* out, const float* rho, const float* dx0, int64_t n,
-                            float omega_m, float max_residual, const float* stats, const float* gain_params,
-                            float* rho_bar, float* omega_cot, void* workspace, size_t workspace_bytes, void* stream);
+int pm_correction_input_bwd(const float* rho_cot, const float* out, const float* rho, const float* phi,
+                            const float* dx0, int64_t n, float omega_m, float max_residual, const float* stats,
+                            const float* gain_params, float* rho_bar, float* phi_bar, float* omega_cot,
+                            void* workspace, size_t workspace_bytes, void* stream);
 int pm_force_loss(const float* acc, const float* target, int64_t n, double* loss, float* acc_cot, void* workspace,
                   size_t workspace_bytes, void* stream);
 int pm_adamax_step(float* params, const float* grads, float* mu, float* nu, int32_t* count, int64_t n,

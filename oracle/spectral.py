@@ -19,6 +19,8 @@ in host memory; chunking changes only the float64 summation order.  The
 force is pinned by the closed form of a single plane wave (a = -A k0 sin(k0.x)/k0^2 of the
 source A cos(k0.x)), by zero force on a uniform lattice, by momentum conservation, and by the
 linearity in Omega_m (gravity.py:630).  ``gravity_vjp`` is pinned by central finite differences.
+``potential`` is pinned by the closed form of a plane wave (phi = -A cos(k0.x)/k0^2) and by
+the force meshes being its spectral gradient, F_i = irfftn(-i k_i phi_hat) term by term.
 """
 
 from __future__ import annotations
@@ -137,6 +139,14 @@ def force_meshes(density, omega_m, mesh_shape, cell_size, masks=()):
     nk = neg_k(mesh_shape, cell_size)
     shapes = [(-1, 1, 1), (1, -1, 1), (1, 1, -1)]
     return tuple(irfftn(1j * nk[i].reshape(shapes[i]) * pot, tuple(mesh_shape)) for i in range(3))
+
+
+def potential(density, omega_m, mesh_shape, cell_size, masks=()):
+    """Real-space potential irfftn(mask * (-1/k^2) * rfftn(1.5 Omega_m (density - 1)))
+    (gravity.py:419-430, read back in real space as corrections/core.py:286-334 does for the
+    mesh CNN's potential channel, mesh_cnn.py:76-84)."""
+    src = (np.asarray(density, dtype=np.float64) - 1.0) * (1.5 * float(omega_m))
+    return irfftn(rfftn(src) * green_multiplier(mesh_shape, cell_size, masks), tuple(mesh_shape))
 
 
 def default_value(n, mesh_shape, ptcl_num, valid=None):

@@ -14,7 +14,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 PM_SUCCESS = 0
 PM_ERROR_INVALID_ARGUMENT = 1
@@ -32,8 +32,9 @@ EXPORTED_SYMBOLS = (
     "pm_fft_workspace", "pm_rfft3d", "pm_irfft3d",
     "pm_spectral_workspace", "pm_spectral_force_multiply",
     "pm_gravity_workspace", "pm_gravity_vjp_workspace", "pm_gravity", "pm_gravity_vjp",
-    "pm_gravity_density", "pm_gravity_corrected", "pm_gravity_corrected_resume", "pm_gravity_corrected_vjp",
     "pm_gravity_keep", "pm_gravity_vjp_resume",
+    "pm_gravity_density", "pm_gravity_corrected", "pm_gravity_corrected_resume", "pm_gravity_corrected_vjp",
+    "pm_potential_workspace", "pm_gravity_potential",
     "pm_kick", "pm_drift", "pm_nbody_adj_workspace", "pm_kick_adj", "pm_drift_adj",
     "pm_power_spectrum", "pm_power_spectrum_bwd",
     "pm_conv3d_fwd", "pm_conv3d_bwd_data", "pm_conv3d_bwd_weight_workspace", "pm_conv3d_bwd_weight",
@@ -123,6 +124,9 @@ _SIGNATURES = {
                               _vp], ctypes.c_int),
     "pm_gravity_corrected_vjp": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, _vp, _vp, _vp,
                                   ctypes.c_size_t, _vp], ctypes.c_int),
+    "pm_potential_workspace": ([ctypes.POINTER(GravityDesc), _sz_p], ctypes.c_int),
+    "pm_gravity_potential": ([_vp, _f32, _f32, _vp, ctypes.POINTER(GravityDesc), _vp, _vp, ctypes.c_size_t, _vp],
+                             ctypes.c_int),
     "pm_kick": ([_vp, _vp, _vp, _i64, _f32, _vp], ctypes.c_int),
     "pm_drift": ([_vp, _vp, _vp, _i64, _f32, _vp], ctypes.c_int),
     "pm_nbody_adj_workspace": ([_sz_p], ctypes.c_int),
@@ -134,17 +138,18 @@ _SIGNATURES = {
     "pm_conv3d_bwd_data": ([_vp, _vp, _vp, _vp, _vp, _i32x3, _vp], ctypes.c_int),
     "pm_conv3d_bwd_weight_workspace": ([_i32x3, _i32, _sz_p], ctypes.c_int),
     "pm_conv3d_bwd_weight": ([_vp, _vp, _i32, _i32x3, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
-    "pm_conv3d_stem": ([_vp, _i32x3, _f32, _f32, _f32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "pm_conv3d_stem": ([_vp, _vp, _i32x3, _f32, _f32, _f32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "pm_conv3d_out_residual": ([_vp, _vp, _vp, _vp, _i32x3, _f32, _f32, _vp, _vp, _vp], ctypes.c_int),
-    "pm_conv3d_stem_hidden": ([_vp, _i32x3, _f32, _f32, _f32, _vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "pm_conv3d_stem_hidden": ([_vp, _vp, _i32x3, _f32, _f32, _f32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+                              ctypes.c_int),
     "pm_correction_workspace": ([_sz_p], ctypes.c_int),
-    "pm_correction_features": ([_vp, _i64, _f32, _f32, _f32, _f32, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
+    "pm_correction_features": ([_vp, _vp, _i64, _f32, _f32, _f32, _f32, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
                                ctypes.c_int),
     "pm_correction_residual": ([_vp, _vp, _i64, _f32, _f32, _vp, _vp, _vp], ctypes.c_int),
     "pm_correction_residual_bwd": ([_vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _vp, _vp, _vp, _vp,
                                     ctypes.c_size_t, _vp], ctypes.c_int),
-    "pm_correction_input_bwd": ([_vp, _vp, _vp, _vp, _i64, _f32, _f32, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t,
-                                 _vp], ctypes.c_int),
+    "pm_correction_input_bwd": ([_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_force_loss": ([_vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_adamax_step": ([_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _f32, _vp, _vp, ctypes.c_size_t,
                         _vp], ctypes.c_int),

@@ -10,8 +10,10 @@ deliberate ways:
 
 * hidden width is 16 channels (one MFMA tile of output channels) and activations are bf16
   with float32 accumulation (``pm_conv3d_*``);
-* the input channels are [S / sigma, a, Omega_m, sigma8] padded to 8 -- the potential channel
-  of the reference needs an extra irfftn per force and is left out;
+* the input channels are [S / sigma, a, Omega_m, sigma8] padded to 8; with ``potential=True``
+  the reference's potential channel is added, [S / sigma, phi / sigma_phi, a, Omega_m, sigma8]
+  (``mesh_cnn.py:20-27, 76-84``), phi being the force step's own filtered potential of S
+  (``gravity_potential``, one extra banded r2c/c2r pair per force);
 * the residual is applied to the source of the Poisson solve, not to the potential:
   ``S' = (1 + g) S + max_residual tanh(o) sigma`` enters ``pm_gravity_corrected`` as the
   density residual ``rho_add = (S' - S) / (1.5 Omega_m)``.  The reference's ``g phi`` term is
@@ -38,7 +40,7 @@ from .conv import conv3d, conv3d_bwd_data, conv3d_bwd_weight
 
 CHANNELS = 16
 IN_CHANNELS = 8
-LIVE_IN_CHANNELS = 4  # S / sigma, a, Omega_m, sigma8
+LIVE_IN_CHANNELS = 4  # S / sigma, a, Omega_m, sigma8 (5 with the potential channel)
 TAPS = 27
 
 
@@ -120,6 +122,7 @@ class MeshCNNPotentialCorrection:
     params_bf16: Optional[torch.Tensor] = None
     block: str = "residual"
     symmetry: str = "none"
+    potential: bool = False
 
     def __post_init__(self):
         require_device(self.params, "params")
@@ -161,15 +164,16 @@ def _truncated_lecun(gen, fan_in, n):
     return t
 
 
-def initial_params(seed=0, depth=2, output_init_scale=1e-2, block="residual") -> torch.Tensor:
+def initial_params(seed=0, depth=2, output_init_scale=1e-2, block="residual", potential=False) -> torch.Tensor:
     """The flat float32 initial parameters on the host (see ``init_mesh_cnn_potential_correction``)."""
     layers, gain, total = param_layout(depth, block)
+    stem_live = LIVE_IN_CHANNELS + (1 if potential else 0)
     gen = torch.Generator().manual_seed(int(seed))
     host = torch.zeros(total, dtype=torch.float32)
     for layer in layers:
         if layer.name == "out":
             continue
-        live = LIVE_IN_CHANNELS if layer.name == "stem" else CHANNELS
+        live = stem_live if layer.name == "stem" else CHANNELS
         w = _truncated_lecun(gen, TAPS * live, TAPS * CHANNELS * layer.cin).view(TAPS, CHANNELS, layer.cin)
         w[:, :, live:] = 0.0
         host[layer.offset:layer.offset + layer.nweights] = w.reshape(-1)
@@ -178,13 +182,13 @@ def initial_params(seed=0, depth=2, output_init_scale=1e-2, block="residual") ->
 
 
 def init_mesh_cnn_potential_correction(seed=0, depth=2, max_residual=0.1, output_init_scale=1e-2, sigma8_value=0.8,
-                                       device="cuda", block="residual", symmetry="none"):
+                                       device="cuda", block="residual", symmetry="none", potential=False):
     """A freshly initialised correction (``mesh_cnn.py:163-214``): LeCun truncated-normal
     convolutions with zero biases, a zero output layer and a small random gain layer, so the
     initial correction is exactly zero.  ``symmetry="cubic"`` projects the kernels onto the
     cube-symmetric ones (``project_cubic``), rescaled to the unprojected initial variance, and
-    training keeps them there."""
-    host = initial_params(seed, depth, output_init_scale, block)
+    training keeps them there.  ``potential=True`` adds the potential input channel."""
+    host = initial_params(seed, depth, output_init_scale, block, potential)
     if symmetry == "cubic":
         layers, _, _ = param_layout(depth, block)
         for layer in layers:
@@ -196,7 +200,7 @@ def init_mesh_cnn_potential_correction(seed=0, depth=2, max_residual=0.1, output
                 ws.mul_(std0 / float(ws.std()))
     return MeshCNNPotentialCorrection(params=host.to(device), depth=depth, max_residual=max_residual,
                                       output_init_scale=output_init_scale, sigma8_value=sigma8_value, block=block,
-                                      symmetry=symmetry)
+                                      symmetry=symmetry, potential=potential)
 
 
 @dataclasses.dataclass
@@ -211,6 +215,8 @@ class Tape:
     out: torch.Tensor
     a: float
     omega_m: float
+    phi: Optional[torch.Tensor] = None  # the potential channel's mesh (None: no such channel)
+    conf: object = None  # the configuration the potential was computed with
 
 
 def _workspace(dev):
@@ -224,15 +230,36 @@ class OutputTape:
 
     stats: torch.Tensor
     out: torch.Tensor
+    phi: Optional[torch.Tensor] = None
+    conf: object = None
 
 
-def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=False, stats=None, keep_out=False):
+def _potential(model, rho, om, conf, phi):
+    """The potential channel's mesh for ``rho``: ``phi`` when given (a slab of the whole mesh's),
+    else the filtered potential of the whole mesh ``rho`` (``gravity_potential``)."""
+    if not model.potential:
+        return None
+    if phi is not None:
+        if tuple(phi.shape) != tuple(rho.shape) or phi.dtype != torch.float32:
+            raise ValueError("phi must be a float32 mesh shaped like rho")
+        return phi.contiguous()
+    if conf is None:
+        raise ValueError("a correction with the potential channel needs conf (or phi) to evaluate")
+    from ..nbody.gravity import gravity_potential
+
+    return gravity_potential(rho, om, conf)
+
+
+def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=False, stats=None, keep_out=False,
+                      conf=None, phi=None):
     """Density residual ``rho_add`` of the correction for the density mesh ``rho`` (X, Y, Z).
 
     Returns ``(rho_add, tape)``; ``tape`` is None unless ``keep`` (then it holds the
     activations for ``residual_backward``) or ``keep_out`` (then an ``OutputTape``).  ``stats``
-    (the [sigma, gain, tanh] device floats of a forward on the whole mesh) are used instead of
-    this mesh's own: ``rho`` is then a slab of that mesh."""
+    (the [sigma, gain, tanh, sigma_phi] device floats of a forward on the whole mesh) are used
+    instead of this mesh's own: ``rho`` is then a slab of that mesh.  A model with the potential
+    channel needs ``conf`` (the force step's configuration, for the potential of ``rho``) or
+    ``phi`` (the potential mesh, or the slab of it that matches ``rho``)."""
     require_device(rho, "rho")
     if rho.dtype != torch.float32 or rho.ndim != 3:
         raise ValueError("rho must be a float32 (X, Y, Z) mesh")
@@ -242,6 +269,7 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     shape = tuple(rho.shape)
     n = rho.numel()
     a, om = float(a), float(omega_m)
+    phi = _potential(model, rho, om, conf, phi)
     ws = _workspace(dev)
     given = stats is not None
     if not given:
@@ -255,24 +283,24 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     if keep:
         # the backward's stem weight gradient needs the network input x0
         x0 = torch.empty(shape + (IN_CHANNELS,), dtype=torch.bfloat16, device=dev)
-        _lib.call("pm_correction_features", ptr(rho), n, om, a, s8, float(model.max_residual), gain, ptr(x0),
-                  ptr(stats), ptr(ws), ws.numel(), s)
+        _lib.call("pm_correction_features", ptr(rho), ptr(phi), n, om, a, s8, float(model.max_residual), gain,
+                  ptr(x0), ptr(stats), ptr(ws), ws.numel(), s)
         pre, x = conv3d(x0, model.weight(layers[0]), model.bias(layers[0]), gelu=True, want_pre=True)
     else:
         # inference: the stem reads the density directly (constant input channels folded
         # into its bias), so x0 is never written
         x0, pre = None, None
         if not given:
-            _lib.call("pm_correction_features", ptr(rho), n, om, a, s8, float(model.max_residual), gain, None,
-                      ptr(stats), ptr(ws), ws.numel(), s)
+            _lib.call("pm_correction_features", ptr(rho), ptr(phi), n, om, a, s8, float(model.max_residual), gain,
+                      None, ptr(stats), ptr(ws), ws.numel(), s)
         x = torch.empty(shape + (CHANNELS,), dtype=torch.bfloat16, device=dev)
         if fused:
             # stem and first (plain) hidden layer in one kernel: the stem's activation stays in LDS
-            _lib.call("pm_conv3d_stem_hidden", ptr(rho), _lib.int3(shape), om, a, s8, ptr(stats),
+            _lib.call("pm_conv3d_stem_hidden", ptr(rho), ptr(phi), _lib.int3(shape), om, a, s8, ptr(stats),
                       ptr(model.weight(layers[0])), ptr(model.bias(layers[0])), ptr(model.weight(layers[1])),
                       ptr(model.bias(layers[1])), ptr(x), s)
         else:
-            _lib.call("pm_conv3d_stem", ptr(rho), _lib.int3(shape), om, a, s8, ptr(stats),
+            _lib.call("pm_conv3d_stem", ptr(rho), ptr(phi), _lib.int3(shape), om, a, s8, ptr(stats),
                       ptr(model.weight(layers[0])), ptr(model.bias(layers[0])), ptr(x), s)
     pres.append(pre)
     acts.append(x)
@@ -299,8 +327,8 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     _lib.call("pm_correction_residual", ptr(out), ptr(rho), n, om, float(model.max_residual), ptr(stats),
               ptr(rho_add), s)
     if not keep:
-        return rho_add, OutputTape(stats, out)
-    return rho_add, Tape(rho, stats, x0, pres, acts, out, a, om)
+        return rho_add, OutputTape(stats, out, phi, conf)
+    return rho_add, Tape(rho, stats, x0, pres, acts, out, a, om, phi, conf)
 
 
 def _stem_weight_16(model):
@@ -365,18 +393,25 @@ def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, gr
     if input_grad:
         return dx0
     rho_bar, omega_bar = _input_bwd(model, tape.rho, rc, tape.out, tape.stats, dx0[..., :4].contiguous(),
-                                    tape.omega_m)
+                                    tape.omega_m, tape.phi, tape.conf)
     return grads, rho_bar, omega_bar
 
 
-def _input_bwd(model, rho, rho_cot, out, stats, dx0c, omega_m):
+def _input_bwd(model, rho, rho_cot, out, stats, dx0c, omega_m, phi=None, conf=None):
+    """``pm_correction_input_bwd``; with the potential channel, the potential cotangent's path to
+    the density (1.5 Omega_m times the symmetric potential operator) is added to rho_bar."""
     dev = rho.device
     ws = _workspace(dev)
     rho_bar = torch.empty_like(rho)
     omega_bar = torch.zeros((), dtype=torch.float32, device=dev)
-    _lib.call("pm_correction_input_bwd", ptr(rho_cot), ptr(out), ptr(rho), ptr(dx0c), rho.numel(), float(omega_m),
-              float(model.max_residual), ptr(stats), ptr(model.gain_params), ptr(rho_bar), ptr(omega_bar),
-              ptr(ws), ws.numel(), stream_handle(dev))
+    phi_bar = None if phi is None else torch.empty_like(rho)
+    _lib.call("pm_correction_input_bwd", ptr(rho_cot), ptr(out), ptr(rho), ptr(phi), ptr(dx0c), rho.numel(),
+              float(omega_m), float(model.max_residual), ptr(stats), ptr(model.gain_params), ptr(rho_bar),
+              ptr(phi_bar), ptr(omega_bar), ptr(ws), ws.numel(), stream_handle(dev))
+    if phi is not None:
+        from ..nbody.gravity import gravity_potential
+
+        gravity_potential(phi_bar, omega_m, conf, out=rho_bar, shift=0.0, add=rho_bar)
     return rho_bar, omega_bar
 
 
@@ -411,8 +446,9 @@ def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_
     if slab % 8 or slab < 8:
         raise ValueError("slab must be a positive multiple of 8 planes")
     dx0c = torch.empty(shape + (4,), dtype=torch.float32, device=dev)
+    phi = tape.phi
     if slab >= n0:
-        _, full = evaluate_residual(model, rho, a, om, keep=True, stats=tape.stats)
+        _, full = evaluate_residual(model, rho, a, om, keep=True, stats=tape.stats, phi=phi)
         dx0c.copy_(residual_backward(model, full, rc, weights=False, input_grad=True)[..., :4])
         del full
     else:
@@ -420,12 +456,13 @@ def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_
             s = min(slab, n0 - x0)
             idx = torch.arange(x0 - halo, x0 + s + halo, device=dev).remainder_(n0)
             rho_e = rho.index_select(0, idx)
-            _, ext = evaluate_residual(model, rho_e, a, om, keep=True, stats=tape.stats)
+            phi_e = None if phi is None else phi.index_select(0, idx)
+            _, ext = evaluate_residual(model, rho_e, a, om, keep=True, stats=tape.stats, phi=phi_e)
             dx0_e = residual_backward(model, ext, rc.index_select(0, idx), weights=False, input_grad=True)
             del ext
             dx0c[x0:x0 + s] = dx0_e[halo:halo + s, ..., :4]
-            del dx0_e, rho_e
-    return _input_bwd(model, rho, rc, tape.out, tape.stats, dx0c, om)
+            del dx0_e, rho_e, phi_e
+    return _input_bwd(model, rho, rc, tape.out, tape.stats, dx0c, om, phi, tape.conf)
 
 
 TRAINED_DIR = __import__("pathlib").Path(__file__).resolve().parent / "weights"
@@ -441,7 +478,7 @@ def save_correction(model: MeshCNNPotentialCorrection, path, **meta) -> None:
               "depth": np.int64(model.depth), "max_residual": np.float64(model.max_residual),
               "output_init_scale": np.float64(model.output_init_scale),
               "sigma8_value": np.float64(model.sigma8_value), "block": np.str_(model.block),
-              "symmetry": np.str_(model.symmetry)}
+              "symmetry": np.str_(model.symmetry), "potential": np.bool_(model.potential)}
     for k, v in meta.items():
         fields[f"meta_{k}"] = np.asarray(v)
     with open(path, "wb") as f:
@@ -460,7 +497,8 @@ def load_correction(path=None, device="cuda") -> MeshCNNPotentialCorrection:
         return MeshCNNPotentialCorrection(params=params, depth=int(z["depth"]), max_residual=float(z["max_residual"]),
                                           output_init_scale=float(z["output_init_scale"]),
                                           sigma8_value=float(z["sigma8_value"]), block=str(z["block"]),
-                                          symmetry=str(z["symmetry"]) if "symmetry" in z.files else "none")
+                                          symmetry=str(z["symmetry"]) if "symmetry" in z.files else "none",
+                                          potential=bool(z["potential"]) if "potential" in z.files else False)
 
 
 def correction_meta(path=None) -> dict:

@@ -161,7 +161,7 @@ def corrected_gravity(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrecti
     s = stream_handle(dev)
     rho = torch.empty(tuple(int(v) for v in conf.mesh_shape), dtype=torch.float32, device=dev)
     _lib.call("pm_gravity_density", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(rho), ptr(ws), nbytes, s)
-    rho_add, _ = evaluate_residual(correction, rho, a, _omega(cosmo))
+    rho_add, _ = evaluate_residual(correction, rho, a, _omega(cosmo), conf=conf)
     del rho
     acc = torch.empty((n, 3), dtype=torch.float32, device=dev)
     _lib.call("pm_gravity_corrected_resume", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(rho_add), ptr(acc), ptr(ws),
@@ -181,14 +181,14 @@ def corrected_gravity_vjp(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorr
     om = _omega(cosmo)
     rho = density(ptcl, cosmo, conf)
     if param_grads is not None:
-        rho_add, full = evaluate_residual(correction, rho, a, om, keep=True)
+        rho_add, full = evaluate_residual(correction, rho, a, om, keep=True, conf=conf)
         disp_cot, omega_cot, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
         del rho_add
         g, rho_bar, omega_bar = residual_backward(correction, full, rho_cot, inputs=True)
         del full
         param_grads += g
     else:
-        rho_add, tape = evaluate_residual(correction, rho, a, om, keep_out=True)
+        rho_add, tape = evaluate_residual(correction, rho, a, om, keep_out=True, conf=conf)
         disp_cot, omega_cot, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
         del rho_add
         rho_bar, omega_bar = network_input_vjp(correction, rho, rho_cot, a, om, tape, slab=slab)
@@ -219,7 +219,7 @@ def train_step(model, optimizer, state, ptcl, a, cosmo, conf, target, grads=None
     A model with ``symmetry="cubic"`` has its gradient projected onto the cube-symmetric
     kernels (``project_cubic``) before the step, so its weights stay in that subspace."""
     rho = density(ptcl, cosmo, conf)
-    rho_add, tape = evaluate_residual(model, rho, a, _omega(cosmo), keep=True)
+    rho_add, tape = evaluate_residual(model, rho, a, _omega(cosmo), keep=True, conf=conf)
     acc = gravity_with_residual(ptcl, cosmo, conf, rho_add)
     loss, acc_cot = force_loss(acc, target)
     _, _, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
@@ -303,7 +303,8 @@ def mean_force_loss(model, pairs, cosmo, conf):
 
 
 def train_correction(pairs, cosmo, conf, *, iters, learning_rate, max_residual, depth=2, block="plain", seed=0,
-                     gradient_clip_norm=100.0, symmetry="none", device="cuda", progress=None, progress_every=500):
+                     gradient_clip_norm=100.0, symmetry="none", device="cuda", progress=None, progress_every=500,
+                     potential=False):
     """Trains a freshly initialised correction on ``pairs`` ([(a, coarse particles, target acc)],
     ``fine_run_pairs``): ``iters`` Adamax steps (``build_correction_optimizer``), one pair per
     step drawn by numpy's PCG64(``seed``).  ``progress(it, mean loss of the last steps)`` is
@@ -314,7 +315,7 @@ def train_correction(pairs, cosmo, conf, *, iters, learning_rate, max_residual, 
     from .mesh_cnn import init_mesh_cnn_potential_correction
 
     model = init_mesh_cnn_potential_correction(seed=seed, depth=depth, max_residual=max_residual, device=device,
-                                               block=block, symmetry=symmetry)
+                                               block=block, symmetry=symmetry, potential=potential)
     opt = build_correction_optimizer(learning_rate, gradient_clip_norm)
     state = opt.init(model.params)
     rng = np.random.default_rng(seed)

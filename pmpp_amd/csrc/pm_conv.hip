@@ -424,46 +424,63 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
 
 // Inference stem straight from the density mesh: act = gelu(conv(x0; w) + bias) where x0 is
 // the 8-channel input [bf16((rho - 1) c / sigma), bf16(a), bf16(Omega_m), bf16(sigma8), 0 ...]
-// that pm_correction_features writes.  Channels 1..3 are the same at every voxel, so their
-// periodic convolution is sum_tap w[tap][cout][1..3] * value, a constant per output channel
-// that is folded into the bias; channels 4..7 are zero.  What remains is a one-channel
-// convolution whose 27 taps fit one K = 32 MFMA step (5 padded taps with zero weights): lane
-// (g, m) feeds taps 8g .. 8g+7 of voxel m, read as eight 2-byte LDS values.  The staged tile is
-// the bf16 source of the block's 10^3 halo, computed from float32 density on load with the
-// same arithmetic as the feature kernel.  Same persistent, prefetching block walk as
-// k_conv_fwd.
-__global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restrict__ rho, float c,
+// that pm_correction_features writes (PHI: [source, bf16(phi / sigma_p), a, Omega_m, sigma8,
+// 0 ...] with the potential channel).  The a, Omega_m and sigma8 channels are the same at
+// every voxel, so their periodic convolution is sum_tap w[tap][cout][ch] * value, a constant
+// per output channel that is folded into the bias; the last channels are zero.  What remains
+// is a one-channel convolution (two with PHI) whose 27 taps fit one K = 32 MFMA step per
+// channel (5 padded taps with zero weights): lane (g, m) feeds taps 8g .. 8g+7 of voxel m, read
+// as eight 2-byte LDS values.  The staged tiles are the bf16 inputs of the block's 10^3 halo,
+// computed from the float32 meshes on load with the same arithmetic as the feature kernel.
+// Same persistent, prefetching block walk as k_conv_fwd.
+constexpr int kStemCin = 8;  // channels of the stem's weight tensor
+
+// the stem's bias with the constant input channels folded in (fixed order), for output channel co
+template <bool PHI>
+__device__ __forceinline__ float stem_bias(const __bf16* __restrict__ w, const float* __restrict__ bias, int co,
+                                           float a, float om, float s8) {
+  constexpr int C0 = PHI ? 2 : 1;  // first constant channel
+  const __bf16 ba = (__bf16)a, bo = (__bf16)om, bs = (__bf16)s8;
+  float t = 0.f;
+  for (int tap = 0; tap < kTaps; ++tap) {
+    const __bf16* wt = w + (tap * kCout + co) * kStemCin;
+    t += (float)wt[C0] * (float)ba + (float)wt[C0 + 1] * (float)bo + (float)wt[C0 + 2] * (float)bs;
+  }
+  return bias[co] + t;
+}
+
+// A fragment of the stem's one-channel MFMA: channel ch's weights of taps 8g .. 8g+7 for output
+// channel m (the padded taps 27 .. 31 are zero)
+__device__ __forceinline__ bf16x8 stem_fragment(const __bf16* __restrict__ w, int ch, int g, int m) {
+  bf16x8 f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int tap = 8 * g + j;
+    f[j] = tap < kTaps ? w[(tap * kCout + m) * kStemCin + ch] : (__bf16)0.f;
+  }
+  return f;
+}
+
+template <bool PHI>
+__global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restrict__ rho,
+                                                           const float* __restrict__ phi, float c,
                                                            const float* __restrict__ stats, float a, float om,
                                                            float s8, const __bf16* __restrict__ w,
                                                            const float* __restrict__ bias, __bf16* __restrict__ act,
                                                            int n0, int n1, int n2) {
   constexpr int NS = kH * kH * kH;
   constexpr int PER = (NS + kThreads - 1) / kThreads;
-  constexpr int CIN = 8;  // channels of the stem's weight tensor
-  __shared__ __align__(16) __bf16 xin[PER * kThreads];
+  constexpr int NT = PHI ? 2 : 1;  // staged input channels
+  __shared__ __align__(16) __bf16 xin[NT][PER * kThreads];
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int m = lane % 16, g = lane / 16;
-  // A fragment: channel-0 weights of taps 8g .. 8g+7 for output channel m
-  bf16x8 wa;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int tap = 8 * g + j;
-    wa[j] = tap < kTaps ? w[(tap * kCout + m) * CIN] : (__bf16)0.f;
-  }
-  // bias of output channels 4g .. 4g+3 with the constant channels folded in (fixed order)
-  const __bf16 ba = (__bf16)a, bo = (__bf16)om, bs = (__bf16)s8;
+  const bf16x8 wa = stem_fragment(w, 0, g, m);
+  const bf16x8 wp = PHI ? stem_fragment(w, 1, g, m) : bf16x8{};
   float bsv[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int co = 4 * g + i;
-    float t = 0.f;
-    for (int tap = 0; tap < kTaps; ++tap) {
-      const __bf16* wt = w + (tap * kCout + co) * CIN;
-      t += (float)wt[1] * (float)ba + (float)wt[2] * (float)bo + (float)wt[3] * (float)bs;
-    }
-    bsv[i] = bias[co] + t;
-  }
+  for (int i = 0; i < 4; ++i) bsv[i] = stem_bias<PHI>(w, bias, 4 * g + i, a, om, s8);
   const float inv = 1.f / stats[0];
+  const float pinv = PHI ? 1.f / stats[3] : 0.f;
   __builtin_amdgcn_s_waitcnt(0);
   // staged offsets of this lane's 8 taps (padded taps read tap 26's voxel; zero weights)
   typedef int TapOffsets __attribute__((ext_vector_type(8)));
@@ -491,22 +508,28 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
   typedef float Prefetch __attribute__((ext_vector_type(PER)));
   int cur = -1, nxt = first;
   for (;;) {
-    Prefetch pf;
+    Prefetch pf, pp;
     if (nxt < nblk) {
       const int bz = nxt % nbz, by = (nxt / nbz) % nby, bx = nxt / (nbz * nby);
       const int ox = bx * kB, oy = by * kB, oz = bz * kB;
       const bool inner = ox > 0 && ox + kB < n0 && oy > 0 && oy + kB < n1 && oz > 0 && oz + kB < n2;
       if (inner) {
-        const float* base = rho + ((int64_t)(ox - 1) * n1 + (oy - 1)) * n2 + (oz - 1);
+        const int64_t o = ((int64_t)(ox - 1) * n1 + (oy - 1)) * n2 + (oz - 1);
 #pragma unroll
-        for (int u = 0; u < PER; ++u) pf[u] = base[loc[u]];
+        for (int u = 0; u < PER; ++u) pf[u] = rho[o + loc[u]];
+        if constexpr (PHI) {
+#pragma unroll
+          for (int u = 0; u < PER; ++u) pp[u] = phi[o + loc[u]];
+        }
       } else {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
           const int vox = min((int)threadIdx.x + u * kThreads, NS - 1);
           const int z = vox % kH, y = (vox / kH) % kH, xx = vox / (kH * kH);
           const int gx = wrap(ox + xx - 1, n0), gy = wrap(oy + y - 1, n1), gz = wrap(oz + z - 1, n2);
-          pf[u] = rho[((int64_t)gx * n1 + gy) * n2 + gz];
+          const int64_t o = ((int64_t)gx * n1 + gy) * n2 + gz;
+          pf[u] = rho[o];
+          if constexpr (PHI) pp[u] = phi[o];
         }
       }
     }
@@ -518,8 +541,14 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
       for (int q = 0; q < kB; ++q) {  // output x plane q: staged planes q .. q+2
         bf16x8 b;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) b[j] = xin[toff[j] + q * kH * kH];
-        const f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        for (int j = 0; j < 8; ++j) b[j] = xin[0][toff[j] + q * kH * kH];
+        f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        if constexpr (PHI) {
+          bf16x8 bp;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) bp[j] = xin[1][toff[j] + q * kH * kH];
+          cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wp, bp, cv, 0, 0, 0);
+        }
         const int64_t e = el + q * plane;
         bf16x4 o;
 #pragma unroll
@@ -532,7 +561,10 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int i = threadIdx.x + u * kThreads;
-      if (i < NS) xin[i] = (__bf16)(((pf[u] - 1.f) * c) * inv);
+      if (i < NS) {
+        xin[0][i] = (__bf16)(((pf[u] - 1.f) * c) * inv);
+        if constexpr (PHI) xin[1][i] = (__bf16)(pp[u] * pinv);
+      }
     }
     __syncthreads();
     cur = nxt;
@@ -547,7 +579,9 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
 // (two-voxel halo), runs the stem's one-step MFMA over the 1080 staged voxels (68 tiles of 16
 // voxels; lane (g, m) feeds taps 8g .. 8g+7 of voxel m with the same arithmetic as
 // k_conv_stem, so the activations are k_conv_stem's bit for bit), writes their GELU as bf16
-// into the tile and walks the hidden conv over it exactly as k_conv_fwd does.  The stem
+// into the tile and walks the hidden conv over it exactly as k_conv_fwd does (PHI: the
+// potential's tile is staged beside the density's and the stem adds its second MFMA step, as
+// k_conv_stem<true>).  The stem
 // activation (32 bytes per voxel written, then read back 2.8x over by the hidden layer's halo
 // tiles) never reaches HBM; the price is the stem's MFMA on 2.1x the block's voxels, one MFMA
 // per 16 of them.  Persistent and prefetching (the next block's density tile in registers
@@ -556,7 +590,9 @@ constexpr int kFRX = 8, kFRY = 12, kFRZ = 20;                // density tile: bl
 constexpr int kFRN = kFRX * kFRY * kFRZ;                     // 1920
 constexpr int kFRPer = (kFRN + kThreads - 1) / kThreads;     // 8
 
-__global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* __restrict__ rho, float c,
+template <bool PHI>
+__global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* __restrict__ rho,
+                                                                  const float* __restrict__ phi, float c,
                                                                   const float* __restrict__ stats, float a, float om,
                                                                   float s8, const __bf16* __restrict__ ws,
                                                                   const float* __restrict__ bs,
@@ -570,11 +606,11 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
   constexpr int NS = Geo::NS, SZ = Geo::SZ, SY = Geo::SY;
   constexpr int BX = Geo::BX, BY = Geo::BY, BZ = Geo::BZ;
   constexpr int STEM_TILES = (NS + 15) / 16;  // 68
-  constexpr int CS = 8;                       // channels of the stem's weight tensor
+  constexpr int NT = PHI ? 2 : 1;             // staged input channels of the stem
   constexpr Taps taps{};
   static_assert(taps.ncls <= 4, "step classes");
   __shared__ __align__(16) __bf16 xin[2 * NS * 8];
-  __shared__ __align__(16) __bf16 rt[kFRN];
+  __shared__ __align__(16) __bf16 rt[NT][kFRN];
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int m = lane % 16, g = lane / 16;
   const int gt = g / 2, gh = g % 2;  // hidden K group: tap slot, channel half
@@ -592,27 +628,15 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) wk.bsv[i] = bh[4 * g + i];
-  // stem: A fragment (channel-0 weights of taps 8g .. 8g+7 for output channel m) and the bias of
-  // output channels 4g .. 4g+3 with the constant input channels folded in, as k_conv_stem
-  bf16x8 wsa;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int tap = 8 * g + j;
-    wsa[j] = tap < kTaps ? ws[(tap * kCout + m) * CS] : (__bf16)0.f;
-  }
-  const __bf16 ba = (__bf16)a, bo = (__bf16)om, bss = (__bf16)s8;
+  // stem: A fragments and the bias of output channels 4g .. 4g+3 with the constant input
+  // channels folded in, as k_conv_stem
+  const bf16x8 wsa = stem_fragment(ws, 0, g, m);
+  const bf16x8 wsp = PHI ? stem_fragment(ws, 1, g, m) : bf16x8{};
   float sbv[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int co = 4 * g + i;
-    float t = 0.f;
-    for (int tap = 0; tap < kTaps; ++tap) {
-      const __bf16* wt = ws + (tap * kCout + co) * CS;
-      t += (float)wt[1] * (float)ba + (float)wt[2] * (float)bo + (float)wt[3] * (float)bss;
-    }
-    sbv[i] = bs[co] + t;
-  }
+  for (int i = 0; i < 4; ++i) sbv[i] = stem_bias<PHI>(ws, bs, 4 * g + i, a, om, s8);
   const float inv = 1.f / stats[0];
+  const float pinv = PHI ? 1.f / stats[3] : 0.f;
   __builtin_amdgcn_s_waitcnt(0);
   // density-tile offsets of this lane's 8 stem taps relative to a staged voxel's own cell
   // (padded taps read tap 26's cell; their weights are zero)
@@ -659,23 +683,29 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
   typedef float Prefetch __attribute__((ext_vector_type(kFRPer)));
   int cur = -1, nxt = first;
   for (;;) {
-    Prefetch pf;
+    Prefetch pf, pp;
     if (nxt < nblk) {
       int bx, by, bz;
       block_of(nxt, n0 / BX, nby, nbz, bx, by, bz);
       const int ox = bx * BX, oy = by * BY, oz = bz * BZ;
       const bool inner = ox >= 2 && ox + BX + 2 <= n0 && oy >= 2 && oy + BY + 2 <= n1 && oz >= 2 && oz + BZ + 2 <= n2;
       if (inner) {
-        const float* base = rho + ((int64_t)(ox - 2) * n1 + (oy - 2)) * n2 + (oz - 2);
+        const int64_t o = ((int64_t)(ox - 2) * n1 + (oy - 2)) * n2 + (oz - 2);
 #pragma unroll
-        for (int u = 0; u < kFRPer; ++u) pf[u] = base[loc[u]];
+        for (int u = 0; u < kFRPer; ++u) pf[u] = rho[o + loc[u]];
+        if constexpr (PHI) {
+#pragma unroll
+          for (int u = 0; u < kFRPer; ++u) pp[u] = phi[o + loc[u]];
+        }
       } else {
 #pragma unroll
         for (int u = 0; u < kFRPer; ++u) {
           const int it = min((int)threadIdx.x + u * kThreads, kFRN - 1);
           const int z = it % kFRZ, y = (it / kFRZ) % kFRY, xx = it / (kFRZ * kFRY);
           const int gx = wrap(ox + xx - 2, n0), gy = wrap(oy + y - 2, n1), gz = wrap(oz + z - 2, n2);
-          pf[u] = rho[((int64_t)gx * n1 + gy) * n2 + gz];
+          const int64_t o = ((int64_t)gx * n1 + gy) * n2 + gz;
+          pf[u] = rho[o];
+          if constexpr (PHI) pp[u] = phi[o];
         }
       }
     }
@@ -692,7 +722,10 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
 #pragma unroll
     for (int u = 0; u < kFRPer; ++u) {
       const int i = threadIdx.x + u * kThreads;
-      if (i < kFRN) rt[i] = (__bf16)(((pf[u] - 1.f) * c) * inv);
+      if (i < kFRN) {
+        rt[0][i] = (__bf16)(((pf[u] - 1.f) * c) * inv);
+        if constexpr (PHI) rt[1][i] = (__bf16)(pp[u] * pinv);
+      }
     }
     __syncthreads();
     // the stem on the staged voxels of block nxt, into the hidden layer's tile
@@ -703,8 +736,14 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
       const int own = ((sx + 1) * kFRY + (sy + 1)) * kFRZ + (sz + 1);
       bf16x8 b;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) b[j] = rt[own + soff[j]];
-      const f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsa, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      for (int j = 0; j < 8; ++j) b[j] = rt[0][own + soff[j]];
+      f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsa, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      if constexpr (PHI) {
+        bf16x8 bp;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bp[j] = rt[1][own + soff[j]];
+        cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsp, bp, cv, 0, 0, 0);
+      }
       bf16x4 o;
 #pragma unroll
       for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i] + sbv[i]);
@@ -1179,7 +1218,7 @@ int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, 
   return PM_SUCCESS;
 }
 
-int pm_conv3d_stem(const float* rho, const int32_t shape[3], float omega_m, float a, float sigma8,
+int pm_conv3d_stem(const float* rho, const float* phi, const int32_t shape[3], float omega_m, float a, float sigma8,
                    const float* stats, const void* w, const float* bias, void* act, void* stream) {
   PM_TRY(check_shape(shape));
   PM_RETURN_IF(rho == nullptr || stats == nullptr || w == nullptr || bias == nullptr, PM_ERROR_INVALID_ARGUMENT,
@@ -1190,15 +1229,17 @@ int pm_conv3d_stem(const float* rho, const int32_t shape[3], float omega_m, floa
                PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
   hipStream_t s = as_stream(stream);
   unsigned grid = 0;
-  PM_TRY(persistent_grid(reinterpret_cast<const void*>(conv::k_conv_stem), nblocks(shape), &grid));
-  hipLaunchKernelGGL(conv::k_conv_stem, dim3(grid), dim3(conv::kThreads), 0, s, rho, 1.5f * omega_m, stats, a,
-                     omega_m, sigma8, static_cast<const __bf16*>(w), bias, static_cast<__bf16*>(act), shape[0],
-                     shape[1], shape[2]);
+  auto kern = phi ? conv::k_conv_stem<true> : conv::k_conv_stem<false>;
+  PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, rho, phi, 1.5f * omega_m, stats, a, omega_m,
+                     sigma8, static_cast<const __bf16*>(w), bias, static_cast<__bf16*>(act), shape[0], shape[1],
+                     shape[2]);
   PM_LAUNCHED("k_conv_stem", s);
   return PM_SUCCESS;
 }
 
-int pm_conv3d_stem_hidden(const float* rho, const int32_t shape[3], float omega_m, float a, float sigma8,
+int pm_conv3d_stem_hidden(const float* rho, const float* phi, const int32_t shape[3], float omega_m, float a,
+                          float sigma8,
                           const float* stats, const void* w_stem, const float* b_stem, const void* w_hidden,
                           const float* b_hidden, void* act, void* stream) {
   PM_TRY(check_shape(shape));
@@ -1213,9 +1254,10 @@ int pm_conv3d_stem_hidden(const float* rho, const int32_t shape[3], float omega_
                PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
   hipStream_t s = as_stream(stream);
   unsigned grid = 0;
-  PM_TRY(persistent_grid(reinterpret_cast<const void*>(conv::k_conv_stem_hidden), nblocks(shape), &grid));
-  hipLaunchKernelGGL(conv::k_conv_stem_hidden, dim3(grid), dim3(conv::kThreads), 0, s, rho, 1.5f * omega_m, stats, a,
-                     omega_m, sigma8, static_cast<const __bf16*>(w_stem), b_stem, static_cast<const __bf16*>(w_hidden),
+  auto kern = phi ? conv::k_conv_stem_hidden<true> : conv::k_conv_stem_hidden<false>;
+  PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, rho, phi, 1.5f * omega_m, stats, a, omega_m,
+                     sigma8, static_cast<const __bf16*>(w_stem), b_stem, static_cast<const __bf16*>(w_hidden),
                      b_hidden, static_cast<__bf16*>(act), shape[0], shape[1], shape[2]);
   PM_LAUNCHED("k_conv_stem_hidden", s);
   return PM_SUCCESS;

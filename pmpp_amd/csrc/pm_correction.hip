@@ -5,6 +5,9 @@
 //   S       = c (rho - 1),  c = 1.5 Omega_m                       (the source of the solve)
 //   sigma   = sqrt(mean(S^2) + 1e-6)                              (mesh_cnn.py:79-80)
 //   x0      = bf16 [S / sigma, a, Omega_m, sigma8, 0, 0, 0, 0]     (mesh_cnn.py:21-27)
+// or, with the potential channel (phi the filtered potential of S, pm_gravity_potential):
+//   sigma_p = sqrt(mean(phi^2) + 1e-6)                            (mesh_cnn.py:81-82)
+//   x0      = bf16 [S / sigma, phi / sigma_p, a, Omega_m, sigma8, 0, 0, 0]
 //   o       = conv stack(x0), channel 0 of the output layer (stored alone, n floats)
 //   g       = r tanh(w . [a, Omega_m, sigma8] + b)                (the direct gain, :98-103)
 //   rho_add = g (rho - 1) + r tanh(o) sigma / c                   (:97, :104 moved to the source)
@@ -50,24 +53,25 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ p, int
 }
 
 // ----------------------------------------------------------- features ----
-// float4 loads with four independent float64 chains per thread (rho is 16-byte aligned); the
-// partition of the mesh over threads is fixed by the grid, so the sum is deterministic
-__global__ void __launch_bounds__(kBlock) k_source_sumsq(const float* __restrict__ rho, int64_t n, float c,
-                                                         double* __restrict__ partials) {
+// sum ((rho + shift) c)^2: float4 loads with four independent float64 chains per thread (rho
+// is 16-byte aligned); the partition of the mesh over threads is fixed by the grid, so the sum
+// is deterministic
+__global__ void __launch_bounds__(kBlock) k_source_sumsq(const float* __restrict__ rho, int64_t n, float shift,
+                                                         float c, double* __restrict__ partials) {
   __shared__ double lds[kBlock / 64];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   const int64_t n4 = n / 4;
   const float4* r4 = reinterpret_cast<const float4*>(rho);
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += (int64_t)gridDim.x * kBlock) {
     const float4 r = r4[i];
-    const float s0 = (r.x - 1.f) * c, s1 = (r.y - 1.f) * c, s2 = (r.z - 1.f) * c, s3 = (r.w - 1.f) * c;
+    const float s0 = (r.x + shift) * c, s1 = (r.y + shift) * c, s2 = (r.z + shift) * c, s3 = (r.w + shift) * c;
     a0 += (double)(s0 * s0);
     a1 += (double)(s1 * s1);
     a2 += (double)(s2 * s2);
     a3 += (double)(s3 * s3);
   }
   for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const float s = (rho[i] - 1.f) * c;
+    const float s = (rho[i] + shift) * c;
     a0 += (double)(s * s);
   }
   const double t = block_sum((a0 + a1) + (a2 + a3), lds);
@@ -87,11 +91,26 @@ __global__ void k_source_scale(const double* __restrict__ partials, int nb, int6
   stats[2] = th;
 }
 
-__global__ void __launch_bounds__(kBlock) k_features(const float* __restrict__ rho, int64_t n, float c,
-                                                     const float* __restrict__ stats, float a, float om, float s8,
-                                                     uint4* __restrict__ x0) {
+// stats[3] = sigma_p = sqrt(mean phi^2 + 1e-6)
+__global__ void k_potential_scale(const double* __restrict__ partials, int nb, int64_t n, float* __restrict__ stats) {
+  if (threadIdx.x != 0) return;
+  stats[3] = sqrtf((float)(sum_partials(partials, nb) / (double)n) + 1e-6f);
+}
+
+__global__ void __launch_bounds__(kBlock) k_features(const float* __restrict__ rho, const float* __restrict__ phi,
+                                                     int64_t n, float c, const float* __restrict__ stats, float a,
+                                                     float om, float s8, uint4* __restrict__ x0) {
   const float inv = 1.f / stats[0];
   const __bf16 ba = (__bf16)a, bo = (__bf16)om, bs = (__bf16)s8, z = (__bf16)0.f;
+  if (phi) {  // uniform
+    const float pinv = 1.f / stats[3];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+      const float s = (rho[i] - 1.f) * c;
+      __bf16 v[8] = {(__bf16)(s * inv), (__bf16)(phi[i] * pinv), ba, bo, bs, z, z, z};
+      x0[i] = *reinterpret_cast<const uint4*>(v);
+    }
+    return;
+  }
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const float s = (rho[i] - 1.f) * c;
     __bf16 v[8] = {(__bf16)(s * inv), ba, bo, bs, z, z, z, z};
@@ -148,45 +167,57 @@ __global__ void k_gain_bwd(const double* __restrict__ partials, int nb, float r,
 
 // ------------------------------------------------------ input-path reverse ----
 // The density and Omega_m cotangents of rho_add through the network's inputs.  With
-// T = tanh(o), S = c (rho - 1), x0_0 = S / sigma, x0_2 = Omega_m and dx0 the gradient of the
-// network input (channels 0 and 2 used):
+// T = tanh(o), S = c (rho - 1), x0_0 = S / sigma, x0_w = Omega_m (w = 2, or 3 with the
+// potential channel) and dx0 the gradient of the network input (channels 0..3):
 //   sigma_bar = (r / c) sum rho_cot T - sum dx0_0 S / sigma^2
 //   S_bar     = dx0_0 / sigma + sigma_bar S / (N sigma)
 //   rho_bar   = g rho_cot + c S_bar
-//   Om_bar    = 1.5 sum S_bar (rho - 1) + sum dx0_2 - 1.5 (r sigma / c^2) sum rho_cot T
+//   Om_bar    = 1.5 sum S_bar (rho - 1) + sum dx0_w - 1.5 (r sigma / c^2) sum rho_cot T
 //               + r (1 - tanh_g^2) w_1 sum rho_cot (rho - 1)
-// Pass 1 reduces the four sums, pass 2 writes rho_bar and reduces sum S_bar (rho - 1).
+// With the potential channel x0_1 = phi / sigma_p (phi linear in S, so d phi / d Omega_m =
+// phi / Omega_m) also
+//   phi_bar   = dx0_1 / sigma_p - phi sum dx0_1 phi / (N sigma_p^3)
+//   Om_bar   += sum phi_bar phi / Omega_m
+// and phi_bar's path to the density (c times the potential operator, which is symmetric)
+// is added by the caller.  Pass 1 reduces the sums, pass 2 writes rho_bar (and phi_bar) and
+// reduces sum S_bar (rho - 1) (and sum phi_bar phi).
 __global__ void __launch_bounds__(kBlock) k_input_sums(const float* __restrict__ rho_cot,
                                                        const float* __restrict__ out,
                                                        const float* __restrict__ rho,
+                                                       const float* __restrict__ phi,
                                                        const float4* __restrict__ dx0, int64_t n, float c,
                                                        double* __restrict__ partials) {
   __shared__ double lds[kBlock / 64];
-  double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const float rc = rho_cot[i];
     const float d = rho[i] - 1.f;
     const float4 g = dx0[i];  // channels 0..3 of the input gradient
     s1 += (double)(rc * tanhf(out[i]));
     s2 += (double)(g.x * (d * c));
-    s3 += (double)g.z;
+    s3 += (double)(phi ? g.w : g.z);
     s4 += (double)(rc * d);
+    if (phi) s5 += (double)(g.y * phi[i]);
   }
   const double t1 = block_sum(s1, lds);
   const double t2 = block_sum(s2, lds);
   const double t3 = block_sum(s3, lds);
   const double t4 = block_sum(s4, lds);
+  const double t5 = block_sum(s5, lds);
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = t1;
     partials[kMaxBlocks + blockIdx.x] = t2;
     partials[2 * kMaxBlocks + blockIdx.x] = t3;
     partials[3 * kMaxBlocks + blockIdx.x] = t4;
+    partials[4 * kMaxBlocks + blockIdx.x] = t5;
   }
 }
 
-// scal[0] = sigma_bar, scal[1] = the Omega_m terms known after pass 1
-__global__ void k_input_scalars(const double* __restrict__ partials, int nb, float c, float r,
-                                const float* __restrict__ stats, const float* __restrict__ lin, float* __restrict__ scal) {
+// scal[0] = sigma_bar, scal[1] = the Omega_m terms known after pass 1, scal[2] = phi_bar's
+// coefficient of phi (potential channel)
+__global__ void k_input_scalars(const double* __restrict__ partials, int nb, int64_t n, float c, float r,
+                                const float* __restrict__ stats, const float* __restrict__ lin, int potential,
+                                float* __restrict__ scal) {
   if (threadIdx.x != 0) return;
   const double p1 = sum_partials(partials, nb);
   const double p2 = sum_partials(partials + kMaxBlocks, nb);
@@ -196,34 +227,54 @@ __global__ void k_input_scalars(const double* __restrict__ partials, int nb, flo
   const double th = stats[2];
   scal[0] = (float)((double)r / c * p1 - p2 / (sigma * sigma));
   scal[1] = (float)(p3 - 1.5 * (double)r * sigma / ((double)c * c) * p1 + (double)r * (1.0 - th * th) * lin[1] * p4);
+  if (potential) {
+    const double sp = stats[3];
+    scal[2] = (float)(sum_partials(partials + 4 * kMaxBlocks, nb) / ((double)n * sp * sp * sp));
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_input_bwd(const float* __restrict__ rho_cot,
                                                       const float* __restrict__ rho,
+                                                      const float* __restrict__ phi,
                                                       const float4* __restrict__ dx0, int64_t n, float c,
                                                       const float* __restrict__ stats,
                                                       const float* __restrict__ scal, float* __restrict__ rho_bar,
-                                                      double* __restrict__ partials) {
+                                                      float* __restrict__ phi_bar, double* __restrict__ partials) {
   __shared__ double lds[kBlock / 64];
   const float sigma = stats[0], g = stats[1];
   const float sb = scal[0] / ((float)n * sigma);
   const float inv = 1.f / sigma;
-  double acc = 0.0;
+  const float pinv = phi ? 1.f / stats[3] : 0.f;
+  const float pc = phi ? scal[2] : 0.f;
+  double acc = 0.0, accp = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const float d = rho[i] - 1.f;
     const float S = d * c;
-    const float Sbar = dx0[i].x * inv + sb * S;
+    const float4 gx = dx0[i];
+    const float Sbar = gx.x * inv + sb * S;
     rho_bar[i] = g * rho_cot[i] + c * Sbar;
     acc += (double)(Sbar * d);
+    if (phi) {  // uniform
+      const float p = phi[i];
+      const float pb = gx.y * pinv - pc * p;
+      phi_bar[i] = pb;
+      accp += (double)(pb * p);
+    }
   }
   const double t = block_sum(acc, lds);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  const double tp = block_sum(accp, lds);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = t;
+    partials[kMaxBlocks + blockIdx.x] = tp;
+  }
 }
 
-__global__ void k_input_omega(const double* __restrict__ partials, int nb, const float* __restrict__ scal,
-                              float* __restrict__ omega_cot) {
+__global__ void k_input_omega(const double* __restrict__ partials, int nb, const float* __restrict__ scal, float om,
+                              int potential, float* __restrict__ omega_cot) {
   if (threadIdx.x != 0) return;
-  omega_cot[0] += (float)(1.5 * sum_partials(partials, nb) + (double)scal[1]);
+  double t = 1.5 * sum_partials(partials, nb) + (double)scal[1];
+  if (potential) t += sum_partials(partials + kMaxBlocks, nb) / (double)om;
+  omega_cot[0] += (float)t;
 }
 
 // ---------------------------------------------------------------- loss ----
@@ -324,7 +375,7 @@ __global__ void __launch_bounds__(kBlock) k_to_bf16(const float* __restrict__ sr
 using namespace pm;
 
 namespace {
-constexpr size_t kPartialBytes = sizeof(double) * 4 * corr::kMaxBlocks + 256;
+constexpr size_t kPartialBytes = sizeof(double) * 5 * corr::kMaxBlocks + 256;
 
 int check_mesh(int64_t n) {
   PM_RETURN_IF(n < 1, PM_ERROR_INVALID_ARGUMENT, "mesh must have at least one cell");
@@ -340,8 +391,8 @@ int pm_correction_workspace(size_t* bytes) {
   return PM_SUCCESS;
 }
 
-int pm_correction_features(const float* rho, int64_t n, float omega_m, float a, float sigma8, float max_residual,
-                           const float* gain_params, void* x0, float* stats, void* workspace,
+int pm_correction_features(const float* rho, const float* phi, int64_t n, float omega_m, float a, float sigma8,
+                           float max_residual, const float* gain_params, void* x0, float* stats, void* workspace,
                            size_t workspace_bytes, void* stream) {
   PM_TRY(check_mesh(n));
   PM_RETURN_IF(rho == nullptr || stats == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL argument");
@@ -351,24 +402,31 @@ int pm_correction_features(const float* rho, int64_t n, float omega_m, float a, 
   hipStream_t s = as_stream(stream);
   const float c = 1.5f * omega_m;
   if (gain_params == nullptr) {  // stats given (a slab of a mesh whose statistics are known)
-    hipLaunchKernelGGL(corr::k_features, dim3(corr::grid_for(n, 8)), dim3(corr::kBlock), 0, s, rho, n, c, stats, a,
-                       omega_m, sigma8, static_cast<uint4*>(x0));
+    hipLaunchKernelGGL(corr::k_features, dim3(corr::grid_for(n, 8)), dim3(corr::kBlock), 0, s, rho, phi, n, c,
+                       stats, a, omega_m, sigma8, static_cast<uint4*>(x0));
     PM_LAUNCHED("k_features", s);
     return PM_SUCCESS;
   }
   PM_RETURN_IF(workspace == nullptr || workspace_bytes < kPartialBytes, PM_ERROR_WORKSPACE,
                "correction workspace too small");
   double* part = static_cast<double*>(workspace);
-  PM_RETURN_IF(reinterpret_cast<uintptr_t>(rho) % 16 != 0, PM_ERROR_INVALID_ARGUMENT, "rho must be 16-byte aligned");
+  PM_RETURN_IF(reinterpret_cast<uintptr_t>(rho) % 16 != 0 || reinterpret_cast<uintptr_t>(phi) % 16 != 0,
+               PM_ERROR_INVALID_ARGUMENT, "rho and phi must be 16-byte aligned");
   const unsigned nb = corr::grid_for(n, 16);
-  hipLaunchKernelGGL(corr::k_source_sumsq, dim3(nb), dim3(corr::kBlock), 0, s, rho, n, c, part);
+  hipLaunchKernelGGL(corr::k_source_sumsq, dim3(nb), dim3(corr::kBlock), 0, s, rho, n, -1.f, c, part);
   PM_LAUNCHED("k_source_sumsq", s);
   hipLaunchKernelGGL(corr::k_source_scale, dim3(1), dim3(64), 0, s, part, (int)nb, n, max_residual, gain_params, a,
                      omega_m, sigma8, stats);
   PM_LAUNCHED("k_source_scale", s);
+  if (phi) {
+    hipLaunchKernelGGL(corr::k_source_sumsq, dim3(nb), dim3(corr::kBlock), 0, s, phi, n, 0.f, 1.f, part);
+    PM_LAUNCHED("k_source_sumsq", s);
+    hipLaunchKernelGGL(corr::k_potential_scale, dim3(1), dim3(64), 0, s, part, (int)nb, n, stats);
+    PM_LAUNCHED("k_potential_scale", s);
+  }
   if (x0 == nullptr) return PM_SUCCESS;  // stats only (the fused inference stem reads rho)
-  hipLaunchKernelGGL(corr::k_features, dim3(corr::grid_for(n, 8)), dim3(corr::kBlock), 0, s, rho, n, c, stats, a,
-                     omega_m, sigma8, static_cast<uint4*>(x0));
+  hipLaunchKernelGGL(corr::k_features, dim3(corr::grid_for(n, 8)), dim3(corr::kBlock), 0, s, rho, phi, n, c, stats,
+                     a, omega_m, sigma8, static_cast<uint4*>(x0));
   PM_LAUNCHED("k_features", s);
   return PM_SUCCESS;
 }
@@ -408,31 +466,35 @@ int pm_correction_residual_bwd(const float* rho_cot, const float* out, const flo
   return PM_SUCCESS;
 }
 
-int pm_correction_input_bwd(const float* rho_cot, const float* out, const float* rho, const float* dx0, int64_t n,
-                            float omega_m, float max_residual, const float* stats, const float* gain_params,
-                            float* rho_bar, float* omega_cot, void* workspace, size_t workspace_bytes, void* stream) {
+int pm_correction_input_bwd(const float* rho_cot, const float* out, const float* rho, const float* phi,
+                            const float* dx0, int64_t n, float omega_m, float max_residual, const float* stats,
+                            const float* gain_params, float* rho_bar, float* phi_bar, float* omega_cot,
+                            void* workspace, size_t workspace_bytes, void* stream) {
   PM_TRY(check_mesh(n));
   PM_RETURN_IF(rho_cot == nullptr || out == nullptr || rho == nullptr || dx0 == nullptr || stats == nullptr ||
                    gain_params == nullptr || rho_bar == nullptr || omega_cot == nullptr,
                PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF(phi != nullptr && phi_bar == nullptr, PM_ERROR_INVALID_ARGUMENT, "phi given without phi_bar");
   PM_RETURN_IF(reinterpret_cast<uintptr_t>(dx0) % 16 != 0, PM_ERROR_INVALID_ARGUMENT, "dx0 must be 16-byte aligned");
   PM_RETURN_IF(workspace == nullptr || workspace_bytes < kPartialBytes, PM_ERROR_WORKSPACE,
                "correction workspace too small");
   hipStream_t s = as_stream(stream);
   const float c = 1.5f * omega_m;
   double* part = static_cast<double*>(workspace);
-  float* scal = reinterpret_cast<float*>(static_cast<char*>(workspace) + sizeof(double) * 4 * corr::kMaxBlocks);
+  float* scal = reinterpret_cast<float*>(static_cast<char*>(workspace) + sizeof(double) * 5 * corr::kMaxBlocks);
   const unsigned nb = corr::grid_for(n);
   const float4* dx4 = reinterpret_cast<const float4*>(dx0);
-  hipLaunchKernelGGL(corr::k_input_sums, dim3(nb), dim3(corr::kBlock), 0, s, rho_cot, out, rho, dx4, n, c, part);
+  const int pot = phi != nullptr;
+  hipLaunchKernelGGL(corr::k_input_sums, dim3(nb), dim3(corr::kBlock), 0, s, rho_cot, out, rho, phi, dx4, n, c,
+                     part);
   PM_LAUNCHED("k_input_sums", s);
-  hipLaunchKernelGGL(corr::k_input_scalars, dim3(1), dim3(64), 0, s, part, (int)nb, c, max_residual, stats,
-                     gain_params, scal);
+  hipLaunchKernelGGL(corr::k_input_scalars, dim3(1), dim3(64), 0, s, part, (int)nb, n, c, max_residual, stats,
+                     gain_params, pot, scal);
   PM_LAUNCHED("k_input_scalars", s);
-  hipLaunchKernelGGL(corr::k_input_bwd, dim3(nb), dim3(corr::kBlock), 0, s, rho_cot, rho, dx4, n, c, stats, scal,
-                     rho_bar, part);
+  hipLaunchKernelGGL(corr::k_input_bwd, dim3(nb), dim3(corr::kBlock), 0, s, rho_cot, rho, phi, dx4, n, c, stats, scal,
+                     rho_bar, phi_bar, part);
   PM_LAUNCHED("k_input_bwd", s);
-  hipLaunchKernelGGL(corr::k_input_omega, dim3(1), dim3(64), 0, s, part, (int)nb, scal, omega_cot);
+  hipLaunchKernelGGL(corr::k_input_omega, dim3(1), dim3(64), 0, s, part, (int)nb, scal, omega_m, pot, omega_cot);
   PM_LAUNCHED("k_input_omega", s);
   return PM_SUCCESS;
 }

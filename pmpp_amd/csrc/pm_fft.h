@@ -311,7 +311,8 @@ struct ColPass {
 
 // kColAdj: the x pass of the force step's spectral adjoint: forward transforms of the three
 // cotangent spectra, out = mask * (-1/k^2) * sum_i conj(D_i) in_i, inverse transform
-enum ColMode { kColFwd = 0, kColInv = 1, kColInv2 = 2, kColGreen = 3, kColAdj = 4 };
+// kColPot: x forward, mask * (-1/k^2), x inverse (the potential, one output)
+enum ColMode { kColFwd = 0, kColInv = 1, kColInv2 = 2, kColGreen = 3, kColAdj = 4, kColPot = 5 };
 
 // Line pass along the contiguous last axis (real <-> half spectrum, split main + Nyquist).
 struct ZPass {
@@ -332,6 +333,8 @@ struct ZPass {
   int nyq = 1;
   // R2C: optional second real input (same layout) added before the pre-op
   const float* rin_add = nullptr;
+  // C2R: optional real mesh (same layout as rout, may alias it) added to the scaled output
+  const float* rout_add = nullptr;
 };
 
 }  // namespace fft

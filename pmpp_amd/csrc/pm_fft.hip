@@ -151,7 +151,7 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
       for (int s = 0; s < E; ++s) io.out0[obase + (int64_t)nrow<N>(p, s) * o_row] = v[s];
     }
     return;
-  } else if constexpr (MODE == kColFwd || MODE == kColGreen) {
+  } else if constexpr (MODE == kColFwd || MODE == kColGreen || MODE == kColPot) {
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = io.in[lbase + (int64_t)nrow<N>(p, s) * io.s_row];
     forward_core<N>(v, my, p, idx, tw);
@@ -181,9 +181,9 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
     }
     return;
   } else {
-    if constexpr (MODE == kColGreen) {
-      // mask x Poisson symbol (gravity.py:174-196, 263-287); the second half also applies
-      // D_x = i*negk_x (configuration.py:365-376)
+    if constexpr (MODE == kColGreen || MODE == kColPot) {
+      // mask x Poisson symbol (gravity.py:174-196, 263-287); kColGreen's second half also
+      // applies D_x = i*negk_x (configuration.py:365-376), kColPot has one half (the potential)
       int ky = (int)(b * io.ky_b) + cgl * io.ky_c;
       ky += ky > io.ky_split ? io.ky_shift : 0;
       int kz = (int)(b * io.kz_b) + cgl * io.kz_c + io.kz0;
@@ -236,7 +236,9 @@ struct ZCfg {
   static constexpr int LDS_FLOATS = LINES * LSTR;
 };
 
-template <int M, bool C2R>
+// ADD (C2R only): the output is rout_add + the scaled line (a separate instantiation, so the
+// force step's c2r keeps its register budget)
+template <int M, bool C2R, bool ADD = false>
 __global__ void __launch_bounds__((ZCfg<M, C2R>::THREADS)) k_zpass(ZPass zp, SpecTables tb,
                                                                  const float2* __restrict__ tw) {
   using PL = Plan<M>;
@@ -339,11 +341,20 @@ __global__ void __launch_bounds__((ZCfg<M, C2R>::THREADS)) k_zpass(ZPass zp, Spe
     inverse_core<M>(v, lds, p, idx, tw);
     if (ok) {
       float* dst = zp.rout + lg * zp.pitch_r;
+      if constexpr (ADD) {
+        const float* add = zp.rout_add + lg * zp.pitch_r;
 #pragma unroll
-      for (int s = 0; s < E; ++s) {
-        int m = nrow<M>(p, s);
-        *reinterpret_cast<float2*>(dst + 2 * m) = make_float2(v[s].x * zp.out_scale, v[s].y * zp.out_scale);
+        for (int s = 0; s < E; ++s) {
+          int m = nrow<M>(p, s);
+          const float2 w = *reinterpret_cast<const float2*>(add + 2 * m);
+          v[s] = make_float2(w.x + v[s].x * zp.out_scale, w.y + v[s].y * zp.out_scale);
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = make_float2(v[s].x * zp.out_scale, v[s].y * zp.out_scale);
       }
+#pragma unroll
+      for (int s = 0; s < E; ++s) *reinterpret_cast<float2*>(dst + 2 * nrow<M>(p, s)) = v[s];
     }
   }
 }
@@ -421,6 +432,7 @@ int launch_cols_n(int mode, const ColPass& cp, int batches, const SpecTables& tb
     case kColInv2: return launch_cols_cfg<N, T2, kColInv2, 2>(cp, batches, tb, dax, s, "fft_cols_inv2");
     case kColGreen: return launch_cols_cfg<N, T2, kColGreen, 2>(cp, batches, tb, dax, s, "fft_cols_green");
     case kColAdj: return launch_cols_cfg<N, T1, kColAdj, 1>(cp, batches, tb, dax, s, "fft_cols_adjoint");
+    case kColPot: return launch_cols_cfg<N, T2, kColPot, 1>(cp, batches, tb, dax, s, "fft_cols_potential");
     default: return fail(PM_ERROR_INVALID_ARGUMENT, "bad column mode");
   }
 }
@@ -432,7 +444,10 @@ int launch_z_n(bool c2r, const ZPass& zp, const SpecTables& tb, hipStream_t s) {
   PM_RETURN_IF(tw == nullptr, PM_ERROR_HIP, "twiddle table not initialised");
   if (zp.nlines == 0) return PM_SUCCESS;
   dim3 grid((unsigned)ceil_div(zp.nlines, ZC::LINES));
-  if (c2r) {
+  if (c2r && zp.rout_add) {
+    hipLaunchKernelGGL((k_zpass<M, true, true>), grid, dim3(ZC::THREADS), 0, s, zp, tb, tw);
+    PM_LAUNCHED("fft_z_c2r_add", s);
+  } else if (c2r) {
     hipLaunchKernelGGL((k_zpass<M, true>), grid, dim3(ZC::THREADS), 0, s, zp, tb, tw);
     PM_LAUNCHED("fft_z_c2r", s);
   } else {

@@ -1290,6 +1290,72 @@ int adjoint_spectral_stage(const pm_gravity_desc* d, const Layout& L, Buffers& b
   return launch_z(L.nz, true, zc, tb, s);
 }
 
+// The particle-Nyquist-filtered potential of a real mesh on the force step's band:
+// out = add + irfftn(mask * (-1/k^2) * rfftn(scale * (in + shift))).  With shift = -1 and
+// scale = 1.5 Omega_m on a density mesh it is the potential of the force's source
+// (gravity.py:626-633 with the Poisson and filter symbols of :174-196, 263-321); the operator
+// is real and symmetric, so with shift = 0 it is also its own adjoint (a cotangent's path back
+// to the source).  The passes are force_mesh_stage's for one output: z r2c storing the kz band
+// (A), y forward packing the |ky| band (A -> B), the x pass with the Green multiply and one
+// inverse (kColPot, B -> G), the y inverse expanding the band (G -> A), z c2r (+ add).
+int potential_stage(const Layout& L, float2* A, float2* B, float2* G, const fft::SpecTables& tb, const float* in,
+                    float shift, float scale, const float* add, float* out, hipStream_t s) {
+  using namespace fft;
+  const bool zband = L.nkz <= L.m;
+  const int64_t p1 = L.pitch1;
+  const int ncols = L.nkz;
+  ZPass zr{};
+  zr.rin = in;
+  zr.cmain = A;
+  zr.cnyq = zband ? A : A + L.m;
+  zr.pitch_r = L.nz;
+  zr.pitch_c = p1;
+  zr.nyq_stride = p1;
+  zr.nlines = (int64_t)L.nx * L.ny;
+  zr.src_shift = shift;
+  zr.src_scale = scale;
+  zr.kz_lim = L.nkz;
+  zr.nyq = zband ? 0 : 1;
+  PM_TRY(launch_z(L.nz, false, zr, tb, s));
+  ColPass yf{A, B, nullptr, p1, 1, (int64_t)L.ny * p1, ncols, 0, 0, 0, 0, 0};
+  yf.o_row = p1;
+  yf.o_batch = (int64_t)L.nky * p1;
+  yf.keep = L.ky_keep;
+  PM_TRY(launch_cols(L.ny, kColFwd, yf, L.nx, tb, 1, s));
+  ColPass xp{B, G, nullptr, (int64_t)L.nky * p1, 1, p1, ncols, 1, 0, 0, 1, 0};
+  if (L.ky_keep >= 0) {
+    xp.ky_split = L.ky_keep;
+    xp.ky_shift = L.ny - L.nky;
+  }
+  PM_TRY(launch_cols(L.nx, kColPot, xp, L.nky, tb, 0, s));
+  ColPass yi{G, A, nullptr, p1, 1, (int64_t)L.nky * p1, ncols, 0, 0, 0, 0, 0};
+  yi.o_row = p1;
+  yi.o_batch = (int64_t)L.ny * p1;
+  yi.keep = L.ky_keep;
+  yi.compact_in = 1;
+  PM_TRY(launch_cols(L.ny, kColInv, yi, L.nx, tb, 1, s));
+  ZPass zc{};
+  zc.pitch_r = L.nz;
+  zc.pitch_c = p1;
+  zc.nyq_stride = p1;
+  zc.nlines = (int64_t)L.nx * L.ny;
+  zc.out_scale = (float)(2.0 / ((double)L.nx * L.ny * L.nz));
+  zc.kz_lim = L.nkz;
+  zc.nyq = zband ? 0 : 1;
+  zc.cmain = A;
+  zc.cnyq = zband ? A : A + L.m;
+  zc.rout = out;
+  zc.rout_add = add;
+  zc.deriv = 0;
+  return launch_z(L.nz, true, zc, tb, s);
+}
+
+size_t potential_workspace_bytes(const pm_gravity_desc* d) {
+  const Layout L = make_layout(d);
+  return align_up(sizeof(float) * fft::spec_table_floats(d->mesh_shape), 256) +
+         align_up(sizeof(float2) * L.s1, 256) * 2 + align_up(sizeof(float2) * L.s2, 256) + 256;
+}
+
 int prepare(const pm_gravity_desc* d, int64_t n, void* workspace, size_t workspace_bytes_, bool reverse,
             Layout& L, Buffers& b, Arena& ws, fft::SpecTables& tb, hipStream_t s) {
   PM_TRY(check_desc(d, n));
@@ -1508,6 +1574,35 @@ int pm_gravity_vjp_resume(const int16_t* pmid, const float* disp, const uint8_t*
                           void* workspace, size_t workspace_bytes, void* stream) {
   return vjp_impl(pmid, disp, valid, n, desc, nullptr, acc_cot, disp_cot, omega_cot, nullptr, workspace,
                   workspace_bytes, stream, true);
+}
+
+int pm_potential_workspace(const pm_gravity_desc* desc, size_t* bytes) {
+  PM_TRY(grav::check_desc(desc, 0));
+  PM_RETURN_IF(bytes == nullptr, PM_ERROR_INVALID_ARGUMENT, "bytes is NULL");
+  *bytes = grav::potential_workspace_bytes(desc);
+  return PM_SUCCESS;
+}
+
+int pm_gravity_potential(const float* in, float shift, float scale, const float* add, const pm_gravity_desc* desc,
+                         float* out, void* workspace, size_t workspace_bytes, void* stream) {
+  PM_TRY(grav::check_desc(desc, 0));
+  PM_TRY(require_device_tables());
+  PM_RETURN_IF(in == nullptr || out == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL mesh");
+  const size_t need = grav::potential_workspace_bytes(desc);
+  PM_RETURN_IF(workspace == nullptr || workspace_bytes < need, PM_ERROR_WORKSPACE,
+               "potential workspace too small: need " + std::to_string(need) + " bytes");
+  hipStream_t s = as_stream(stream);
+  const grav::Layout L = grav::make_layout(desc);
+  Arena ws(workspace, workspace_bytes);
+  float* tables = ws.take<float>(fft::spec_table_floats(desc->mesh_shape));
+  float2* A = ws.take<float2>(L.s1);
+  float2* G = ws.take<float2>(L.s1);
+  float2* B = ws.take<float2>(L.s2);
+  PM_RETURN_IF(!ws.ok(), PM_ERROR_WORKSPACE, "potential workspace carve failed");
+  const fft::SpecTables sc = fft::spec_scalars(desc->cell_size, desc->ptcl_spacing, desc->mask_active);
+  PM_TRY(fft::launch_spec_tables(tables, desc->mesh_shape, sc, s));
+  const fft::SpecTables tb = fft::spec_tables_view(tables, desc->mesh_shape, sc);
+  return grav::potential_stage(L, A, B, G, tb, in, shift, scale, add, out, s);
 }
 
 int pm_spectral_workspace(const pm_gravity_desc* desc, size_t* bytes) {

@@ -219,3 +219,33 @@ def gravity_keep(ptcl, cosmo, conf, kept):
     kept.key = KeptForce.key_of(ptcl, omega)
     return acc
 
+
+
+def gravity_potential(dens, omega_m, conf, out=None, shift=-1.0, scale=None, add=None):
+    """Real-space potential of a density mesh on the force step's spectral band:
+    ``irfftn(mask * (-1/k^2) * rfftn(1.5 Omega_m (dens - 1)))``, the potential the reference's
+    corrected force builds (``nbody/gravity.py:419-430``) and reads back in real space for the
+    mesh CNN's potential channel (``corrections/core.py:286-334``, ``mesh_cnn.py:76-84``).
+
+    ``shift`` and ``scale`` (default -1 and 1.5 Omega_m) set the source ``scale * (dens + shift)``;
+    the operator is symmetric, so ``shift=0`` applies it to a cotangent.  ``add`` (a mesh, may be
+    ``out``) is added to the result.  One ``pm_gravity_potential`` call."""
+    dev = dens.device
+    ensure_init(dev)
+    d = gravity_desc(conf, omega_m)
+    x = as_contig(dens, torch.float32, "dens")
+    shape = tuple(int(v) for v in conf.mesh_shape)
+    if tuple(x.shape) != shape:
+        raise ValueError(f"mesh {tuple(x.shape)} does not match the configuration's {shape}")
+    if out is None:
+        out = torch.empty_like(x)
+    if tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float32 mesh of the configuration's shape")
+    if add is not None and (tuple(add.shape) != shape or add.dtype != torch.float32 or not add.is_contiguous()):
+        raise ValueError("add must be a contiguous float32 mesh of the configuration's shape")
+    sc = 1.5 * d.omega_m if scale is None else float(scale)
+    nbytes = _lib.size_query("pm_potential_workspace", d)
+    ws = workspace(nbytes, dev)
+    _lib.call("pm_gravity_potential", ptr(x), float(shift), sc, ptr(add), d, ptr(out), ptr(ws), nbytes,
+              stream_handle(dev))
+    return out

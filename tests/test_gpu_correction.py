@@ -75,6 +75,35 @@ def test_density_is_the_step_cic_mesh(device):
     assert rho.sum() == pytest.approx(np.prod(conf.mesh_shape), rel=1e-6)
 
 
+@pytest.mark.parametrize("ng,ratio,box", [(16, 1, 32.0), (8, 2, 16.0), (16, 2, 48.0), (32, 2, 64.0)])
+def test_potential_matches_oracle_and_is_symmetric(device, ng, ratio, box):
+    """pm_gravity_potential (the network's potential channel) against the float64 oracle at the
+    force step's float32 bar; the operator with shift 0 is symmetric (<P u, v> = <u, P v>, so it
+    is its own adjoint); the accumulate form equals the separate sum (up to the fused
+    multiply-add of the output scaling)."""
+    from pmpp_amd.nbody.gravity import gravity_potential
+
+    conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio)
+    ms = conf.mesh_shape
+    rng = np.random.default_rng(ng + ratio)
+    rho = (1 + 0.5 * rng.normal(size=ms)).astype(np.float32)
+    om = 0.3
+    got = _np(gravity_potential(torch.from_numpy(rho).to(device), om, conf))
+    ref = spectral.potential(rho, om, ms, conf.cell_size, _masks(conf))
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+    u = torch.from_numpy(rng.normal(size=ms).astype(np.float32)).to(device)
+    v = torch.from_numpy(rng.normal(size=ms).astype(np.float32)).to(device)
+    pu = gravity_potential(u, om, conf, shift=0.0, scale=1.0)
+    pv = gravity_potential(v, om, conf, shift=0.0, scale=1.0)
+    lhs, rhs = float((pu.double() * v.double()).sum()), float((u.double() * pv.double()).sum())
+    assert lhs == pytest.approx(rhs, rel=1e-5)
+    w = torch.from_numpy(rng.normal(size=ms).astype(np.float32)).to(device)
+    acc = w.clone()
+    gravity_potential(u, om, conf, out=acc, shift=0.0, scale=0.45, add=acc)
+    sep = _np(w + gravity_potential(u, om, conf, shift=0.0, scale=0.45))
+    np.testing.assert_allclose(_np(acc), sep, rtol=1e-6, atol=1e-6 * np.abs(sep).max())
+
+
 def _randomised(model, seed, out_scale=0.05):
     """Non-zero output and gain layers so that every parameter carries gradient."""
     g = torch.Generator().manual_seed(seed)
@@ -101,16 +130,27 @@ def _torch_conv(x, w, b, cin):
     return F.conv3d(xc, k, b)[0].permute(1, 2, 3, 0)
 
 
-def _torch_residual(model, rho, a, om, params):
+def _green(conf):
+    """mask * (-1/k^2) of the configuration (the oracle's float64 symbol) as a torch tensor."""
+    return torch.from_numpy(spectral.green_multiplier(conf.mesh_shape, conf.cell_size, _masks(conf)))
+
+
+def _torch_residual(model, rho, a, om, params, green=None):
     """float32 restatement of the network with the kernels' bf16 roundings (differentiable in
-    the float32 parameter vector ``params``, in ``rho`` and in a tensor ``om``)."""
+    the float32 parameter vector ``params``, in ``rho`` and in a tensor ``om``).  A model with
+    the potential channel reads phi = irfftn(green * rfftn(S)) (``green`` from ``_green``)."""
     r, s8 = model.max_residual, model.sigma8_value
     om = torch.as_tensor(om, dtype=torch.float32)
     c = 1.5 * om
     S = (rho - 1.0) * c
     sigma = torch.sqrt((S.double() ** 2).mean().float() + 1e-6)
     ones = torch.ones_like(rho)
-    x0 = torch.stack([S / sigma, a * ones, om * ones, s8 * ones] + [0 * ones] * 4, -1)
+    if model.potential:
+        phi = torch.fft.irfftn(torch.fft.rfftn(S.double()) * green, s=tuple(rho.shape)).float()
+        sp = torch.sqrt((phi.double() ** 2).mean().float() + 1e-6)
+        x0 = torch.stack([S / sigma, phi / sp, a * ones, om * ones, s8 * ones] + [0 * ones] * 3, -1)
+    else:
+        x0 = torch.stack([S / sigma, a * ones, om * ones, s8 * ones] + [0 * ones] * 4, -1)
     x0 = _bf(x0)
 
     def layer(i):
@@ -136,18 +176,20 @@ def _torch_residual(model, rho, a, om, params):
     return g * (rho - 1.0) + r * torch.tanh(out) * sigma / c
 
 
-@pytest.mark.parametrize("depth,shape,block", [(1, (8, 8, 8), "residual"), (2, (16, 8, 24), "residual"),
-                                               (3, (16, 16, 16), "residual"), (2, (16, 8, 24), "plain"),
-                                               (3, (8, 16, 16), "plain")])
-def test_network_and_backward_match_float32_torch(device, depth, shape, block):
-    model = _randomised(init_mesh_cnn_potential_correction(seed=depth, depth=depth, device=device, block=block),
-                        depth)
+@pytest.mark.parametrize("depth,shape,block,potential", [
+    (1, (8, 8, 8), "residual", False), (2, (16, 8, 24), "residual", False), (3, (16, 16, 16), "residual", False),
+    (2, (16, 8, 24), "plain", False), (3, (8, 16, 16), "plain", False), (2, (16, 8, 32), "plain", True),
+    (3, (16, 16, 16), "residual", True)])
+def test_network_and_backward_match_float32_torch(device, depth, shape, block, potential):
+    model = _randomised(init_mesh_cnn_potential_correction(seed=depth, depth=depth, device=device, block=block,
+                                                           potential=potential), depth)
     rng = np.random.default_rng(depth)
     rho = torch.from_numpy((1 + 0.5 * rng.normal(size=shape)).astype(np.float32)).to(device)
     a, om = 0.7, 0.31
-    rho_add, tape = evaluate_residual(model, rho, a, om, keep=True)
+    conf = Configuration(2.0, tuple(v // 2 for v in shape), mesh_shape=2) if potential else None
+    rho_add, tape = evaluate_residual(model, rho, a, om, keep=True, conf=conf)
     params = model.params.detach().cpu().clone().requires_grad_(True)
-    ref = _torch_residual(model, rho.cpu(), a, om, params)
+    ref = _torch_residual(model, rho.cpu(), a, om, params, _green(conf) if potential else None)
     got = rho_add.cpu()
     assert torch.linalg.norm(got - ref.detach()) / torch.linalg.norm(ref.detach()) < 2e-2
     rho_cot = torch.from_numpy(rng.normal(size=shape).astype(np.float32))
@@ -168,27 +210,30 @@ def test_network_and_backward_match_float32_torch(device, depth, shape, block):
     assert torch.all(grads[~live] == 0)
 
 
-@pytest.mark.parametrize("shape", [(16, 8, 24), (32, 32, 48), (64, 64, 64)])
-def test_inference_stem_from_density_matches_the_feature_path(device, shape):
+@pytest.mark.parametrize("shape,potential", [((16, 8, 24), False), ((32, 32, 48), False), ((64, 64, 64), False),
+                                             ((16, 8, 32), True), ((64, 64, 64), True)])
+def test_inference_stem_from_density_matches_the_feature_path(device, shape, potential):
     """Inference reads the density straight into the stem, with the constant input channels
     folded into its bias; training writes the 8-channel input x0 and runs the generic conv.
     Both compute the same float32 pre-activations up to summation order, so the stem's bf16
     activations agree within one bf16 ulp (at most 2^-7 relative, plus an absolute floor for
     values near zero), on a few per cent of the values at most, and the residual within the
     network's bf16 bar.  The meshes hold wrapped
-    and interior blocks and, at 64^3, more blocks than the persistent grid's workgroups."""
-    model = _randomised(init_mesh_cnn_potential_correction(seed=7, depth=2, device=device), 7)
+    and interior blocks and, at 64^3, more blocks than the persistent grid's workgroups.  With
+    the potential channel the stem adds a second MFMA step over the staged potential."""
+    model = _randomised(init_mesh_cnn_potential_correction(seed=7, depth=2, device=device, potential=potential), 7)
     rng = np.random.default_rng(8)
     rho = torch.from_numpy((1 + 0.5 * rng.normal(size=shape)).astype(np.float32)).to(device)
     a, om = 0.45, 0.29
-    fused, _ = evaluate_residual(model, rho, a, om)
-    ref, tape = evaluate_residual(model, rho, a, om, keep=True)
+    conf = Configuration(1.0, tuple(v // 2 for v in shape), mesh_shape=2) if potential else None
+    fused, _ = evaluate_residual(model, rho, a, om, conf=conf)
+    ref, tape = evaluate_residual(model, rho, a, om, keep=True, conf=conf)
     # stem activations of both paths (the tape keeps the feature path's)
     from pmpp_amd import _lib
     from pmpp_amd._torch import ptr, stream_handle
     act = torch.empty(shape + (16,), dtype=torch.bfloat16, device=device)
-    _lib.call("pm_conv3d_stem", ptr(rho), _lib.int3(shape), om, a, float(model.sigma8_value), ptr(tape.stats),
-              ptr(model.weight(model.layers[0])), ptr(model.bias(model.layers[0])), ptr(act),
+    _lib.call("pm_conv3d_stem", ptr(rho), ptr(tape.phi), _lib.int3(shape), om, a, float(model.sigma8_value),
+              ptr(tape.stats), ptr(model.weight(model.layers[0])), ptr(model.bias(model.layers[0])), ptr(act),
               stream_handle(device))
     got, exp = act.float().cpu(), tape.acts[0].float().cpu()
     tol = 2.0 ** -7 * exp.abs() + 1e-3 * exp.abs().max()
@@ -367,7 +412,8 @@ def test_correction_trained_on_fine_runs_moves_pk_towards_the_fine_run(device):
     assert r_corr > r_plain - 0.03  # the phases stay as close to the fine run's
 
 
-def test_corrected_force_reverse_through_the_network_inputs(device):
+@pytest.mark.parametrize("potential", [False, True])
+def test_corrected_force_reverse_through_the_network_inputs(device, potential):
     # disp and Omega_m cotangents of the corrected force: the HIP chain (force-step adjoint at the
     # corrected source, network backward to its input, input-path kernel, scatter adjoint)
     # against the float64 oracle force adjoint plus float32 torch autograd of the network
@@ -376,7 +422,8 @@ def test_corrected_force_reverse_through_the_network_inputs(device):
 
     conf = Configuration(2.0, (16, 16, 16), mesh_shape=1)
     ptcl = _ptcl(conf, 0.6, 21, device)
-    model = _randomised(init_mesh_cnn_potential_correction(seed=4, depth=2, device=device), 4, out_scale=0.2)
+    model = _randomised(init_mesh_cnn_potential_correction(seed=4, depth=2, device=device, potential=potential), 4,
+                        out_scale=0.2)
     om, a = 0.3, 0.8
     rng = np.random.default_rng(12)
     ac = rng.normal(size=(conf.ptcl_num, 3)).astype(np.float32)
@@ -386,7 +433,7 @@ def test_corrected_force_reverse_through_the_network_inputs(device):
     params = model.params.detach().cpu()
     rho_t = rho.clone().requires_grad_(True)
     om_t = torch.tensor(om, dtype=torch.float32, requires_grad=True)
-    rho_add = _torch_residual(model, rho_t, a, om_t, params)
+    rho_add = _torch_residual(model, rho_t, a, om_t, params, _green(conf))
     pm, dp = ptcl.pmid.cpu().numpy().astype(np.int64), _np(ptcl.disp)
     dc0, oc0, rc0 = spectral.gravity_vjp(pm, dp, om, ac, conf.mesh_shape, conf.cell_size, conf.ptcl_num,
                                          _masks(conf), rho_add=_np(rho_add))
@@ -435,8 +482,8 @@ def test_nbody_reverse_sweep_with_a_correction_matches_finite_differences(device
     assert float((disp.grad * v).sum()) == pytest.approx(fd, rel=5e-2)
 
 
-@pytest.mark.parametrize("slab", [8, 16])
-def test_network_reverse_by_slabs_equals_the_whole_mesh(device, slab):
+@pytest.mark.parametrize("slab,potential", [(8, False), (16, False), (8, True)])
+def test_network_reverse_by_slabs_equals_the_whole_mesh(device, slab, potential):
     """The corrected force's reverse with the network's backward run over x-slabs with a halo
     (network_input_vjp) against the same backward on the whole mesh: the slab's own planes
     see exactly the whole-mesh arithmetic, so the cotangents agree to summation order."""
@@ -444,8 +491,8 @@ def test_network_reverse_by_slabs_equals_the_whole_mesh(device, slab):
 
     conf = Configuration(2.0, (32, 16, 16), mesh_shape=1)
     ptcl = _ptcl(conf, 0.6, 41, device)
-    model = _randomised(init_mesh_cnn_potential_correction(seed=6, depth=2, device=device, block="plain"), 6,
-                        out_scale=0.2)
+    model = _randomised(init_mesh_cnn_potential_correction(seed=6, depth=2, device=device, block="plain",
+                                                           potential=potential), 6, out_scale=0.2)
     rng = np.random.default_rng(14)
     ac = torch.from_numpy(rng.normal(size=(conf.ptcl_num, 3)).astype(np.float32)).to(device)
     d_full, o_full = corrected_gravity_vjp(0.7, ptcl, 0.3, conf, model, ac)
@@ -489,28 +536,32 @@ def test_whole_run_loss_parameter_gradient_matches_central_differences(device):
     assert got == pytest.approx(fd, rel=5e-2)
 
 
-@pytest.mark.parametrize("shape", [(8, 8, 16), (32, 32, 48), (64, 64, 64)])
-def test_fused_stem_and_hidden_layer_equal_the_two_kernels(device, shape):
+@pytest.mark.parametrize("shape,potential", [((8, 8, 16), False), ((32, 32, 48), False), ((64, 64, 64), False),
+                                             ((8, 8, 16), True), ((64, 64, 64), True)])
+def test_fused_stem_and_hidden_layer_equal_the_two_kernels(device, shape, potential):
     """pm_conv3d_stem_hidden (the stem computed on the hidden layer's staged tile in LDS) against
     pm_conv3d_stem followed by the hidden conv: the same arithmetic in the same order, so the
     activations agree bit for bit; meshes of one block along y and z (every halo wraps) and of
-    several blocks along each axis."""
+    several blocks along each axis; with and without the potential channel."""
     from pmpp_amd import _lib
     from pmpp_amd._torch import ptr, stream_handle
     from pmpp_amd.corrections import conv3d
 
-    model = _randomised(init_mesh_cnn_potential_correction(seed=9, depth=2, block="plain", device=device), 9)
+    model = _randomised(init_mesh_cnn_potential_correction(seed=9, depth=2, block="plain", device=device,
+                                                           potential=potential), 9)
     rng = np.random.default_rng(10)
     rho = torch.from_numpy((1 + 0.5 * rng.normal(size=shape)).astype(np.float32)).to(device)
     a, om = 0.55, 0.31
-    _, tape = evaluate_residual(model, rho, a, om, keep_out=True)  # the statistics
+    conf = Configuration(1.0, tuple(v // 2 for v in shape), mesh_shape=2) if potential else None
+    _, tape = evaluate_residual(model, rho, a, om, keep_out=True, conf=conf)  # the statistics (and phi)
     L0, L1 = model.layers[0], model.layers[1]
     s = stream_handle(device)
     stem = torch.empty(shape + (16,), dtype=torch.bfloat16, device=device)
-    _lib.call("pm_conv3d_stem", ptr(rho), _lib.int3(shape), om, a, float(model.sigma8_value), ptr(tape.stats),
-              ptr(model.weight(L0)), ptr(model.bias(L0)), ptr(stem), s)
+    _lib.call("pm_conv3d_stem", ptr(rho), ptr(tape.phi), _lib.int3(shape), om, a, float(model.sigma8_value),
+              ptr(tape.stats), ptr(model.weight(L0)), ptr(model.bias(L0)), ptr(stem), s)
     _, ref = conv3d(stem, model.weight(L1), model.bias(L1), gelu=True, want_pre=False)
     got = torch.empty(shape + (16,), dtype=torch.bfloat16, device=device)
-    _lib.call("pm_conv3d_stem_hidden", ptr(rho), _lib.int3(shape), om, a, float(model.sigma8_value), ptr(tape.stats),
-              ptr(model.weight(L0)), ptr(model.bias(L0)), ptr(model.weight(L1)), ptr(model.bias(L1)), ptr(got), s)
+    _lib.call("pm_conv3d_stem_hidden", ptr(rho), ptr(tape.phi), _lib.int3(shape), om, a, float(model.sigma8_value),
+              ptr(tape.stats), ptr(model.weight(L0)), ptr(model.bias(L0)), ptr(model.weight(L1)), ptr(model.bias(L1)),
+              ptr(got), s)
     np.testing.assert_array_equal(got.float().cpu().numpy(), ref.float().cpu().numpy())

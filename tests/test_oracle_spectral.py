@@ -80,6 +80,32 @@ def test_single_plane_wave_force_closed_form():
     np.testing.assert_allclose(fz, 0, atol=1e-15)
 
 
+def test_potential_plane_wave_and_gradient_consistency():
+    # S = A cos(k0 . x) along y  =>  phi = -A cos(k0 y) / k0^2
+    ms, cell = (8, 16, 8), 0.5
+    A, om = 0.02, 0.3
+    y = np.arange(16) * cell
+    k0 = 2 * np.pi * 3 / (16 * cell)
+    density = 1 + (A * np.cos(k0 * y))[None, :, None] * np.ones(ms) / (1.5 * om)
+    phi = spectral.potential(density, om, ms, cell)
+    np.testing.assert_allclose(phi, (-A * np.cos(k0 * y) / k0 ** 2)[None, :, None] * np.ones(ms), atol=1e-14)
+    # on a random density with particle-Nyquist masks, the force meshes are the spectral
+    # gradient of the potential (F_i = irfftn(-i k_i phi_hat) with the cached -k_i)
+    rng = np.random.default_rng(3)
+    ms, cell = (16, 8, 16), 1.0
+    masks = spectral.particle_nyquist_masks(ms, (8, 4, 8), 2.0, cell)
+    dens = 1 + 0.3 * rng.normal(size=ms)
+    phi = spectral.potential(dens, om, ms, cell, masks)
+    forces = spectral.force_meshes(dens, om, ms, cell, masks)
+    nk = spectral.neg_k(ms, cell)
+    ph = spectral.rfftn(phi)
+    shapes = [(-1, 1, 1), (1, -1, 1), (1, 1, -1)]
+    for i in range(3):
+        ref = spectral.irfftn(1j * nk[i].reshape(shapes[i]) * ph, ms)
+        np.testing.assert_allclose(forces[i], ref, atol=1e-12 * np.abs(ref).max())
+    assert abs(phi.mean()) < 1e-15  # the k = 0 mode is removed
+
+
 def _lattice(ng, ratio):
     q = np.stack(np.meshgrid(*[np.arange(ng) * ratio] * 3, indexing="ij"), -1).reshape(-1, 3)
     return q
