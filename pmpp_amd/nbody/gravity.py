@@ -220,7 +220,6 @@ def gravity_keep(ptcl, cosmo, conf, kept):
     return acc
 
 
-
 def gravity_potential(dens, omega_m, conf, out=None, shift=-1.0, scale=None, add=None):
     """Real-space potential of a density mesh on the force step's spectral band:
     ``irfftn(mask * (-1/k^2) * rfftn(1.5 Omega_m (dens - 1)))``, the potential the reference's
