@@ -729,27 +729,42 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
       }
     }
     __syncthreads();
-    // the stem on the staged voxels of block nxt, into the hidden layer's tile
-    for (int t = wave; t < STEM_TILES; t += kThreads / 64) {
-      const int v = 16 * t + m;
-      const int vc = min(v, NS - 1);
-      const int sz = vc % SZ, sy = (vc / SZ) % SY, sx = vc / (SZ * SY);
-      const int own = ((sx + 1) * kFRY + (sy + 1)) * kFRZ + (sz + 1);
+    // the stem on the staged voxels of block nxt, into the hidden layer's tile: a wave takes the
+    // tiles wave, wave + 4, ... two at a time, so one tile's LDS reads and MFMA overlap the
+    // other's GELU (the same arithmetic per voxel as one at a time)
+    auto stem_mfma = [&](int t) {
+      // density-tile cell of staged voxel vc = (sx, sy, sz), q = vc / SZ = sx SY + sy:
+      // ((sx + 1) kFRY + sy + 1) kFRZ + sz + 1 = vc + q (kFRZ - SZ) + sx (kFRY - SY) kFRZ + c0
+      const int vc = min(16 * t + m, NS - 1);
+      const int q = vc / SZ;
+      const int c = vc + q * (kFRZ - SZ) + (q / SY) * ((kFRY - SY) * kFRZ) + (kFRY * kFRZ + kFRZ + 1);
       bf16x8 b;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) b[j] = rt[0][own + soff[j]];
+      for (int j = 0; j < 8; ++j) b[j] = rt[0][c + soff[j]];
       f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsa, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       if constexpr (PHI) {
         bf16x8 bp;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) bp[j] = rt[1][own + soff[j]];
+        for (int j = 0; j < 8; ++j) bp[j] = rt[1][c + soff[j]];
         cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsp, bp, cv, 0, 0, 0);
       }
+      return cv;
+    };
+    auto stem_store = [&](int t, const f32x4& cv) {
+      const int v = 16 * t + m;
       bf16x4 o;
 #pragma unroll
       for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i] + sbv[i]);
       if (v < NS) *reinterpret_cast<bf16x4*>(xin + ((g / 2) * NS + v) * 8 + 4 * (g % 2)) = o;
+    };
+    int t = wave;
+    for (; t + 4 < STEM_TILES; t += 8) {
+      const f32x4 c0 = stem_mfma(t);
+      const f32x4 c1 = stem_mfma(t + 4);
+      stem_store(t, c0);
+      stem_store(t + 4, c1);
     }
+    if (t < STEM_TILES) stem_store(t, stem_mfma(t));
     __syncthreads();
     cur = nxt;
     nxt += G;
