@@ -91,7 +91,7 @@ def test_cic_window_of_a_randomly_placed_particle_on_the_gpu(device):
     pks = []
     for o in offs:
         p = Particles(conf, torch.tensor([[3, 5, 7]], dtype=torch.int16, device=device),
-                      torch.tensor([o], dtype=torch.float32, device=device))
+                      torch.from_numpy(o[None].astype(np.float32)).to(device))
         mesh = scatter(p, conf, val=torch.ones(1, device=device))
         pks.append(delta_to_pk(mesh, conf, mas="CIC")[1].cpu().numpy())
 
