@@ -782,23 +782,28 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
 // tile, which is no longer needed) and each output voxel sums its nine, in tap order.  Two
 // MFMAs per staged column replace 15 per 16 output voxels.
 //
-// Blocks are 4 x 8 x 16 output voxels (the mesh must divide into them) staged as 6 x 10 x 18
-// with a one-voxel periodic halo: 1080 voxels of 32 bytes, stored as one plane per 8-channel
-// half; the 60 staged columns x 9 offsets x 16 z of D take exactly the same 34560 bytes.
+// Blocks are 8 x 8 x 16 output voxels (the mesh must divide into them) staged as 10 x 10 x 18
+// with a one-voxel periodic halo: 1800 voxels of 32 bytes, stored as one plane per 8-channel
+// half; the 100 staged columns x 9 offsets x 16 z of D take exactly the same 57600 bytes.  The
+// layer is bound by HBM, and the halo sets its traffic: 8 x 8 x 16 blocks on 512-thread
+// workgroups stage 1.76 voxels per output voxel against 2.11 for 4 x 8 x 16 blocks on 256
+// threads, at the same two waves per SIMD (1024^3: 16.2 -> 15.5 ms, fetch 72 -> 53 GiB).
 // Epilogues: kPlain writes the pre-activation out + bias[0] (pre, one channel); kOutResidual
 // writes rho_add = gain (rho - 1) + amp tanh(out + bias[0]) (the residual stage).
-constexpr int kOBX = 4, kOBY = 8, kOBZ = 16;
+constexpr int kOBX = 8, kOBY = 8, kOBZ = 16;
+constexpr int kOThreads = 512;                              // 8 waves
+constexpr int kOWaves = kOThreads / 64;
 constexpr int kOSX = kOBX + 2, kOSY = kOBY + 2, kOSZ = kOBZ + 2;
-constexpr int kOCols = kOSX * kOSY;                        // 60 staged columns
-constexpr int kONS = kOCols * kOSZ;                        // 1080 staged voxels
-constexpr int kOItems = 2 * kONS;                          // 16-byte vectors staged
-constexpr int kOPer = (kOItems + kThreads - 1) / kThreads;  // 9 per thread
-constexpr int kOColsPerWave = kOCols / (kThreads / 64);     // 15
+constexpr int kOCols = kOSX * kOSY;                         // 100 staged columns
+constexpr int kONS = kOCols * kOSZ;                         // 1800 staged voxels
+constexpr int kOItems = 2 * kONS;                           // 16-byte vectors staged
+constexpr int kOPer = (kOItems + kOThreads - 1) / kOThreads;  // 8 per thread
+constexpr int kOColsPerWave = (kOCols + kOWaves - 1) / kOWaves;  // 13 (the last round: waves 0..3)
 static_assert(kOCols * 9 * kOBZ == kONS * 8, "D fills the staged tile exactly");
-static_assert(kOCols % (kThreads / 64) == 0, "columns per wave");
+static_assert((kOBX * kOBY * kOBZ) % kOThreads == 0, "output voxels per thread");
 
 template <int EPI>
-__global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
+__global__ void __launch_bounds__(kOThreads) k_conv_out1(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
                                                         const float* __restrict__ bias, float* __restrict__ pre,
                                                         const float* __restrict__ rho, const float* __restrict__ stats,
                                                         float c, float r, float* __restrict__ rho_add, int n0, int n1,
@@ -836,7 +841,7 @@ __global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict
   Offsets loc;
 #pragma unroll
   for (int u = 0; u < kOPer; ++u) {
-    const int it = min((int)threadIdx.x + u * kThreads, kOItems - 1);
+    const int it = min((int)threadIdx.x + u * kOThreads, kOItems - 1);
     const int h = it % 2, vox = it / 2;
     const int sz = vox % kOSZ, sy = (vox / kOSZ) % kOSY, sx = vox / (kOSZ * kOSY);
     loc[u] = ((sx * n1 + sy) * n2 + sz) * kCout + 8 * h;
@@ -845,7 +850,7 @@ __global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict
   // channel half g % 2; step 0 reads dz = -1 (g < 2) or 0, step 1 reads dz = +1
   const int h = g % 2;
   const int sz0 = g < 2 ? m : m + 1, sz1 = m + 2;
-  constexpr int VPT = (kOBX * kOBY * kOBZ) / kThreads;  // output voxels per thread
+  constexpr int VPT = (kOBX * kOBY * kOBZ) / kOThreads;  // output voxels per thread
   const int G = gridDim.x, per = G / 8, first = (blockIdx.x % 8) * per + blockIdx.x / 8;
   typedef uint32_t Prefetch __attribute__((ext_vector_type(4 * kOPer)));  // a value, never a private array
   int cur = -1, nxt = first;
@@ -856,7 +861,7 @@ __global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict
     if (cur >= 0) {
 #pragma unroll
       for (int s = 0; s < VPT; ++s) {
-        const int v = threadIdx.x + s * kThreads;
+        const int v = threadIdx.x + s * kOThreads;
         const int z = v % kOBZ, y = (v / kOBZ) % kOBY, xx = v / (kOBZ * kOBY);
         const int64_t e = ecur + ((int64_t)xx * n1 + y) * n2 + z;
         rv[s] = EPI == kOutResidual ? rho[e] : 0.f;
@@ -881,7 +886,7 @@ __global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict
       } else {
 #pragma unroll
         for (int u = 0; u < kOPer; ++u) {
-          const int it = min((int)threadIdx.x + u * kThreads, kOItems - 1);
+          const int it = min((int)threadIdx.x + u * kOThreads, kOItems - 1);
           const int hh = it % 2, vox = it / 2;
           const int sz = vox % kOSZ, sy = (vox / kOSZ) % kOSY, sx = vox / (kOSZ * kOSY);
           const int gx = wrap(ox + sx - 1, n0), gy = wrap(oy + sy - 1, n1), gz = wrap(oz + sz - 1, n2);
@@ -897,7 +902,8 @@ __global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict
       f32x4 acc[kOColsPerWave];
 #pragma unroll
       for (int k = 0; k < kOColsPerWave; ++k) {
-        const int col = wave + 4 * k;
+        const int col = wave + kOWaves * k;
+        if (col >= kOCols) break;  // wave-uniform: the last round has fewer columns than waves
         const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(xin + (h * kONS + col * kOSZ + sz0) * 8);
         const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(xin + (h * kONS + col * kOSZ + sz1) * 8);
         f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa0, b0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
@@ -907,7 +913,8 @@ __global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict
       // D over the tile: lds[(col * 9 + t) * 16 + z]; lane (g, m) holds t = 4 g + i of voxel z = m
 #pragma unroll
       for (int k = 0; k < kOColsPerWave; ++k) {
-        const int col = wave + 4 * k;
+        const int col = wave + kOWaves * k;
+        if (col >= kOCols) break;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
           if (4 * g + i < 9) lds[(col * 9 + 4 * g + i) * kOBZ + m] = acc[k][i];
@@ -915,7 +922,7 @@ __global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict
       __syncthreads();
 #pragma unroll
       for (int s = 0; s < VPT; ++s) {
-        const int v = threadIdx.x + s * kThreads;
+        const int v = threadIdx.x + s * kOThreads;
         const int z = v % kOBZ, y = (v / kOBZ) % kOBY, xx = v / (kOBZ * kOBY);
         float o = 0.f;
 #pragma unroll
@@ -935,7 +942,7 @@ __global__ void __launch_bounds__(kThreads) k_conv_out1(const __bf16* __restrict
     __syncthreads();  // the reduction is done with D
 #pragma unroll
     for (int u = 0; u < kOPer; ++u) {
-      const int i = threadIdx.x + u * kThreads;
+      const int i = threadIdx.x + u * kOThreads;
       if (i < kOItems)
         *reinterpret_cast<uint4*>(xin + ((i % 2) * kONS + i / 2) * 8) =
             make_uint4(pf[4 * u], pf[4 * u + 1], pf[4 * u + 2], pf[4 * u + 3]);
@@ -1137,11 +1144,11 @@ unsigned nblocks(const int32_t* shape) {
 // Workgroups of a persistent conv grid: what fits on the device at once (the kernel's
 // occupancy times the CU count), at most the number of blocks, rounded up to a multiple of 8
 // (one share per XCD).
-int persistent_grid(const void* kern, int64_t nblk, unsigned* grid) {
+int persistent_grid(const void* kern, int64_t nblk, unsigned* grid, int threads = conv::kThreads) {
   int dev = 0, cus = 0, per_cu = 0;
   PM_HIP_TRY(hipGetDevice(&dev));
   PM_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  PM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, conv::kThreads, 0));
+  PM_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0));
   int64_t g = (int64_t)std::max(cus, 1) * std::max(per_cu, 1);
   g = std::min<int64_t>(g, nblk);
   *grid = (unsigned)(ceil_div(g, 8) * 8);
@@ -1149,7 +1156,7 @@ int persistent_grid(const void* kern, int64_t nblk, unsigned* grid) {
 }
 
 // The one-output-channel layer with the taps on the matrix rows (k_conv_out1): a persistent
-// grid over the 4 x 8 x 16 blocks (the next block's tile prefetched while one is computed)
+// grid over the 8 x 8 x 16 blocks (the next block's tile prefetched while one is computed)
 int launch_out1(const __bf16* x, const __bf16* w, const float* bias, float* pre, const float* rho,
                 const float* stats, float c, float r, float* rho_add, const int32_t* shape, hipStream_t s) {
   const int64_t nblk = (int64_t)shape[0] / conv::kOBX * (shape[1] / conv::kOBY) * (shape[2] / conv::kOBZ);
@@ -1159,13 +1166,13 @@ int launch_out1(const __bf16* x, const __bf16* w, const float* bias, float* pre,
   unsigned grid = 0;
   if (rho_add) {
     auto kern = conv::k_conv_out1<conv::kOutResidual>;
-    PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblk, &grid));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, x, w, bias, pre, rho, stats, c, r, rho_add,
+    PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblk, &grid, conv::kOThreads));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kOThreads), 0, s, x, w, bias, pre, rho, stats, c, r, rho_add,
                        shape[0], shape[1], shape[2]);
   } else {
     auto kern = conv::k_conv_out1<conv::kPlain>;
-    PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblk, &grid));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, x, w, bias, pre, rho, stats, c, r, rho_add,
+    PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblk, &grid, conv::kOThreads));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kOThreads), 0, s, x, w, bias, pre, rho, stats, c, r, rho_add,
                        shape[0], shape[1], shape[2]);
   }
   PM_LAUNCHED("k_conv_out1", s);
