@@ -308,7 +308,8 @@ int pm_power_spectrum_bwd(const float* spec, const int32_t shape[3], int32_t mas
  *                      pre (float32, the first pre_channels = 1 or 16 channels per voxel) and
  *                      act (bf16, 16 channels) are each optional outputs.
  * pm_conv3d_bwd_data   dx = conv^T(dz; w) (+ add), times gelu'(gact) when gact is given
- *                      (16 -> 16 layers).
+ *                      (16 -> 16 layers; float32 meshes, 16-byte aligned; dx may alias add or
+ *                      gact but not dz).
  * pm_conv3d_bwd_weight dw_db = [dW (27*16*cin floats), db (16 floats)] for pre-activation
  *                      gradient dz and layer input x; workspace from the query.
  * pm_conv3d_stem       act = gelu(conv(x0; w[27][16][8]) + bias) for the network input x0 that

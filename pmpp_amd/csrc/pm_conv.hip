@@ -97,8 +97,9 @@ __device__ __forceinline__ float gelu_fast(float x) {
 // and an epilogue store covers 512 contiguous bytes.
 // Epilogue modes of the forward: kPlain writes pre and/or act; kOutResidual is the output
 // layer fused with the correction's residual stage (inference): rho_add = gain (rho - 1) +
-// amp tanh(pre channel 0), the same arithmetic as k_residual of pm_correction.hip.
-enum Epilogue { kPlain = 0, kOutResidual = 1 };
+// amp tanh(pre channel 0), the same arithmetic as k_residual of pm_correction.hip; kBwdData
+// is a 16 -> 16 layer's data gradient run as a forward convolution (see k_conv_fwd).
+enum Epilogue { kPlain = 0, kOutResidual = 1, kBwdData = 2 };
 
 template <int BZ_>
 struct FwdGeom {
@@ -178,6 +179,8 @@ struct PlaneWalk {
   const float* rho;
   float* rho_add;
   float gain, amp;
+  const float* addf;  // kBwdData: gradient added before the GELU derivative (or null)
+  const float* gact;  // kBwdData: pre-activation whose GELU derivative scales the result (or null)
 
   __device__ __forceinline__ int64_t voxel(int r, int t) const { return el + r * rowz + t * plane; }
 
@@ -209,6 +212,22 @@ struct PlaneWalk {
     // lane holds output channels 4g .. 4g+3 of its voxel in output plane t of row r
     const int64_t e = voxel(r, t);
     const int q = r * BX + t;
+    if constexpr (EPI == kBwdData) {
+      // dx = (sum + add) * gelu'(gact), channels 4g .. 4g+3 of the voxel, float32
+      f32x4 v = c;
+      if (addf) {
+        const f32x4 a = *reinterpret_cast<const f32x4*>(addf + e * kCout + 4 * g);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] += a[i];
+      }
+      if (gact) {
+        const f32x4 ga = *reinterpret_cast<const f32x4*>(gact + e * kCout + 4 * g);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] *= gelu_grad(ga[i]);
+      }
+      *reinterpret_cast<f32x4*>(pre + e * kCout + 4 * g) = v;
+      return;
+    }
     float zv[4];
     bf16x4 rv;
     if constexpr (RES) rv = __builtin_bit_cast(bf16x4, make_uint2(early[2 * q], early[2 * q + 1]));
@@ -277,6 +296,13 @@ struct PlaneWalk {
 // as one plane per 8-channel half ([half][voxel][8]).  The global addresses of a block are the
 // thread's fixed offsets plus the block origin, except for blocks whose halo wraps around the
 // mesh.
+//
+// EPI = kBwdData runs a 16 -> 16 layer's data gradient through the same walk: dx[v] =
+// sum_tap dz[v - off(tap)] w[tap]^T is the forward convolution of dz with the taps mirrored
+// (tap -> 26 - tap, off -> -off) and each tap's weight matrix transposed, so the A fragments
+// are loaded as A[ci][(tap, co)] = w[26 - tap][co][ci] (once per workgroup), x is the float32
+// dz (rounded to bf16 when staged, as the matrix cores read it), there is no bias, and the
+// epilogue writes dx = (sum + addf) * gelu'(gact) to pre as float32.
 template <int CIN, bool GELU, bool RES, int EPI = kPlain, int BZ = 8>
 __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
                                                           const float* __restrict__ bias,
@@ -284,9 +310,15 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
                                                           __bf16* __restrict__ act, int n0, int n1, int n2, int pch,
                                                           const float* __restrict__ rho = nullptr,
                                                           const float* __restrict__ stats = nullptr, float c = 0.f,
-                                                          float r = 0.f, float* __restrict__ rho_add = nullptr) {
+                                                          float r = 0.f, float* __restrict__ rho_add = nullptr,
+                                                          const float* __restrict__ addf = nullptr,
+                                                          const float* __restrict__ gact = nullptr) {
   using Geo = FwdGeom<BZ>;
   using Walk = PlaneWalk<CIN, GELU, RES, EPI, BZ>;
+  constexpr bool BWD = EPI == kBwdData;
+  static_assert(!BWD || (CIN == 16 && !GELU && !RES), "data gradient: 16 -> 16, plain");
+  constexpr int PW = BWD ? 8 : 4;  // prefetch words per staged 8-channel vector (float32 or bf16)
+  const float* __restrict__ xf = reinterpret_cast<const float*>(x);
   using Taps = PlaneTaps<CIN, Geo::SZ>;
   constexpr int TPS = Taps::TPS, KS = Taps::KS;
   constexpr int V = CIN / 8;  // 8-channel halves of a voxel (LDS planes)
@@ -309,13 +341,19 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
     for (int k = 0; k < KS; ++k) {
       const int j = k * TPS + gt;
       const int tap = d * 9 + (j < 9 ? j : 0);
-      bf16x8 v = *reinterpret_cast<const bf16x8*>(w + (tap * kCout + m) * CIN + 8 * gh);
+      if constexpr (BWD) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) wk.wa[d * KS + k][i] = j < 9 ? v[i] : (__bf16)0.f;
+        for (int i = 0; i < 8; ++i)
+          wk.wa[d * KS + k][i] = j < 9 ? w[((kTaps - 1 - tap) * kCout + 8 * gh + i) * kCout + m] : (__bf16)0.f;
+      } else {
+        bf16x8 v = *reinterpret_cast<const bf16x8*>(w + (tap * kCout + m) * CIN + 8 * gh);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) wk.wa[d * KS + k][i] = j < 9 ? v[i] : (__bf16)0.f;
+      }
     }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) wk.bsv[i] = bias[4 * g + i];
+  for (int i = 0; i < 4; ++i) wk.bsv[i] = BWD ? 0.f : bias[4 * g + i];
   // weights and bias are in registers before the block loop starts (otherwise the wait for
   // them at the loop head also waits for the prefetch in flight)
   __builtin_amdgcn_s_waitcnt(0);
@@ -343,6 +381,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
   wk.pch = pch;
   wk.rho = rho;
   wk.rho_add = rho_add;
+  wk.addf = addf;
+  wk.gact = gact;
   if constexpr (EPI == kOutResidual) {
     wk.gain = stats[1];
     wk.amp = r * stats[0] / c;
@@ -363,7 +403,28 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
   const int G = gridDim.x, per = G / 8, first = (blockIdx.x % 8) * per + blockIdx.x / 8;
   // iteration i loads block nxt into registers, computes block cur (staged in LDS by the
   // previous iteration) while those loads are in flight, then stages nxt
-  typedef uint32_t Prefetch __attribute__((ext_vector_type(4 * PER)));  // a value, never a private array
+  typedef uint32_t Prefetch __attribute__((ext_vector_type(PW * PER)));  // a value, never a private array
+  // one staged 8-channel vector at element offset o of x: 16 bytes of bf16, or 32 of float32
+  auto fetch = [&](Prefetch& pf, int u, int64_t o) {
+    if constexpr (BWD) {
+      const float4 t0 = *reinterpret_cast<const float4*>(xf + o);
+      const float4 t1 = *reinterpret_cast<const float4*>(xf + o + 4);
+      pf[8 * u] = __float_as_uint(t0.x);
+      pf[8 * u + 1] = __float_as_uint(t0.y);
+      pf[8 * u + 2] = __float_as_uint(t0.z);
+      pf[8 * u + 3] = __float_as_uint(t0.w);
+      pf[8 * u + 4] = __float_as_uint(t1.x);
+      pf[8 * u + 5] = __float_as_uint(t1.y);
+      pf[8 * u + 6] = __float_as_uint(t1.z);
+      pf[8 * u + 7] = __float_as_uint(t1.w);
+    } else {
+      const uint4 t = *reinterpret_cast<const uint4*>(x + o);
+      pf[4 * u] = t.x;
+      pf[4 * u + 1] = t.y;
+      pf[4 * u + 2] = t.z;
+      pf[4 * u + 3] = t.w;
+    }
+  };
   int cur = -1, nxt = first;
   for (;;) {
     if (cur >= 0) {
@@ -380,15 +441,9 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
       const int ox = bx * BX, oy = by * BY, oz = bz * BZ;
       const bool inner = ox > 0 && ox + BX < n0 && oy > 0 && oy + BY < n1 && oz > 0 && oz + BZ < n2;
       if (inner) {
-        const __bf16* base = x + (((int64_t)(ox - 1) * n1 + (oy - 1)) * n2 + (oz - 1)) * CIN;
+        const int64_t base = (((int64_t)(ox - 1) * n1 + (oy - 1)) * n2 + (oz - 1)) * CIN;
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {  // unconditional (clamped) loads, all in flight together
-          const uint4 t = *reinterpret_cast<const uint4*>(base + loc[u]);
-          pf[4 * u] = t.x;
-          pf[4 * u + 1] = t.y;
-          pf[4 * u + 2] = t.z;
-          pf[4 * u + 3] = t.w;
-        }
+        for (int u = 0; u < PER; ++u) fetch(pf, u, base + loc[u]);  // unconditional (clamped) loads, all in flight
       } else {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
@@ -396,11 +451,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
           const int v = it % V, vox = it / V;
           const int z = vox % SZ, y = (vox / SZ) % SY, xx = vox / (SZ * SY);
           const int gx = wrap(ox + xx - 1, n0), gy = wrap(oy + y - 1, n1), gz = wrap(oz + z - 1, n2);
-          const uint4 t = *reinterpret_cast<const uint4*>(x + (((int64_t)gx * n1 + gy) * n2 + gz) * CIN + 8 * v);
-          pf[4 * u] = t.x;
-          pf[4 * u + 1] = t.y;
-          pf[4 * u + 2] = t.z;
-          pf[4 * u + 3] = t.w;
+          fetch(pf, u, (((int64_t)gx * n1 + gy) * n2 + gz) * CIN + 8 * v);
         }
       }
     }
@@ -413,9 +464,17 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int i = threadIdx.x + u * kThreads;
-      if (i < ITEMS)
-        *reinterpret_cast<uint4*>(xin + ((i % V) * NS + i / V) * 8) =
-            make_uint4(pf[4 * u], pf[4 * u + 1], pf[4 * u + 2], pf[4 * u + 3]);
+      if (i < ITEMS) {
+        if constexpr (BWD) {
+          bf16x8 b;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) b[q] = (__bf16)__uint_as_float(pf[8 * u + q]);
+          *reinterpret_cast<bf16x8*>(xin + ((i % V) * NS + i / V) * 8) = b;
+        } else {
+          *reinterpret_cast<uint4*>(xin + ((i % V) * NS + i / V) * 8) =
+              make_uint4(pf[4 * u], pf[4 * u + 1], pf[4 * u + 2], pf[4 * u + 3]);
+        }
+      }
     }
     __syncthreads();
     cur = nxt;
@@ -958,63 +1017,6 @@ __global__ void __launch_bounds__(kOThreads) k_conv_out1(const __bf16* __restric
   }
 }
 
-// Data gradient of a 16 -> 16 layer: dx[v][ci] = sum_tap sum_co dz[v - off(tap)][co] w[tap][co][ci]
-// (the same implicit GEMM with the taps mirrored and the weight matrix transposed); dz is the
-// float32 gradient of the pre-activations, rounded to bf16 for the matrix cores.  When gact
-// is set, the result is multiplied by gelu'(gact) (the previous layer's pre-activation) so the
-// kernel returns that layer's dz directly; `add` (optional) is added first (residual path).
-__global__ void __launch_bounds__(kThreads) k_conv_bwd_data(const float* __restrict__ dz, const __bf16* __restrict__ w,
-                                                            const float* __restrict__ add,
-                                                            const float* __restrict__ gact, float* __restrict__ dx,
-                                                            int n0, int n1, int n2) {
-  constexpr int CIN = 16, TPS = 2, STEPS = 14;
-  __shared__ __align__(16) __bf16 gin[kH * kH * kH * kCout];
-  __shared__ __align__(16) __bf16 wt[STEPS * TPS * CIN * kCout];  // [tap][ci][co]
-  const int nbz = n2 / kB, nby = n1 / kB;
-  const int blk = blockIdx.x;
-  const int bz = blk % nbz, by = (blk / nbz) % nby, bx = blk / (nbz * nby);
-  const int ox = bx * kB, oy = by * kB, oz = bz * kB;
-  for (int it = threadIdx.x; it < kH * kH * kH * kCout; it += kThreads) {
-    const int c = it % kCout, vox = it / kCout;
-    const int z = vox % kH, y = (vox / kH) % kH, xx = vox / (kH * kH);
-    const int gx = wrap(ox + xx - 1, n0), gy = wrap(oy + y - 1, n1), gz = wrap(oz + z - 1, n2);
-    gin[it] = (__bf16)dz[(((int64_t)gx * n1 + gy) * n2 + gz) * kCout + c];
-  }
-  for (int i = threadIdx.x; i < STEPS * TPS * CIN * kCout; i += kThreads) {
-    const int co = i % kCout, ci = (i / kCout) % CIN, tap = i / (kCout * CIN);
-    wt[i] = tap < kTaps ? w[(tap * kCout + co) * CIN + ci] : (__bf16)0.f;
-  }
-  __syncthreads();
-  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-  const int m = lane % 16, g = lane / 16;
-  const int gt = g / 2, gc = 8 * (g % 2);
-  for (int tile = wave; tile < kVox / 16; tile += 4) {
-    int vx, vy, vz;
-    tile_voxel(tile, m, vx, vy, vz);
-    f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int s = 0; s < STEPS; ++s) {
-      const int tap = s * TPS + gt;
-      const int t = tap < kTaps ? tap : 0;
-      const int dx_ = t / 9 - 1, dy = (t / 3) % 3 - 1, dz_ = t % 3 - 1;
-      // output voxel v receives from v - off(tap)
-      const int lv = ((vx + 1 - dx_) * kH + (vy + 1 - dy)) * kH + (vz + 1 - dz_);
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(gin + lv * kCout + gc);
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(wt + (tap * CIN + m) * kCout + gc);
-      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int qx, qy, qz;
-      tile_voxel(tile, 4 * g + i, qx, qy, qz);
-      const int64_t e = (((int64_t)(ox + qx) * n1 + (oy + qy)) * n2 + (oz + qz)) * CIN + m;
-      float v = c[i];
-      if (add) v += add[e];
-      if (gact) v *= gelu_grad(gact[e]);
-      dx[e] = v;
-    }
-  }
-}
 
 // Weight and bias gradients: dw[tap][co][ci] = sum_v dz[v][co] x[v + off(tap)][ci],
 // db[co] = sum_v dz[v][co].  A persistent grid walks the voxel blocks; each workgroup keeps
@@ -1213,7 +1215,7 @@ int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, 
   const int64_t nblk = nblocks(shape);
   const int key = (cin == 16 ? 4 : 0) + (gelu ? 2 : 0) + (res ? 1 : 0) + (wide_blocks(shape) ? 8 : 0);
   void (*kern)(const __bf16*, const __bf16*, const float*, const __bf16*, float*, __bf16*, int, int, int, int,
-               const float*, const float*, float, float, float*) = nullptr;
+               const float*, const float*, float, float, float*, const float*, const float*) = nullptr;
   switch (key) {
 #define PM_CONV_CASE(K, C, G, R, Z) \
   case K: kern = conv::k_conv_fwd<C, G, R, conv::kPlain, Z>; break;
@@ -1238,7 +1240,7 @@ int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, 
   unsigned grid = 0;
   PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblk, &grid));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, xb, wb, bias, rb, pre, ab, shape[0], shape[1],
-                     shape[2], (int)pre_channels, nullptr, nullptr, 0.f, 0.f, nullptr);
+                     shape[2], (int)pre_channels, nullptr, nullptr, 0.f, 0.f, nullptr, nullptr, nullptr);
   PM_LAUNCHED("k_conv_fwd", s);
   return PM_SUCCESS;
 }
@@ -1308,7 +1310,7 @@ int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, cons
   PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, static_cast<const __bf16*>(x),
                      static_cast<const __bf16*>(w), bias, nullptr, nullptr, nullptr, shape[0], shape[1], shape[2], 1,
-                     rho, stats, 1.5f * omega_m, max_residual, rho_add);
+                     rho, stats, 1.5f * omega_m, max_residual, rho_add, nullptr, nullptr);
   PM_LAUNCHED("k_conv_out_residual", s);
   return PM_SUCCESS;
 }
@@ -1317,9 +1319,22 @@ int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const f
                        const int32_t shape[3], void* stream) {
   PM_TRY(check_shape(shape));
   PM_RETURN_IF(dz == nullptr || w == nullptr || dx == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF(reinterpret_cast<uintptr_t>(dz) % 16 != 0 || reinterpret_cast<uintptr_t>(dx) % 16 != 0 ||
+                   reinterpret_cast<uintptr_t>(add) % 16 != 0 || reinterpret_cast<uintptr_t>(gact) % 16 != 0,
+               PM_ERROR_INVALID_ARGUMENT, "conv3d data gradient: meshes must be 16-byte aligned");
+  PM_RETURN_IF((int64_t)shape[0] * shape[1] * shape[2] / (conv::kB * conv::kB * conv::kB) >= (int64_t(1) << 31) ||
+                   (int64_t)(conv::kB + 2) * shape[1] * shape[2] * conv::kCout >= (int64_t(1) << 31),
+               PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
+  PM_RETURN_IF(dz == dx, PM_ERROR_INVALID_ARGUMENT, "conv3d data gradient: dx must not alias dz");
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(conv::k_conv_bwd_data, dim3(nblocks(shape)), dim3(conv::kThreads), 0, s, dz,
-                     static_cast<const __bf16*>(w), add, gact, dx, shape[0], shape[1], shape[2]);
+  const auto wb = static_cast<const __bf16*>(w);
+  const auto xb = reinterpret_cast<const __bf16*>(dz);  // read as float32 by the kBwdData walk
+  auto kern = wide_blocks(shape) ? conv::k_conv_fwd<16, false, false, conv::kBwdData, 16>
+                                 : conv::k_conv_fwd<16, false, false, conv::kBwdData, 8>;
+  unsigned grid = 0;
+  PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, xb, wb, nullptr, nullptr, dx, nullptr, shape[0],
+                     shape[1], shape[2], (int)conv::kCout, nullptr, nullptr, 0.f, 0.f, nullptr, add, gact);
   PM_LAUNCHED("k_conv_bwd_data", s);
   return PM_SUCCESS;
 }
