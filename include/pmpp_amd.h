@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 6
+#define PM_ABI_VERSION 7
 
 enum pm_status {
   PM_SUCCESS = 0,
@@ -309,7 +309,11 @@ int pm_power_spectrum_bwd(const float* spec, const int32_t shape[3], int32_t mas
  *                      act (bf16, 16 channels) are each optional outputs.
  * pm_conv3d_bwd_data   dx = conv^T(dz; w) (+ add), times gelu'(gact) when gact is given
  *                      (16 -> 16 layers; float32 meshes, 16-byte aligned; dx may alias add or
- *                      gact but not dz).
+ *                      gact but not dz); dx_channels = 16, or 4: only the first four channels
+ *                      of dx are written, as an n x 4 mesh (what the input path reads).
+ * pm_conv3d_bwd_data_out1  the same for a one-output-channel layer (w[27][16][16], channel 0
+ *                      used) from its one-channel gradient dz0 (n floats): dx (n x 16) =
+ *                      conv^T(dz0; w[:, 0, :]) times gelu'(gact) when gact is given.
  * pm_conv3d_bwd_weight dw_db = [dW (27*16*cin floats), db (16 floats)] for pre-activation
  *                      gradient dz and layer input x; workspace from the query.
  * pm_conv3d_stem       act = gelu(conv(x0; w[27][16][8]) + bias) for the network input x0 that
@@ -324,7 +328,9 @@ int pm_power_spectrum_bwd(const float* spec, const int32_t shape[3], int32_t mas
 int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, const void* res, int32_t gelu,
                   float* pre, int32_t pre_channels, void* act, const int32_t shape[3], void* stream);
 int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const float* gact, float* dx,
-                       const int32_t shape[3], void* stream);
+                       int32_t dx_channels, const int32_t shape[3], void* stream);
+int pm_conv3d_bwd_data_out1(const float* dz0, const void* w, const float* gact, float* dx, const int32_t shape[3],
+                            void* stream);
 int pm_conv3d_bwd_weight_workspace(const int32_t shape[3], int32_t cin, size_t* bytes);
 int pm_conv3d_bwd_weight(const float* dz, const void* x, int32_t cin, const int32_t shape[3], float* dw_db,
                          void* workspace, size_t workspace_bytes, void* stream);
@@ -359,9 +365,9 @@ int pm_conv3d_stem_hidden(const float* rho, const float* phi, const int32_t shap
  *                         slabs of it) and only x0 is written.
  * pm_correction_residual  rho_add = g (rho - 1) + max_residual tanh(out) sigma / c, out the
  *                         output layer's float32 pre-activation of channel 0 (n floats).
- * pm_correction_residual_bwd  the output layer's pre-activation gradient dz_out (n x 16,
- *                         channel 0 live) and the gain parameters' gradient (4 floats) from
- *                         rho_cot.
+ * pm_correction_residual_bwd  the output layer's pre-activation gradient dz_out (n x 16 with
+ *                         channel 0 live, or that channel alone, n floats, when dz_channels = 1)
+ *                         and the gain parameters' gradient (4 floats) from rho_cot.
  * pm_correction_input_bwd the cotangents through the network's inputs: rho_bar (n floats,
  *                         overwritten) = d<rho_cot, rho_add>/d rho and omega_cot[0] += its
  *                         Omega_m derivative, from dx0 (n x 4 float32, channels 0..3 of the
@@ -387,7 +393,8 @@ int pm_correction_residual(const float* out, const float* rho, int64_t n, float 
                            const float* stats, float* rho_add, void* stream);
 int pm_correction_residual_bwd(const float* rho_cot, const float* out, const float* rho, int64_t n, float omega_m,
                                float a, float sigma8, float max_residual, const float* stats, float* dz_out,
-                               float* gain_grad, void* workspace, size_t workspace_bytes, void* stream);
+                               int32_t dz_channels, float* gain_grad, void* workspace, size_t workspace_bytes,
+                               void* stream);
 int pm_correction_input_bwd(const float* rho_cot, const float* out, const float* rho, const float* phi,
                             const float* dx0, int64_t n, float omega_m, float max_residual, const float* stats,
                             const float* gain_params, float* rho_bar, float* phi_bar, float* omega_cot,

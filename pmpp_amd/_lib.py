@@ -14,7 +14,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 PM_SUCCESS = 0
 PM_ERROR_INVALID_ARGUMENT = 1
@@ -37,7 +37,8 @@ EXPORTED_SYMBOLS = (
     "pm_potential_workspace", "pm_gravity_potential",
     "pm_kick", "pm_drift", "pm_nbody_adj_workspace", "pm_kick_adj", "pm_drift_adj",
     "pm_power_spectrum", "pm_power_spectrum_bwd",
-    "pm_conv3d_fwd", "pm_conv3d_bwd_data", "pm_conv3d_bwd_weight_workspace", "pm_conv3d_bwd_weight",
+    "pm_conv3d_fwd", "pm_conv3d_bwd_data", "pm_conv3d_bwd_data_out1", "pm_conv3d_bwd_weight_workspace",
+    "pm_conv3d_bwd_weight",
     "pm_conv3d_stem", "pm_conv3d_out_residual", "pm_conv3d_stem_hidden",
     "pm_correction_workspace", "pm_correction_features", "pm_correction_residual", "pm_correction_residual_bwd",
     "pm_correction_input_bwd", "pm_force_loss", "pm_adamax_step", "pm_to_bf16",
@@ -135,7 +136,8 @@ _SIGNATURES = {
     "pm_power_spectrum": ([_vp, _vp, _i32x3, _i32, _i32, _i32, _vp, _vp], ctypes.c_int),
     "pm_power_spectrum_bwd": ([_vp, _i32x3, _i32, _i32, _i32, _vp, _vp, _vp], ctypes.c_int),
     "pm_conv3d_fwd": ([_vp, _i32, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32x3, _vp], ctypes.c_int),
-    "pm_conv3d_bwd_data": ([_vp, _vp, _vp, _vp, _vp, _i32x3, _vp], ctypes.c_int),
+    "pm_conv3d_bwd_data": ([_vp, _vp, _vp, _vp, _vp, _i32, _i32x3, _vp], ctypes.c_int),
+    "pm_conv3d_bwd_data_out1": ([_vp, _vp, _vp, _vp, _i32x3, _vp], ctypes.c_int),
     "pm_conv3d_bwd_weight_workspace": ([_i32x3, _i32, _sz_p], ctypes.c_int),
     "pm_conv3d_bwd_weight": ([_vp, _vp, _i32, _i32x3, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_conv3d_stem": ([_vp, _vp, _i32x3, _f32, _f32, _f32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
@@ -146,7 +148,7 @@ _SIGNATURES = {
     "pm_correction_features": ([_vp, _vp, _i64, _f32, _f32, _f32, _f32, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
                                ctypes.c_int),
     "pm_correction_residual": ([_vp, _vp, _i64, _f32, _f32, _vp, _vp, _vp], ctypes.c_int),
-    "pm_correction_residual_bwd": ([_vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _vp, _vp, _vp, _vp,
+    "pm_correction_residual_bwd": ([_vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _vp, _vp, _i32, _vp, _vp,
                                     ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_correction_input_bwd": ([_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _vp, _vp, _vp, _vp, _vp, _vp,
                                  ctypes.c_size_t, _vp], ctypes.c_int),

@@ -36,7 +36,7 @@ import torch
 
 from .. import _lib
 from .._torch import ensure_init, ptr, require_device, stream_handle, workspace
-from .conv import conv3d, conv3d_bwd_data, conv3d_bwd_weight
+from .conv import conv3d, conv3d_bwd_data, conv3d_bwd_data_out1, conv3d_bwd_weight
 
 CHANNELS = 16
 IN_CHANNELS = 8
@@ -346,8 +346,10 @@ def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, gr
     parameter layout); written into ``grads`` when given.  With ``inputs`` also the cotangents
     through the network's inputs: returns ``(grads, rho_bar, omega_bar)`` with ``rho_bar`` the
     density cotangent (mesh) and ``omega_bar`` a device float (``pm_correction_input_bwd``).
-    With ``input_grad`` it returns only the gradient of the network input x0 (16-channel
-    float32 mesh, channels 0..7 live); ``weights=False`` skips the weight gradients."""
+    With ``input_grad`` it returns only the gradient of the network input x0's first four
+    channels (a float32 (X, Y, Z, 4) mesh, what the input path reads); ``weights=False`` skips
+    the weight gradients (the output layer's gradient then stays one channel wide:
+    ``pm_conv3d_bwd_data_out1``)."""
     dev = tape.rho.device
     n = tape.rho.numel()
     shape = tuple(tape.rho.shape)
@@ -358,11 +360,13 @@ def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, gr
     if rc.dtype != torch.float32 or tuple(rc.shape) != shape:
         raise ValueError("rho_cot must be a float32 mesh shaped like rho")
     ws = _workspace(dev)
-    dz = torch.empty(shape + (CHANNELS,), dtype=torch.float32, device=dev)
+    # the output layer's weight gradient reads its 16-channel dz; without it one channel suffices
+    dz_ch = CHANNELS if weights else 1
+    dz = torch.empty(shape + ((CHANNELS,) if weights else ()), dtype=torch.float32, device=dev)
     g = model.gain_offset if weights else 0
     _lib.call("pm_correction_residual_bwd", ptr(rc), ptr(tape.out), ptr(tape.rho), n, tape.omega_m, tape.a,
-              float(model.sigma8_value), float(model.max_residual), ptr(tape.stats), ptr(dz), ptr(grads[g:g + 4]),
-              ptr(ws), ws.numel(), stream_handle(dev))
+              float(model.sigma8_value), float(model.max_residual), ptr(tape.stats), ptr(dz), dz_ch,
+              ptr(grads[g:g + 4]), ptr(ws), ws.numel(), stream_handle(dev))
     layers = model.layers
 
     def wgrad(layer, dzl, x):
@@ -371,7 +375,10 @@ def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, gr
 
     # output layer: input acts[-1] = gelu(pres[-1])
     wgrad(layers[-1], dz, tape.acts[-1])
-    dz = conv3d_bwd_data(dz, model.weight(layers[-1]), gact=tape.pres[-1])
+    if weights:
+        dz = conv3d_bwd_data(dz, model.weight(layers[-1]), gact=tape.pres[-1])
+    else:
+        dz = conv3d_bwd_data_out1(dz, model.weight(layers[-1]), gact=tape.pres[-1])
     for i in reversed(range(model.depth - 1)):
         if model.block == "plain":
             # dz is the gradient of block i's pre-activation; its input is acts[i] = gelu(pres[i])
@@ -389,11 +396,11 @@ def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, gr
     wgrad(layers[0], dz, tape.x0)
     if not inputs and not input_grad:
         return grads
-    dx0 = conv3d_bwd_data(dz, _stem_weight_16(model))
+    dx0 = conv3d_bwd_data(dz, _stem_weight_16(model), channels=4)
     if input_grad:
         return dx0
-    rho_bar, omega_bar = _input_bwd(model, tape.rho, rc, tape.out, tape.stats, dx0[..., :4].contiguous(),
-                                    tape.omega_m, tape.phi, tape.conf)
+    rho_bar, omega_bar = _input_bwd(model, tape.rho, rc, tape.out, tape.stats, dx0, tape.omega_m, tape.phi,
+                                    tape.conf)
     return grads, rho_bar, omega_bar
 
 
@@ -432,8 +439,9 @@ def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_
     ``tape``) and its data backward run on the extended slab as if it were periodic.  Each
     conv reaches one plane further, so after the 2 x layers convolutions of the forward and
     the backward only the outer H planes hold wrapped values and the slab's own planes equal
-    the whole-mesh computation.  The input gradient of those planes is kept (4 of its 16
-    channels); the global terms (through sigma, the gain and Omega_m) follow on the whole mesh."""
+    the whole-mesh computation.  The input gradient of those planes is kept (its first 4
+    channels, all the input path reads); the global terms (through sigma, the gain and Omega_m)
+    follow on the whole mesh."""
     require_device(rho, "rho")
     dev = rho.device
     shape = tuple(rho.shape)
@@ -449,7 +457,7 @@ def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_
     phi = tape.phi
     if slab >= n0:
         _, full = evaluate_residual(model, rho, a, om, keep=True, stats=tape.stats, phi=phi)
-        dx0c.copy_(residual_backward(model, full, rc, weights=False, input_grad=True)[..., :4])
+        dx0c.copy_(residual_backward(model, full, rc, weights=False, input_grad=True))
         del full
     else:
         for x0 in range(0, n0, slab):
@@ -460,7 +468,7 @@ def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_
             _, ext = evaluate_residual(model, rho_e, a, om, keep=True, stats=tape.stats, phi=phi_e)
             dx0_e = residual_backward(model, ext, rc.index_select(0, idx), weights=False, input_grad=True)
             del ext
-            dx0c[x0:x0 + s] = dx0_e[halo:halo + s, ..., :4]
+            dx0c[x0:x0 + s] = dx0_e[halo:halo + s]
             del dx0_e, rho_e, phi_e
     return _input_bwd(model, rho, rc, tape.out, tape.stats, dx0c, om, phi, tape.conf)
 

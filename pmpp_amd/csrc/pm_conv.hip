@@ -213,7 +213,9 @@ struct PlaneWalk {
     const int64_t e = voxel(r, t);
     const int q = r * BX + t;
     if constexpr (EPI == kBwdData) {
-      // dx = (sum + add) * gelu'(gact), channels 4g .. 4g+3 of the voxel, float32
+      // dx = (sum + add) * gelu'(gact), channels 4g .. 4g+3 of the voxel, float32; with pch = 4
+      // only channels 0 .. 3 are written (dx has 4 channels per voxel)
+      if (pch != kCout && g != 0) return;
       f32x4 v = c;
       if (addf) {
         const f32x4 a = *reinterpret_cast<const f32x4*>(addf + e * kCout + 4 * g);
@@ -225,7 +227,7 @@ struct PlaneWalk {
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] *= gelu_grad(ga[i]);
       }
-      *reinterpret_cast<f32x4*>(pre + e * kCout + 4 * g) = v;
+      *reinterpret_cast<f32x4*>(pre + e * pch + 4 * g) = v;
       return;
     }
     float zv[4];
@@ -302,7 +304,8 @@ struct PlaneWalk {
 // (tap -> 26 - tap, off -> -off) and each tap's weight matrix transposed, so the A fragments
 // are loaded as A[ci][(tap, co)] = w[26 - tap][co][ci] (once per workgroup), x is the float32
 // dz (rounded to bf16 when staged, as the matrix cores read it), there is no bias, and the
-// epilogue writes dx = (sum + addf) * gelu'(gact) to pre as float32.
+// epilogue writes dx = (sum + addf) * gelu'(gact) to pre as float32 (pch = 16 channels per
+// voxel, or 4: only the first four).
 template <int CIN, bool GELU, bool RES, int EPI = kPlain, int BZ = 8>
 __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
                                                           const float* __restrict__ bias,
@@ -521,25 +524,44 @@ __device__ __forceinline__ bf16x8 stem_fragment(const __bf16* __restrict__ w, in
   return f;
 }
 
-template <bool PHI>
+// The same one-channel walk computes the data gradient of the one-output-channel layer (MODE
+// kStemBwdOut1): dx[v][ci] = gelu'(gact[v][ci]) sum_tap dz0[v - off(tap)] w[tap][0][ci] is a
+// one-channel convolution of dz0 with the taps mirrored, so lane (g, m) holds
+// w[26 - tap][0][m] for taps 8g .. 8g+7, the staged tile is bf16(dz0), and the epilogue writes
+// float32 (no bias, no GELU).
+enum StemMode { kStemDensity = 0, kStemPotential = 1, kStemBwdOut1 = 2 };
+
+template <int MODE>
 __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restrict__ rho,
                                                            const float* __restrict__ phi, float c,
                                                            const float* __restrict__ stats, float a, float om,
                                                            float s8, const __bf16* __restrict__ w,
                                                            const float* __restrict__ bias, __bf16* __restrict__ act,
-                                                           int n0, int n1, int n2) {
+                                                           int n0, int n1, int n2, float* __restrict__ dxo = nullptr,
+                                                           const float* __restrict__ gact = nullptr) {
+  constexpr bool PHI = MODE == kStemPotential;
+  constexpr bool BWD1 = MODE == kStemBwdOut1;
   constexpr int NS = kH * kH * kH;
   constexpr int PER = (NS + kThreads - 1) / kThreads;
   constexpr int NT = PHI ? 2 : 1;  // staged input channels
   __shared__ __align__(16) __bf16 xin[NT][PER * kThreads];
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int m = lane % 16, g = lane / 16;
-  const bf16x8 wa = stem_fragment(w, 0, g, m);
+  bf16x8 wa;
+  if constexpr (BWD1) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int tap = 8 * g + j;
+      wa[j] = tap < kTaps ? w[((kTaps - 1 - tap) * kCout) * kCout + m] : (__bf16)0.f;
+    }
+  } else {
+    wa = stem_fragment(w, 0, g, m);
+  }
   const bf16x8 wp = PHI ? stem_fragment(w, 1, g, m) : bf16x8{};
   float bsv[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) bsv[i] = stem_bias<PHI>(w, bias, 4 * g + i, a, om, s8);
-  const float inv = 1.f / stats[0];
+  for (int i = 0; i < 4; ++i) bsv[i] = BWD1 ? 0.f : stem_bias<PHI>(w, bias, 4 * g + i, a, om, s8);
+  const float inv = BWD1 ? 1.f : 1.f / stats[0];
   const float pinv = PHI ? 1.f / stats[3] : 0.f;
   __builtin_amdgcn_s_waitcnt(0);
   // staged offsets of this lane's 8 taps (padded taps read tap 26's voxel; zero weights)
@@ -610,10 +632,20 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
           cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wp, bp, cv, 0, 0, 0);
         }
         const int64_t e = el + q * plane;
-        bf16x4 o;
+        if constexpr (BWD1) {
+          f32x4 v = cv;
+          if (gact) {
+            const f32x4 ga = *reinterpret_cast<const f32x4*>(gact + e * kCout + 4 * g);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i] + bsv[i]);
-        *reinterpret_cast<bf16x4*>(act + e * kCout + 4 * g) = o;
+            for (int i = 0; i < 4; ++i) v[i] *= gelu_grad(ga[i]);
+          }
+          *reinterpret_cast<f32x4*>(dxo + e * kCout + 4 * g) = v;
+        } else {
+          bf16x4 o;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i] + bsv[i]);
+          *reinterpret_cast<bf16x4*>(act + e * kCout + 4 * g) = o;
+        }
       }
     }
     if (nxt >= nblk) break;
@@ -622,7 +654,7 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
     for (int u = 0; u < PER; ++u) {
       const int i = threadIdx.x + u * kThreads;
       if (i < NS) {
-        xin[0][i] = (__bf16)(((pf[u] - 1.f) * c) * inv);
+        xin[0][i] = BWD1 ? (__bf16)pf[u] : (__bf16)(((pf[u] - 1.f) * c) * inv);
         if constexpr (PHI) xin[1][i] = (__bf16)(pp[u] * pinv);
       }
     }
@@ -641,7 +673,7 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
 // k_conv_stem, so the activations are k_conv_stem's bit for bit), writes their GELU as bf16
 // into the tile and walks the hidden conv over it exactly as k_conv_fwd does (PHI: the
 // potential's tile is staged beside the density's and the stem adds its second MFMA step, as
-// k_conv_stem<true>).  The stem
+// k_conv_stem<kStemPotential>).  The stem
 // activation (32 bytes per voxel written, then read back 2.8x over by the hidden layer's halo
 // tiles) never reaches HBM; the price is the stem's MFMA on 2.1x the block's voxels, one MFMA
 // per 16 of them.  Persistent and prefetching (the next block's density tile in registers
@@ -1256,11 +1288,11 @@ int pm_conv3d_stem(const float* rho, const float* phi, const int32_t shape[3], f
                PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
   hipStream_t s = as_stream(stream);
   unsigned grid = 0;
-  auto kern = phi ? conv::k_conv_stem<true> : conv::k_conv_stem<false>;
+  auto kern = phi ? conv::k_conv_stem<conv::kStemPotential> : conv::k_conv_stem<conv::kStemDensity>;
   PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, rho, phi, 1.5f * omega_m, stats, a, omega_m,
                      sigma8, static_cast<const __bf16*>(w), bias, static_cast<__bf16*>(act), shape[0], shape[1],
-                     shape[2]);
+                     shape[2], nullptr, nullptr);
   PM_LAUNCHED("k_conv_stem", s);
   return PM_SUCCESS;
 }
@@ -1316,9 +1348,10 @@ int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, cons
 }
 
 int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const float* gact, float* dx,
-                       const int32_t shape[3], void* stream) {
+                       int32_t dx_channels, const int32_t shape[3], void* stream) {
   PM_TRY(check_shape(shape));
   PM_RETURN_IF(dz == nullptr || w == nullptr || dx == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF(dx_channels != 16 && dx_channels != 4, PM_ERROR_INVALID_ARGUMENT, "dx_channels must be 16 or 4");
   PM_RETURN_IF(reinterpret_cast<uintptr_t>(dz) % 16 != 0 || reinterpret_cast<uintptr_t>(dx) % 16 != 0 ||
                    reinterpret_cast<uintptr_t>(add) % 16 != 0 || reinterpret_cast<uintptr_t>(gact) % 16 != 0,
                PM_ERROR_INVALID_ARGUMENT, "conv3d data gradient: meshes must be 16-byte aligned");
@@ -1334,8 +1367,28 @@ int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const f
   unsigned grid = 0;
   PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, xb, wb, nullptr, nullptr, dx, nullptr, shape[0],
-                     shape[1], shape[2], (int)conv::kCout, nullptr, nullptr, 0.f, 0.f, nullptr, add, gact);
+                     shape[1], shape[2], (int)dx_channels, nullptr, nullptr, 0.f, 0.f, nullptr, add, gact);
   PM_LAUNCHED("k_conv_bwd_data", s);
+  return PM_SUCCESS;
+}
+
+int pm_conv3d_bwd_data_out1(const float* dz0, const void* w, const float* gact, float* dx, const int32_t shape[3],
+                            void* stream) {
+  PM_TRY(check_shape(shape));
+  PM_RETURN_IF(dz0 == nullptr || w == nullptr || dx == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL argument");
+  PM_RETURN_IF(reinterpret_cast<uintptr_t>(dx) % 16 != 0 || reinterpret_cast<uintptr_t>(gact) % 16 != 0,
+               PM_ERROR_INVALID_ARGUMENT, "dx and gact must be 16-byte aligned");
+  PM_RETURN_IF(dz0 == dx, PM_ERROR_INVALID_ARGUMENT, "dx must not alias dz0");
+  PM_RETURN_IF((int64_t)shape[0] * shape[1] * shape[2] / (conv::kB * conv::kB * conv::kB) >= (int64_t(1) << 31) ||
+                   (int64_t)conv::kH * shape[1] * shape[2] >= (int64_t(1) << 31),
+               PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
+  hipStream_t s = as_stream(stream);
+  auto kern = conv::k_conv_stem<conv::kStemBwdOut1>;
+  unsigned grid = 0;
+  PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, dz0, nullptr, 0.f, nullptr, 0.f, 0.f, 0.f,
+                     static_cast<const __bf16*>(w), nullptr, nullptr, shape[0], shape[1], shape[2], dx, gact);
+  PM_LAUNCHED("k_conv_bwd_out1", s);
   return PM_SUCCESS;
 }
 

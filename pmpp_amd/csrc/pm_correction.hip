@@ -128,13 +128,15 @@ __global__ void __launch_bounds__(kBlock) k_residual(const float* __restrict__ o
     rho_add[i] = g * (rho[i] - 1.f) + amp * tanhf(out[i]);
 }
 
-// dz (the output layer's pre-activation gradient, 16 channels, channel 0 live) from rho_cot,
-// and per-block partials of sum(rho_cot (rho - 1)) for the gain.
+// dz (the output layer's pre-activation gradient: 16 channels with channel 0 live, or that
+// channel alone when one_channel) from rho_cot, and per-block partials of
+// sum(rho_cot (rho - 1)) for the gain.
 __global__ void __launch_bounds__(kBlock) k_residual_bwd(const float* __restrict__ rho_cot,
                                                          const float* __restrict__ out,
                                                          const float* __restrict__ rho, int64_t n, float c, float r,
-                                                         const float* __restrict__ stats, float4* __restrict__ dz,
-                                                         double* __restrict__ partials) {
+                                                         const float* __restrict__ stats, float* __restrict__ dzf,
+                                                         int one_channel, double* __restrict__ partials) {
+  float4* __restrict__ dz = reinterpret_cast<float4*>(dzf);
   __shared__ double lds[kBlock / 64];
   const float amp = r * stats[0] / c;
   double acc = 0.0;
@@ -142,8 +144,13 @@ __global__ void __launch_bounds__(kBlock) k_residual_bwd(const float* __restrict
     const float rc = rho_cot[i];
     const float t = tanhf(out[i]);
     acc += (double)(rc * (rho[i] - 1.f));
+    const float d0 = rc * amp * (1.f - t * t);
+    if (one_channel) {  // uniform
+      dzf[i] = d0;
+      continue;
+    }
     const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-    dz[i * 4 + 0] = make_float4(rc * amp * (1.f - t * t), 0.f, 0.f, 0.f);
+    dz[i * 4 + 0] = make_float4(d0, 0.f, 0.f, 0.f);
     dz[i * 4 + 1] = zero;
     dz[i * 4 + 2] = zero;
     dz[i * 4 + 3] = zero;
@@ -445,8 +452,10 @@ int pm_correction_residual(const float* out, const float* rho, int64_t n, float 
 
 int pm_correction_residual_bwd(const float* rho_cot, const float* out, const float* rho, int64_t n, float omega_m,
                                float a, float sigma8, float max_residual, const float* stats, float* dz_out,
-                               float* gain_grad, void* workspace, size_t workspace_bytes, void* stream) {
+                               int32_t dz_channels, float* gain_grad, void* workspace, size_t workspace_bytes,
+                               void* stream) {
   PM_TRY(check_mesh(n));
+  PM_RETURN_IF(dz_channels != 1 && dz_channels != 16, PM_ERROR_INVALID_ARGUMENT, "dz_channels must be 1 or 16");
   PM_RETURN_IF(rho_cot == nullptr || out == nullptr || rho == nullptr || stats == nullptr || dz_out == nullptr ||
                    gain_grad == nullptr,
                PM_ERROR_INVALID_ARGUMENT, "NULL argument");
@@ -458,7 +467,7 @@ int pm_correction_residual_bwd(const float* rho_cot, const float* out, const flo
   double* part = static_cast<double*>(workspace);
   const unsigned nb = corr::grid_for(n);
   hipLaunchKernelGGL(corr::k_residual_bwd, dim3(nb), dim3(corr::kBlock), 0, s, rho_cot, out, rho, n,
-                     1.5f * omega_m, max_residual, stats, reinterpret_cast<float4*>(dz_out), part);
+                     1.5f * omega_m, max_residual, stats, dz_out, dz_channels == 1 ? 1 : 0, part);
   PM_LAUNCHED("k_residual_bwd", s);
   hipLaunchKernelGGL(corr::k_gain_bwd, dim3(1), dim3(64), 0, s, part, (int)nb, max_residual, stats, a, omega_m,
                      sigma8, gain_grad);

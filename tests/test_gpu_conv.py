@@ -10,6 +10,7 @@ import torch
 import torch.nn.functional as Fn
 
 from pmpp_amd.corrections import conv3d, conv3d_bwd_data, conv3d_bwd_weight
+from pmpp_amd.corrections.conv import conv3d_bwd_data_out1
 
 pytestmark = pytest.mark.gpu
 
@@ -68,8 +69,9 @@ def test_forward_persistent_grid_walks_every_block(device, cin, res, shape):
                                atol=1e-2 * scale)
 
 
-def test_backward_data_and_weight_match_autograd(device):
-    shape = (16, 16, 8)
+@pytest.mark.parametrize("shape", [(16, 16, 8), (8, 16, 32), (24, 8, 48)])
+def test_backward_data_and_weight_match_autograd(device, shape):
+    # (16, 16, 8) walks 8^3 blocks, the others 4 x 8 x 16 blocks (x a multiple of 4, z of 16)
     g = torch.Generator().manual_seed(3)
     x = _bf(torch.randn(shape + (16,), generator=g))
     w = _bf(torch.randn(27, 16, 16, generator=g) * 0.2)
@@ -91,6 +93,36 @@ def test_backward_data_and_weight_match_autograd(device):
     t = torch.tanh(0.7978845608028654 * (gact + 0.044715 * gact ** 3))
     gg = 0.5 * (1 + t) + 0.5 * gact * (1 - t * t) * 0.7978845608028654 * (1 + 3 * 0.044715 * gact ** 2)
     np.testing.assert_allclose(dx2.cpu().numpy(), ((xr.grad + add) * gg).numpy(), rtol=1e-4, atol=1e-4)
+    # only the first four channels (the input path's): the same values
+    dx4 = conv3d_bwd_data(dz.to(device), w.to(device, torch.bfloat16), add=add.to(device), gact=gact.to(device),
+                          channels=4)
+    assert tuple(dx4.shape) == shape + (4,)
+    np.testing.assert_array_equal(dx4.cpu().numpy(), dx2[..., :4].cpu().numpy())
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 24, 32)])
+def test_one_channel_layer_data_gradient(device, shape):
+    """pm_conv3d_bwd_data_out1 (the data gradient of the one-output-channel layer from its
+    one-channel gradient) against autograd of the float32 conv with the same bf16 operands, and
+    against the 16 -> 16 data gradient of a dz whose channel 0 is that gradient."""
+    g = torch.Generator().manual_seed(6)
+    w = _bf(torch.randn(27, 16, 16, generator=g) * 0.2)
+    dz0 = _bf(torch.randn(shape, generator=g))
+    gact = torch.randn(shape + (16,), generator=g)
+    xr = torch.zeros(shape + (16,), requires_grad=True)
+    w1 = torch.zeros(27, 16, 16)
+    w1[:, 0, :] = w[:, 0, :]
+    y = _ref_conv(xr, w1, torch.zeros(16))
+    dz16 = torch.zeros(shape + (16,))
+    dz16[..., 0] = dz0
+    y.backward(dz16)
+    t = torch.tanh(0.7978845608028654 * (gact + 0.044715 * gact ** 3))
+    gg = 0.5 * (1 + t) + 0.5 * gact * (1 - t * t) * 0.7978845608028654 * (1 + 3 * 0.044715 * gact ** 2)
+    got = conv3d_bwd_data_out1(dz0.to(device), w.to(device, torch.bfloat16), gact=gact.to(device)).cpu()
+    ref = xr.grad * gg
+    np.testing.assert_allclose(got.numpy(), ref.numpy(), rtol=1e-4, atol=1e-5 * ref.abs().max().item())
+    wide = conv3d_bwd_data(dz16.to(device), w.to(device, torch.bfloat16), gact=gact.to(device)).cpu()
+    np.testing.assert_allclose(got.numpy(), wide.numpy(), rtol=1e-5, atol=1e-6 * wide.abs().max().item())
 
 
 def test_weight_gradient_cin8(device):
