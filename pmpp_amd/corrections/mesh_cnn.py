@@ -251,7 +251,7 @@ def _potential(model, rho, om, conf, phi):
 
 
 def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=False, stats=None, keep_out=False,
-                      conf=None, phi=None):
+                      conf=None, phi=None, output=None):
     """Density residual ``rho_add`` of the correction for the density mesh ``rho`` (X, Y, Z).
 
     Returns ``(rho_add, tape)``; ``tape`` is None unless ``keep`` (then it holds the
@@ -259,7 +259,11 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     (the [sigma, gain, tanh, sigma_phi] device floats of a forward on the whole mesh) are used
     instead of this mesh's own: ``rho`` is then a slab of that mesh.  A model with the potential
     channel needs ``conf`` (the force step's configuration, for the potential of ``rho``) or
-    ``phi`` (the potential mesh, or the slab of it that matches ``rho``)."""
+    ``phi`` (the potential mesh, or the slab of it that matches ``rho``).  With ``keep`` and
+    ``output`` (this mesh's output-layer pre-activation, e.g. the slab of a whole-mesh
+    ``OutputTape.out``) the output layer and the activation only it reads are not computed:
+    the tape then serves the inputs' cotangent (``residual_backward(weights=False)``) and
+    ``rho_add`` is None."""
     require_device(rho, "rho")
     if rho.dtype != torch.float32 or rho.ndim != 3:
         raise ValueError("rho must be a float32 (X, Y, Z) mesh")
@@ -280,12 +284,14 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     pres, acts = [], []
     gain = None if given else ptr(model.gain_params)
     fused = (not keep and model.block == "plain" and model.depth >= 2 and shape[0] % 4 == 0 and shape[2] % 16 == 0)
+    skip_out = keep and output is not None  # the last activation feeds only the output layer
     if keep:
         # the backward's stem weight gradient needs the network input x0
         x0 = torch.empty(shape + (IN_CHANNELS,), dtype=torch.bfloat16, device=dev)
         _lib.call("pm_correction_features", ptr(rho), ptr(phi), n, om, a, s8, float(model.max_residual), gain,
                   ptr(x0), ptr(stats), ptr(ws), ws.numel(), s)
-        pre, x = conv3d(x0, model.weight(layers[0]), model.bias(layers[0]), gelu=True, want_pre=True)
+        pre, x = conv3d(x0, model.weight(layers[0]), model.bias(layers[0]), gelu=True, want_pre=True,
+                        want_act=not (skip_out and model.depth == 1))
     else:
         # inference: the stem reads the density directly (constant input channels folded
         # into its bias), so x0 is never written
@@ -305,17 +311,23 @@ def evaluate_residual(model: MeshCNNPotentialCorrection, rho, a, omega_m, keep=F
     pres.append(pre)
     acts.append(x)
     for i in range(1 if fused else 0, model.depth - 1):
+        last_act = not (skip_out and i == model.depth - 2)
         if model.block == "plain":
             l1 = layers[1 + i]
-            p1, x = conv3d(x, model.weight(l1), model.bias(l1), gelu=True, want_pre=keep)
+            p1, x = conv3d(x, model.weight(l1), model.bias(l1), gelu=True, want_pre=keep, want_act=last_act)
             pres.append(p1)
             acts.append(x)
             continue
         l1, l2 = layers[1 + 2 * i], layers[2 + 2 * i]
         p1, h = conv3d(x, model.weight(l1), model.bias(l1), gelu=True, want_pre=keep)
-        p2, x = conv3d(h, model.weight(l2), model.bias(l2), res=x, gelu=True, want_pre=keep)
+        p2, x = conv3d(h, model.weight(l2), model.bias(l2), res=x, gelu=True, want_pre=keep, want_act=last_act)
         pres += [p1, p2]
         acts += [h, x]
+    if skip_out:
+        out = output.contiguous()
+        if out.dtype != torch.float32 or tuple(out.shape[:3]) != shape:
+            raise ValueError("output must be a float32 mesh shaped like rho")
+        return None, Tape(rho, stats, x0, pres, acts, out, a, om, phi, conf)
     rho_add = torch.empty_like(rho)
     if not keep and not keep_out:
         # inference: the output layer writes rho_add directly (its pre-activation is not kept)
@@ -435,11 +447,12 @@ def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_
     Each slab of ``slab`` x planes (a multiple of 8; default: the largest whose extended slab
     holds at most ``SLAB_CELLS`` cells, or the whole mesh when that fits) is extended by a halo
     of H planes on both sides (wrapped periodically), H = 2 x the number of conv layers rounded
-    up to a multiple of 8; the network's forward (with the whole mesh's statistics from
-    ``tape``) and its data backward run on the extended slab as if it were periodic.  Each
-    conv reaches one plane further, so after the 2 x layers convolutions of the forward and
-    the backward only the outer H planes hold wrapped values and the slab's own planes equal
-    the whole-mesh computation.  The input gradient of those planes is kept (its first 4
+    up to a multiple of 8; the network's forward up to the output layer (with the whole
+    mesh's statistics from ``tape``; the output layer's pre-activation is the slab of the whole
+    mesh's, ``tape.out``) and its data backward run on the extended slab as if it were periodic.
+    Each conv reaches one plane further, so after the at most 2 x layers convolutions of the
+    forward and the backward only the outer H planes hold wrapped values and the slab's own
+    planes equal the whole-mesh computation.  The input gradient of those planes is kept (its first 4
     channels, all the input path reads); the global terms (through sigma, the gain and Omega_m)
     follow on the whole mesh."""
     require_device(rho, "rho")
@@ -456,7 +469,7 @@ def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_
     dx0c = torch.empty(shape + (4,), dtype=torch.float32, device=dev)
     phi = tape.phi
     if slab >= n0:
-        _, full = evaluate_residual(model, rho, a, om, keep=True, stats=tape.stats, phi=phi)
+        _, full = evaluate_residual(model, rho, a, om, keep=True, stats=tape.stats, phi=phi, output=tape.out)
         dx0c.copy_(residual_backward(model, full, rc, weights=False, input_grad=True))
         del full
     else:
@@ -465,7 +478,8 @@ def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_
             idx = torch.arange(x0 - halo, x0 + s + halo, device=dev).remainder_(n0)
             rho_e = rho.index_select(0, idx)
             phi_e = None if phi is None else phi.index_select(0, idx)
-            _, ext = evaluate_residual(model, rho_e, a, om, keep=True, stats=tape.stats, phi=phi_e)
+            _, ext = evaluate_residual(model, rho_e, a, om, keep=True, stats=tape.stats, phi=phi_e,
+                                       output=tape.out.index_select(0, idx))
             dx0_e = residual_backward(model, ext, rc.index_select(0, idx), weights=False, input_grad=True)
             del ext
             dx0c[x0:x0 + s] = dx0_e[halo:halo + s]
