@@ -142,13 +142,37 @@ def gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot):
     return disp_cot, omega_cot, rho_cot
 
 
-def corrected_gravity(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrection):
+class KeptCorrection:
+    """The density, the network's residual and its output tape of one corrected force
+    evaluation (``corrected_gravity(..., kept=)``), for ``corrected_gravity_vjp`` at the same
+    particles, scale factor and Omega_m: the reverse sweep recomputes the force at the
+    reconstructed positions and pulls the next cotangent back there (integrator.py:781-837),
+    so the VJP need not run the density and the network's forward again.  ``key`` names the
+    particle arrays, a and Omega_m; None means nothing usable is held."""
+
+    def __init__(self):
+        self.key = None
+        self.rho = self.rho_add = self.tape = None
+
+    @staticmethod
+    def key_of(ptcl, a, omega):
+        from ..nbody.gravity import KeptForce
+
+        return KeptForce.key_of(ptcl, omega) + (float(a),)
+
+    def clear(self):
+        self.key = None
+        self.rho = self.rho_add = self.tape = None
+
+
+def corrected_gravity(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrection, kept: KeptCorrection = None):
     """Coarse-mesh accelerations with the learned correction (no gradient).
 
     One force step: the density (``pm_gravity_density``), the network on it, then the same
     step resumed with the residual (``pm_gravity_corrected_resume``), which reuses the
     bucketed particles and the density the first call left in the workspace (held here
-    across the network's own allocations)."""
+    across the network's own allocations).  With ``kept`` the density, the residual and the
+    network's output tape are kept there for a following ``corrected_gravity_vjp``."""
     dev = ptcl.disp.device
     ensure_init(dev)
     d = gravity_desc(conf, _omega(cosmo))
@@ -161,26 +185,35 @@ def corrected_gravity(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrecti
     s = stream_handle(dev)
     rho = torch.empty(tuple(int(v) for v in conf.mesh_shape), dtype=torch.float32, device=dev)
     _lib.call("pm_gravity_density", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(rho), ptr(ws), nbytes, s)
-    rho_add, _ = evaluate_residual(correction, rho, a, _omega(cosmo), conf=conf)
-    del rho
+    if kept is None:
+        rho_add, _ = evaluate_residual(correction, rho, a, _omega(cosmo), conf=conf)
+        del rho
+    else:
+        kept.clear()
+        rho_add, tape = evaluate_residual(correction, rho, a, _omega(cosmo), keep_out=True, conf=conf)
+        kept.rho, kept.rho_add, kept.tape = rho, rho_add, tape
     acc = torch.empty((n, 3), dtype=torch.float32, device=dev)
     _lib.call("pm_gravity_corrected_resume", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(rho_add), ptr(acc), ptr(ws),
               nbytes, s)
+    if kept is not None:
+        kept.key = KeptCorrection.key_of(ptcl, a, _omega(cosmo))
     return acc
 
 
 def corrected_gravity_vjp(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrection, acc_cot, slab=None,
-                          param_grads=None):
+                          param_grads=None, kept: KeptCorrection = None):
     """(disp_cot, omega_cot float) of ``corrected_gravity``: the force step's adjoint at the
     corrected source (``pm_gravity_corrected_vjp``) plus the path through the network's inputs
     (``pm_correction_input_bwd``), whose density cotangent reaches the particles through the
     CIC scatter adjoint.  The network's reverse runs over x-slabs (``network_input_vjp``);
-    with ``param_grads`` it runs on the whole mesh and also adds the parameters' gradient."""
+    with ``param_grads`` it runs on the whole mesh and also adds the parameters' gradient.  A
+    ``kept`` evaluation of these same particles (``corrected_gravity(..., kept=)``) supplies the
+    density, the residual and the network's output tape instead of recomputing them."""
     from ..cic.kernels import hip_scatter_bwd
 
     om = _omega(cosmo)
-    rho = density(ptcl, cosmo, conf)
     if param_grads is not None:
+        rho = density(ptcl, cosmo, conf)
         rho_add, full = evaluate_residual(correction, rho, a, om, keep=True, conf=conf)
         disp_cot, omega_cot, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
         del rho_add
@@ -188,7 +221,12 @@ def corrected_gravity_vjp(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorr
         del full
         param_grads += g
     else:
-        rho_add, tape = evaluate_residual(correction, rho, a, om, keep_out=True, conf=conf)
+        if kept is not None and kept.key is not None and kept.key == KeptCorrection.key_of(ptcl, a, om):
+            rho, rho_add, tape = kept.rho, kept.rho_add, kept.tape
+            kept.clear()
+        else:
+            rho = density(ptcl, cosmo, conf)
+            rho_add, tape = evaluate_residual(correction, rho, a, om, keep_out=True, conf=conf)
         disp_cot, omega_cot, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
         del rho_add
         rho_bar, omega_bar = network_input_vjp(correction, rho, rho_cot, a, om, tape, slab=slab)

@@ -170,6 +170,9 @@ class AdjointState:
         from .gravity import KeptForce
 
         self.kept = KeptForce()  # forward meshes of the last force recompute, for the next VJP
+        from ..corrections.training import KeptCorrection
+
+        self.kept_corr = KeptCorrection()  # the same for a corrected force: density, residual, tape
         # with a learned correction: accumulate the gradient of its parameters here (flat
         # float32, the parameter layout) when set
         self.param_grads = None
@@ -199,6 +202,7 @@ def drift_adj_from_output(a_vel, a_prev, a_next, st: AdjointState, cosmo, conf):
     """Reverse a drift from its output: disp -= D vel, vel_cot += D disp_cot (:429-474)."""
     D = drift_factor(a_vel, a_prev, a_next, cosmo, conf)
     st.kept.key = None  # the positions change under any kept force meshes
+    st.kept_corr.clear()
     _lib.call("pm_drift_adj", ptr(st.disp), ptr(st.vel), ptr(st.disp_cot), ptr(st.vel_cot), _n(st.ptcl), _f32(D),
               ptr(st.dot), ptr(st.ws), st.ws_bytes, st.stream)
     st.add_dot(_factor_dOm("drift_factor", a_vel, a_prev, a_next, cosmo, conf))
@@ -215,7 +219,8 @@ def force_adj(a, st: AdjointState, cosmo, conf, correction=None):
         from ..corrections.training import corrected_gravity_vjp
 
         d_disp, d_om = corrected_gravity_vjp(a, st.particles(), cosmo, conf, correction, st.acc_cot,
-                                             param_grads=st.param_grads)
+                                             param_grads=st.param_grads,
+                                             kept=None if st.param_grads is not None else st.kept_corr)
     st.disp_cot += d_disp
     st.omega_cot_dev += d_om
     st.acc_cot.zero_()
@@ -238,4 +243,9 @@ def integrate_adj(a_prev, a_next, st: AdjointState, cosmo, conf, correction=None
 
             st.acc = gravity_keep(st.particles(), cosmo, conf, st.kept).contiguous()
         else:
-            st.acc = force(a_acc_in, st.particles(), cosmo, conf, correction=correction).acc.contiguous()
+            from ..corrections.training import corrected_gravity
+
+            # the density, residual and network tape stay for the next stage's VJP at these positions
+            keep = st.kept_corr if st.param_grads is None else None
+            with torch.no_grad():
+                st.acc = corrected_gravity(a_acc_in, st.particles(), cosmo, conf, correction, kept=keep).contiguous()
