@@ -247,6 +247,16 @@ int pm_gravity_corrected_vjp(const int16_t* pmid, const float* disp, const uint8
                              float* disp_cot, float* omega_cot, float* rho_cot, void* workspace,
                              size_t workspace_bytes, void* stream);
 
+/* pm_gravity_density_cot adds the CIC scatter adjoint's displacement term for a density
+ * cotangent dens_cot (the real mesh; cic/pallas.py:358-409 with the particle value N_m/N_p) to
+ * disp_cot (n x 3, caller order; invalid particles' rows are left as they are): the particles
+ * are binned by brick and the term is taken in the tiled column pass the VJP uses for its own
+ * density term.  The workspace is that of pm_gravity_workspace.  The corrected force's reverse
+ * uses it for the density cotangent through the network's inputs, known only after the VJP. */
+int pm_gravity_density_cot(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                           const pm_gravity_desc* desc, const float* dens_cot, float* disp_cot, void* workspace,
+                           size_t workspace_bytes, void* stream);
+
 /* The particle-Nyquist-filtered potential of a real mesh, on the force step's spectral band:
  *   out = add + irfftn(mask(k) * (-1/k^2) * rfftn(scale * (in + shift)))
  * (nbody/gravity.py:174-196, 263-321, 626-633: the potential the reference's corrected force

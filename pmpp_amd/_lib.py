@@ -34,7 +34,7 @@ EXPORTED_SYMBOLS = (
     "pm_gravity_workspace", "pm_gravity_vjp_workspace", "pm_gravity", "pm_gravity_vjp",
     "pm_gravity_keep", "pm_gravity_vjp_resume",
     "pm_gravity_density", "pm_gravity_corrected", "pm_gravity_corrected_resume", "pm_gravity_corrected_vjp",
-    "pm_potential_workspace", "pm_gravity_potential",
+    "pm_gravity_density_cot", "pm_potential_workspace", "pm_gravity_potential",
     "pm_kick", "pm_drift", "pm_nbody_adj_workspace", "pm_kick_adj", "pm_drift_adj",
     "pm_power_spectrum", "pm_power_spectrum_bwd",
     "pm_conv3d_fwd", "pm_conv3d_bwd_data", "pm_conv3d_bwd_data_out1", "pm_conv3d_bwd_weight_workspace",
@@ -125,6 +125,8 @@ _SIGNATURES = {
                               _vp], ctypes.c_int),
     "pm_gravity_corrected_vjp": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, _vp, _vp, _vp,
                                   ctypes.c_size_t, _vp], ctypes.c_int),
+    "pm_gravity_density_cot": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, _vp, ctypes.c_size_t,
+                                _vp], ctypes.c_int),
     "pm_potential_workspace": ([ctypes.POINTER(GravityDesc), _sz_p], ctypes.c_int),
     "pm_gravity_potential": ([_vp, _f32, _f32, _vp, ctypes.POINTER(GravityDesc), _vp, _vp, ctypes.c_size_t, _vp],
                              ctypes.c_int),

@@ -142,6 +142,28 @@ def gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot):
     return disp_cot, omega_cot, rho_cot
 
 
+def density_cot_add(ptcl, cosmo, conf, dens_cot, disp_cot):
+    """disp_cot += the CIC scatter adjoint's displacement term for the density cotangent
+    ``dens_cot`` (``pm_gravity_density_cot``: the particles binned by brick, the tiled column
+    pass); invalid particles' rows are left as they are.  Returns ``disp_cot``."""
+    dev = ptcl.disp.device
+    ensure_init(dev)
+    d = gravity_desc(conf, _omega(cosmo))
+    pm = as_contig(ptcl.pmid, torch.int16, "pmid")
+    dp = as_contig(ptcl.disp.detach(), torch.float32, "disp")
+    n = _check_particles(pm, dp)
+    dc = as_contig(dens_cot, torch.float32, "dens_cot")
+    if tuple(dc.shape) != tuple(int(v) for v in conf.mesh_shape):
+        raise ValueError("dens_cot must be a mesh of the configuration's shape")
+    if disp_cot.dtype != torch.float32 or tuple(disp_cot.shape) != (n, 3) or not disp_cot.is_contiguous():
+        raise ValueError("disp_cot must be a contiguous float32 (N, 3) tensor")
+    nbytes = _lib.size_query("pm_gravity_workspace", d, n)
+    ws = workspace(nbytes, dev)
+    _lib.call("pm_gravity_density_cot", ptr(pm), ptr(dp), ptr(_valid_u8(ptcl)), n, d, ptr(dc), ptr(disp_cot),
+              ptr(ws), nbytes, stream_handle(dev))
+    return disp_cot
+
+
 class KeptCorrection:
     """The density, the network's residual and its output tape of one corrected force
     evaluation (``corrected_gravity(..., kept=)``), for ``corrected_gravity_vjp`` at the same
@@ -205,12 +227,11 @@ def corrected_gravity_vjp(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorr
     """(disp_cot, omega_cot float) of ``corrected_gravity``: the force step's adjoint at the
     corrected source (``pm_gravity_corrected_vjp``) plus the path through the network's inputs
     (``pm_correction_input_bwd``), whose density cotangent reaches the particles through the
-    CIC scatter adjoint.  The network's reverse runs over x-slabs (``network_input_vjp``);
-    with ``param_grads`` it runs on the whole mesh and also adds the parameters' gradient.  A
+    CIC scatter adjoint (``density_cot_add``).  The network's reverse runs over x-slabs
+    (``network_input_vjp``); with ``param_grads`` it runs on the whole mesh and also adds the
+    parameters' gradient.  A
     ``kept`` evaluation of these same particles (``corrected_gravity(..., kept=)``) supplies the
     density, the residual and the network's output tape instead of recomputing them."""
-    from ..cic.kernels import hip_scatter_bwd
-
     om = _omega(cosmo)
     if param_grads is not None:
         rho = density(ptcl, cosmo, conf)
@@ -230,11 +251,8 @@ def corrected_gravity_vjp(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorr
         disp_cot, omega_cot, rho_cot = gravity_with_residual_vjp(ptcl, cosmo, conf, rho_add, acc_cot)
         del rho_add
         rho_bar, omega_bar = network_input_vjp(correction, rho, rho_cot, a, om, tape, slab=slab)
-    valid = _valid_u8(ptcl)
-    dc, _ = hip_scatter_bwd(ptcl.pmid, ptcl.disp.detach(), conf.mesh_size / conf.ptcl_num, rho_bar, offset=0.0,
-                            particle_cell_size=conf.cell_size, cell_size=None, global_shape=tuple(conf.mesh_shape),
-                            valid_mask=None if valid is None else valid.bool())
-    return disp_cot + dc, float(omega_cot.item()) + float(omega_bar.item())
+    density_cot_add(ptcl, cosmo, conf, rho_bar, disp_cot)
+    return disp_cot, float(omega_cot.item()) + float(omega_bar.item())
 
 
 def force_loss(acc, target, want_grad=True):

@@ -601,7 +601,7 @@ __global__ void __launch_bounds__(kTileThreads) k_disp_cot_columns(
     const int16_t* __restrict__ pmid, const float* __restrict__ disp, const int32_t* __restrict__ bin_off,
     const int32_t* __restrict__ perm, Geom g, Bricks k, const float* __restrict__ f0, const float* __restrict__ f1,
     const float* __restrict__ f2, const float* __restrict__ dcot, const float* __restrict__ cot_s, int64_t n,
-    float val, float* __restrict__ disp_cot) {
+    float val, float* __restrict__ disp_cot, int accum) {
   constexpr int kLds = BX ? (BX + 1) * (BY + 1) * (BZ + 2) : kTileMaxFloats;
   constexpr int kPl = BX ? (BX + 1) * (BY + 1) : kPlaneMax;
   __shared__ __align__(16) float tile[kLds];
@@ -612,8 +612,9 @@ __global__ void __launch_bounds__(kTileThreads) k_disp_cot_columns(
   const int cx = col / k.nb[1], cy = col - cx * k.nb[1];
   const int rows = B.rows(), P = B.pitch(), nb2 = k.nb[2];
   const int nm = val != 0.f ? 4 : 3;
+  const int m0 = f0 ? 0 : 3;  // the first mesh: the force components, or the density cotangent alone
   // the brick above the top one is brick 0 (periodic)
-  for (int m = 0; m < nm; ++m)
+  for (int m = m0; m < nm; ++m)
     load_plane(plane[nb2 & 1][m], m == 0 ? f0 : (m == 1 ? f1 : (m == 2 ? f2 : dcot)), B, k, cx, cy, 0);
   for (int bz = nb2 - 1; bz >= 0; --bz) {
     const int o[3] = {cx * B.b0, cy * B.b1, bz * B.b2};
@@ -622,7 +623,7 @@ __global__ void __launch_bounds__(kTileThreads) k_disp_cot_columns(
     const int up = (bz + 1) & 1, me = bz & 1;
     if (j0 == j1) {  // no particles: only this brick's lower planes are needed (by the brick below)
       __syncthreads();
-      for (int m = 0; m < nm; ++m)
+      for (int m = m0; m < nm; ++m)
         load_plane(plane[me][m], m == 0 ? f0 : (m == 1 ? f1 : (m == 2 ? f2 : dcot)), B, k, cx, cy, bz);
       continue;
     }
@@ -639,7 +640,7 @@ __global__ void __launch_bounds__(kTileThreads) k_disp_cot_columns(
         c0[q] = local_cell(B, bs, o);
       }
 #pragma unroll 1
-      for (int m = 0; m < nm; ++m) {
+      for (int m = m0; m < nm; ++m) {
         const float* __restrict__ F = m == 0 ? f0 : (m == 1 ? f1 : (m == 2 ? f2 : dcot));
         __syncthreads();  // the previous mesh's pass is done with the tile
         stage_rows<BX, BY, BZ>(tile, F, B, k, cx, cy, bz);
@@ -678,7 +679,7 @@ __global__ void __launch_bounds__(kTileThreads) k_disp_cot_columns(
             gs[2] = gs[2] + v * (w[0][bx] * w[1][by] * sg[2][bzz]);
           }
 #pragma unroll
-          for (int a = 0; a < 3; ++a) res[a][jl] = (m == 0 ? 0.f : res[a][jl]) + cf[q] * gs[a];
+          for (int a = 0; a < 3; ++a) res[a][jl] = (m == m0 ? 0.f : res[a][jl]) + cf[q] * gs[a];
         }
       }
       __syncthreads();
@@ -687,9 +688,8 @@ __global__ void __launch_bounds__(kTileThreads) k_disp_cot_columns(
         const int jl = threadIdx.x + q * kTileThreads;
         if (s0 + jl < s1) {
           const int64_t i = perm[s0 + jl];
-          disp_cot[i * 3 + 0] = res[0][jl] / g.a1;
-          disp_cot[i * 3 + 1] = res[1][jl] / g.a1;
-          disp_cot[i * 3 + 2] = res[2][jl] / g.a1;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) disp_cot[i * 3 + a] = (accum ? disp_cot[i * 3 + a] : 0.f) + res[a][jl] / g.a1;
         }
       }
     }
@@ -806,7 +806,8 @@ __global__ void __launch_bounds__(kBlock) k_scatter_bwd_sorted(
   }
   int64_t i = perm[j];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) disp_cot[i * 3 + a] = disp_cot_s[j * 3 + a] + dc[a] / g.a1;
+  for (int a = 0; a < 3; ++a)  // added to the gather's terms (disp_cot_s), or to disp_cot itself when null
+    disp_cot[i * 3 + a] = (disp_cot_s ? disp_cot_s[j * 3 + a] : disp_cot[i * 3 + a]) + dc[a] / g.a1;
 }
 
 // omega_cot += 1.5 * sum_x src_cot(x) * (rho(x) [+ rho_add(x)] - 1); dens_cot = src_cot * scale,
@@ -1061,9 +1062,10 @@ int scatter_tiled(const pm_gravity_desc* d, const Buffers& b, float* mesh, const
   return PM_SUCCESS;
 }
 
-int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
-                  const pm_gravity_desc* d, const Layout& L, Buffers& b, Arena& ws, hipStream_t s) {
-  const Geom g = step_geom(d);
+// The particles binned by brick into the workspace's sorted copies (b.perm, b.bins, b.pmid_s,
+// b.disp_s, b.valid_s).
+int bucket_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                 const pm_gravity_desc* d, Buffers& b, Arena& ws, hipStream_t s) {
   // the Lagrangian grid the particle count implies (mesh * cell / spacing per axis)
   int32_t grid[3] = {0, 0, 0};
   if (d->ptcl_spacing > 0.f)
@@ -1071,8 +1073,14 @@ int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, 
       const double v = (double)d->mesh_shape[a] * d->cell_size / d->ptcl_spacing;
       grid[a] = (v >= 1.0 && v < 1e9) ? (int32_t)lrint(v) : 0;
     }
-  PM_TRY(sort::bucket_cells(pmid, disp, valid, n, d->mesh_shape, d->cell_size, d->brick, grid, b.perm, b.bins,
-                            b.pmid_s, b.disp_s, b.valid_s, ws, s));
+  return sort::bucket_cells(pmid, disp, valid, n, d->mesh_shape, d->cell_size, d->brick, grid, b.perm, b.bins,
+                            b.pmid_s, b.disp_s, b.valid_s, ws, s);
+}
+
+int density_stage(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                  const pm_gravity_desc* d, const Layout& L, Buffers& b, Arena& ws, hipStream_t s) {
+  const Geom g = step_geom(d);
+  PM_TRY(bucket_stage(pmid, disp, valid, n, d, b, ws, s));
   if (b.tiled) return scatter_tiled(d, b, b.r0, nullptr, nullptr, s);
   PM_TRY(fill(b.r0, L.nreal, s));
   if (n > 0) {
@@ -1124,10 +1132,10 @@ int disp_cot_tiled(const pm_gravity_desc* d, const Buffers& b, int64_t n, const 
   const float val = d->particle_value;
   if (is_brick(k, 16, 16, 32))
     hipLaunchKernelGGL((k_disp_cot_columns<16, 16, 32>), dim3(ncols), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
-                       b.bins, b.perm, g, k, b.g0, b.r1, b.r2, b.g1, b.cot_s, n, val, disp_cot);
+                       b.bins, b.perm, g, k, b.g0, b.r1, b.r2, b.g1, b.cot_s, n, val, disp_cot, 0);
   else
     hipLaunchKernelGGL((k_disp_cot_columns<0, 0, 0>), dim3(ncols), dim3(kTileThreads), 0, s, b.pmid_s, b.disp_s,
-                       b.bins, b.perm, g, k, b.g0, b.r1, b.r2, b.g1, b.cot_s, n, val, disp_cot);
+                       b.bins, b.perm, g, k, b.g0, b.r1, b.r2, b.g1, b.cot_s, n, val, disp_cot, 0);
   PM_LAUNCHED("k_disp_cot_columns", s);
   if (valid) {
     unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), 4096);
@@ -1574,6 +1582,42 @@ int pm_gravity_vjp_resume(const int16_t* pmid, const float* disp, const uint8_t*
                           void* workspace, size_t workspace_bytes, void* stream) {
   return vjp_impl(pmid, disp, valid, n, desc, nullptr, acc_cot, disp_cot, omega_cot, nullptr, workspace,
                   workspace_bytes, stream, true);
+}
+
+int pm_gravity_density_cot(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
+                           const pm_gravity_desc* desc, const float* dens_cot, float* disp_cot, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+  hipStream_t s = as_stream(stream);
+  grav::Layout L;
+  grav::Buffers b{};
+  Arena ws(workspace, workspace_bytes);
+  fft::SpecTables tb;
+  int st = grav::prepare(desc, n, workspace, workspace_bytes, false, L, b, ws, tb, s);
+  if (st) return st;
+  PM_RETURN_IF(n > 0 && (pmid == nullptr || disp == nullptr || dens_cot == nullptr || disp_cot == nullptr),
+               PM_ERROR_INVALID_ARGUMENT, "pmid/disp/dens_cot/disp_cot NULL");
+  if (n == 0) return PM_SUCCESS;
+  PM_TRY(grav::bucket_stage(pmid, disp, valid, n, desc, b, ws, s));
+  const Geom g = grav::step_geom(desc);
+  const float val = desc->particle_value;
+  if (b.tiled) {  // invalid particles are in no bin: their disp_cot stays as it is
+    grav::Bricks k = grav::make_bricks(desc->mesh_shape, desc->brick);
+    const unsigned ncols = (unsigned)((int64_t)k.nb[0] * k.nb[1]);
+    if (grav::is_brick(k, 16, 16, 32))
+      hipLaunchKernelGGL((grav::k_disp_cot_columns<16, 16, 32>), dim3(ncols), dim3(grav::kTileThreads), 0, s,
+                         b.pmid_s, b.disp_s, b.bins, b.perm, g, k, nullptr, nullptr, nullptr, dens_cot, nullptr, n,
+                         val, disp_cot, 1);
+    else
+      hipLaunchKernelGGL((grav::k_disp_cot_columns<0, 0, 0>), dim3(ncols), dim3(grav::kTileThreads), 0, s, b.pmid_s,
+                         b.disp_s, b.bins, b.perm, g, k, nullptr, nullptr, nullptr, dens_cot, nullptr, n, val,
+                         disp_cot, 1);
+    PM_LAUNCHED("k_disp_cot_columns", s);
+    return PM_SUCCESS;
+  }
+  hipLaunchKernelGGL(grav::k_scatter_bwd_sorted, dim3(grav::grid_for(n)), dim3(grav::kBlock), 0, s, b.pmid_s,
+                     b.disp_s, b.valid_s, b.perm, n, g, val, dens_cot, nullptr, disp_cot);
+  PM_LAUNCHED("k_scatter_bwd_sorted", s);
+  return PM_SUCCESS;
 }
 
 int pm_potential_workspace(const pm_gravity_desc* desc, size_t* bytes) {

@@ -191,6 +191,44 @@ def test_force_vjp_across_brick_tilings(device, brick, clustered, valid_frac):
         assert np.all(got[~valid] == 0)
 
 
+@pytest.mark.parametrize("brick,clustered,valid_frac", [
+    ((8, 8, 8), False, 0.8),
+    ((16, 16, 32), False, None),
+    ((16, 16, 32), True, 0.9),
+    ((3, 3, 3), False, 0.8),
+])
+def test_density_cotangent_pass_matches_oracle_scatter_adjoint(device, brick, clustered, valid_frac):
+    """pm_gravity_density_cot (the particles binned by brick, the tiled density term of the
+    VJP's displacement pass, added to disp_cot) against the float64 oracle's CIC scatter adjoint
+    with the step's particle value; invalid particles' rows keep what disp_cot held."""
+    from oracle import cic
+    from pmpp_amd.corrections.training import density_cot_add
+
+    conf = Configuration(2.0, (16, 16, 16), mesh_shape=2, brick=brick)
+    rng = np.random.default_rng(37)
+    n = 16 ** 3
+    if clustered:
+        pm = rng.integers(0, 12, (n, 3))
+        base = Particles(conf, torch.as_tensor(pm.astype(np.int16), device=device),
+                         torch.as_tensor(rng.uniform(-1.5, 1.5, (n, 3)).astype(np.float32), device=device))
+    else:
+        base = _ptcl(conf, 0.6, 41, device)
+    unused = None if valid_frac is None else torch.as_tensor(rng.random(n) > valid_frac, device=device)
+    ptcl = base.replace(unused_index=unused)
+    dens_cot = rng.normal(size=conf.mesh_shape).astype(np.float32)
+    start = rng.normal(size=(n, 3)).astype(np.float32)
+    got = density_cot_add(ptcl, 0.3, conf, torch.from_numpy(dens_cot).to(device),
+                          torch.from_numpy(start).to(device)).cpu().numpy().astype(np.float64)
+    valid = None if unused is None else ~unused.cpu().numpy()
+    pm64, dp64 = ptcl.pmid.cpu().numpy().astype(np.int64), ptcl.disp.cpu().numpy().astype(np.float64)
+    val = spectral.default_value(n, conf.mesh_shape, conf.ptcl_num, valid)
+    dref, _ = cic.scatter_adj(pm64, dp64, val, dens_cot.astype(np.float64), conf.cell_size, conf.mesh_shape)
+    ref = start.astype(np.float64) + dref
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-5 * np.abs(dref).max())
+    if valid is not None:
+        np.testing.assert_array_equal(got[~valid], start[~valid])
+
+
 def test_force_and_vjp_on_1024_point_z_lines(device):
     # an 8 x 8 x 1024 mesh: the banded z passes of 1024-point lines (r2c, the derivative c2r,
     # and the adjoint's plain c2r) against the oracle
