@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 7
+#define PM_ABI_VERSION 8
 
 enum pm_status {
   PM_SUCCESS = 0,
@@ -318,12 +318,16 @@ int pm_power_spectrum_bwd(const float* spec, const int32_t shape[3], int32_t mas
  *                      pre (float32, the first pre_channels = 1 or 16 channels per voxel) and
  *                      act (bf16, 16 channels) are each optional outputs.
  * pm_conv3d_bwd_data   dx = conv^T(dz; w) (+ add), times gelu'(gact) when gact is given
- *                      (16 -> 16 layers; float32 meshes, 16-byte aligned; dx may alias add or
- *                      gact but not dz); dx_channels = 16, or 4: only the first four channels
- *                      of dx are written, as an n x 4 mesh (what the input path reads).
+ *                      (16 -> 16 layers; meshes 16-byte aligned; dx may alias add or gact but
+ *                      not dz); dx_channels = 16, or 4: only the first four channels of dx are
+ *                      written, as an n x 4 mesh (what the input path reads).  bf16_io: 0 =
+ *                      dz, add and dx float32; 1 = dz and add bf16, dx float32; 3 = dz, add
+ *                      and dx bf16 (the matrix cores read dz as bf16 in every case; gact is
+ *                      float32).
  * pm_conv3d_bwd_data_out1  the same for a one-output-channel layer (w[27][16][16], channel 0
- *                      used) from its one-channel gradient dz0 (n floats): dx (n x 16) =
- *                      conv^T(dz0; w[:, 0, :]) times gelu'(gact) when gact is given.
+ *                      used) from its one-channel gradient dz0 (n floats): dx (n x 16, float32,
+ *                      or bf16 when dx_bf16) = conv^T(dz0; w[:, 0, :]) times gelu'(gact) when
+ *                      gact is given.
  * pm_conv3d_bwd_weight dw_db = [dW (27*16*cin floats), db (16 floats)] for pre-activation
  *                      gradient dz and layer input x; workspace from the query.
  * pm_conv3d_stem       act = gelu(conv(x0; w[27][16][8]) + bias) for the network input x0 that
@@ -337,10 +341,10 @@ int pm_power_spectrum_bwd(const float* spec, const int32_t shape[3], int32_t mas
  *                      channel 0) sigma / c, without writing the output pre-activation. */
 int pm_conv3d_fwd(const void* x, int32_t cin, const void* w, const float* bias, const void* res, int32_t gelu,
                   float* pre, int32_t pre_channels, void* act, const int32_t shape[3], void* stream);
-int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const float* gact, float* dx,
-                       int32_t dx_channels, const int32_t shape[3], void* stream);
-int pm_conv3d_bwd_data_out1(const float* dz0, const void* w, const float* gact, float* dx, const int32_t shape[3],
-                            void* stream);
+int pm_conv3d_bwd_data(const void* dz, const void* w, const void* add, const float* gact, void* dx,
+                       int32_t dx_channels, int32_t bf16_io, const int32_t shape[3], void* stream);
+int pm_conv3d_bwd_data_out1(const float* dz0, const void* w, const float* gact, void* dx, int32_t dx_bf16,
+                            const int32_t shape[3], void* stream);
 int pm_conv3d_bwd_weight_workspace(const int32_t shape[3], int32_t cin, size_t* bytes);
 int pm_conv3d_bwd_weight(const float* dz, const void* x, int32_t cin, const int32_t shape[3], float* dw_db,
                          void* workspace, size_t workspace_bytes, void* stream);

@@ -14,7 +14,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 PM_SUCCESS = 0
 PM_ERROR_INVALID_ARGUMENT = 1
@@ -138,8 +138,8 @@ _SIGNATURES = {
     "pm_power_spectrum": ([_vp, _vp, _i32x3, _i32, _i32, _i32, _vp, _vp], ctypes.c_int),
     "pm_power_spectrum_bwd": ([_vp, _i32x3, _i32, _i32, _i32, _vp, _vp, _vp], ctypes.c_int),
     "pm_conv3d_fwd": ([_vp, _i32, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32x3, _vp], ctypes.c_int),
-    "pm_conv3d_bwd_data": ([_vp, _vp, _vp, _vp, _vp, _i32, _i32x3, _vp], ctypes.c_int),
-    "pm_conv3d_bwd_data_out1": ([_vp, _vp, _vp, _vp, _i32x3, _vp], ctypes.c_int),
+    "pm_conv3d_bwd_data": ([_vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32x3, _vp], ctypes.c_int),
+    "pm_conv3d_bwd_data_out1": ([_vp, _vp, _vp, _vp, _i32, _i32x3, _vp], ctypes.c_int),
     "pm_conv3d_bwd_weight_workspace": ([_i32x3, _i32, _sz_p], ctypes.c_int),
     "pm_conv3d_bwd_weight": ([_vp, _vp, _i32, _i32x3, _vp, _vp, ctypes.c_size_t, _vp], ctypes.c_int),
     "pm_conv3d_stem": ([_vp, _vp, _i32x3, _f32, _f32, _f32, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),

@@ -39,27 +39,35 @@ def conv3d(x, w, b, res=None, gelu=False, want_pre=True, want_act=True, pre_chan
     return pre, act
 
 
-def conv3d_bwd_data(dz, w, add=None, gact=None, channels=16):
+def conv3d_bwd_data(dz, w, add=None, gact=None, channels=16, dtype=torch.float32):
     """dx = conv^T(dz; w) (+ add), times gelu'(gact) if given (16 -> 16 layers); ``channels=4``
-    returns only the first four channels of dx (an (X, Y, Z, 4) mesh)."""
+    returns only the first four channels of dx (an (X, Y, Z, 4) mesh).  dz (and add) may be
+    float32 or bf16; ``dtype`` is dx's (bf16 only from a bf16 dz)."""
     shape = _shape(dz)
-    dx = torch.empty(shape + (channels,), dtype=torch.float32, device=dz.device)
+    if dz.dtype not in (torch.float32, torch.bfloat16):
+        raise ValueError("dz must be float32 or bf16")
+    if add is not None and add.dtype != dz.dtype:
+        raise ValueError("add must have dz's dtype")
+    bf16_io = (1 if dz.dtype == torch.bfloat16 else 0) | (2 if dtype == torch.bfloat16 else 0)
+    if bf16_io == 2:
+        raise ValueError("a bf16 dx needs a bf16 dz")
+    dx = torch.empty(shape + (channels,), dtype=dtype, device=dz.device)
     _lib.call("pm_conv3d_bwd_data", ptr(dz.contiguous()), ptr(w.contiguous()),
               ptr(add.contiguous()) if add is not None else None, ptr(gact.contiguous()) if gact is not None else None,
-              ptr(dx), int(channels), _lib.int3(shape), stream_handle(dz.device))
+              ptr(dx), int(channels), bf16_io, _lib.int3(shape), stream_handle(dz.device))
     return dx
 
 
-def conv3d_bwd_data_out1(dz0, w, gact=None):
+def conv3d_bwd_data_out1(dz0, w, gact=None, dtype=torch.float32):
     """dx (X, Y, Z, 16) of a one-output-channel layer (weights [27][16][16], channel 0) from its
-    one-channel gradient dz0 (X, Y, Z), times gelu'(gact) if given."""
+    one-channel gradient dz0 (X, Y, Z), times gelu'(gact) if given; ``dtype`` float32 or bf16."""
     if dz0.ndim != 3 or dz0.dtype != torch.float32:
         raise ValueError("dz0 must be a float32 (X, Y, Z) mesh")
     shape = tuple(dz0.shape)
-    dx = torch.empty(shape + (16,), dtype=torch.float32, device=dz0.device)
+    dx = torch.empty(shape + (16,), dtype=dtype, device=dz0.device)
     _lib.call("pm_conv3d_bwd_data_out1", ptr(dz0.contiguous()), ptr(w.contiguous()),
-              ptr(gact.contiguous()) if gact is not None else None, ptr(dx), _lib.int3(shape),
-              stream_handle(dz0.device))
+              ptr(gact.contiguous()) if gact is not None else None, ptr(dx), int(dtype == torch.bfloat16),
+              _lib.int3(shape), stream_handle(dz0.device))
     return dx
 
 

@@ -387,24 +387,27 @@ def residual_backward(model: MeshCNNPotentialCorrection, tape: Tape, rho_cot, gr
 
     # output layer: input acts[-1] = gelu(pres[-1])
     wgrad(layers[-1], dz, tape.acts[-1])
+    # without the weight gradients, the pre-activation gradients only feed further data
+    # gradients, whose matrix cores read them as bf16: they are stored as bf16
+    gdt = torch.float32 if weights else torch.bfloat16
     if weights:
         dz = conv3d_bwd_data(dz, model.weight(layers[-1]), gact=tape.pres[-1])
     else:
-        dz = conv3d_bwd_data_out1(dz, model.weight(layers[-1]), gact=tape.pres[-1])
+        dz = conv3d_bwd_data_out1(dz, model.weight(layers[-1]), gact=tape.pres[-1], dtype=gdt)
     for i in reversed(range(model.depth - 1)):
         if model.block == "plain":
             # dz is the gradient of block i's pre-activation; its input is acts[i] = gelu(pres[i])
             l1 = layers[1 + i]
             wgrad(l1, dz, tape.acts[i])
-            dz = conv3d_bwd_data(dz, model.weight(l1), gact=tape.pres[i])
+            dz = conv3d_bwd_data(dz, model.weight(l1), gact=tape.pres[i], dtype=gdt)
             continue
         l1, l2 = layers[1 + 2 * i], layers[2 + 2 * i]
         x_in, pre_in = tape.acts[2 * i], tape.pres[2 * i]
         h, pre1 = tape.acts[2 * i + 1], tape.pres[2 * i + 1]
         wgrad(l2, dz, h)
-        dz1 = conv3d_bwd_data(dz, model.weight(l2), gact=pre1)
+        dz1 = conv3d_bwd_data(dz, model.weight(l2), gact=pre1, dtype=gdt)
         wgrad(l1, dz1, x_in)
-        dz = conv3d_bwd_data(dz1, model.weight(l1), add=dz, gact=pre_in)
+        dz = conv3d_bwd_data(dz1, model.weight(l1), add=dz, gact=pre_in, dtype=gdt)
     wgrad(layers[0], dz, tape.x0)
     if not inputs and not input_grad:
         return grads

@@ -97,9 +97,10 @@ __device__ __forceinline__ float gelu_fast(float x) {
 // and an epilogue store covers 512 contiguous bytes.
 // Epilogue modes of the forward: kPlain writes pre and/or act; kOutResidual is the output
 // layer fused with the correction's residual stage (inference): rho_add = gain (rho - 1) +
-// amp tanh(pre channel 0), the same arithmetic as k_residual of pm_correction.hip; kBwdData
-// is a 16 -> 16 layer's data gradient run as a forward convolution (see k_conv_fwd).
-enum Epilogue { kPlain = 0, kOutResidual = 1, kBwdData = 2 };
+// amp tanh(pre channel 0), the same arithmetic as k_residual of pm_correction.hip; kBwdData*
+// are a 16 -> 16 layer's data gradient run as a forward convolution (see k_conv_fwd): dz and
+// dx float32 (kBwdData), both bf16 (kBwdData16), or dz bf16 and dx float32 (kBwdData16F).
+enum Epilogue { kPlain = 0, kOutResidual = 1, kBwdData = 2, kBwdData16 = 3, kBwdData16F = 4 };
 
 template <int BZ_>
 struct FwdGeom {
@@ -179,8 +180,9 @@ struct PlaneWalk {
   const float* rho;
   float* rho_add;
   float gain, amp;
-  const float* addf;  // kBwdData: gradient added before the GELU derivative (or null)
-  const float* gact;  // kBwdData: pre-activation whose GELU derivative scales the result (or null)
+  const float* addf;  // kBwdData*: gradient added before the GELU derivative (or null; bf16 data
+                      // when dz is bf16)
+  const float* gact;  // kBwdData*: pre-activation whose GELU derivative scales the result (or null)
 
   __device__ __forceinline__ int64_t voxel(int r, int t) const { return el + r * rowz + t * plane; }
 
@@ -212,22 +214,35 @@ struct PlaneWalk {
     // lane holds output channels 4g .. 4g+3 of its voxel in output plane t of row r
     const int64_t e = voxel(r, t);
     const int q = r * BX + t;
-    if constexpr (EPI == kBwdData) {
-      // dx = (sum + add) * gelu'(gact), channels 4g .. 4g+3 of the voxel, float32; with pch = 4
-      // only channels 0 .. 3 are written (dx has 4 channels per voxel)
+    if constexpr (EPI >= kBwdData) {
+      // dx = (sum + add) * gelu'(gact), channels 4g .. 4g+3 of the voxel; with pch = 4 only
+      // channels 0 .. 3 are written (dx has 4 channels per voxel)
       if (pch != kCout && g != 0) return;
       f32x4 v = c;
       if (addf) {
-        const f32x4 a = *reinterpret_cast<const f32x4*>(addf + e * kCout + 4 * g);
+        if constexpr (EPI == kBwdData) {
+          const f32x4 a = *reinterpret_cast<const f32x4*>(addf + e * kCout + 4 * g);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] += a[i];
+          for (int i = 0; i < 4; ++i) v[i] += a[i];
+        } else {
+          const bf16x4 a = *reinterpret_cast<const bf16x4*>(reinterpret_cast<const __bf16*>(addf) + e * kCout + 4 * g);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] += (float)a[i];
+        }
       }
       if (gact) {
         const f32x4 ga = *reinterpret_cast<const f32x4*>(gact + e * kCout + 4 * g);
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] *= gelu_grad(ga[i]);
       }
-      *reinterpret_cast<f32x4*>(pre + e * pch + 4 * g) = v;
+      if constexpr (EPI == kBwdData16) {
+        bf16x4 o;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[i] = (__bf16)v[i];
+        *reinterpret_cast<bf16x4*>(act + e * pch + 4 * g) = o;
+      } else {
+        *reinterpret_cast<f32x4*>(pre + e * pch + 4 * g) = v;
+      }
       return;
     }
     float zv[4];
@@ -305,7 +320,9 @@ struct PlaneWalk {
 // are loaded as A[ci][(tap, co)] = w[26 - tap][co][ci] (once per workgroup), x is the float32
 // dz (rounded to bf16 when staged, as the matrix cores read it), there is no bias, and the
 // epilogue writes dx = (sum + addf) * gelu'(gact) to pre as float32 (pch = 16 channels per
-// voxel, or 4: only the first four).
+// voxel, or 4: only the first four).  kBwdData16 reads a bf16 dz (and add) and writes dx to
+// act as bf16; kBwdData16F reads a bf16 dz and writes float32: a gradient that goes on to
+// another data gradient only is rounded to bf16 there anyway, so it is stored as bf16.
 template <int CIN, bool GELU, bool RES, int EPI = kPlain, int BZ = 8>
 __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restrict__ x, const __bf16* __restrict__ w,
                                                           const float* __restrict__ bias,
@@ -318,9 +335,10 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
                                                           const float* __restrict__ gact = nullptr) {
   using Geo = FwdGeom<BZ>;
   using Walk = PlaneWalk<CIN, GELU, RES, EPI, BZ>;
-  constexpr bool BWD = EPI == kBwdData;
+  constexpr bool BWD = EPI >= kBwdData;
+  constexpr bool F32IN = EPI == kBwdData;  // dz staged from float32
   static_assert(!BWD || (CIN == 16 && !GELU && !RES), "data gradient: 16 -> 16, plain");
-  constexpr int PW = BWD ? 8 : 4;  // prefetch words per staged 8-channel vector (float32 or bf16)
+  constexpr int PW = F32IN ? 8 : 4;  // prefetch words per staged 8-channel vector (float32 or bf16)
   const float* __restrict__ xf = reinterpret_cast<const float*>(x);
   using Taps = PlaneTaps<CIN, Geo::SZ>;
   constexpr int TPS = Taps::TPS, KS = Taps::KS;
@@ -409,7 +427,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
   typedef uint32_t Prefetch __attribute__((ext_vector_type(PW * PER)));  // a value, never a private array
   // one staged 8-channel vector at element offset o of x: 16 bytes of bf16, or 32 of float32
   auto fetch = [&](Prefetch& pf, int u, int64_t o) {
-    if constexpr (BWD) {
+    if constexpr (F32IN) {
       const float4 t0 = *reinterpret_cast<const float4*>(xf + o);
       const float4 t1 = *reinterpret_cast<const float4*>(xf + o + 4);
       pf[8 * u] = __float_as_uint(t0.x);
@@ -468,7 +486,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_fwd(const __bf16* __restri
     for (int u = 0; u < PER; ++u) {
       const int i = threadIdx.x + u * kThreads;
       if (i < ITEMS) {
-        if constexpr (BWD) {
+        if constexpr (F32IN) {
           bf16x8 b;
 #pragma unroll
           for (int q = 0; q < 8; ++q) b[q] = (__bf16)__uint_as_float(pf[8 * u + q]);
@@ -528,8 +546,8 @@ __device__ __forceinline__ bf16x8 stem_fragment(const __bf16* __restrict__ w, in
 // kStemBwdOut1): dx[v][ci] = gelu'(gact[v][ci]) sum_tap dz0[v - off(tap)] w[tap][0][ci] is a
 // one-channel convolution of dz0 with the taps mirrored, so lane (g, m) holds
 // w[26 - tap][0][m] for taps 8g .. 8g+7, the staged tile is bf16(dz0), and the epilogue writes
-// float32 (no bias, no GELU).
-enum StemMode { kStemDensity = 0, kStemPotential = 1, kStemBwdOut1 = 2 };
+// float32 (kStemBwdOut1Bf16: bf16, to act; no bias, no GELU).
+enum StemMode { kStemDensity = 0, kStemPotential = 1, kStemBwdOut1 = 2, kStemBwdOut1Bf16 = 3 };
 
 template <int MODE>
 __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restrict__ rho,
@@ -540,7 +558,7 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
                                                            int n0, int n1, int n2, float* __restrict__ dxo = nullptr,
                                                            const float* __restrict__ gact = nullptr) {
   constexpr bool PHI = MODE == kStemPotential;
-  constexpr bool BWD1 = MODE == kStemBwdOut1;
+  constexpr bool BWD1 = MODE == kStemBwdOut1 || MODE == kStemBwdOut1Bf16;
   constexpr int NS = kH * kH * kH;
   constexpr int PER = (NS + kThreads - 1) / kThreads;
   constexpr int NT = PHI ? 2 : 1;  // staged input channels
@@ -639,7 +657,14 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
 #pragma unroll
             for (int i = 0; i < 4; ++i) v[i] *= gelu_grad(ga[i]);
           }
-          *reinterpret_cast<f32x4*>(dxo + e * kCout + 4 * g) = v;
+          if constexpr (MODE == kStemBwdOut1Bf16) {
+            bf16x4 o;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o[i] = (__bf16)v[i];
+            *reinterpret_cast<bf16x4*>(act + e * kCout + 4 * g) = o;
+          } else {
+            *reinterpret_cast<f32x4*>(dxo + e * kCout + 4 * g) = v;
+          }
         } else {
           bf16x4 o;
 #pragma unroll
@@ -1347,11 +1372,13 @@ int pm_conv3d_out_residual(const void* x, const void* w, const float* bias, cons
   return PM_SUCCESS;
 }
 
-int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const float* gact, float* dx,
-                       int32_t dx_channels, const int32_t shape[3], void* stream) {
+int pm_conv3d_bwd_data(const void* dz, const void* w, const void* add, const float* gact, void* dx,
+                       int32_t dx_channels, int32_t bf16_io, const int32_t shape[3], void* stream) {
   PM_TRY(check_shape(shape));
   PM_RETURN_IF(dz == nullptr || w == nullptr || dx == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL argument");
   PM_RETURN_IF(dx_channels != 16 && dx_channels != 4, PM_ERROR_INVALID_ARGUMENT, "dx_channels must be 16 or 4");
+  PM_RETURN_IF(bf16_io != 0 && bf16_io != 1 && bf16_io != 3, PM_ERROR_INVALID_ARGUMENT,
+               "bf16_io must be 0 (float32 dz and dx), 1 (bf16 dz, float32 dx) or 3 (bf16 dz and dx)");
   PM_RETURN_IF(reinterpret_cast<uintptr_t>(dz) % 16 != 0 || reinterpret_cast<uintptr_t>(dx) % 16 != 0 ||
                    reinterpret_cast<uintptr_t>(add) % 16 != 0 || reinterpret_cast<uintptr_t>(gact) % 16 != 0,
                PM_ERROR_INVALID_ARGUMENT, "conv3d data gradient: meshes must be 16-byte aligned");
@@ -1361,19 +1388,30 @@ int pm_conv3d_bwd_data(const float* dz, const void* w, const float* add, const f
   PM_RETURN_IF(dz == dx, PM_ERROR_INVALID_ARGUMENT, "conv3d data gradient: dx must not alias dz");
   hipStream_t s = as_stream(stream);
   const auto wb = static_cast<const __bf16*>(w);
-  const auto xb = reinterpret_cast<const __bf16*>(dz);  // read as float32 by the kBwdData walk
-  auto kern = wide_blocks(shape) ? conv::k_conv_fwd<16, false, false, conv::kBwdData, 16>
-                                 : conv::k_conv_fwd<16, false, false, conv::kBwdData, 8>;
+  const auto xb = static_cast<const __bf16*>(dz);  // read as float32 by the kBwdData walk
+  const bool wide = wide_blocks(shape);
+  decltype(&conv::k_conv_fwd<16, false, false, conv::kBwdData, 8>) kern;
+  if (bf16_io == 0)
+    kern = wide ? conv::k_conv_fwd<16, false, false, conv::kBwdData, 16> : conv::k_conv_fwd<16, false, false, conv::kBwdData, 8>;
+  else if (bf16_io == 1)
+    kern = wide ? conv::k_conv_fwd<16, false, false, conv::kBwdData16F, 16>
+                : conv::k_conv_fwd<16, false, false, conv::kBwdData16F, 8>;
+  else
+    kern = wide ? conv::k_conv_fwd<16, false, false, conv::kBwdData16, 16>
+                : conv::k_conv_fwd<16, false, false, conv::kBwdData16, 8>;
+  float* dxf = bf16_io == 3 ? nullptr : static_cast<float*>(dx);
+  __bf16* dxb = bf16_io == 3 ? static_cast<__bf16*>(dx) : nullptr;
   unsigned grid = 0;
   PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, xb, wb, nullptr, nullptr, dx, nullptr, shape[0],
-                     shape[1], shape[2], (int)dx_channels, nullptr, nullptr, 0.f, 0.f, nullptr, add, gact);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, xb, wb, nullptr, nullptr, dxf, dxb, shape[0],
+                     shape[1], shape[2], (int)dx_channels, nullptr, nullptr, 0.f, 0.f, nullptr,
+                     static_cast<const float*>(add), gact);
   PM_LAUNCHED("k_conv_bwd_data", s);
   return PM_SUCCESS;
 }
 
-int pm_conv3d_bwd_data_out1(const float* dz0, const void* w, const float* gact, float* dx, const int32_t shape[3],
-                            void* stream) {
+int pm_conv3d_bwd_data_out1(const float* dz0, const void* w, const float* gact, void* dx, int32_t dx_bf16,
+                            const int32_t shape[3], void* stream) {
   PM_TRY(check_shape(shape));
   PM_RETURN_IF(dz0 == nullptr || w == nullptr || dx == nullptr, PM_ERROR_INVALID_ARGUMENT, "NULL argument");
   PM_RETURN_IF(reinterpret_cast<uintptr_t>(dx) % 16 != 0 || reinterpret_cast<uintptr_t>(gact) % 16 != 0,
@@ -1383,11 +1421,12 @@ int pm_conv3d_bwd_data_out1(const float* dz0, const void* w, const float* gact, 
                    (int64_t)conv::kH * shape[1] * shape[2] >= (int64_t(1) << 31),
                PM_ERROR_UNSUPPORTED, "conv3d: mesh too large for 32-bit tile offsets");
   hipStream_t s = as_stream(stream);
-  auto kern = conv::k_conv_stem<conv::kStemBwdOut1>;
+  auto kern = dx_bf16 ? conv::k_conv_stem<conv::kStemBwdOut1Bf16> : conv::k_conv_stem<conv::kStemBwdOut1>;
   unsigned grid = 0;
   PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, dz0, nullptr, 0.f, nullptr, 0.f, 0.f, 0.f,
-                     static_cast<const __bf16*>(w), nullptr, nullptr, shape[0], shape[1], shape[2], dx, gact);
+                     static_cast<const __bf16*>(w), nullptr, dx_bf16 ? static_cast<__bf16*>(dx) : nullptr, shape[0],
+                     shape[1], shape[2], dx_bf16 ? nullptr : static_cast<float*>(dx), gact);
   PM_LAUNCHED("k_conv_bwd_out1", s);
   return PM_SUCCESS;
 }

@@ -98,6 +98,18 @@ def test_backward_data_and_weight_match_autograd(device, shape):
                           channels=4)
     assert tuple(dx4.shape) == shape + (4,)
     np.testing.assert_array_equal(dx4.cpu().numpy(), dx2[..., :4].cpu().numpy())
+    # bf16 dz and add (dz is bf16-valued already, as the matrix cores read it): the same sums,
+    # with dx float32 or rounded to bf16
+    addb = _bf(add)
+    wd = w.to(device, torch.bfloat16)
+    ref32 = conv3d_bwd_data(dz.to(device), wd, add=addb.to(device), gact=gact.to(device))
+    got32 = conv3d_bwd_data(dz.to(device, torch.bfloat16), wd, add=addb.to(device, torch.bfloat16),
+                            gact=gact.to(device))
+    np.testing.assert_array_equal(got32.cpu().numpy(), ref32.cpu().numpy())
+    got16 = conv3d_bwd_data(dz.to(device, torch.bfloat16), wd, add=addb.to(device, torch.bfloat16),
+                            gact=gact.to(device), dtype=torch.bfloat16)
+    assert got16.dtype == torch.bfloat16
+    np.testing.assert_array_equal(got16.float().cpu().numpy(), ref32.to(torch.bfloat16).float().cpu().numpy())
 
 
 @pytest.mark.parametrize("shape", [(8, 8, 8), (16, 24, 32)])
@@ -123,6 +135,9 @@ def test_one_channel_layer_data_gradient(device, shape):
     np.testing.assert_allclose(got.numpy(), ref.numpy(), rtol=1e-4, atol=1e-5 * ref.abs().max().item())
     wide = conv3d_bwd_data(dz16.to(device), w.to(device, torch.bfloat16), gact=gact.to(device)).cpu()
     np.testing.assert_allclose(got.numpy(), wide.numpy(), rtol=1e-5, atol=1e-6 * wide.abs().max().item())
+    got16 = conv3d_bwd_data_out1(dz0.to(device), w.to(device, torch.bfloat16), gact=gact.to(device),
+                                 dtype=torch.bfloat16).cpu()
+    np.testing.assert_array_equal(got16.float().numpy(), got.to(torch.bfloat16).float().numpy())
 
 
 def test_weight_gradient_cin8(device):
