@@ -292,14 +292,16 @@ def train_correction_run(args):
     keeping the checkpoint with the best validation score.  Every choice uses the validation
     seed 999 alone (score: largest |P/P_fine - 1| up to k = 1 h/Mpc plus 30 x any shortfall of
     the cross-correlation below the plain run's); only the final network is judged, on the
-    held-out seeds 1000 and 1001.  Densities are measured on the coarse mesh.  ``--save PATH``
+    held-out seeds 1000 and 1001.  Densities are measured on the coarse mesh.  ``--init-from PATH``
+    skips stage 1 and fine-tunes saved weights instead (with ``--potential``, a network without
+    the potential channel gets it with zero weights first).  ``--save PATH``
     writes the weights there (the package ships them as
     ``pmpp_amd/corrections/weights/mesh_cnn_2x_from_4x.npz``).  Prints one JSON line."""
     import torch
 
     from pmpp_amd import Configuration
-    from pmpp_amd.corrections import (compare_runs, fine_run_pairs, mean_force_loss, pk_errors, save_correction,
-                                      train_correction)
+    from pmpp_amd.corrections import (compare_runs, correction_meta, fine_run_pairs, load_correction, mean_force_loss,
+                                      pk_errors, save_correction, train_correction, with_potential_channel)
     from pmpp_amd.corrections.training import RunTarget, train_correction_runs
     from pmpp_amd.initial_conditions import LCDM
 
@@ -317,14 +319,29 @@ def train_correction_run(args):
         return c["pk_err_max"] + 30.0 * max(0.0, p["r_mean"] - c["r_mean"])
 
     t0 = time.perf_counter()
-    pairs = fine_run_pairs(cosmo, conf, conf_fine, seeds=range(100, 100 + args.train_seeds), every=2, device=dev)
+    stage1 = args.init_from is None
+    init_meta = {} if stage1 else correction_meta(args.init_from)
+    pairs = (fine_run_pairs(cosmo, conf, conf_fine, seeds=range(100, 100 + args.train_seeds), every=2, device=dev)
+             if stage1 else [])
     val_pairs = fine_run_pairs(cosmo, conf, conf_fine, seeds=[999], every=2, device=dev)
     torch.cuda.synchronize()
-    log(f"[train] {len(pairs)} training pairs from {args.train_seeds} fine runs in {time.perf_counter() - t0:.1f} s")
+    log(f"[train] {len(pairs)} training pairs from {args.train_seeds if stage1 else 0} fine runs in "
+        f"{time.perf_counter() - t0:.1f} s")
     val_plain = mean_force_loss(None, val_pairs, cosmo, conf)
     cands = []
     grid = [(sym, mr, lr) for sym in args.symmetry.split(",") for mr in _parse_floats(args.max_residual)
-            for lr in _parse_floats(args.lr)]
+            for lr in _parse_floats(args.lr)] if stage1 else []
+    if not stage1:
+        init = load_correction(args.init_from, device=dev)
+        if args.potential:
+            init = with_potential_channel(init)
+        val = judge(init, 999)
+        cands.append({"symmetry": init.symmetry, "max_residual": init.max_residual,
+                      "lr": float(init_meta.get("lr", 3e-4)), "train_s": 0.0,
+                      "init_from": str(args.init_from), "val_force_loss": mean_force_loss(init, val_pairs, cosmo, conf),
+                      "val_force_loss_plain": val_plain, "val": val, "score": score(val), "model": init})
+        log(f"[train] start from {args.init_from}: val P(k) err to k=1 max {val['corrected']['pk_err_max']:.4f}, "
+            f"r {val['corrected']['r_mean']:.6f} (plain {val['plain']['r_mean']:.6f})")
     for sym, mr, lr in grid:
         tag = f"symmetry {sym} max_residual {mr:g} lr {lr:g}"
         t0 = time.perf_counter()
@@ -382,11 +399,14 @@ def train_correction_run(args):
                       "force_loss_plain": mean_force_loss(None, judge_pairs, cosmo, conf)}
         log(f"[train] judge seed {sd}: {json.dumps(judged[sd])}")
     recipe = {"particles": ng, "box": box, "coarse_mesh": 2 * ng, "fine_mesh": 4 * ng, "kdk_steps": args.run_steps,
-              "a_start": a0, "train_seeds": f"100-{99 + args.train_seeds}", "pairs": n_pairs,
-              "iters": args.train_iters, "validation_seed": 999, "judge_seed": 1000,
+              "a_start": a0, "train_seeds": f"100-{99 + args.train_seeds}",
+              "pairs": n_pairs if stage1 else int(init_meta.get("pairs", 0)),
+              "iters": args.train_iters if stage1 else int(init_meta.get("iters", 0)), "validation_seed": 999,
+              "judge_seed": 1000,
               "max_residual": best["max_residual"], "lr": best["lr"], "symmetry": best["symmetry"],
               "e2e_iters": args.e2e_iters, "e2e_seeds": f"100-{99 + args.e2e_seeds}", "e2e_lr": args.e2e_lr,
-              "r_weight": args.r_weight, "potential": int(args.potential)}
+              "r_weight": args.r_weight, "potential": int(args.potential),
+              "init_from": "" if stage1 else str(args.init_from)}
     if args.save:
         save_correction(model, args.save, **recipe)
         log(f"[train] wrote {args.save}")
@@ -594,6 +614,8 @@ def main():
     ap.add_argument("--e2e-lr", type=float, default=1e-4)
     ap.add_argument("--e2e-validate-every", type=int, default=50)
     ap.add_argument("--r-weight", type=float, default=1.0)
+    ap.add_argument("--init-from", default=None,
+                    help="with --train-correction: skip stage 1 and fine-tune these saved weights")
     ap.add_argument("--potential", action="store_true",
                     help="with --train-correction or --network: the network also reads the potential channel")
     ap.add_argument("--save", default=None,

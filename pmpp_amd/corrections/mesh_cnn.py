@@ -203,6 +203,25 @@ def init_mesh_cnn_potential_correction(seed=0, depth=2, max_residual=0.1, output
                                       symmetry=symmetry, potential=potential)
 
 
+def with_potential_channel(model: MeshCNNPotentialCorrection) -> MeshCNNPotentialCorrection:
+    """The same network reading the potential channel as well: the stem's weights of the
+    constant channels (a, Omega_m, sigma8) move from input channels 1..3 to 2..4 and the
+    potential's (channel 1) start at zero, so it computes what ``model`` does until those are
+    trained (the starting point of a fine-tuning with the potential)."""
+    if model.potential:
+        return model
+    stem = model.layers[0]
+    p = model.params.detach().clone()
+    w = p[stem.offset:stem.offset + stem.nweights].view(TAPS, CHANNELS, IN_CHANNELS)
+    w2 = torch.zeros_like(w)
+    w2[:, :, 0] = w[:, :, 0]
+    w2[:, :, 2:2 + LIVE_IN_CHANNELS - 1] = w[:, :, 1:LIVE_IN_CHANNELS]
+    p[stem.offset:stem.offset + stem.nweights] = w2.reshape(-1)
+    return MeshCNNPotentialCorrection(params=p, depth=model.depth, max_residual=model.max_residual,
+                                      output_init_scale=model.output_init_scale, sigma8_value=model.sigma8_value,
+                                      block=model.block, symmetry=model.symmetry, potential=True)
+
+
 @dataclasses.dataclass
 class Tape:
     """What the backward pass needs from a forward evaluation."""
