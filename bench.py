@@ -234,7 +234,7 @@ def nbody_run(args):
     log(f"[run] reverse sweep in {t_adj:.3f} s")
     del cot, out
     corrected = {}
-    model = load_correction(device=dev)
+    model = load_correction(args.weights, device=dev)
     nbody(ptcl, cosmo, conf, correction=model, observer=lambda a, p: None)  # warm (allocator)
     out, t_corr = timed(lambda: nbody(ptcl, cosmo, conf, correction=model, observer=lambda a, p: None))
     corrected = {"conv_layers": len(model.layers), "steps_per_s": nsteps / t_corr, "s": t_corr,
@@ -422,13 +422,14 @@ def train_correction_run(args):
     print(json.dumps(out), flush=True)
 
 
-def pk_error_run(dev, nsteps=20, seed=0):
+def pk_error_run(dev, nsteps=20, seed=0, weights=None):
     """The second half of the headline metric at full scale: from one seed's white noise, the
     512^3-particle / 1000 Mpc/h run (2LPT at a = 1/64, ``nsteps`` KDK steps to a = 1) on the
     1024^3 mesh with the shipped trained correction, and the plain run on the 2048^3 mesh (4x
     the particle grid); both final densities are measured on the 1024^3 mesh (CIC, deconvolved),
     and each run is timed after one warm run.  Returns a dict (P(k) error and cross-correlation
-    over 0 < k <= 1 h/Mpc for the corrected and the plain 2x run, and the wall times)."""
+    over 0 < k <= 1 h/Mpc for the corrected and the plain 2x run, and the wall times).
+    ``weights`` (a path) replaces the shipped correction."""
     import torch
 
     from pmpp_amd import Configuration
@@ -444,7 +445,7 @@ def pk_error_run(dev, nsteps=20, seed=0):
     cosmo = LCDM()
     conf = Configuration(box / ng, (ng,) * 3, mesh_shape=ratio, a_start=a0, a_nbody_maxstep=(1 - a0) / nsteps)
     conf4 = conf.replace(mesh_shape=4)
-    model = load_correction(device=dev)
+    model = load_correction(weights, device=dev)
     modes = linear_modes(white_noise(seed, conf, device=dev), cosmo, conf)
 
     def timed(c, corr):
@@ -474,7 +475,8 @@ def pk_error_run(dev, nsteps=20, seed=0):
     k = k.cpu().numpy()
     sel = (k > 0) & (k <= 1.0)
     res = {"kmax_h_per_Mpc": 1.0, "shells": int(sel.sum()), "seed": seed, "kdk_steps": nsteps,
-           "measured_on": "1024^3 mesh, CIC deconvolved", "weights": correction_meta()}
+           "measured_on": "1024^3 mesh, CIC deconvolved", "weights": correction_meta(weights),
+           "potential_channel": bool(model.potential)}
     for name, pk, r in (("corrected_2x", pk_c, r_c), ("plain_2x", pk_p, r_p)):
         e = abs(pk.cpu().numpy()[sel] / pk_fine.cpu().numpy()[sel] - 1)
         res[name] = {"pk_err_mean": round(float(e.mean()), 5), "pk_err_max": round(float(e.max()), 5),
@@ -614,6 +616,8 @@ def main():
     ap.add_argument("--e2e-lr", type=float, default=1e-4)
     ap.add_argument("--e2e-validate-every", type=int, default=50)
     ap.add_argument("--r-weight", type=float, default=1.0)
+    ap.add_argument("--weights", default=None,
+                    help="a saved correction for the P(k) comparison and --nbody-run instead of the shipped one")
     ap.add_argument("--init-from", default=None,
                     help="with --train-correction: skip stage 1 and fine-tune these saved weights")
     ap.add_argument("--potential", action="store_true",
@@ -747,7 +751,7 @@ def main():
     if not args.no_pk_error and args.config == "c5":
         if (TRAINED_DIR / TRAINED_DEFAULT).exists():
             log("[bench] P(k) of the corrected 1024^3 run against the plain 2048^3 run")
-            pk_err = pk_error_run(dev)
+            pk_err = pk_error_run(dev, weights=args.weights)
         else:
             log("[bench] no trained correction weights shipped: P(k) comparison skipped")
         torch.cuda.empty_cache()
