@@ -511,6 +511,7 @@ def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_
 
 TRAINED_DIR = __import__("pathlib").Path(__file__).resolve().parent / "weights"
 TRAINED_DEFAULT = "mesh_cnn_2x_from_4x.npz"
+TRAINED_DENSITY_ONLY = "mesh_cnn_2x_from_4x_density_only.npz"
 
 
 def save_correction(model: MeshCNNPotentialCorrection, path, **meta) -> None:
@@ -531,8 +532,9 @@ def save_correction(model: MeshCNNPotentialCorrection, path, **meta) -> None:
 
 def load_correction(path=None, device="cuda") -> MeshCNNPotentialCorrection:
     """A correction saved by ``save_correction`` (default: the weights shipped with the package,
-    trained on coarse (2x) / fine (4x particle-grid) force pairs at 1.95 Mpc/h particle spacing).
-    Loaded with ``numpy.load(allow_pickle=False)``."""
+    trained on coarse (2x) / fine (4x particle-grid) force pairs and whole runs at 1.95 Mpc/h
+    particle spacing, reading the potential channel; ``TRAINED_DENSITY_ONLY`` in the same folder
+    is the network without it).  Loaded with ``numpy.load(allow_pickle=False)``."""
     import numpy as np
 
     path = TRAINED_DIR / TRAINED_DEFAULT if path is None else path

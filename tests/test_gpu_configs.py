@@ -270,7 +270,9 @@ def test_c5_reverse_with_the_trained_correction_matches_central_differences(devi
     shipped trained correction: the derivative of <cot, final disp> along a random direction of
     the initial displacement, from the hand-written reverse sweep (force VJP at the corrected
     source, the network's reverse over x-slabs, scatter adjoint, per-step force recomputation),
-    against float32 central differences of the corrected forward run."""
+    against float32 central differences of the corrected forward run.  The step is 2.5e-3: the
+    central difference's own error along this direction was measured at 3.7 % for a step of
+    1e-2, 2.3 % for 5e-3 and 0.07 % for 2.5e-3 (converging onto the adjoint)."""
     from pmpp_amd.corrections import load_correction
 
     a0 = 1 / 64
@@ -291,7 +293,7 @@ def test_c5_reverse_with_the_trained_correction_matches_central_differences(devi
     got = float((disp.grad.double() * v.double()).sum())
     del out, disp
     print(f"[c5] reverse sweep with the correction done; memory high-water mark {peak:.1f} GiB", flush=True)
-    eps = 1e-2
+    eps = 2.5e-3
     with torch.no_grad():
         fp = nbody(p0.replace(disp=p0.disp + eps * v), cos, conf, correction=model).disp
         lp = float((fp.double() * cot.double()).sum())

@@ -412,6 +412,43 @@ def test_correction_trained_on_fine_runs_moves_pk_towards_the_fine_run(device):
     assert r_corr > r_plain - 0.03  # the phases stay as close to the fine run's
 
 
+@pytest.mark.parametrize("case", ["density", "potential", "potential_only"])
+def test_input_path_cotangents_match_torch(device, case):
+    """The density and Omega_m cotangents through the network's inputs alone (residual_backward
+    with inputs: the network's data gradient, pm_correction_input_bwd and, with the potential
+    channel, the potential's adjoint) against float32 torch autograd of the restated network.
+    ``potential_only`` zeroes the stem's density-channel weights, so every density cotangent
+    passes through the potential (and the statistics)."""
+    potential = case != "density"
+    conf = Configuration(2.0, (16, 16, 16), mesh_shape=2)
+    model = _randomised(init_mesh_cnn_potential_correction(seed=11, depth=2, device=device, block="plain",
+                                                           potential=potential), 11, out_scale=0.3)
+    if case == "potential_only":
+        p = model.params.detach().clone()
+        stem = model.layers[0]
+        w = p[stem.offset:stem.offset + stem.nweights].view(27, 16, 8)
+        w[:, :, 0] = 0
+        model.params.copy_(p)
+        model.sync_bf16()
+    rng = np.random.default_rng(13)
+    rho = torch.from_numpy((1 + 0.6 * rng.normal(size=conf.mesh_shape)).astype(np.float32)).to(device)
+    rho_cot = torch.from_numpy(rng.normal(size=conf.mesh_shape).astype(np.float32)).to(device)
+    a, om = 0.6, 0.3
+    _, tape = evaluate_residual(model, rho, a, om, keep=True, conf=conf)
+    _, rho_bar, omega_bar = residual_backward(model, tape, rho_cot, inputs=True)
+    params = model.params.detach().cpu()
+    rho_t = rho.cpu().clone().requires_grad_(True)
+    om_t = torch.tensor(om, dtype=torch.float32, requires_grad=True)
+    ref = _torch_residual(model, rho_t, a, om_t, params, _green(conf) if potential else None)
+    (ref * rho_cot.cpu()).sum().backward()
+    got, exp = rho_bar.cpu(), rho_t.grad
+    err = float(torch.linalg.norm(got - exp) / torch.linalg.norm(exp))
+    print(f"[input path] {case}: rho_bar rel err {err:.3e}, omega {float(omega_bar):.6e} vs {float(om_t.grad):.6e}",
+          flush=True)
+    assert err < 1e-2
+    assert float(omega_bar) == pytest.approx(float(om_t.grad), rel=2e-2, abs=1e-3 * float(exp.abs().sum()))
+
+
 @pytest.mark.parametrize("potential", [False, True])
 def test_corrected_force_reverse_through_the_network_inputs(device, potential):
     # disp and Omega_m cotangents of the corrected force: the HIP chain (force-step adjoint at the

@@ -2,17 +2,20 @@
 250 Mpc/h box (1.95 Mpc/h particle spacing, as the full-scale workload), coarse mesh 256^3
 (2x the particle grid), fine mesh 512^3 (4x), 2LPT at a = 1/64 then 20 KDK steps to a = 1.
 
-* The shipped weights (trained by ``bench.py --train-correction --save`` on force pairs and
-  whole runs of seeds 100-131, chosen on seed 999) judged on held-out seeds 1000 and 1001,
+* Both shipped networks (trained by ``bench.py --train-correction --save`` on force pairs and
+  whole runs of seeds 100-131, chosen on seed 999: the density-only network, and the default
+  one that reads the potential channel too, fine-tuned from it on whole runs) judged on
+  held-out seeds 1000 and 1001,
   with every final density measured on the coarse mesh: over 0 < k <= 1 h/Mpc the corrected
   coarse run's largest per-shell P(k) error against the fine run is below 2.5 % and 0.6 of the
   plain coarse run's, its mean error below 0.7 of the plain run's, its cross-correlation with
   the fine run no worse than the plain run's, and its relative force error on the held-out
   run's snapshots below a quarter of the plain force's.  (In this 250 Mpc/h box the largest
-  shell error is 2.0-2.3 %, the mean 1.0-1.1 %, against 4.5-5.2 % and 1.8-2.0 % plain; the 2 %
-  target of the full-scale 1000 Mpc/h workload is measured there by ``bench.py``.)
-* The training recipe itself (a short run on 4 seeds) lowers the force loss of a validation
-  seed below the plain force's.
+  shell error is 2.0-2.3 % (1.97-2.10 % with the potential channel), the mean 0.95-1.1 %,
+  against 4.5-5.2 % and 1.8-2.0 % plain; the 2 % target of the full-scale 1000 Mpc/h workload
+  is measured there by ``bench.py``.)
+* The training recipe's first stage (a short run on 4 seeds) lowers the force loss of a
+  validation seed below the plain force's.
 """
 
 import numpy as np
@@ -21,6 +24,7 @@ import pytest
 from pmpp_amd import Configuration
 from pmpp_amd.corrections import (compare_runs, correction_meta, fine_run_pairs, load_correction, mean_force_loss,
                                   pk_errors, train_correction)
+from pmpp_amd.corrections.mesh_cnn import TRAINED_DENSITY_ONLY, TRAINED_DIR
 from pmpp_amd.initial_conditions import LCDM
 
 pytestmark = pytest.mark.gpu
@@ -32,18 +36,21 @@ def _confs():
     return conf, conf.replace(mesh_shape=4)
 
 
+@pytest.mark.parametrize("weights", ["default", "density_only"])
 @pytest.mark.parametrize("seed", [1000, 1001])
-def test_shipped_correction_on_held_out_seeds(device, seed):
+def test_shipped_correction_on_held_out_seeds(device, seed, weights):
     conf, conf_fine = _confs()
     cosmo = LCDM()
-    meta = correction_meta()
+    path = None if weights == "default" else TRAINED_DIR / TRAINED_DENSITY_ONLY
+    meta = correction_meta(path)
     assert meta["validation_seed"] == 999 and meta["judge_seed"] == 1000
-    model = load_correction(device=device)
+    model = load_correction(path, device=device)
+    assert model.potential == (weights == "default")
     res = pk_errors(compare_runs(model, cosmo, conf, conf_fine, seed=seed, device=device, measure_conf=conf))
     pairs = fine_run_pairs(cosmo, conf, conf_fine, seeds=[seed], every=4, device=device)
     loss_c = mean_force_loss(model, pairs, cosmo, conf)
     loss_p = mean_force_loss(None, pairs, cosmo, conf)
-    print(f"[c4] seed {seed}: P(k) err to k=1 max {res['corrected']['pk_err_max']:.4f} "
+    print(f"[c4] {weights} seed {seed}: P(k) err to k=1 max {res['corrected']['pk_err_max']:.4f} "
           f"(plain {res['plain']['pk_err_max']:.4f}), mean {res['corrected']['pk_err_mean']:.4f} "
           f"(plain {res['plain']['pk_err_mean']:.4f}); r {res['corrected']['r_mean']:.6f} "
           f"(plain {res['plain']['r_mean']:.6f}); force loss {loss_c:.6f} (plain {loss_p:.6f})", flush=True)
@@ -57,7 +64,7 @@ def test_shipped_correction_on_held_out_seeds(device, seed):
 def test_training_recipe_lowers_validation_force_loss(device):
     conf, conf_fine = _confs()
     cosmo = LCDM()
-    meta = correction_meta()
+    meta = correction_meta(TRAINED_DIR / TRAINED_DENSITY_ONLY)
     pairs = fine_run_pairs(cosmo, conf, conf_fine, seeds=range(100, 104), every=2, device=device)
     val = fine_run_pairs(cosmo, conf, conf_fine, seeds=[999], every=4, device=device)
     model, losses = train_correction(pairs, cosmo, conf, iters=400, learning_rate=float(meta["lr"]),
