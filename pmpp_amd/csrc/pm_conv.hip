@@ -700,9 +700,15 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
 // potential's tile is staged beside the density's and the stem adds its second MFMA step, as
 // k_conv_stem<kStemPotential>).  The stem
 // activation (32 bytes per voxel written, then read back 2.8x over by the hidden layer's halo
-// tiles) never reaches HBM; the price is the stem's MFMA on 2.1x the block's voxels, one MFMA
-// per 16 of them.  Persistent and prefetching (the next block's density tile in registers
-// while the current block is computed), with the hidden layer's block order.
+// tiles) never reaches HBM; the price is the stem's MFMA and GELU on more voxels than the
+// block has.  The stem's GELUs, not the matrix cores, bound the kernel (per 16 voxels a stem
+// tile issues ~38 VALU instructions and 8 quarter-rate transcendentals against one MFMA), so
+// the grid walks runs of consecutive blocks along x: block bx's staged planes 0 and 1 are
+// block bx - 1's planes 4 and 5, copied within LDS, and only planes 2 .. 5 (720 voxels, 1.41
+// per output voxel instead of 2.11) go through the stem.  A work item is one (y, z) column
+// of blocks and a range of `run` x blocks; the workgroups one XCD runs at once take
+// consecutive items (neighbouring columns).  Persistent and prefetching (the next block's
+// density tile in registers while the current block is computed).
 constexpr int kFRX = 8, kFRY = 12, kFRZ = 20;                // density tile: block + two-voxel halo
 constexpr int kFRN = kFRX * kFRY * kFRZ;                     // 1920
 constexpr int kFRPer = (kFRN + kThreads - 1) / kThreads;     // 8
@@ -715,13 +721,14 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
                                                                   const float* __restrict__ bs,
                                                                   const __bf16* __restrict__ wh,
                                                                   const float* __restrict__ bh,
-                                                                  __bf16* __restrict__ act, int n0, int n1, int n2) {
+                                                                  __bf16* __restrict__ act, int n0, int n1, int n2,
+                                                                  int run) {
   using Geo = FwdGeom<16>;
   using Walk = PlaneWalk<16, true, false, kPlain, 16>;
   using Taps = PlaneTaps<16, Geo::SZ>;
   constexpr int TPS = Taps::TPS, KS = Taps::KS;
   constexpr int NS = Geo::NS, SZ = Geo::SZ, SY = Geo::SY;
-  constexpr int BX = Geo::BX, BY = Geo::BY, BZ = Geo::BZ;
+  constexpr int BX = Geo::BX, BY = Geo::BY, BZ = Geo::BZ, SX = Geo::SX;
   constexpr int STEM_TILES = (NS + 15) / 16;  // 68
   constexpr int NT = PHI ? 2 : 1;             // staged input channels of the stem
   constexpr Taps taps{};
@@ -794,17 +801,28 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
     const int z = it % kFRZ, y = (it / kFRZ) % kFRY, xx = it / (kFRZ * kFRY);
     loc[u] = (xx * n1 + y) * n2 + z;
   }
-  const int nbz = n2 / BZ, nby = n1 / BY;
-  const int nblk = (n0 / BX) * nby * nbz;
-  const int G = gridDim.x, per = G / 8, first = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int nbz = n2 / BZ, nby = n1 / BY, nbx = n0 / BX;
+  const int ncol = nby * nbz;
+  const int nitems = ncol * ((nbx + run - 1) / run);
+  const int G = gridDim.x, per = G / 8;
+  // the next block to stage: item `item` (column item % ncol, x range item / ncol), x block nbxi
+  int item = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  int nbxi = (item / ncol) * run;
+  constexpr int CP = 2 * SY * SZ;  // voxels of the two carried planes (per channel half)
   typedef float Prefetch __attribute__((ext_vector_type(kFRPer)));
-  int cur = -1, nxt = first;
+  bool have_cur = false;
+  int64_t ecur = 0;  // origin voxel of the block being walked
   for (;;) {
     Prefetch pf, pp;
-    if (nxt < nblk) {
-      int bx, by, bz;
-      block_of(nxt, n0 / BX, nby, nbz, bx, by, bz);
+    const bool have_nxt = item < nitems;
+    bool carry = false;
+    int64_t enxt = 0;
+    if (have_nxt) {
+      const int col = item % ncol;
+      const int by = col / nbz, bz = col % nbz, bx = nbxi;
+      carry = nbxi > (item / ncol) * run;  // block bx - 1 of the same run was staged last
       const int ox = bx * BX, oy = by * BY, oz = bz * BZ;
+      enxt = ((int64_t)ox * n1 + oy) * n2 + oz;
       const bool inner = ox >= 2 && ox + BX + 2 <= n0 && oy >= 2 && oy + BY + 2 <= n1 && oz >= 2 && oz + BZ + 2 <= n2;
       if (inner) {
         const int64_t o = ((int64_t)(ox - 2) * n1 + (oy - 2)) * n2 + (oz - 2);
@@ -826,16 +844,13 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
         }
       }
     }
-    if (cur >= 0) {
-      int bx, by, bz;
-      block_of(cur, n0 / BX, nby, nbz, bx, by, bz);
-      const int64_t e0 = (((int64_t)bx * BX * n1 + by * BY) * n2 + bz * BZ);
-      wk.el = e0 + (int64_t)ly * n2 + lz;
+    if (have_cur) {
+      wk.el = ecur + (int64_t)ly * n2 + lz;
       wk.read_plane(0, 0, 0);
       wk.template walk<0, 0>();
     }
-    if (nxt >= nblk) break;
-    __syncthreads();  // the hidden layer's MFMAs of block cur are done with the tile
+    if (!have_nxt) break;
+    __syncthreads();  // the hidden layer's MFMAs of the current block are done with the tile
 #pragma unroll
     for (int u = 0; u < kFRPer; ++u) {
       const int i = threadIdx.x + u * kThreads;
@@ -844,10 +859,20 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
         if constexpr (PHI) rt[1][i] = (__bf16)(pp[u] * pinv);
       }
     }
+    if (carry) {
+      // staged planes SX - 2, SX - 1 of the block just walked are planes 0, 1 of this one
+      for (int i = threadIdx.x; i < 2 * CP; i += kThreads) {
+        const int h = i / CP, v = i % CP;
+        *reinterpret_cast<uint4*>(xin + (h * NS + v) * 8) =
+            *reinterpret_cast<const uint4*>(xin + (h * NS + (SX - 2) * SY * SZ + v) * 8);
+      }
+    }
     __syncthreads();
-    // the stem on the staged voxels of block nxt, into the hidden layer's tile: a wave takes the
-    // tiles wave, wave + 4, ... two at a time, so one tile's LDS reads and MFMA overlap the
-    // other's GELU (the same arithmetic per voxel as one at a time)
+    // the stem on the staged voxels of the next block (from plane 2 on when planes 0, 1 were
+    // carried), into the hidden layer's tile: a wave takes the tiles t0 + wave, t0 + wave + 4,
+    // ... two at a time, so one tile's LDS reads and MFMA overlap the other's GELU (the same
+    // arithmetic per voxel as one at a time)
+    const int v0 = carry ? CP : 0;
     auto stem_mfma = [&](int t) {
       // density-tile cell of staged voxel vc = (sx, sy, sz), q = vc / SZ = sx SY + sy:
       // ((sx + 1) kFRY + sy + 1) kFRZ + sz + 1 = vc + q (kFRZ - SZ) + sx (kFRY - SY) kFRZ + c0
@@ -871,9 +896,9 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
       bf16x4 o;
 #pragma unroll
       for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i] + sbv[i]);
-      if (v < NS) *reinterpret_cast<bf16x4*>(xin + ((g / 2) * NS + v) * 8 + 4 * (g % 2)) = o;
+      if (v >= v0 && v < NS) *reinterpret_cast<bf16x4*>(xin + ((g / 2) * NS + v) * 8 + 4 * (g % 2)) = o;
     };
-    int t = wave;
+    int t = v0 / 16 + wave;
     for (; t + 4 < STEM_TILES; t += 8) {
       const f32x4 c0 = stem_mfma(t);
       const f32x4 c1 = stem_mfma(t + 4);
@@ -882,8 +907,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
     }
     if (t < STEM_TILES) stem_store(t, stem_mfma(t));
     __syncthreads();
-    cur = nxt;
-    nxt += G;
+    have_cur = true;
+    ecur = enxt;
+    if (++nbxi == min(nbx, (item / ncol + 1) * run)) {
+      item += G;
+      nbxi = (item / ncol) * run;
+    }
   }
 }
 
@@ -1214,6 +1243,16 @@ int persistent_grid(const void* kern, int64_t nblk, unsigned* grid, int threads 
   return PM_SUCCESS;
 }
 
+// x runs of a persistent grid over (y, z) columns of blocks (k_conv_stem_hidden):
+// whole x lines when there are enough columns for every workgroup the grid holds, halved until
+// there are work items for all of them; the grid is cut to the work items there are.
+int64_t x_run(int64_t nbx, int64_t ncol, unsigned* grid) {
+  int64_t run = nbx;
+  while (run > 1 && ncol * ceil_div(nbx, run) < (int64_t)*grid) run = ceil_div(run, 2);
+  *grid = (unsigned)std::min<int64_t>(*grid, ceil_div(ncol * ceil_div(nbx, run), 8) * 8);
+  return run;
+}
+
 // The one-output-channel layer with the taps on the matrix rows (k_conv_out1): a persistent
 // grid over the 8 x 8 x 16 blocks (the next block's tile prefetched while one is computed)
 int launch_out1(const __bf16* x, const __bf16* w, const float* bias, float* pre, const float* rho,
@@ -1340,9 +1379,10 @@ int pm_conv3d_stem_hidden(const float* rho, const float* phi, const int32_t shap
   unsigned grid = 0;
   auto kern = phi ? conv::k_conv_stem_hidden<true> : conv::k_conv_stem_hidden<false>;
   PM_TRY(persistent_grid(reinterpret_cast<const void*>(kern), nblocks(shape), &grid));
+  const int64_t run = x_run(shape[0] / 4, (int64_t)(shape[1] / 8) * (shape[2] / 16), &grid);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(conv::kThreads), 0, s, rho, phi, 1.5f * omega_m, stats, a, omega_m,
                      sigma8, static_cast<const __bf16*>(w_stem), b_stem, static_cast<const __bf16*>(w_hidden),
-                     b_hidden, static_cast<__bf16*>(act), shape[0], shape[1], shape[2]);
+                     b_hidden, static_cast<__bf16*>(act), shape[0], shape[1], shape[2], (int)run);
   PM_LAUNCHED("k_conv_stem_hidden", s);
   return PM_SUCCESS;
 }

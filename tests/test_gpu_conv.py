@@ -152,11 +152,12 @@ def test_weight_gradient_cin8(device):
     np.testing.assert_allclose(dw.cpu().numpy(), wr.grad.numpy(), atol=1e-4 * wr.grad.abs().max().item())
 
 
-@pytest.mark.parametrize("shape", [(8, 8, 16), (24, 16, 32), (32, 16, 48)])
+@pytest.mark.parametrize("shape", [(8, 8, 16), (24, 16, 32), (32, 16, 48), (256, 32, 64), (256, 64, 128)])
 def test_one_channel_output_layer_matches_torch(device, shape):
     """The one-output-channel layer (taps on the matrix rows, k_conv_out1: meshes that divide
-    into 4 x 8 x 16 blocks) against channel 0 of the float32 convolution, on meshes of one
-    block along y and z (every y and z halo wraps) and of several blocks along each axis."""
+    into 8 x 8 x 16 blocks) against channel 0 of the float32 convolution, on meshes of one
+    block along y and z (every y and z halo wraps) and of several blocks along each axis, up to
+    meshes of more blocks than the persistent grid has workgroups."""
     g = torch.Generator().manual_seed(sum(shape))
     x = _bf(torch.randn(shape + (16,), generator=g))
     w = _bf(torch.randn(27, 16, 16, generator=g) * 0.2)
