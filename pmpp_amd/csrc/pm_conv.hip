@@ -61,11 +61,18 @@ __device__ __forceinline__ float gelu(float x) {
   return 0.5f * x * (1.f + tanhf(c * (x + 0.044715f * x * x * x)));
 }
 
+// Derivative of the tanh form: with s = sigmoid(2u) = 0.5 (1 + tanh u) and 1 - tanh^2 u =
+// 4 s (1 - s), d/dx [0.5 x (1 + tanh u)] = s + 2 x s (1 - s) u', u' = sqrt(2/pi) (1 + 3 *
+// 0.044715 x^2): one hardware exp2 and one reciprocal, as gelu_fast, instead of the library
+// tanh (about a third of the instructions).  Saturates to 0 and 1.
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float c = 0.7978845608028654f;
-  const float u = c * (x + 0.044715f * x * x * x);
-  const float t = tanhf(u);
-  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * c * (1.f + 3.f * 0.044715f * x * x);
+  constexpr float c = 0.7978845608028654f;
+  constexpr float k1 = -2.f * c * 1.4426950408889634f;  // -2 sqrt(2/pi) log2(e)
+  constexpr float k3 = k1 * 0.044715f;
+  const float x2 = x * x;
+  const float s = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(x * (k1 + k3 * x2)));
+  const float du = c + (3.f * 0.044715f * c) * x2;
+  return s + 2.f * x * (s * (1.f - s)) * du;
 }
 
 // Voxel m (0..15) of M tile `tile`: tiles are 2 y-rows x 8 z of one x plane.
