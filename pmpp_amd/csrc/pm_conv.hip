@@ -257,7 +257,7 @@ struct PlaneWalk {
     if constexpr (RES) rv = __builtin_bit_cast(bf16x4, make_uint2(early[2 * q], early[2 * q + 1]));
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      zv[i] = c[i] + bsv[i];
+      zv[i] = c[i];  // the bias was the accumulator's initial value
       if constexpr (RES) zv[i] += (float)rv[i];
     }
     if constexpr (EPI == kOutResidual) {
@@ -286,7 +286,7 @@ struct PlaneWalk {
       read_plane((P + 1) % 2, R, P + 1);
     else if constexpr (R + 1 < Geo::ROWS)
       read_plane((P + 1) % 2, R + 1, 0);
-    if constexpr (P < BX) acc[P % 3] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (P < BX) acc[P % 3] = f32x4{bsv[0], bsv[1], bsv[2], bsv[3]};  // the bias enters as C
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       // output plane t = P - dx' takes dx' = P - t
@@ -649,7 +649,7 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
         bf16x8 b;
 #pragma unroll
         for (int j = 0; j < 8; ++j) b[j] = xin[0][toff[j] + q * kH * kH];
-        f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, b, f32x4{bsv[0], bsv[1], bsv[2], bsv[3]}, 0, 0, 0);
         if constexpr (PHI) {
           bf16x8 bp;
 #pragma unroll
@@ -675,7 +675,7 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
         } else {
           bf16x4 o;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i] + bsv[i]);
+          for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i]);
           *reinterpret_cast<bf16x4*>(act + e * kCout + 4 * g) = o;
         }
       }
@@ -889,7 +889,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
       bf16x8 b;
 #pragma unroll
       for (int j = 0; j < 8; ++j) b[j] = rt[0][c + soff[j]];
-      f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsa, b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsa, b, f32x4{sbv[0], sbv[1], sbv[2], sbv[3]}, 0, 0, 0);
       if constexpr (PHI) {
         bf16x8 bp;
 #pragma unroll
@@ -902,7 +902,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
       const int v = 16 * t + m;
       bf16x4 o;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i] + sbv[i]);
+      for (int i = 0; i < 4; ++i) o[i] = (__bf16)gelu_fast(cv[i]);
       if (v >= v0 && v < NS) *reinterpret_cast<bf16x4*>(xin + ((g / 2) * NS + v) * 8 + 4 * (g % 2)) = o;
     };
     int t = v0 / 16 + wave;
