@@ -497,12 +497,20 @@ def network_input_vjp(model: MeshCNNPotentialCorrection, rho, rho_cot, a, omega_
     else:
         for x0 in range(0, n0, slab):
             s = min(slab, n0 - x0)
-            idx = torch.arange(x0 - halo, x0 + s + halo, device=dev).remainder_(n0)
-            rho_e = rho.index_select(0, idx)
-            phi_e = None if phi is None else phi.index_select(0, idx)
+            lo, hi = x0 - halo, x0 + s + halo
+            if lo >= 0 and hi <= n0:  # the extended slab does not wrap: contiguous views, read only
+                def ext_of(t, lo=lo, hi=hi):
+                    return t[lo:hi]
+            else:
+                idx = torch.arange(lo, hi, device=dev).remainder_(n0)
+
+                def ext_of(t, idx=idx):
+                    return t.index_select(0, idx)
+            rho_e = ext_of(rho)
+            phi_e = None if phi is None else ext_of(phi)
             _, ext = evaluate_residual(model, rho_e, a, om, keep=True, stats=tape.stats, phi=phi_e,
-                                       output=tape.out.index_select(0, idx))
-            dx0_e = residual_backward(model, ext, rc.index_select(0, idx), weights=False, input_grad=True)
+                                       output=ext_of(tape.out))
+            dx0_e = residual_backward(model, ext, ext_of(rc), weights=False, input_grad=True)
             del ext
             dx0c[x0:x0 + s] = dx0_e[halo:halo + s]
             del dx0_e, rho_e, phi_e
