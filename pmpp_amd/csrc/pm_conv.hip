@@ -719,6 +719,12 @@ __global__ void __launch_bounds__(kThreads, 4) k_conv_stem(const float* __restri
 constexpr int kFRX = 8, kFRY = 12, kFRZ = 20;                // density tile: block + two-voxel halo
 constexpr int kFRN = kFRX * kFRY * kFRZ;                     // 1920
 constexpr int kFRPer = (kFRN + kThreads - 1) / kThreads;     // 8
+// LDS plane stride of the density tile (bf16 elements; rows stay kFRZ apart): with the
+// unpadded 240 the four lane groups of a stem gather (taps 8g + j, one x plane apart at the
+// extremes) overlap in bank windows, 2.14 distinct dwords per bank on average over all tiles
+// and taps, up to 3; with 292 it is 1.31 (an exhaustive search over row and plane strides)
+constexpr int kFRP = 292;
+constexpr int kFRL = kFRX * kFRP;
 
 template <bool PHI>
 __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* __restrict__ rho,
@@ -741,7 +747,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
   constexpr Taps taps{};
   static_assert(taps.ncls <= 4, "step classes");
   __shared__ __align__(16) __bf16 xin[2 * NS * 8];
-  __shared__ __align__(16) __bf16 rt[NT][kFRN];
+  __shared__ __align__(16) __bf16 rt[NT][kFRL];
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int m = lane % 16, g = lane / 16;
   const int gt = g / 2, gh = g % 2;  // hidden K group: tap slot, channel half
@@ -776,7 +782,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const int t = min(8 * g + j, kTaps - 1);
-    soff[j] = ((t / 9 - 1) * kFRY + ((t / 3) % 3 - 1)) * kFRZ + (t % 3 - 1);
+    soff[j] = (t / 9 - 1) * kFRP + ((t / 3) % 3 - 1) * kFRZ + (t % 3 - 1);
   }
   const int ly = 2 * wave, lz = m;
   {
@@ -862,8 +868,9 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
     for (int u = 0; u < kFRPer; ++u) {
       const int i = threadIdx.x + u * kThreads;
       if (i < kFRN) {
-        rt[0][i] = (__bf16)(((pf[u] - 1.f) * c) * inv);
-        if constexpr (PHI) rt[1][i] = (__bf16)(pp[u] * pinv);
+        const int l = i + (i / (kFRY * kFRZ)) * (kFRP - kFRY * kFRZ);  // padded planes
+        rt[0][l] = (__bf16)(((pf[u] - 1.f) * c) * inv);
+        if constexpr (PHI) rt[1][l] = (__bf16)(pp[u] * pinv);
       }
     }
     if (carry) {
@@ -882,10 +889,10 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
     const int v0 = carry ? CP : 0;
     auto stem_mfma = [&](int t) {
       // density-tile cell of staged voxel vc = (sx, sy, sz), q = vc / SZ = sx SY + sy:
-      // ((sx + 1) kFRY + sy + 1) kFRZ + sz + 1 = vc + q (kFRZ - SZ) + sx (kFRY - SY) kFRZ + c0
+      // (sx + 1) kFRP + (sy + 1) kFRZ + sz + 1 = vc + q (kFRZ - SZ) + sx (kFRP - SY kFRZ) + c0
       const int vc = min(16 * t + m, NS - 1);
       const int q = vc / SZ;
-      const int c = vc + q * (kFRZ - SZ) + (q / SY) * ((kFRY - SY) * kFRZ) + (kFRY * kFRZ + kFRZ + 1);
+      const int c = vc + q * (kFRZ - SZ) + (q / SY) * (kFRP - SY * kFRZ) + (kFRP + kFRZ + 1);
       bf16x8 b;
 #pragma unroll
       for (int j = 0; j < 8; ++j) b[j] = rt[0][c + soff[j]];
