@@ -574,15 +574,16 @@ def test_whole_run_loss_parameter_gradient_matches_central_differences(device):
 
 
 @pytest.mark.parametrize("shape,potential", [((8, 8, 16), False), ((32, 32, 48), False), ((64, 64, 64), False),
-                                             ((128, 128, 128), False), ((256, 64, 64), False),
+                                             ((128, 128, 128), False), ((256, 64, 64), False), ((200, 128, 128), False),
                                              ((8, 8, 16), True), ((64, 64, 64), True), ((128, 128, 128), True)])
 def test_fused_stem_and_hidden_layer_equal_the_two_kernels(device, shape, potential):
     """pm_conv3d_stem_hidden (the stem computed on the hidden layer's staged tile in LDS) against
     pm_conv3d_stem followed by the hidden conv: the same arithmetic in the same order, so the
     activations agree bit for bit; meshes of one block along y and z (every halo wraps) and of
     several blocks along each axis; with and without the potential channel.  On the small meshes
-    every block is its own x run; at 128^3 the runs are 8 blocks long and at 256 x 64 x 64 four
-    (two staged planes carried from the previous block in LDS for 7 of 8 and 3 of 4 blocks)."""
+    every block is its own x run; at 128^3 the runs are 8 blocks long, at 256 x 64 x 64 four
+    (two staged planes carried from the previous block in LDS for 7 of 8 and 3 of 4 blocks) and
+    at 200 x 128 x 128 thirteen, the last run of each x line eleven."""
     from pmpp_amd import _lib
     from pmpp_amd._torch import ptr, stream_handle
     from pmpp_amd.corrections import conv3d
