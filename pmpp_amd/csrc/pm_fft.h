@@ -165,6 +165,26 @@ struct Plan {
   static_assert(N1 * N2 == N, "power of two");
 };
 
+// The twiddle table through a pointer the compiler cannot see through: table loads made
+// through it are not merged with loads of the same entries made before it.  A kernel with a
+// forward and an inverse transform otherwise keeps the forward's N1 - 1 twiddles live
+// (2 (N1 - 1) VGPRs) until the inverse uses the same entries again.
+__device__ __forceinline__ const float2* fresh_table(const float2* t) {
+  uint64_t u = reinterpret_cast<uint64_t>(t);
+  asm volatile("" : "+s"(u));
+  return reinterpret_cast<const float2*>(u);
+}
+
+// 1/x from the hardware reciprocal and one Newton step: correctly rounded but for a few
+// arguments per million (1 ulp off then), with a fraction of the registers of an IEEE
+// division, whose temporaries the compiler interleaves across a column's unrolled slots
+// (the x Green pass: 258 -> 200 VGPRs, the potential pass 226 -> 164).  x = 0 gives NaN; the
+// callers select 0 for that mode.
+__device__ __forceinline__ float recip_nr(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, r, 1.f), r, r);
+}
+
 // W_N^j (forward sign) from the table; INV conjugates.
 template <int N, bool INV>
 __device__ __forceinline__ float2 table_tw(const float2* __restrict__ tw, int j) {

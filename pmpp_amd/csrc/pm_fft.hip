@@ -63,10 +63,12 @@ namespace fft {
 namespace {
 
 // ----------------------------------------------------------- column passes ----
-// T adjacent columns per workgroup.  Two-output passes run two workgroups per tile (HALVES),
-// each of which loads the same input tile (the second read is served by L2: the pair runs
-// on one XCD at nearly the same time), runs the forward transform and inverts one of the
-// two outputs, so no thread keeps both spectra live; their outputs must not alias the input.
+// T adjacent columns per workgroup.  The y inverse with two outputs (kColInv2) runs two
+// workgroups per tile (HALVES), each of which loads the same input tile (the second read is
+// served by L2: the pair runs on one XCD at nearly the same time) and inverts one of the two
+// outputs, so no thread keeps both spectra live.  The x Green pass (kColGreen) runs one
+// workgroup per tile that parks its multiplied spectrum in its first output between the two
+// inverses.  Outputs of two-output passes must not alias the input or each other.
 template <int N, int T, int HALVES>
 struct ColCfg {
   static constexpr int P = Plan<N>::P;
@@ -103,6 +105,11 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
   const int64_t lbase = b * io.s_batch + (int64_t)cgl * io.s_col;
   const int64_t o_row = io.o_row >= 0 ? io.o_row : io.s_row;
   const int64_t obase = b * (io.o_batch >= 0 ? io.o_batch : io.s_batch) + (int64_t)cg * io.s_col;
+  // row r = p + d of this thread's column: base + d * stride, with d a compile-time multiple of
+  // the row step for every unrolled slot, so the per-slot offsets are uniform (scalar) and no
+  // per-slot row index stays live in a VGPR across the transforms
+  const int64_t ibase = lbase + (int64_t)p * io.s_row;
+  const int64_t obp = obase + (int64_t)p * o_row;
   const int keep = io.keep;
   // band compaction of the transformed axis (see ColPass::keep)
   auto in_band = [&](int k) { return keep < 0 || k <= keep || k >= N - keep; };
@@ -126,7 +133,7 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
     const bool keep_yz = cok && tb.mask[1][ky] != 0.f && tb.mask[2][kz] != 0.f;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const int64_t o = lbase + (int64_t)nrow<N>(p, s) * io.s_row;
+      const int64_t o = ibase + (int64_t)(nrow<N>(p, s) - p) * io.s_row;
       const float2 gy = io.in1[o], gz = io.in2[o];
       v[s] = make_float2(nky * gy.x + nkz * gz.x, nky * gy.y + nkz * gz.y);
     }
@@ -134,7 +141,7 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
     asm volatile("" ::: "memory");  // the x component's loads after U's transform
     float2 w[E];
 #pragma unroll
-    for (int s = 0; s < E; ++s) w[s] = io.in[lbase + (int64_t)nrow<N>(p, s) * io.s_row];
+    for (int s = 0; s < E; ++s) w[s] = io.in[ibase + (int64_t)(nrow<N>(p, s) - p) * io.s_row];
     forward_core<N>(w, my, p, idx, tw);
 #pragma unroll
     for (int s = 0; s < E; ++s) {
@@ -143,17 +150,18 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
       const float tx = nkx * w[s].x + v[s].x, ty = nkx * w[s].y + v[s].y;  // s = -i t
       const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
       const bool keep = keep_yz && spec_keep(kxv, tb) && k2 != 0.f;
-      v[s] = keep ? make_float2(-ty / k2, tx / k2) : make_float2(0.f, 0.f);  // -s / k^2
+      const float r = recip_nr(k2);
+      v[s] = keep ? make_float2(-ty * r, tx * r) : make_float2(0.f, 0.f);  // -s / k^2
     }
     inverse_core<N>(v, my, p, idx, tw);
     if (cok) {
 #pragma unroll
-      for (int s = 0; s < E; ++s) io.out0[obase + (int64_t)nrow<N>(p, s) * o_row] = v[s];
+      for (int s = 0; s < E; ++s) io.out0[obp + (int64_t)(nrow<N>(p, s) - p) * o_row] = v[s];
     }
     return;
   } else if constexpr (MODE == kColFwd || MODE == kColGreen || MODE == kColPot) {
 #pragma unroll
-    for (int s = 0; s < E; ++s) v[s] = io.in[lbase + (int64_t)nrow<N>(p, s) * io.s_row];
+    for (int s = 0; s < E; ++s) v[s] = io.in[ibase + (int64_t)(nrow<N>(p, s) - p) * io.s_row];
     forward_core<N>(v, my, p, idx, tw);
   } else {
     if (keep >= 0 && io.compact_in) {  // packed band in, zero outside it
@@ -167,7 +175,7 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
         if (!in_band(xrow<N>(p, s))) v[s] = make_float2(0.f, 0.f);
     } else {
 #pragma unroll
-      for (int s = 0; s < E; ++s) v[s] = io.in[lbase + (int64_t)xrow<N>(p, s) * io.s_row];
+      for (int s = 0; s < E; ++s) v[s] = io.in[ibase + (int64_t)(xrow<N>(p, s) - p) * io.s_row];
     }
   }
 
@@ -197,9 +205,32 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
         float kxv = spec_k(kx, N, false, tb.period);
         float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
         bool keep = keep_yz && spec_keep(kxv, tb) && k2 != 0.f;
-        float g = keep ? -1.f / k2 : 0.f;  // one division per mode, shared by both parts
+        const float r = recip_nr(k2);
+        float g = keep ? -r : 0.f;  // one reciprocal per mode, shared by both parts
         float2 phi = make_float2(v[s].x * g, v[s].y * g);
         v[s] = (half == 1) ? c_imul(spec_negk(kxv, tb), phi) : phi;
+      }
+      if constexpr (MODE == kColGreen && HALVES == 1) {
+        // one workgroup per tile: the multiplied spectrum phi is parked in out0 at this
+        // thread's own output rows (where it writes its out0 values at the end, so no other
+        // thread touches them; the tile round-trips through L2), D_x phi is inverted into
+        // out1, then phi is read back and inverted into out0 -- three transforms per tile
+        // instead of four (two workgroups that each redo the forward)
+        if (cok) {
+#pragma unroll
+          for (int s = 0; s < E; ++s) io.out0[obp + (int64_t)(nrow<N>(p, s) - p) * o_row] = v[s];
+        }
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = c_imul(spec_negk(spec_k(xrow<N>(p, s), N, false, tb.period), tb), v[s]);
+        inverse_core<N>(v, my, p, idx, fresh_table(tw));
+        if (cok) {
+#pragma unroll
+          for (int s = 0; s < E; ++s) io.out1[obp + (int64_t)(nrow<N>(p, s) - p) * o_row] = v[s];
+        }
+        asm volatile("" ::: "memory");  // no twiddle of the first inverse kept live for the second
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+          v[s] = cok ? io.out0[obp + (int64_t)(nrow<N>(p, s) - p) * o_row] : make_float2(0.f, 0.f);
       }
     } else if constexpr (MODE == kColInv2) {
       if (half == 1) {
@@ -207,11 +238,13 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
         for (int s = 0; s < E; ++s) v[s] = c_imul(tb.negk[dax][xrow<N>(p, s)], v[s]);
       }
     }
-    inverse_core<N>(v, my, p, idx, tw);
+    // after a forward transform in the same kernel (kColGreen, kColPot) the inverse's twiddles
+    // are loaded afresh instead of kept live from the forward
+    inverse_core<N>(v, my, p, idx, (MODE == kColGreen || MODE == kColPot) ? fresh_table(tw) : tw);
     float2* out = (half == 1) ? io.out1 : io.out0;
     if (cok) {
 #pragma unroll
-      for (int s = 0; s < E; ++s) out[obase + (int64_t)nrow<N>(p, s) * o_row] = v[s];
+      for (int s = 0; s < E; ++s) out[obp + (int64_t)(nrow<N>(p, s) - p) * o_row] = v[s];
     }
   }
 }
@@ -411,8 +444,8 @@ int launch_cols_cfg(const ColPass& cp, int batches, const SpecTables& tb, int da
   const int gx = (int)ceil_div(cp.ncols, T);
   const int64_t tiles = (int64_t)gx * batches;
   const int64_t g = ceil_div(tiles * HALVES, 8) * 8;
-  PM_RETURN_IF(HALVES > 1 && (cp.in == cp.out0 || cp.in == cp.out1), PM_ERROR_INVALID_ARGUMENT,
-               "two-output FFT passes need outputs distinct from their input");
+  PM_RETURN_IF((HALVES > 1 || MODE == kColGreen) && (cp.in == cp.out0 || cp.in == cp.out1 || cp.out0 == cp.out1),
+               PM_ERROR_INVALID_ARGUMENT, "two-output FFT passes need outputs distinct from their input");
   PM_RETURN_IF(g > INT32_MAX, PM_ERROR_UNSUPPORTED, "too many FFT column tiles");
   hipLaunchKernelGGL((k_cols<N, T, MODE, HALVES>), dim3((unsigned)g), dim3(CF::THREADS), 0, s, cp, tb, dax, tw,
                      gx, batches, (int)(g / 8));
@@ -430,7 +463,7 @@ int launch_cols_n(int mode, const ColPass& cp, int batches, const SpecTables& tb
     case kColFwd: return launch_cols_cfg<N, T1, kColFwd, 1>(cp, batches, tb, dax, s, "fft_cols_fwd");
     case kColInv: return launch_cols_cfg<N, T1, kColInv, 1>(cp, batches, tb, dax, s, "fft_cols_inv");
     case kColInv2: return launch_cols_cfg<N, T2, kColInv2, 2>(cp, batches, tb, dax, s, "fft_cols_inv2");
-    case kColGreen: return launch_cols_cfg<N, T2, kColGreen, 2>(cp, batches, tb, dax, s, "fft_cols_green");
+    case kColGreen: return launch_cols_cfg<N, T2, kColGreen, 1>(cp, batches, tb, dax, s, "fft_cols_green");
     case kColAdj: return launch_cols_cfg<N, T1, kColAdj, 1>(cp, batches, tb, dax, s, "fft_cols_adjoint");
     case kColPot: return launch_cols_cfg<N, T2, kColPot, 1>(cp, batches, tb, dax, s, "fft_cols_potential");
     default: return fail(PM_ERROR_INVALID_ARGUMENT, "bad column mode");
