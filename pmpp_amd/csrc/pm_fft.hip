@@ -213,9 +213,10 @@ __global__ void __launch_bounds__((ColCfg<N, T, HALVES>::THREADS))
       if constexpr (MODE == kColGreen && HALVES == 1) {
         // one workgroup per tile: the multiplied spectrum phi is parked in out0 at this
         // thread's own output rows (where it writes its out0 values at the end, so no other
-        // thread touches them; the tile round-trips through L2), D_x phi is inverted into
-        // out1, then phi is read back and inverted into out0 -- three transforms per tile
-        // instead of four (two workgroups that each redo the forward)
+        // thread touches them), D_x phi is inverted into out1, then phi is read back and
+        // inverted into out0 -- three transforms per tile instead of four (two workgroups that
+        // each redo the forward); the parked tiles overflow L2, so the pass moves 1.8x its
+        // algorithmic bytes, and is faster anyway: the transforms bound it, not HBM
         if (cok) {
 #pragma unroll
           for (int s = 0; s < E; ++s) io.out0[obp + (int64_t)(nrow<N>(p, s) - p) * o_row] = v[s];
