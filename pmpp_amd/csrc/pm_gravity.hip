@@ -32,6 +32,26 @@ __global__ void k_fill(float* __restrict__ p, int64_t n, float v) {
     p[i] = v;
 }
 
+// dst = src for a mesh handed back to the caller: 16-byte accesses, four in flight per
+// thread (a device-to-device hipMemcpyAsync of the 4 GiB density at 1024^3 ran at 2.1 TB/s)
+__global__ void __launch_bounds__(kBlock) k_copy_mesh(const float* __restrict__ src, float* __restrict__ dst,
+                                                      int64_t n) {
+  const int64_t n4 = n / 4;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  const float4* s4 = reinterpret_cast<const float4*>(src);
+  float4* d4 = reinterpret_cast<float4*>(dst);
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    const float4 a = s4[i], b = s4[i + stride], c = s4[i + 2 * stride], d = s4[i + 3 * stride];
+    d4[i] = a;
+    d4[i + stride] = b;
+    d4[i + 2 * stride] = c;
+    d4[i + 3 * stride] = d;
+  }
+  for (; i < n4; i += stride) d4[i] = s4[i];
+  for (int64_t j = 4 * n4 + (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += stride) dst[j] = src[j];
+}
+
 __global__ void __launch_bounds__(kBlock) k_scatter_sorted(const int16_t* __restrict__ pmid,
                                                            const float* __restrict__ disp,
                                                            const uint8_t* __restrict__ valid, int64_t n,
@@ -1462,7 +1482,12 @@ int pm_gravity_density(const int16_t* pmid, const float* disp, const uint8_t* va
                "pmid/disp/rho NULL");
   st = grav::density_stage(pmid, disp, valid, n, desc, L, b, ws, s);
   if (st) return st;
-  PM_HIP_TRY(hipMemcpyAsync(rho, b.r0, sizeof(float) * L.nreal, hipMemcpyDeviceToDevice, s));
+  if (((reinterpret_cast<uintptr_t>(rho) | reinterpret_cast<uintptr_t>(b.r0)) & 15) == 0) {
+    hipLaunchKernelGGL(grav::k_copy_mesh, dim3(2048), dim3(grav::kBlock), 0, s, b.r0, rho, (int64_t)L.nreal);
+    PM_LAUNCHED("k_copy_mesh", s);
+  } else {
+    PM_HIP_TRY(hipMemcpyAsync(rho, b.r0, sizeof(float) * L.nreal, hipMemcpyDeviceToDevice, s));
+  }
   return PM_SUCCESS;
 }
 

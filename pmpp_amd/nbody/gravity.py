@@ -10,6 +10,8 @@ through ``torch.autograd`` with cotangents for ``disp`` and ``Omega_m``.
 
 from __future__ import annotations
 
+import weakref
+
 import torch
 
 from .. import _lib
@@ -36,12 +38,25 @@ def _check_particles(pm, dp):
     return n
 
 
+# the uint8 validity mask of an unused_index tensor, kept while that tensor lives and is not
+# modified in place (its _version), so that a run whose particles carry the same mask through
+# every step converts it once instead of at every force
+_VALID_MASKS = {}  # id(unused_index) -> (weak reference to it, its _version, the mask)
+
+
 def _valid_u8(ptcl):
-    if ptcl.unused_index is None:
+    ui = ptcl.unused_index
+    if ui is None:
         return None
-    if ptcl.unused_index.shape != (ptcl.pmid.shape[0],):
-        raise ValueError(f"unused_index has shape {tuple(ptcl.unused_index.shape)}, expected ({ptcl.pmid.shape[0]},)")
-    return (~ptcl.unused_index.detach()).to(torch.uint8).contiguous()
+    if ui.shape != (ptcl.pmid.shape[0],):
+        raise ValueError(f"unused_index has shape {tuple(ui.shape)}, expected ({ptcl.pmid.shape[0]},)")
+    key = id(ui)
+    hit = _VALID_MASKS.get(key)
+    if hit is not None and hit[0]() is ui and hit[1] == ui._version:
+        return hit[2]
+    valid = (~ui.detach()).to(torch.uint8).contiguous()
+    _VALID_MASKS[key] = (weakref.ref(ui, lambda _r, k=key: _VALID_MASKS.pop(k, None)), ui._version, valid)
+    return valid
 
 
 class _GravityFn(torch.autograd.Function):

@@ -95,3 +95,33 @@ def test_correction_parameter_layout_and_initialisation():
     assert abs(float(conv1.std()) - math.sqrt(1 / (27 * 16))) < 0.05 * math.sqrt(1 / (27 * 16))
     assert torch.all(p[layers[0].nweights:layers[0].size] == 0)  # zero biases
     assert 0 < float(p[gain:gain + 3].abs().max()) < 0.05 and p[gain + 3] == 0
+
+
+def test_validity_mask_is_converted_once_per_mask_and_follows_in_place_changes():
+    """The force step's uint8 validity mask (1 = valid) is the negation of ``unused_index``; it
+    is converted once while the same mask tensor is passed unchanged (an N-body run carries one
+    mask through every step), again after an in-place change, and separately for another mask."""
+    import gc
+    from types import SimpleNamespace
+
+    import torch
+
+    from pmpp_amd.nbody.gravity import _VALID_MASKS, _valid_u8
+
+    ui = torch.tensor([False, True, False, False])
+    p = SimpleNamespace(unused_index=ui, pmid=torch.zeros(4, 3, dtype=torch.int16))
+    first = _valid_u8(p)
+    assert first.dtype == torch.uint8 and first.tolist() == [1, 0, 1, 1]
+    assert _valid_u8(p) is first
+    ui[3] = True
+    changed = _valid_u8(p)
+    assert changed is not first and changed.tolist() == [1, 0, 1, 0]
+    other = SimpleNamespace(unused_index=ui.clone(), pmid=p.pmid)
+    assert _valid_u8(other) is not changed and _valid_u8(other).tolist() == [1, 0, 1, 0]
+    assert _valid_u8(SimpleNamespace(unused_index=None, pmid=p.pmid)) is None
+    with pytest.raises(ValueError, match="unused_index has shape"):
+        _valid_u8(SimpleNamespace(unused_index=ui[:2], pmid=p.pmid))
+    n = len(_VALID_MASKS)
+    del p, other, ui, first, changed
+    gc.collect()
+    assert len(_VALID_MASKS) < n  # entries go with their masks
