@@ -748,6 +748,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
   static_assert(taps.ncls <= 4, "step classes");
   __shared__ __align__(16) __bf16 xin[2 * NS * 8];
   __shared__ __align__(16) __bf16 rt[NT][kFRL];
+  uint32_t* rtw = reinterpret_cast<uint32_t*>(&rt[0][0]);  // PHI: kFRL (source, potential) words
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int m = lane % 16, g = lane / 16;
   const int gt = g / 2, gh = g % 2;  // hidden K group: tap slot, channel half
@@ -869,8 +870,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
       const int i = threadIdx.x + u * kThreads;
       if (i < kFRN) {
         const int l = i + (i / (kFRY * kFRZ)) * (kFRP - kFRY * kFRZ);  // padded planes
-        rt[0][l] = (__bf16)(((pf[u] - 1.f) * c) * inv);
-        if constexpr (PHI) rt[1][l] = (__bf16)(pp[u] * pinv);
+        if constexpr (PHI) {  // (source, potential) bf16 pairs: one 4-byte LDS word per cell
+          const __bf16 sv = (__bf16)(((pf[u] - 1.f) * c) * inv), pv = (__bf16)(pp[u] * pinv);
+          rtw[l] = (uint32_t)__builtin_bit_cast(uint16_t, sv) | ((uint32_t)__builtin_bit_cast(uint16_t, pv) << 16);
+        } else {
+          rt[0][l] = (__bf16)(((pf[u] - 1.f) * c) * inv);
+        }
       }
     }
     if (carry) {
@@ -893,17 +898,26 @@ __global__ void __launch_bounds__(kThreads, 2) k_conv_stem_hidden(const float* _
       const int vc = min(16 * t + m, NS - 1);
       const int q = vc / SZ;
       const int c = vc + q * (kFRZ - SZ) + (q / SY) * (kFRP - SY * kFRZ) + (kFRP + kFRZ + 1);
-      bf16x8 b;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) b[j] = rt[0][c + soff[j]];
-      f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsa, b, f32x4{sbv[0], sbv[1], sbv[2], sbv[3]}, 0, 0, 0);
       if constexpr (PHI) {
-        bf16x8 bp;
+        // taps 2i, 2i+1: the low halves of their two words are the source fragment's pair,
+        // the high halves the potential's
+        typedef uint32_t Words __attribute__((ext_vector_type(4)));
+        Words lo, hi;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) bp[j] = rt[1][c + soff[j]];
-        cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsp, bp, cv, 0, 0, 0);
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t w0 = rtw[c + soff[2 * i]], w1 = rtw[c + soff[2 * i + 1]];
+          lo[i] = __builtin_amdgcn_perm(w1, w0, 0x05040100u);
+          hi[i] = __builtin_amdgcn_perm(w1, w0, 0x07060302u);
+        }
+        const f32x4 cv = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsa, __builtin_bit_cast(bf16x8, lo),
+                                                               f32x4{sbv[0], sbv[1], sbv[2], sbv[3]}, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsp, __builtin_bit_cast(bf16x8, hi), cv, 0, 0, 0);
+      } else {
+        bf16x8 b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = rt[0][c + soff[j]];
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(wsa, b, f32x4{sbv[0], sbv[1], sbv[2], sbv[3]}, 0, 0, 0);
       }
-      return cv;
     };
     auto stem_store = [&](int t, const f32x4& cv) {
       const int v = 16 * t + m;
