@@ -2,7 +2,7 @@
 
 Mirrors ``src/pmpp/core/configuration.py`` (field names, defaults and validation messages of
 :172-304, the derived properties :463-574) for the single-device force path.  Multi-GPU fields,
-transfer-function and LPT knobs that this build does not use are not carried.
+transfer-function and LPT knobs that the single-device force path does not use are not carried.
 """
 
 from __future__ import annotations
