@@ -627,6 +627,8 @@ def main():
                          "pmpp_amd/corrections/weights/mesh_cnn_2x_from_4x.npz)")
     ap.add_argument("--no-pk-error", action="store_true",
                     help="skip the corrected 1024^3 vs plain 2048^3 P(k) comparison of the default run")
+    ap.add_argument("--pk-seed", type=int, default=0,
+                    help="white-noise seed of that P(k) comparison (default 0)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (the known-size kernel that "
                          "calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
@@ -751,7 +753,7 @@ def main():
     if not args.no_pk_error and args.config == "c5":
         if (TRAINED_DIR / TRAINED_DEFAULT).exists():
             log("[bench] P(k) of the corrected 1024^3 run against the plain 2048^3 run")
-            pk_err = pk_error_run(dev, weights=args.weights)
+            pk_err = pk_error_run(dev, seed=args.pk_seed, weights=args.weights)
         else:
             log("[bench] no trained correction weights shipped: P(k) comparison skipped")
         torch.cuda.empty_cache()
