@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PM_ABI_VERSION 8
+#define PM_ABI_VERSION 9
 
 enum pm_status {
   PM_SUCCESS = 0,
@@ -236,6 +236,14 @@ int pm_gravity_vjp_resume(const int16_t* pmid, const float* disp, const uint8_t*
 int pm_gravity_density(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                        const pm_gravity_desc* desc, float* rho, void* workspace, size_t workspace_bytes,
                        void* stream);
+/* pm_gravity_density_offset  the byte offset inside a pm_gravity_workspace (of the same desc
+ *                            and n, 256-byte aligned base) where pm_gravity_density leaves the
+ *                            density.  Passing workspace + offset as rho skips the copy out (a
+ *                            4 GiB mesh at 1024^3); the mesh there is then overwritten by
+ *                            pm_gravity_corrected_resume, so a caller that needs the density
+ *                            after the resume passes its own buffer.  No reference counterpart:
+ *                            a layout query of this library's workspace. */
+int pm_gravity_density_offset(const pm_gravity_desc* desc, int64_t n, size_t* offset);
 int pm_gravity_corrected(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                          const pm_gravity_desc* desc, const float* rho_add, float* acc, void* workspace,
                          size_t workspace_bytes, void* stream);

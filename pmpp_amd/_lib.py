@@ -14,7 +14,7 @@ import ctypes
 import os
 from pathlib import Path
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 PM_SUCCESS = 0
 PM_ERROR_INVALID_ARGUMENT = 1
@@ -33,7 +33,7 @@ EXPORTED_SYMBOLS = (
     "pm_spectral_workspace", "pm_spectral_force_multiply",
     "pm_gravity_workspace", "pm_gravity_vjp_workspace", "pm_gravity", "pm_gravity_vjp",
     "pm_gravity_keep", "pm_gravity_vjp_resume",
-    "pm_gravity_density", "pm_gravity_corrected", "pm_gravity_corrected_resume", "pm_gravity_corrected_vjp",
+    "pm_gravity_density", "pm_gravity_density_offset", "pm_gravity_corrected", "pm_gravity_corrected_resume", "pm_gravity_corrected_vjp",
     "pm_gravity_density_cot", "pm_potential_workspace", "pm_gravity_potential",
     "pm_kick", "pm_drift", "pm_nbody_adj_workspace", "pm_kick_adj", "pm_drift_adj",
     "pm_power_spectrum", "pm_power_spectrum_bwd",
@@ -108,6 +108,7 @@ _SIGNATURES = {
     "pm_spectral_force_multiply": ([_vp, _vp, ctypes.POINTER(GravityDesc), _i32, _i32, _vp, ctypes.c_size_t, _vp],
                                    ctypes.c_int),
     "pm_gravity_workspace": ([ctypes.POINTER(GravityDesc), _i64, _sz_p], ctypes.c_int),
+    "pm_gravity_density_offset": ([ctypes.POINTER(GravityDesc), _i64, _sz_p], ctypes.c_int),
     "pm_gravity_vjp_workspace": ([ctypes.POINTER(GravityDesc), _i64, _sz_p], ctypes.c_int),
     "pm_gravity": ([_vp, _vp, _vp, _i64, ctypes.POINTER(GravityDesc), _vp, _vp, ctypes.c_size_t, _vp],
                    ctypes.c_int),

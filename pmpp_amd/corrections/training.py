@@ -18,6 +18,8 @@ The optimizer mirrors ``build_correction_optimizer`` (``corrections/common.py:13
 
 from __future__ import annotations
 
+import math
+
 import dataclasses
 
 import torch
@@ -205,7 +207,12 @@ def corrected_gravity(a, ptcl, cosmo, conf, correction: MeshCNNPotentialCorrecti
     nbytes = _lib.size_query("pm_gravity_workspace", d, n)
     ws = workspace(nbytes, dev)
     s = stream_handle(dev)
-    rho = torch.empty(tuple(int(v) for v in conf.mesh_shape), dtype=torch.float32, device=dev)
+    shape = tuple(int(v) for v in conf.mesh_shape)
+    if kept is None:  # read in place: the resume overwrites it only after the network is done with it
+        off = _lib.size_query("pm_gravity_density_offset", d, n)
+        rho = ws[off:off + 4 * math.prod(shape)].view(torch.float32).view(shape)
+    else:  # kept for the VJP after the resume: a buffer of its own
+        rho = torch.empty(shape, dtype=torch.float32, device=dev)
     _lib.call("pm_gravity_density", ptr(pm), ptr(dp), ptr(valid), n, d, ptr(rho), ptr(ws), nbytes, s)
     if kept is None:
         rho_add, _ = evaluate_residual(correction, rho, a, _omega(cosmo), conf=conf)

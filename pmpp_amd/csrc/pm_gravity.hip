@@ -1468,6 +1468,21 @@ int pm_gravity_corrected_resume(const int16_t* pmid, const float* disp, const ui
   return grav::gather_stage(b.pmid_s, b.disp_s, b.valid_s, valid, n, desc, b, b.r0, b.r1, b.r2, acc, s);
 }
 
+int pm_gravity_density_offset(const pm_gravity_desc* desc, int64_t n, size_t* offset) {
+  PM_TRY(grav::check_desc(desc, n));
+  PM_RETURN_IF(offset == nullptr, PM_ERROR_INVALID_ARGUMENT, "offset is NULL");
+  // the forward carve on a stand-in base (never dereferenced): prepare() carves from the
+  // workspace start in the same order, so the density's offset is the same there
+  char* const base = reinterpret_cast<char*>(uintptr_t(1) << 12);
+  Arena ws(base, SIZE_MAX / 2);
+  grav::Buffers b{};
+  const grav::Layout L = grav::make_layout(desc);
+  PM_RETURN_IF(!grav::carve(ws, L, n, desc->mesh_shape, desc->brick, false, b), PM_ERROR_WORKSPACE,
+               "workspace carve failed");
+  *offset = (size_t)(reinterpret_cast<char*>(b.r0) - base);
+  return PM_SUCCESS;
+}
+
 int pm_gravity_density(const int16_t* pmid, const float* disp, const uint8_t* valid, int64_t n,
                        const pm_gravity_desc* desc, float* rho, void* workspace, size_t workspace_bytes,
                        void* stream) {
@@ -1482,6 +1497,7 @@ int pm_gravity_density(const int16_t* pmid, const float* disp, const uint8_t* va
                "pmid/disp/rho NULL");
   st = grav::density_stage(pmid, disp, valid, n, desc, L, b, ws, s);
   if (st) return st;
+  if (rho == b.r0) return PM_SUCCESS;  // the caller reads the density in place (pm_gravity_density_offset)
   if (((reinterpret_cast<uintptr_t>(rho) | reinterpret_cast<uintptr_t>(b.r0)) & 15) == 0) {
     hipLaunchKernelGGL(grav::k_copy_mesh, dim3(2048), dim3(grav::kBlock), 0, s, b.r0, rho, (int64_t)L.nreal);
     PM_LAUNCHED("k_copy_mesh", s);

@@ -50,6 +50,14 @@ def test_queries_validate_without_a_gpu():
     fwd = out.value
     assert lib.pm_gravity_vjp_workspace(d, 4096, ctypes.byref(out)) == _lib.PM_SUCCESS
     assert out.value > fwd + 4 * 4 * 32 ** 3  # plus the forward meshes kept and 3 cotangent meshes
+    # the density's place in the forward workspace: 256-byte aligned, the whole mesh inside it
+    for conf_, n_ in [(Configuration(1.0, (16, 16, 16), mesh_shape=2), 4096),
+                      (Configuration(1.0, (16, 16, 16), mesh_shape=2, brick=(32, 32, 32)), 4096),
+                      (Configuration(1000 / 512, (512,) * 3, mesh_shape=2), 512 ** 3)]:
+        dd = gravity_desc(conf_, 0.3)
+        ws_bytes = _lib.size_query("pm_gravity_workspace", dd, n_)
+        off = _lib.size_query("pm_gravity_density_offset", dd, n_)
+        assert off % 256 == 0 and off + 4 * conf_.mesh_size <= ws_bytes
     with pytest.raises(NotImplementedError, match="powers of two"):
         _lib.call("pm_gravity_workspace", gravity_desc(Configuration(1.0, (12, 12, 12), mesh_shape=1), 0.3),
                   10, ctypes.byref(out))
