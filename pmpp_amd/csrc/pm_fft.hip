@@ -255,8 +255,10 @@ template <int M, bool C2R>
 struct ZCfg {
   static constexpr int P = Plan<M>::P;
   // long lines fill LDS (one slot per line): smaller workgroups let several of them share a CU
-  // and interleave their load / transform / store phases
-  static constexpr int THREADS = M >= 256 ? 128 : 256;
+  // and interleave their load / transform / store phases.  From 1024-point lines on, the two
+  // directions part ways (M = 512 against 128 threads for both: c2r 1.795 -> 1.76 ms with 256,
+  // r2c 1.74 -> 1.63 ms with 64)
+  static constexpr int THREADS = M >= 512 ? (C2R ? 256 : 64) : (M >= 256 ? 128 : 256);
   static constexpr int LINES = THREADS / P;
   // floats per line slot, with LSTR = P (mod 64) so the 64/P lines of a wave start on
   // disjoint groups of the 64 LDS banks (the exchange rows of one line are P apart).  R2C
@@ -473,19 +475,23 @@ int launch_cols_n(int mode, const ColPass& cp, int batches, const SpecTables& tb
 
 template <int M>
 int launch_z_n(bool c2r, const ZPass& zp, const SpecTables& tb, hipStream_t s) {
-  using ZC = ZCfg<M, true>;  // THREADS and LINES are the same for both directions
+  using ZI = ZCfg<M, true>;   // c2r workgroups
+  using ZF = ZCfg<M, false>;  // r2c workgroups
   const float2* tw = twiddle_table();
   PM_RETURN_IF(tw == nullptr, PM_ERROR_HIP, "twiddle table not initialised");
   if (zp.nlines == 0) return PM_SUCCESS;
-  dim3 grid((unsigned)ceil_div(zp.nlines, ZC::LINES));
-  if (c2r && zp.rout_add) {
-    hipLaunchKernelGGL((k_zpass<M, true, true>), grid, dim3(ZC::THREADS), 0, s, zp, tb, tw);
-    PM_LAUNCHED("fft_z_c2r_add", s);
-  } else if (c2r) {
-    hipLaunchKernelGGL((k_zpass<M, true>), grid, dim3(ZC::THREADS), 0, s, zp, tb, tw);
-    PM_LAUNCHED("fft_z_c2r", s);
+  if (c2r) {
+    const dim3 grid((unsigned)ceil_div(zp.nlines, ZI::LINES));
+    if (zp.rout_add) {
+      hipLaunchKernelGGL((k_zpass<M, true, true>), grid, dim3(ZI::THREADS), 0, s, zp, tb, tw);
+      PM_LAUNCHED("fft_z_c2r_add", s);
+    } else {
+      hipLaunchKernelGGL((k_zpass<M, true>), grid, dim3(ZI::THREADS), 0, s, zp, tb, tw);
+      PM_LAUNCHED("fft_z_c2r", s);
+    }
   } else {
-    hipLaunchKernelGGL((k_zpass<M, false>), grid, dim3(ZC::THREADS), 0, s, zp, tb, tw);
+    const dim3 grid((unsigned)ceil_div(zp.nlines, ZF::LINES));
+    hipLaunchKernelGGL((k_zpass<M, false>), grid, dim3(ZF::THREADS), 0, s, zp, tb, tw);
     PM_LAUNCHED("fft_z_r2c", s);
   }
   return PM_SUCCESS;
